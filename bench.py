@@ -1,0 +1,221 @@
+#!/usr/bin/env python3
+"""Benchmark: gray 2-D heterogeneous Marshak IMC, 200x200 cells, 1e7 particles
+per step, FP32-track/FP64-tally against plain FP64, on one MI355X.
+
+One "step" is one IMC time step (Fleck factors, boundary + emission sourcing,
+transport of every particle to census, material update, comb) of the
+200x200 problem with the census resident in HBM.  W warm-up steps run on a
+separate seed; the K timed steps then start from t = 0.
+
+Prints one JSON line.  ``value`` = transported segments per second of the
+FP32-track step (whole step, host orchestration included).  FOM = 1/(R^2 t)
+with R^2 the median per-cell relative variance of the final-step absorbed
+energy over ``--fom-seeds`` independent Philox seeds and t the timed wall time.
+The CPU baseline is the FP64 C oracle (OpenMP) on a bounded sample of the same
+workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "IMC FOM & segments/s, 2D 200² cells 1e7 particles: FP32-track vs FP64 GPU"
+BYTES_PER_SEGMENT = {"fp32": 104, "fp64": 152}
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def run_gpu(prob, precision, steps, warmup, fom_seeds):
+    from radtranbook_amd import IMC2D
+
+    IMC2D.VERBOSE = False
+    e1, e2 = prob.edges1, prob.edges2
+    shape = prob.shape
+    T0 = np.full(shape, prob.T0)
+    src = np.zeros(shape)
+    opac, eos = prob.opacity, prob.eos
+
+    def one_run(seed, nsteps, timed):
+        st = IMC2D.init_simulation(prob.Ntarget, T0, T0, e1, e2, eos.eos, eos.inv_eos, geometry=prob.geometry,
+                                   precision=precision, seed=seed)
+        infos = []
+        t0 = time.perf_counter()
+        for s in range(nsteps):
+            st, info = IMC2D.step(st, prob.Ntarget, prob.Nboundary, 0, prob.Nmax, prob.T_boundary, prob.dt, e1, e2,
+                                  opac, eos.inv_eos, eos.cv_func, src, reflect=prob.reflect, geometry=prob.geometry,
+                                  max_events_per_particle=prob.max_events)
+            infos.append(info)
+            if timed:
+                te = info["profiling"]["transport_events"]
+                log(f"  [{precision}] step {s + 1}/{nsteps}: {info['profiling']['phase_times_s']['total'] * 1e3:.1f} ms,"
+                    f" transport kernel {info['profiling']['transport_kernel_ms']:.1f} ms, segments {te['total']:.3e},"
+                    f" histories {te['n_particles_transported']:.3e}")
+        wall = time.perf_counter() - t0
+        st._engine.close()
+        return infos, wall
+
+    if warmup > 0:
+        log(f"[{precision}] warmup {warmup} steps")
+        one_run(1000, warmup, False)
+    log(f"[{precision}] timed {steps} steps")
+    infos, wall = one_run(1, steps, True)
+    segs = sum(i["profiling"]["transport_events"]["total"] for i in infos)
+    hist = sum(i["profiling"]["transport_events"]["n_particles_transported"] for i in infos)
+    kern_ms = np.array([i["profiling"]["transport_kernel_ms"] for i in infos])
+    seg_per_launch = np.array([i["profiling"]["transport_events"]["total"] for i in infos])
+    residual = max(abs(i["energy_residual"]) / i["total_energy"] for i in infos)
+    deps = [infos[-1]["deposited"]]
+    for s in range(2, fom_seeds + 1):
+        log(f"[{precision}] FOM seed {s}/{fom_seeds}")
+        inf_s, _ = one_run(s, steps, False)
+        deps.append(inf_s[-1]["deposited"])
+    deps = np.array(deps)
+    mean = deps.mean(0)
+    mask = mean > 1e-3 * mean.max()
+    relvar = deps.var(0, ddof=1)[mask] / mean[mask] ** 2 if len(deps) > 1 else np.array([np.nan])
+    R2 = float(np.median(relvar))
+    fom = 1.0 / (R2 * wall) if np.isfinite(R2) and R2 > 0 else None
+    return dict(wall=wall, segments=int(segs), histories=int(hist), kernel_ms=kern_ms, seg_per_launch=seg_per_launch,
+                R2=R2, fom=fom, residual=residual, final_dep=mean, n_cells_fom=int(mask.sum()))
+
+
+def run_cpu_baseline(prob, n_particles, nsteps):
+    from oracle import oracle as O
+
+    O.build()
+    cores = len(os.sched_getaffinity(0))
+    e1, e2 = prob.edges1, prob.edges2
+    shape = prob.shape
+    T0 = np.full(shape, prob.T0)
+    orc = O.Oracle2D(e1, e2, prob.geometry, seed=1, nthreads=cores)
+    N = int(n_particles)
+    st = orc.init_simulation(N // 2, T0, T0, prob.eos.eos, prob.eos.inv_eos)
+    segs = 0
+    tt = 0.0
+    for s in range(nsteps):
+        t0 = time.perf_counter()
+        st, info = orc.step(st, N // 2, N // 2, 0, N, prob.T_boundary, prob.dt, prob.opacity, prob.eos.inv_eos,
+                            prob.eos.cv_func, np.zeros(shape), reflect=prob.reflect)
+        dt = time.perf_counter() - t0
+        if s >= 1:  # first step is the warm-up
+            segs += int(info["stats"][0])
+            tt += dt
+        log(f"  [cpu] step {s + 1}/{nsteps}: {dt:.2f} s, segments {int(info['stats'][0]):.3e}")
+    return dict(value=segs / tt if tt > 0 else None, cores=cores, seconds=tt, segments=segs, N=N, steps=nsteps)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--particles", type=float, default=1e7)
+    ap.add_argument("--cells", type=int, default=200)
+    ap.add_argument("--fom-seeds", type=int, default=8)
+    ap.add_argument("--precision", choices=["both", "fp32", "fp64"], default="both")
+    ap.add_argument("--cpu-particles", type=float, default=2e5)
+    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py runs on exactly one GPU; --gpus {args.gpus} is not supported "
+              "(the IMC step shards by particles with one tally all-reduce per step; that path is not built)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from radtranbook_amd import _lib
+    from radtranbook_amd.problems import marshak_2d_heterogeneous
+
+    _lib.load()
+    _lib.require_gpu()
+    prob = marshak_2d_heterogeneous(n=args.cells, n_particles=int(args.particles), n_steps=args.steps)
+    precs = ["fp32", "fp64"] if args.precision == "both" else [args.precision]
+    res = {}
+    for p in precs:
+        res[p] = run_gpu(prob, p, args.steps, args.warmup, args.fom_seeds)
+    head = res[precs[0]]
+    value = head["segments"] / head["wall"]
+    kern_s = head["kernel_ms"].sum() / 1e3
+    seg_per_launch = head["seg_per_launch"].mean()
+    avg_launch_s = head["kernel_ms"].mean() / 1e3
+    achieved = seg_per_launch * BYTES_PER_SEGMENT[precs[0]] / avg_launch_s / 1e9
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "segments/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["wall"] / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 geometry / f64 weights+tallies" if precs[0] == "fp32" else "f64",
+        "data": "synthetic (seeded sigma0 field, Philox particle streams)",
+        "config": {
+            "workload": f"gray IMC 2D xy {args.cells}x{args.cells} heterogeneous Marshak, "
+                        f"{args.particles:.0e} particles/step, dt=0.005 ns x {args.steps} steps",
+            "cells": args.cells * args.cells,
+            "particles_per_step": int(args.particles),
+            "census_Nmax": int(args.particles),
+        },
+        "histories_per_s": head["histories"] / head["wall"],
+        "transport_segments_per_s": head["segments"] / kern_s,
+        "fom_seeds": args.fom_seeds,
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "kernel": f"k_transport<{'float' if precs[0] == 'fp32' else 'double'},xy>",
+            "bytes_per_segment": BYTES_PER_SEGMENT[precs[0]],
+        },
+    }
+    for p in precs:
+        r = res[p]
+        out[p] = {
+            "segments_per_s": r["segments"] / r["wall"],
+            "ms_per_step": r["wall"] / args.steps * 1e3,
+            "transport_kernel_ms_per_step": float(r["kernel_ms"].mean()),
+            "segments_per_step": float(r["seg_per_launch"].mean()),
+            "R2_median": r["R2"],
+            "fom": r["fom"],
+            "max_energy_residual_rel": r["residual"],
+        }
+    if "fp32" in res and "fp64" in res:
+        if res["fp32"]["fom"] and res["fp64"]["fom"]:
+            out["fom_ratio_fp32_over_fp64"] = res["fp32"]["fom"] / res["fp64"]["fom"]
+        a, b = res["fp32"]["final_dep"], res["fp64"]["final_dep"]
+        m = b > 1e-3 * b.max()
+        out["fp32_vs_fp64_final_dep_max_rel"] = float(np.max(np.abs(a[m] - b[m]) / b[m]))
+    if not args.no_cpu:
+        cb = run_cpu_baseline(prob, args.cpu_particles, args.cpu_steps)
+        out["cpu_baseline"] = {
+            "value": cb["value"],
+            "unit": "segments/s",
+            "cores": cb["cores"],
+            "kind": "port",
+            "sample": f"FP64 C oracle (OpenMP, history-based), same 200x200 problem at {cb['N']:.0e} particles/step, "
+                      f"steps 2..{cb['steps']} timed ({cb['segments']:.3e} segments in {cb['seconds']:.1f} s)",
+        }
+        if cb["value"]:
+            out["gpu_over_cpu_segments_per_s"] = value / cb["value"]
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

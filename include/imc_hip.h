@@ -1,0 +1,123 @@
+/*
+ * imc_hip.h -- C-ABI of libimc_hip.so, the MI355X (gfx950) implicit Monte Carlo
+ * particle engine.  Plain pointers and sizes only; every function returns an int
+ * status (0 = ok, < 0 = error, message from imc_last_error()) unless noted.
+ *
+ * Conventions
+ *  - Host arrays passed in are copied, never retained.  Per-cell arrays are
+ *    (n1, n2) C-order doubles (index i*n2 + j), the layout of the reference's
+ *    NumPy material arrays (IMC/IMC2D.py:65-84).
+ *  - Particles stay resident in device memory owned by the context; the census
+ *    is observable through *_get_census / *_put_census.
+ *  - One host thread per context; calls are ordered on the context's stream and
+ *    every call that returns host data synchronises.
+ *  - Random numbers: counter-based Philox4x32-10 keyed by (seed, stream tag,
+ *    particle index, step, block).  The reference draws from Numba's hidden
+ *    per-thread MT19937 instead (IMC/IMC2D.py:697-707,756-765).
+ */
+#ifndef IMC_HIP_H
+#define IMC_HIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* geometry codes (IMC/IMC2D.py:51-52) */
+#define IMC_GEOM_XY 0
+#define IMC_GEOM_RZ 1
+
+/* arithmetic of the tracker: FP64 throughout, or FP32 geometry with FP64
+ * weights, tallies and bookkeeping */
+#define IMC_FP64 0
+#define IMC_FP32_TRACK 1
+
+/* volume-source kinds */
+#define IMC_SRC_EMISSION 0
+#define IMC_SRC_FIXED 1
+
+typedef struct imc2d_ctx imc2d_ctx;
+
+/* Message of the last failed call on this thread. */
+const char* imc_last_error(void);
+/* Number of visible GPUs (0 if none). */
+int imc_device_count(void);
+
+/* ---- gray 2-D (xy / rz) ------------------------------------------------ */
+
+/* Context for one mesh.  Replaces the module-level state set up by
+ * IMC2D.init_simulation (IMC/IMC2D.py:1514-1588): edges, cell volumes
+ * (IMC/IMC2D.py:87-106) and the particle arrays of SimulationState2D
+ * (IMC/IMC2D.py:65-84). */
+int imc2d_create(imc2d_ctx** out, int geometry, int nx, int ny, const double* edges1, const double* edges2,
+                 int precision, uint64_t seed, int device);
+void imc2d_destroy(imc2d_ctx* ctx);
+
+/* Step number used as the Philox counter word of every draw that follows. */
+int imc2d_set_step(imc2d_ctx* ctx, uint32_t step);
+/* Transport launch shape: grid 0 = occupancy-derived persistent grid;
+ * batch = particles a wave claims per queue visit. */
+int imc2d_set_launch(imc2d_ctx* ctx, int grid, int batch);
+
+/* Census access: the particle arrays of SimulationState2D
+ * (weights, dir1, dir2, times, pos1, pos2, cell_i, cell_j; IMC/IMC2D.py:68-75). */
+int64_t imc2d_census_size(imc2d_ctx* ctx);
+int imc2d_put_census(imc2d_ctx* ctx, int64_t n, const double* w, const double* d1, const double* d2,
+                     const double* t, const double* p1, const double* p2, const int64_t* ci, const int64_t* cj);
+int imc2d_get_census(imc2d_ctx* ctx, int64_t n, double* w, double* d1, double* d2, double* t, double* p1,
+                     double* p2, int64_t* ci, int64_t* cj);
+/* Sum of census weights (the "Total Radiation Energy" of IMC/IMC2D.py:1833). */
+int imc2d_census_energy(imc2d_ctx* ctx, double* out);
+
+/* Census energy per cell (np.bincount of the weights by cell,
+ * IMC/IMC2D.py:1543-1545). */
+int imc2d_census_by_cell(imc2d_ctx* ctx, double* out);
+
+/* Fleck factor and effective opacities, IMC/IMC2D.py:1646-1651:
+ * beta = 4 a T^3 / c_v, f = 1/(1 + theta beta sigma c dt), sigma_s = (1-f) sigma,
+ * sigma_a = f sigma.  clip=1 clips f to [0,1] (2-D); clip=0 fails if f leaves
+ * [0,1] (IMC/IMC1D.py:936).  Outputs are optional. */
+int imc2d_set_material(imc2d_ctx* ctx, const double* T, const double* sigma_true, const double* cv, double theta,
+                       double dt, int clip, double* sigma_a_out, double* sigma_s_out);
+
+/* Equilibrium census at t=0, replacing the pool: _equilibrium_sample_xy/_rz
+ * (IMC/IMC2D.py:1110-1188). */
+int imc2d_equilibrium_sample(imc2d_ctx* ctx, int64_t N, const double* Tr);
+
+/* Half-Lambertian face source appended to the pool: _sample_boundary_xy/_rz
+ * (IMC/IMC2D.py:157-489).  side: 0 left/rmin, 1 right/rmax, 2 bottom/zmin,
+ * 3 top/zmax.  E_total: face energy a c T^4/4 A dt.  prof_cum (nullable):
+ * running sum of per-face-cell emission for a position-dependent face
+ * temperature (boundary_source_func, IMC/IMC2D.py:182-281). */
+int imc2d_sample_boundary(imc2d_ctx* ctx, int side, int64_t n, double E_total, double dt, const double* prof_cum,
+                          int nface);
+
+/* Volume source appended to the pool: material emission
+ * (IMC/IMC2D.py:1732-1764 -> _sample_source_xy 1191-1238,
+ * _sample_source_rz_linear 1326-1445, _sample_source_rz_uniform 1241-1290) or
+ * a fixed source (IMC/IMC2D.py:1708-1728).  Counts ceil(N E_c / E).
+ * E_out (nullable): per-cell source energy; total_out: its sum; count_out:
+ * particles added. */
+int imc2d_sample_volume(imc2d_ctx* ctx, int kind, int64_t N, const double* QV_host, double dt, int rz_linear,
+                        double* E_out, double* total_out, int64_t* count_out);
+
+/* Transport every particle in the pool to census, escape or the weight floor:
+ * _transport_particles_2d (IMC/IMC2D.py:938-1107).  reflect[4] per side;
+ * weight_floor < 0 means 1e-10 * sum(w) / N (IMC/IMC2D.py:1771).  Outputs:
+ * dep, si per cell (nullable), boundary loss, stats[8] = events, boundary
+ * crossings, scatters, census events, captures (0), floor kills,
+ * reflections, event-cap hits. */
+int imc2d_transport(imc2d_ctx* ctx, double dt, const int32_t* reflect, int64_t max_events, double weight_floor,
+                    double* dep_out, double* si_out, double* bl_out, int64_t* stats_out);
+/* Device time of the last transport kernel in milliseconds (HIP events). */
+float imc2d_last_transport_ms(imc2d_ctx* ctx);
+
+/* Per-cell stochastic comb to about Nmax particles: _comb (IMC/IMC2D.py:1448-1511).
+ * ecen_out: pre-comb census energy per cell; disc_out: ecen - ecen_after;
+ * total_rad_out: census energy after the comb.  Times reset to 0. */
+int imc2d_comb(imc2d_ctx* ctx, int64_t Nmax, double* ecen_out, double* disc_out, double* total_rad_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

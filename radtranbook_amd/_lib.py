@@ -1,0 +1,209 @@
+"""ctypes binding of ``radtranbook_amd/lib/libimc_hip.so`` (C-ABI in include/imc_hip.h).
+
+There is no CPU fallback: every entry point needs the HIP library and a GPU,
+and raises ``ImcError`` when either is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libimc_hip.so")
+
+GEOM_XY = 0
+GEOM_RZ = 1
+FP64 = 0
+FP32_TRACK = 1
+SRC_EMISSION = 0
+SRC_FIXED = 1
+
+A_RAD = 0.01372
+C_LIGHT = 29.98
+
+_dp = C.POINTER(C.c_double)
+_i64p = C.POINTER(C.c_int64)
+_i32p = C.POINTER(C.c_int32)
+_vp = C.c_void_p
+
+# name -> (restype, argtypes); mirrors include/imc_hip.h
+SIGNATURES = {
+    "imc_last_error": (C.c_char_p, []),
+    "imc_device_count": (C.c_int, []),
+    "imc2d_create": (C.c_int, [C.POINTER(_vp), C.c_int, C.c_int, C.c_int, _dp, _dp, C.c_int, C.c_uint64, C.c_int]),
+    "imc2d_destroy": (None, [_vp]),
+    "imc2d_set_step": (C.c_int, [_vp, C.c_uint32]),
+    "imc2d_set_launch": (C.c_int, [_vp, C.c_int, C.c_int]),
+    "imc2d_census_size": (C.c_int64, [_vp]),
+    "imc2d_put_census": (C.c_int, [_vp, C.c_int64, _dp, _dp, _dp, _dp, _dp, _dp, _i64p, _i64p]),
+    "imc2d_get_census": (C.c_int, [_vp, C.c_int64, _dp, _dp, _dp, _dp, _dp, _dp, _i64p, _i64p]),
+    "imc2d_census_energy": (C.c_int, [_vp, _dp]),
+    "imc2d_census_by_cell": (C.c_int, [_vp, _dp]),
+    "imc2d_set_material": (C.c_int, [_vp, _dp, _dp, _dp, C.c_double, C.c_double, C.c_int, _dp, _dp]),
+    "imc2d_equilibrium_sample": (C.c_int, [_vp, C.c_int64, _dp]),
+    "imc2d_sample_boundary": (C.c_int, [_vp, C.c_int, C.c_int64, C.c_double, C.c_double, _dp, C.c_int]),
+    "imc2d_sample_volume": (C.c_int, [_vp, C.c_int, C.c_int64, _dp, C.c_double, C.c_int, _dp, _dp, _i64p]),
+    "imc2d_transport": (C.c_int, [_vp, C.c_double, _i32p, C.c_int64, C.c_double, _dp, _dp, _dp, _i64p]),
+    "imc2d_last_transport_ms": (C.c_float, [_vp]),
+    "imc2d_comb": (C.c_int, [_vp, C.c_int64, _dp, _dp, _dp]),
+}
+
+
+class ImcError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load():
+    """Load the HIP library (raises ImcError if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImcError(f"{LIB_PATH} is missing: build it with `python -m radtranbook_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        msg = load().imc_last_error()
+        raise ImcError(f"libimc_hip error {rc}: {msg.decode() if msg else ''}")
+
+
+def require_gpu():
+    n = load().imc_device_count()
+    if n < 1:
+        raise ImcError("no GPU visible: the IMC engine runs only on the GPU")
+    return n
+
+
+def _p(a, ct=_dp):
+    return a.ctypes.data_as(ct) if a is not None else None
+
+
+def _cell(a, shape):
+    return np.ascontiguousarray(np.broadcast_to(np.asarray(a, dtype=np.float64), shape), dtype=np.float64)
+
+
+class Engine2D:
+    """Device context for one 2-D mesh (wraps ``imc2d_ctx``)."""
+
+    def __init__(self, edges1, edges2, geometry="xy", precision="fp64", seed=1, device=0):
+        L = load()
+        require_gpu()
+        self.e1 = np.ascontiguousarray(edges1, dtype=np.float64)
+        self.e2 = np.ascontiguousarray(edges2, dtype=np.float64)
+        self.nx = self.e1.size - 1
+        self.ny = self.e2.size - 1
+        self.shape = (self.nx, self.ny)
+        self.geometry = geometry
+        self.gcode = {"xy": GEOM_XY, "rz": GEOM_RZ}[geometry]
+        self.precision = {"fp64": FP64, "fp32": FP32_TRACK}[precision]
+        self.seed = int(seed)
+        h = _vp()
+        check(L.imc2d_create(C.byref(h), self.gcode, self.nx, self.ny, _p(self.e1), _p(self.e2), self.precision,
+                             self.seed, device))
+        self._h = h
+        self._L = L
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._L.imc2d_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- configuration -------------------------------------------------------
+    def set_step(self, step):
+        check(self._L.imc2d_set_step(self._h, int(step) & 0xFFFFFFFF))
+
+    def set_launch(self, grid=0, batch=256):
+        check(self._L.imc2d_set_launch(self._h, int(grid), int(batch)))
+
+    # -- census ---------------------------------------------------------------
+    @property
+    def n(self):
+        return int(self._L.imc2d_census_size(self._h))
+
+    def put_census(self, w, d1, d2, t, p1, p2, ci, cj):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (w, d1, d2, t, p1, p2)]
+        ints = [np.ascontiguousarray(a, dtype=np.int64) for a in (ci, cj)]
+        n = arrs[0].size
+        check(self._L.imc2d_put_census(self._h, n, *[_p(a) for a in arrs], *[_p(a, _i64p) for a in ints]))
+
+    def get_census(self):
+        n = self.n
+        f = [np.empty(n) for _ in range(6)]
+        i = [np.empty(n, dtype=np.int64) for _ in range(2)]
+        check(self._L.imc2d_get_census(self._h, n, *[_p(a) for a in f], *[_p(a, _i64p) for a in i]))
+        return tuple(f) + tuple(i)
+
+    def census_energy(self):
+        out = C.c_double()
+        check(self._L.imc2d_census_energy(self._h, C.byref(out)))
+        return out.value
+
+    def census_by_cell(self):
+        out = np.empty(self.shape)
+        check(self._L.imc2d_census_by_cell(self._h, _p(out)))
+        return out
+
+    # -- step stages ------------------------------------------------------------
+    def set_material(self, T, sigma_true, cv, theta, dt, clip=True, want_opacity=False):
+        T = _cell(T, self.shape); s = _cell(sigma_true, self.shape); c = _cell(cv, self.shape)
+        sa = np.empty(self.shape) if want_opacity else None
+        ss = np.empty(self.shape) if want_opacity else None
+        check(self._L.imc2d_set_material(self._h, _p(T), _p(s), _p(c), float(theta), float(dt), 1 if clip else 0,
+                                         _p(sa), _p(ss)))
+        return sa, ss
+
+    def equilibrium_sample(self, N, Tr):
+        Tr = _cell(Tr, self.shape)
+        check(self._L.imc2d_equilibrium_sample(self._h, int(N), _p(Tr)))
+
+    def sample_boundary(self, side, n, E_total, dt, prof_cum=None):
+        pc = None if prof_cum is None else np.ascontiguousarray(prof_cum, dtype=np.float64)
+        nface = 0 if pc is None else pc.size
+        check(self._L.imc2d_sample_boundary(self._h, int(side), int(n), float(E_total), float(dt), _p(pc), nface))
+
+    def sample_volume(self, kind, N, QV=None, dt=0.0, rz_linear=True, want_energy=False):
+        q = None if QV is None else _cell(QV, self.shape)
+        E = np.empty(self.shape) if want_energy else None
+        tot = C.c_double()
+        cnt = C.c_int64()
+        check(self._L.imc2d_sample_volume(self._h, int(kind), int(N), _p(q), float(dt), 1 if rz_linear else 0,
+                                          _p(E), C.byref(tot), C.byref(cnt)))
+        return E, tot.value, cnt.value
+
+    def transport(self, dt, reflect, max_events, weight_floor=-1.0, want_tallies=True):
+        refl = np.array([1 if r else 0 for r in reflect], dtype=np.int32)
+        dep = np.empty(self.shape) if want_tallies else None
+        si = np.empty(self.shape) if want_tallies else None
+        bl = C.c_double()
+        st = np.zeros(8, dtype=np.int64)
+        check(self._L.imc2d_transport(self._h, float(dt), _p(refl, _i32p), int(max_events), float(weight_floor),
+                                      _p(dep), _p(si), C.byref(bl), _p(st, _i64p)))
+        return dep, si, bl.value, st
+
+    def last_transport_ms(self):
+        return float(self._L.imc2d_last_transport_ms(self._h))
+
+    def comb(self, Nmax, want_cells=True):
+        ecen = np.empty(self.shape) if want_cells else None
+        disc = np.empty(self.shape) if want_cells else None
+        tr = C.c_double()
+        check(self._L.imc2d_comb(self._h, int(Nmax), _p(ecen), _p(disc), C.byref(tr)))
+        return ecen, disc, tr.value

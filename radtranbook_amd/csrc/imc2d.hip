@@ -1,0 +1,1268 @@
+// imc2d.hip -- gray 2-D (xy / rz) Fleck-Cummings IMC step on one MI355X.
+//
+// Device-resident particle pool (structure of arrays), counter-based Philox
+// sourcing, a persistent wave64 tracker with ballot-driven lane refill, the
+// per-cell census comb and the per-cell Fleck/emission kernels.  The C-ABI at
+// the bottom is declared in include/imc_hip.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "../../include/imc_hip.h"
+#include "imc2d_kernels.hpp"
+#include "imc_device.hpp"
+#include "imc_util.hpp"
+
+namespace imc {
+
+// ---------------------------------------------------------------------------
+// particle pool
+// ---------------------------------------------------------------------------
+template <typename R>
+struct PoolView {
+  double* w;
+  R* d1;
+  R* d2;
+  R* t;
+  R* p1;
+  R* p2;
+  int32_t* ci;
+  int32_t* cj;
+};
+
+struct PoolBuf {
+  double* w = nullptr;
+  void* d1 = nullptr;
+  void* d2 = nullptr;
+  void* t = nullptr;
+  void* p1 = nullptr;
+  void* p2 = nullptr;
+  int32_t* ci = nullptr;
+  int32_t* cj = nullptr;
+  int64_t cap = 0;
+  int rsz = 8;
+
+  template <typename R>
+  PoolView<R> view() const {
+    return PoolView<R>{w, (R*)d1, (R*)d2, (R*)t, (R*)p1, (R*)p2, ci, cj};
+  }
+
+  int grow(int64_t n, int64_t keep, hipStream_t s) {
+    if (n <= cap) return kOk;
+    int64_t nc = std::max<int64_t>(n, cap + cap / 4);
+    void* arr[8] = {w, d1, d2, t, p1, p2, ci, cj};
+    size_t esz[8] = {8, (size_t)rsz, (size_t)rsz, (size_t)rsz, (size_t)rsz, (size_t)rsz, 4, 4};
+    void* nw[8];
+    for (int a = 0; a < 8; ++a) {
+      IMC_HIP(hipMalloc(&nw[a], esz[a] * (size_t)nc));
+      if (arr[a] && keep > 0) IMC_HIP(hipMemcpyAsync(nw[a], arr[a], esz[a] * (size_t)keep, hipMemcpyDeviceToDevice, s));
+    }
+    IMC_HIP(hipStreamSynchronize(s));
+    for (int a = 0; a < 8; ++a)
+      if (arr[a]) IMC_HIP(hipFree(arr[a]));
+    w = (double*)nw[0]; d1 = nw[1]; d2 = nw[2]; t = nw[3]; p1 = nw[4]; p2 = nw[5];
+    ci = (int32_t*)nw[6]; cj = (int32_t*)nw[7];
+    cap = nc;
+    return kOk;
+  }
+  void release() {
+    void* arr[8] = {w, d1, d2, t, p1, p2, ci, cj};
+    for (int a = 0; a < 8; ++a)
+      if (arr[a]) (void)hipFree(arr[a]);
+    w = nullptr; d1 = d2 = t = p1 = p2 = nullptr; ci = cj = nullptr;
+    cap = 0;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// per-cell kernels (no FMA contraction: counts and weights must match the
+// oracle bit for bit)
+// ---------------------------------------------------------------------------
+__global__ void k_fleck(int64_t n, const double* __restrict__ T, const double* __restrict__ sig,
+                        const double* __restrict__ cv, double theta, double dt, int clip, double* __restrict__ sa,
+                        double* __restrict__ ss, float* __restrict__ saf, float* __restrict__ ssf,
+                        int* __restrict__ bad) {
+#pragma clang fp contract(off)
+  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  double t = T[c];
+  double beta = 4.0 * kA * (t * t * t) / cv[c];
+  double f = 1.0 / (1.0 + theta * beta * sig[c] * kC * dt);
+  if (clip) {
+    if (f < 0.0) f = 0.0;
+    if (f > 1.0) f = 1.0;
+  } else if (!(f >= 0.0 && f <= 1.0)) {
+    atomicOr(bad, 1);
+  }
+  double s_s = sig[c] * (1.0 - f);
+  double s_a = sig[c] * f;
+  sa[c] = s_a;
+  ss[c] = s_s;
+  if (saf) {
+    saf[c] = (float)s_a;
+    ssf[c] = (float)s_s;
+  }
+}
+
+__global__ void k_emission_energy(int64_t n, const double* __restrict__ T, const double* __restrict__ sa,
+                                  const double* __restrict__ vol, double dt, double* __restrict__ E) {
+#pragma clang fp contract(off)
+  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  double t = T[c] > 0.0 ? T[c] : 0.0;
+  double t2 = t * t;
+  double t4 = t2 * t2;
+  E[c] = kA * kC * t4 * sa[c] * dt * vol[c];
+}
+
+// a (T_r) source for the equilibrium census: a T^4 V
+__global__ void k_equilibrium_energy(int64_t n, const double* __restrict__ Tr, const double* __restrict__ vol,
+                                     double* __restrict__ E) {
+#pragma clang fp contract(off)
+  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  double t = Tr[c] > 0.0 ? Tr[c] : 0.0;
+  double t2 = t * t;
+  E[c] = kA * (t2 * t2) * vol[c];
+}
+
+__global__ void k_counts(int64_t n, const double* __restrict__ E, int64_t N, int min_one,
+                         const double* __restrict__ Etot, int64_t* __restrict__ counts) {
+#pragma clang fp contract(off)
+  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  double et = *Etot;
+  int64_t k;
+  if (!min_one && !(et > 0.0)) {
+    k = 0;
+  } else {
+    double den = min_one ? (et > 1e-300 ? et : 1e-300) : et;
+    k = (int64_t)ceil((double)N * E[c] / den);
+    if (min_one && k < 1) k = 1;
+  }
+  counts[c] = k;
+}
+
+__device__ __forceinline__ double t4_of(double t) {
+  double u = t > 0.0 ? t : 0.0;
+  double u2 = u * u;
+  return u2 * u2;
+}
+
+// rz linear-in-r / linear-in-z tilt parameters per cell (IMC/IMC2D.py:1364-1418)
+__global__ void k_rz_params(int nx, int ny, const double* __restrict__ T, const double* __restrict__ e1,
+                            const double* __restrict__ e2, double* __restrict__ Ar, double* __restrict__ Mr_out,
+                            double* __restrict__ Pn_out) {
+#pragma clang fp contract(off)
+  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= (int64_t)nx * ny) return;
+  int i = (int)(c / ny), j = (int)(c % ny);
+  auto T4 = [&](int ii, int jj) { return t4_of(T[(int64_t)ii * ny + jj]); };
+  auto RC = [&](int ii) { return 0.5 * (e1[ii] + e1[ii + 1]); };
+  auto ZC = [&](int jj) { return 0.5 * (e2[jj] + e2[jj + 1]); };
+  double mr, mz;
+  if (nx == 1)
+    mr = 0.0;
+  else if (i == 0)
+    mr = (T4(i + 1, j) - T4(i, j)) / (RC(i + 1) - RC(i));
+  else if (i == nx - 1)
+    mr = (T4(i, j) - T4(i - 1, j)) / (RC(i) - RC(i - 1));
+  else
+    mr = (T4(i + 1, j) - T4(i - 1, j)) / (RC(i + 1) - RC(i - 1));
+  if (ny == 1)
+    mz = 0.0;
+  else if (j == 0)
+    mz = (T4(i, j + 1) - T4(i, j)) / (ZC(j + 1) - ZC(j));
+  else if (j == ny - 1)
+    mz = (T4(i, j) - T4(i, j - 1)) / (ZC(j) - ZC(j - 1));
+  else
+    mz = (T4(i, j + 1) - T4(i, j - 1)) / (ZC(j + 1) - ZC(j - 1));
+  double x0 = e1[i], x1 = e1[i + 1], y0 = e2[j], y1 = e2[j + 1];
+  double T4c = T4(i, j);
+  double rcen = 0.5 * (x0 + x1), zcen = 0.5 * (y0 + y1);
+  double a_r = T4c - mr * (rcen - x0);
+  double t4r0 = a_r, t4r1 = a_r + mr * (x1 - x0);
+  double mloc;
+  if (t4r0 < 0.0 || t4r1 < 0.0) {
+    a_r = T4c > 0.0 ? T4c : 0.0;
+    mloc = 0.0;
+  } else {
+    mloc = mr;
+  }
+  double a_z = T4c - mz * (zcen - y0);
+  double t4z0 = a_z > 0.0 ? a_z : 0.0;
+  double t4z1 = a_z + mz * (y1 - y0);
+  t4z1 = t4z1 > 0.0 ? t4z1 : 0.0;
+  double den = t4z0 + t4z1;
+  Ar[c] = a_r;
+  Mr_out[c] = mloc;
+  Pn_out[c] = den > 0.0 ? t4z0 / den : 0.5;
+}
+
+__device__ double solve_r_linear_cdf(double r0, double r1, double a, double M, double xi) {
+#pragma clang fp contract(off)
+  double C = (a * 0.5 * (r1 * r1 - r0 * r0) + M / 3.0 * (r1 * r1 * r1 - r0 * r0 * r0));
+  if (C <= 0.0) return sqrt(r0 * r0 + (r1 * r1 - r0 * r0) * xi);
+  double rhs = xi * C + a * 0.5 * (r0 * r0) + M / 3.0 * (r0 * r0 * r0);
+  double lo = r0, hi = r1;
+  double r = sqrt(r0 * r0 + (r1 * r1 - r0 * r0) * xi);
+  for (int it = 0; it < 50; ++it) {
+    double g = a * 0.5 * r * r + M / 3.0 * (r * r * r) - rhs;
+    if (fabs(g) < 1e-14 * (fabs(rhs) + 1e-30)) break;
+    if (g < 0.0)
+      lo = r;
+    else
+      hi = r;
+    double dg = a * r + M * r * r;
+    double rn = (fabs(dg) > 1e-30) ? r - g / dg : 0.5 * (lo + hi);
+    if (rn <= lo || rn >= hi) rn = 0.5 * (lo + hi);
+    if (fabs(rn - r) < 1e-14 * (r1 - r0 + 1e-30)) {
+      r = rn;
+      break;
+    }
+    r = rn;
+  }
+  return r;
+}
+
+// first index m in [0,n) with offs[m+1] > k  (offs has n+1 entries)
+__device__ __forceinline__ int64_t find_bin(const int64_t* __restrict__ offs, int64_t n, int64_t k) {
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    int64_t mid = (lo + hi) >> 1;
+    if (offs[mid + 1] > k)
+      hi = mid;
+    else
+      lo = mid + 1;
+  }
+  return lo;
+}
+
+// Volume sources: emission, fixed source, equilibrium census.
+// geometry 0 xy, 1 rz uniform, 2 rz linear; kind 1 = equilibrium (t = 0).
+template <typename R>
+__global__ void k_sample_volume(PoolView<R> P, int64_t base, int64_t total, const int64_t* __restrict__ offs,
+                                int64_t ncell, int ny, const double* __restrict__ e1, const double* __restrict__ e2,
+                                const double* __restrict__ QV, int kind, int geometry, const double* __restrict__ Ar,
+                                const double* __restrict__ Mr, const double* __restrict__ Pn, double dt, Key key,
+                                uint32_t step, uint32_t tag) {
+#pragma clang fp contract(off)
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= total) return;
+  int64_t c = find_bin(offs, ncell, k);
+  int64_t n = offs[c + 1] - offs[c];
+  int i = (int)(c / ny), j = (int)(c % ny);
+  double x0 = e1[i], x1 = e1[i + 1], y0 = e2[j], y1 = e2[j + 1];
+  Ctr ctr = make_ctr(tag, (uint64_t)k, step);
+  Pair b0 = block_d(ctr, key, 0), b1 = block_d(ctr, key, 1), b2 = block_d(ctr, key, 2);
+  double t = (kind == 1) ? 0.0 : dt * b0.u0;
+  double p1, p2, d1, d2;
+  if (geometry == 0) {
+    p1 = x0 + (x1 - x0) * b0.u1;
+    p2 = y0 + (y1 - y0) * b1.u0;
+    double uz = 2.0 * b1.u1 - 1.0;
+    double phi = 2.0 * kPi * b2.u0;
+    double rxy = sqrt(fmax(0.0, 1.0 - uz * uz));
+    d1 = rxy * cos(phi);
+    d2 = rxy * sin(phi);
+  } else if (geometry == 1) {
+    p1 = sqrt(x0 * x0 + (x1 * x1 - x0 * x0) * b0.u1);
+    p2 = y0 + (y1 - y0) * b1.u0;
+    d2 = 2.0 * b1.u1 - 1.0;
+    d1 = cos(2.0 * kPi * b2.u0);
+  } else {
+    double dz = y1 - y0;
+    p1 = solve_r_linear_cdf(x0, x1, Ar[c], Mr[c], b0.u1);
+    double xb = b1.u0, xs = b1.u1;
+    p2 = (xb <= Pn[c]) ? (y1 - dz * sqrt(xs)) : (y0 + dz * sqrt(xs));
+    d2 = 2.0 * b2.u0 - 1.0;
+    d1 = cos(2.0 * kPi * b2.u1);
+  }
+  int64_t o = base + k;
+  P.w[o] = QV[c] / (double)n;
+  P.t[o] = (R)t;
+  P.p1[o] = (R)p1;
+  P.p2[o] = (R)p2;
+  P.d1[o] = (R)d1;
+  P.d2[o] = (R)d2;
+  P.ci[o] = i;
+  P.cj[o] = j;
+}
+
+__device__ __forceinline__ int locate_edge(const double* __restrict__ e, int n, double x) {
+  int lo = 0, hi = n + 1;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (e[mid] > x)
+      hi = mid;
+    else
+      lo = mid + 1;
+  }
+  int idx = lo - 1;
+  return idx < 0 ? 0 : (idx > n - 1 ? n - 1 : idx);
+}
+
+// Boundary sources, IMC/IMC2D.py:157-489.
+template <typename R>
+__global__ void k_sample_boundary(PoolView<R> P, int64_t base, int64_t n, int geometry, int side, double E_total,
+                                  double dt, const double* __restrict__ e1, int nx, const double* __restrict__ e2,
+                                  int ny, const double* __restrict__ prof, Key key, uint32_t step) {
+#pragma clang fp contract(off)
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  double x0 = e1[0], x1 = e1[nx], y0 = e2[0], y1 = e2[ny];
+  int nface = side < 2 ? ny : nx;
+  const double* fe = side < 2 ? e2 : e1;
+  Ctr ctr = make_ctr(kTagBoundary0 + side, (uint64_t)k, step);
+  Pair b0 = block_d(ctr, key, 0), b1 = block_d(ctr, key, 1), b2 = block_d(ctr, key, 2);
+  double a = side < 2 ? y0 : x0, b = side < 2 ? y1 : x1;
+  if (prof) {
+    Pair b3 = block_d(ctr, key, 3);
+    double tot = prof[nface - 1];
+    double u = b3.u0 * tot;
+    int m = 0;
+    while (m < nface - 1 && !(u < prof[m])) ++m;
+    while (m > 0 && prof[m] == prof[m - 1]) --m;
+    a = fe[m];
+    b = fe[m + 1];
+  }
+  double t = dt * b0.u0;
+  double p1, p2, d1, d2;
+  if (geometry == 0) {
+    double mun = sqrt(b0.u1);
+    double mut = sqrt(fmax(0.0, 1.0 - mun * mun));
+    double phi = 2.0 * kPi * b1.u0;
+    double along = a + (b - a) * b1.u1;
+    double ct = mut * cos(phi);
+    if (side == 0) {
+      d1 = mun; d2 = ct; p1 = x0 + 1e-12; p2 = along;
+    } else if (side == 1) {
+      d1 = -mun; d2 = ct; p1 = x1 - 1e-12; p2 = along;
+    } else if (side == 2) {
+      d2 = mun; d1 = ct; p2 = y0 + 1e-12; p1 = along;
+    } else {
+      d2 = -mun; d1 = ct; p2 = y1 - 1e-12; p1 = along;
+    }
+  } else {
+    double eta = 2.0 * b0.u1 - 1.0;
+    double muabs = sqrt(b1.u0);
+    if (side == 0) {
+      d1 = muabs; d2 = eta; p1 = x0 + 1e-12; p2 = a + (b - a) * b1.u1;
+    } else if (side == 1) {
+      d1 = -muabs; d2 = eta; p1 = x1 - 1e-12; p2 = a + (b - a) * b1.u1;
+    } else {
+      double e = sqrt(b1.u1);
+      d2 = side == 2 ? e : -e;
+      d1 = cos(2.0 * kPi * b2.u0);
+      p2 = side == 2 ? y0 + 1e-12 : y1 - 1e-12;
+      p1 = sqrt(a * a + (b * b - a * a) * b2.u1);
+    }
+  }
+  int64_t o = base + k;
+  P.w[o] = E_total / (double)n;
+  P.t[o] = (R)t;
+  P.p1[o] = (R)p1;
+  P.p2[o] = (R)p2;
+  P.d1[o] = (R)d1;
+  P.d2[o] = (R)d2;
+  P.ci[o] = locate_edge(e1, nx, p1);
+  P.cj[o] = locate_edge(e2, ny, p2);
+}
+
+// ---------------------------------------------------------------------------
+// transport: persistent waves, one segment per loop trip, lanes refilled from
+// a wave-local queue (ballot + prefix popcount), per-lane same-cell tally
+// accumulation flushed with FP64 atomics.  Follows IMC/IMC2D.py:938-1107.
+// ---------------------------------------------------------------------------
+template <typename R>
+struct TransportArgs {
+  PoolView<R> P;
+  int64_t n;
+  const R* e1;
+  const R* e2;
+  int nx, ny;
+  const R* sa;
+  const R* ss;
+  double dt;
+  int reflect;  // bit s set -> side s reflects
+  int64_t max_events;
+  const double* floor_ptr;
+  Key key;
+  uint32_t step;
+  unsigned long long* next;
+  int batch;
+  double* dep_raw;  // sum of dep_w per cell
+  double* si_raw;   // sum of dep_i per cell
+  double* bl;
+  unsigned long long* stats;  // [8]
+};
+
+template <typename R, int GEOM>
+__global__ __launch_bounds__(256) void k_transport(TransportArgs<R> A) {
+  const double wfloor = *A.floor_ptr;
+  const int ny = A.ny, nx = A.nx;
+  int64_t qbase = 0, qend = 0;
+  bool exhausted = false;
+  bool active = false;
+  int64_t k = 0;
+  double w = 0.0;
+  R d1 = 0, d2 = 0, p1 = 0, p2 = 0, dcen = 0;
+  int i = 0, j = 0;
+  uint32_t ev = 0;
+  double acc_dep = 0.0, acc_si = 0.0;
+  int acc_cell = -1;
+  double bl = 0.0;
+  uint32_t st_ev = 0, st_bnd = 0, st_sc = 0, st_cen = 0, st_floor = 0, st_refl = 0, st_cap = 0;
+
+  while (true) {
+    uint64_t need = __ballot(!active);
+    if (need) {
+      if (qbase >= qend && !exhausted) {
+        unsigned long long b = 0;
+        if (__lane_id() == 0) b = atomicAdd(A.next, (unsigned long long)A.batch);
+        b = __shfl(b, 0, 64);
+        if ((int64_t)b >= A.n) {
+          exhausted = true;
+        } else {
+          qbase = (int64_t)b;
+          qend = ((int64_t)b + A.batch < A.n) ? (int64_t)b + A.batch : A.n;
+        }
+      }
+      int64_t avail = qend - qbase;
+      if (avail > 0) {
+        int64_t rank = __popcll(need & lanemask_lt());
+        if (!active && rank < avail) {
+          k = qbase + rank;
+          w = A.P.w[k];
+          d1 = A.P.d1[k];
+          d2 = A.P.d2[k];
+          p1 = A.P.p1[k];
+          p2 = A.P.p2[k];
+          i = A.P.ci[k];
+          j = A.P.cj[k];
+          dcen = (R)((A.dt - (double)A.P.t[k]) * kC);
+          ev = 0;
+          active = dcen > (R)0;
+        }
+        int64_t take = __popcll(need);
+        qbase += take < avail ? take : avail;
+      }
+    }
+    if (__ballot(active) == 0) {
+      if (exhausted) break;
+      continue;
+    }
+    if (!active) continue;
+
+    // ---- one segment ----
+    bool finish = false;
+    ++ev;
+    ++st_ev;
+    if ((int64_t)ev > A.max_events) {
+      ++st_cap;
+      finish = true;
+    } else if (i < 0 || i >= nx || j < 0 || j >= ny) {
+      finish = true;
+    } else {
+      int c = i * ny + j;
+      R sa = A.sa[c], ss = A.ss[c];
+      MoveOut<R> o;
+      Ctr ctr = make_ctr(kTagTransport, (uint64_t)k, A.step);
+      if (GEOM == 0)
+        move_xy(ctr, A.key, ev - 1u, w, d1, d2, p1, p2, A.e1[i], A.e1[i + 1], A.e2[j], A.e2[j + 1], sa, ss, dcen, o);
+      else
+        move_rz(ctr, A.key, ev - 1u, w, d1, d2, p1, p2, A.e1[i], A.e1[i + 1], A.e2[j], A.e2[j + 1], sa, ss, dcen, o);
+      dcen = o.dcen;
+      st_bnd += o.evt == kEvtBoundary;
+      st_sc += o.evt == kEvtScatter;
+      st_cen += o.evt == kEvtCensus;
+      if (c != acc_cell) {
+        if (acc_cell >= 0) {
+          atomic_add_f64(A.dep_raw + acc_cell, acc_dep);
+          atomic_add_f64(A.si_raw + acc_cell, acc_si);
+        }
+        acc_cell = c;
+        acc_dep = 0.0;
+        acc_si = 0.0;
+      }
+      acc_dep += o.dep_w;
+      acc_si += o.dep_i;
+      w = o.w;
+      if (o.w <= wfloor) {
+        ++st_floor;
+        finish = true;  // weight kept, position and direction not advanced
+      } else {
+        d1 = o.d1; d2 = o.d2; p1 = o.p1; p2 = o.p2;
+        if (o.crossed != kCrossNone) {
+          if (o.crossed == kCrossIPlus) ++i;
+          else if (o.crossed == kCrossIMinus) --i;
+          else if (o.crossed == kCrossJPlus) ++j;
+          else --j;
+          int side = -1;
+          if (i < 0) side = 0;
+          else if (i >= nx) side = 1;
+          else if (j < 0) side = 2;
+          else if (j >= ny) side = 3;
+          if (side >= 0) {
+            if ((A.reflect >> side) & 1) {
+              if (side < 2) { d1 = -d1; i = side == 0 ? 0 : nx - 1; }
+              else { d2 = -d2; j = side == 2 ? 0 : ny - 1; }
+              ++st_refl;
+            } else {
+              bl += w;
+              w = 0.0;
+              dcen = 0;
+            }
+          }
+        }
+        if (w <= 1e-300) {
+          w = 0.0;
+          dcen = 0;
+        }
+        if (!(dcen > (R)0)) finish = true;
+      }
+    }
+    if (finish) {
+      A.P.w[k] = w;
+      A.P.d1[k] = d1;
+      A.P.d2[k] = d2;
+      A.P.p1[k] = p1;
+      A.P.p2[k] = p2;
+      A.P.ci[k] = i;
+      A.P.cj[k] = j;
+      active = false;
+    }
+  }
+  if (acc_cell >= 0) {
+    atomic_add_f64(A.dep_raw + acc_cell, acc_dep);
+    atomic_add_f64(A.si_raw + acc_cell, acc_si);
+  }
+  // wave reductions of counters
+  unsigned long long s0 = wave_sum<unsigned long long>(st_ev);
+  unsigned long long s1 = wave_sum<unsigned long long>(st_bnd);
+  unsigned long long s2 = wave_sum<unsigned long long>(st_sc);
+  unsigned long long s3 = wave_sum<unsigned long long>(st_cen);
+  unsigned long long s5 = wave_sum<unsigned long long>(st_floor);
+  unsigned long long s6 = wave_sum<unsigned long long>(st_refl);
+  unsigned long long s7 = wave_sum<unsigned long long>(st_cap);
+  double blw = wave_sum<double>(bl);
+  if (__lane_id() == 0) {
+    atomicAdd(A.stats + 0, s0);
+    atomicAdd(A.stats + 1, s1);
+    atomicAdd(A.stats + 2, s2);
+    atomicAdd(A.stats + 3, s3);
+    atomicAdd(A.stats + 5, s5);
+    atomicAdd(A.stats + 6, s6);
+    atomicAdd(A.stats + 7, s7);
+    if (blw != 0.0) atomic_add_f64(A.bl, blw);
+  }
+}
+
+__global__ void k_weight_floor(const double* __restrict__ sumw, int64_t n, double* __restrict__ out) {
+#pragma clang fp contract(off)
+  *out = 1e-10 * (*sumw) / (double)(n > 1 ? n : 1);
+}
+
+__global__ void k_finalize_tallies(int64_t n, const double* __restrict__ dep_raw, const double* __restrict__ si_raw,
+                                   const double* __restrict__ vol, double dt, double* __restrict__ dep,
+                                   double* __restrict__ si) {
+  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  dep[c] = dep_raw[c] / vol[c];
+  si[c] = si_raw[c] / (dt * vol[c]);
+}
+
+// ---------------------------------------------------------------------------
+// comb (IMC/IMC2D.py:1448-1511)
+// ---------------------------------------------------------------------------
+// Particles outside the mesh (escaped) or with non-positive weight are dead.
+template <typename R>
+__device__ __forceinline__ bool comb_alive(const PoolView<R>& P, int64_t k, int nx, int ny) {
+  int32_t ci = P.ci[k], cj = P.cj[k];
+  return P.w[k] > 0.0 && ci >= 0 && ci < nx && cj >= 0 && cj < ny;
+}
+
+template <typename R>
+__global__ void k_comb_ecen(PoolView<R> P, int64_t n, int nx, int ny, double* __restrict__ ecen) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  if (comb_alive(P, k, nx, ny)) atomic_add_f64(ecen + (int64_t)P.ci[k] * ny + P.cj[k], P.w[k]);
+}
+
+__global__ void k_comb_ew(int64_t ncell, const double* __restrict__ ecen, const double* __restrict__ Etot,
+                          int64_t Nmax, double* __restrict__ ew) {
+#pragma clang fp contract(off)
+  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  double e = ecen[c];
+  double E = *Etot;
+  if (e > 0.0 && E > 0.0) {
+    double d = rint((double)Nmax * e / E);
+    if (d < 1.0) d = 1.0;
+    ew[c] = e / d;
+  } else {
+    ew[c] = 0.0;
+  }
+}
+
+template <typename R>
+__global__ void k_comb_copies(PoolView<R> P, int64_t n, int nx, int ny, const double* __restrict__ ew,
+                              int passthrough, Key key, uint32_t step, int32_t* __restrict__ copies) {
+#pragma clang fp contract(off)
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  double w = P.w[k];
+  int32_t m = 0;
+  if (comb_alive(P, k, nx, ny)) {
+    if (passthrough) {
+      m = 1;
+    } else {
+      double et = ew[(int64_t)P.ci[k] * ny + P.cj[k]];
+      if (et > 0.0) {
+        double u = slot_d(make_ctr(kTagComb, (uint64_t)k, step), key, 0);
+        m = (int32_t)(w / et + u);
+      }
+    }
+  }
+  copies[k] = m;
+}
+
+template <typename R>
+__global__ void k_comb_write(PoolView<R> S, PoolView<R> D, int64_t n, int ny, const int32_t* __restrict__ copies,
+                             const int64_t* __restrict__ offs, const double* __restrict__ ew, int passthrough,
+                             unsigned long long* __restrict__ cellcount) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  int32_t m = copies[k];
+  if (m <= 0) return;
+  int32_t ci = S.ci[k], cj = S.cj[k];
+  int64_t c = (int64_t)ci * ny + cj;
+  double w = passthrough ? S.w[k] : ew[c];
+  R d1 = S.d1[k], d2 = S.d2[k], p1 = S.p1[k], p2 = S.p2[k];
+  int64_t o = offs[k];
+  for (int32_t r = 0; r < m; ++r) {
+    D.w[o + r] = w;
+    D.d1[o + r] = d1;
+    D.d2[o + r] = d2;
+    D.p1[o + r] = p1;
+    D.p2[o + r] = p2;
+    D.t[o + r] = (R)0;  // times reset after the comb (IMC/IMC2D.py:1831)
+    D.ci[o + r] = ci;
+    D.cj[o + r] = cj;
+  }
+  if (!passthrough) atomicAdd(cellcount + c, (unsigned long long)m);
+}
+
+__global__ void k_comb_disc(int64_t ncell, const double* __restrict__ ecen, const double* __restrict__ ew,
+                            const unsigned long long* __restrict__ cellcount, int passthrough,
+                            double* __restrict__ disc) {
+  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  disc[c] = passthrough ? 0.0 : ecen[c] - ew[c] * (double)cellcount[c];
+}
+
+// census upload / download with precision conversion
+template <typename R>
+__global__ void k_put(PoolView<R> P, int64_t base, int64_t n, const double* __restrict__ w,
+                      const double* __restrict__ d1, const double* __restrict__ d2, const double* __restrict__ t,
+                      const double* __restrict__ p1, const double* __restrict__ p2, const int64_t* __restrict__ ci,
+                      const int64_t* __restrict__ cj) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  int64_t o = base + k;
+  P.w[o] = w[k];
+  P.d1[o] = (R)d1[k];
+  P.d2[o] = (R)d2[k];
+  P.t[o] = (R)t[k];
+  P.p1[o] = (R)p1[k];
+  P.p2[o] = (R)p2[k];
+  P.ci[o] = (int32_t)ci[k];
+  P.cj[o] = (int32_t)cj[k];
+}
+
+template <typename R>
+__global__ void k_get(PoolView<R> P, int64_t n, double* __restrict__ w, double* __restrict__ d1,
+                      double* __restrict__ d2, double* __restrict__ t, double* __restrict__ p1,
+                      double* __restrict__ p2, int64_t* __restrict__ ci, int64_t* __restrict__ cj) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  w[k] = P.w[k];
+  d1[k] = (double)P.d1[k];
+  d2[k] = (double)P.d2[k];
+  t[k] = (double)P.t[k];
+  p1[k] = (double)P.p1[k];
+  p2[k] = (double)P.p2[k];
+  ci[k] = P.ci[k];
+  cj[k] = P.cj[k];
+}
+
+__global__ void k_f64_to_f32(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) b[k] = (float)a[k];
+}
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+struct Imc2dCtx {
+  int geometry = 0;  // 0 xy, 1 rz
+  int nx = 0, ny = 0;
+  int64_t ncell = 0;
+  int precision = 0;  // IMC_FP64 / IMC_FP32_TRACK
+  uint64_t seed = 1;
+  uint32_t step = 0;
+  int device = 0;
+  int n_cu = 256;
+  hipStream_t stream = nullptr;
+  std::vector<double> h_e1, h_e2;
+  double *e1 = nullptr, *e2 = nullptr, *vol = nullptr;
+  float *e1f = nullptr, *e2f = nullptr;
+  DevBuf<double> T, sig, cv, sa, ss, E, Ar, Mr, Pn, dep_raw, si_raw, dep, si, ecen, ew, disc, prof, sum_scr, wbuf,
+      volsrc;
+  DevBuf<float> saf, ssf;
+  DevBuf<int64_t> counts, offs, scan_scr;
+  DevBuf<int32_t> copies;
+  DevBuf<unsigned long long> cellcount;
+  double* scal = nullptr;  // [0] Etot  [1] weight sum  [2] floor  [3] bl  [4] comb E  [5] rad total
+  unsigned long long* stats = nullptr;
+  unsigned long long* next = nullptr;
+  int* bad = nullptr;
+  PoolBuf A, B;
+  int64_t n = 0;
+  int grid_override = 0;
+  int batch = 256;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  float last_transport_ms = 0.0f;
+  double last_energy_emit = 0.0;
+};
+
+static int ensure_sum_scratch(Imc2dCtx* c, int64_t n) { return c->sum_scr.reserve(canonical_sum_scratch(n) + 8); }
+
+static int pool_ensure(Imc2dCtx* c, int64_t n_total) { return c->A.grow(n_total, c->n, c->stream); }
+
+}  // namespace imc
+
+using namespace imc;
+
+extern "C" {
+
+struct imc2d_ctx : public Imc2dCtx {};
+
+const char* imc_last_error(void) { return imc::get_error(); }
+
+int imc_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+int imc2d_create(imc2d_ctx** out, int geometry, int nx, int ny, const double* edges1, const double* edges2,
+                 int precision, uint64_t seed, int device) {
+  IMC_REQUIRE(out && edges1 && edges2, kErrArg, "imc2d_create: null argument");
+  IMC_REQUIRE(geometry == IMC_GEOM_XY || geometry == IMC_GEOM_RZ, kErrArg, "imc2d_create: bad geometry %d", geometry);
+  IMC_REQUIRE(nx > 0 && ny > 0, kErrArg, "imc2d_create: empty mesh");
+  IMC_REQUIRE(precision == IMC_FP64 || precision == IMC_FP32_TRACK, kErrArg, "imc2d_create: bad precision");
+  for (int a = 0; a < nx; ++a) IMC_REQUIRE(edges1[a + 1] > edges1[a], kErrArg, "imc2d_create: edges1 not increasing");
+  for (int a = 0; a < ny; ++a) IMC_REQUIRE(edges2[a + 1] > edges2[a], kErrArg, "imc2d_create: edges2 not increasing");
+  if (geometry == IMC_GEOM_RZ) IMC_REQUIRE(edges1[0] >= 0.0, kErrArg, "imc2d_create: negative radius");
+  IMC_HIP(hipSetDevice(device));
+  imc2d_ctx* c = new imc2d_ctx();
+  c->geometry = geometry;
+  c->nx = nx;
+  c->ny = ny;
+  c->ncell = (int64_t)nx * ny;
+  c->precision = precision;
+  c->seed = seed;
+  c->device = device;
+  hipDeviceProp_t prop;
+  IMC_HIP(hipGetDeviceProperties(&prop, device));
+  c->n_cu = prop.multiProcessorCount;
+  IMC_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  IMC_HIP(hipEventCreate(&c->ev0));
+  IMC_HIP(hipEventCreate(&c->ev1));
+  c->h_e1.assign(edges1, edges1 + nx + 1);
+  c->h_e2.assign(edges2, edges2 + ny + 1);
+  IMC_HIP(hipMalloc(&c->e1, sizeof(double) * (nx + 1)));
+  IMC_HIP(hipMalloc(&c->e2, sizeof(double) * (ny + 1)));
+  IMC_HIP(hipMalloc(&c->e1f, sizeof(float) * (nx + 1)));
+  IMC_HIP(hipMalloc(&c->e2f, sizeof(float) * (ny + 1)));
+  IMC_HIP(hipMemcpy(c->e1, edges1, sizeof(double) * (nx + 1), hipMemcpyHostToDevice));
+  IMC_HIP(hipMemcpy(c->e2, edges2, sizeof(double) * (ny + 1), hipMemcpyHostToDevice));
+  std::vector<float> f1(edges1, edges1 + nx + 1), f2(edges2, edges2 + ny + 1);
+  IMC_HIP(hipMemcpy(c->e1f, f1.data(), sizeof(float) * (nx + 1), hipMemcpyHostToDevice));
+  IMC_HIP(hipMemcpy(c->e2f, f2.data(), sizeof(float) * (ny + 1), hipMemcpyHostToDevice));
+  // volumes (IMC/IMC2D.py:87-99)
+  std::vector<double> vol((size_t)c->ncell);
+  for (int a = 0; a < nx; ++a)
+    for (int b = 0; b < ny; ++b) {
+      double dy = edges2[b + 1] - edges2[b];
+      if (geometry == IMC_GEOM_XY) {
+        vol[(size_t)a * ny + b] = (edges1[a + 1] - edges1[a]) * dy;
+      } else {
+        double dr2 = edges1[a + 1] * edges1[a + 1] - edges1[a] * edges1[a];
+        vol[(size_t)a * ny + b] = M_PI * dr2 * dy;
+      }
+    }
+  IMC_HIP(hipMalloc(&c->vol, sizeof(double) * c->ncell));
+  IMC_HIP(hipMemcpy(c->vol, vol.data(), sizeof(double) * c->ncell, hipMemcpyHostToDevice));
+  int64_t nc = c->ncell;
+  int rc = 0;
+  DevBuf<double>* cellbufs[] = {&c->T, &c->sig, &c->cv, &c->sa, &c->ss, &c->E, &c->Ar, &c->Mr, &c->Pn,
+                                &c->dep_raw, &c->si_raw, &c->dep, &c->si, &c->ecen, &c->ew, &c->disc, &c->volsrc};
+  for (auto* b : cellbufs)
+    if ((rc = b->reserve(nc))) return rc;
+  if ((rc = c->saf.reserve(nc))) return rc;
+  if ((rc = c->ssf.reserve(nc))) return rc;
+  if ((rc = c->counts.reserve(nc + 1))) return rc;
+  if ((rc = c->offs.reserve(nc + 2))) return rc;
+  if ((rc = c->scan_scr.reserve(scan_scratch(nc) + 8))) return rc;
+  if ((rc = c->cellcount.reserve(nc))) return rc;
+  if ((rc = c->prof.reserve(std::max(nx, ny) + 1))) return rc;
+  if ((rc = ensure_sum_scratch(c, nc))) return rc;
+  IMC_HIP(hipMalloc(&c->scal, sizeof(double) * 16));
+  IMC_HIP(hipMemset(c->scal, 0, sizeof(double) * 16));
+  IMC_HIP(hipMalloc(&c->stats, sizeof(unsigned long long) * 8));
+  IMC_HIP(hipMalloc(&c->next, sizeof(unsigned long long)));
+  IMC_HIP(hipMalloc(&c->bad, sizeof(int)));
+  c->A.rsz = c->B.rsz = (precision == IMC_FP64) ? 8 : 4;
+  *out = c;
+  return kOk;
+}
+
+void imc2d_destroy(imc2d_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  DevBuf<double>* cellbufs[] = {&c->T, &c->sig, &c->cv, &c->sa, &c->ss, &c->E, &c->Ar, &c->Mr, &c->Pn,
+                                &c->dep_raw, &c->si_raw, &c->dep, &c->si, &c->ecen, &c->ew, &c->disc, &c->prof,
+                                &c->sum_scr, &c->wbuf, &c->volsrc};
+  for (auto* b : cellbufs) b->release();
+  c->saf.release();
+  c->ssf.release();
+  c->counts.release();
+  c->offs.release();
+  c->scan_scr.release();
+  c->copies.release();
+  c->cellcount.release();
+  c->A.release();
+  c->B.release();
+  void* ptrs[] = {c->e1, c->e2, c->e1f, c->e2f, c->vol, c->scal, c->stats, c->next, c->bad};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  (void)hipEventDestroy(c->ev0);
+  (void)hipEventDestroy(c->ev1);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+int imc2d_set_step(imc2d_ctx* c, uint32_t step) {
+  IMC_REQUIRE(c, kErrArg, "null ctx");
+  c->step = step;
+  return kOk;
+}
+
+int imc2d_set_launch(imc2d_ctx* c, int grid, int batch) {
+  IMC_REQUIRE(c, kErrArg, "null ctx");
+  IMC_REQUIRE(grid >= 0 && batch > 0, kErrArg, "imc2d_set_launch: bad arguments");
+  c->grid_override = grid;
+  c->batch = batch;
+  return kOk;
+}
+
+int64_t imc2d_census_size(imc2d_ctx* c) { return c ? c->n : -1; }
+
+int imc2d_put_census(imc2d_ctx* c, int64_t n, const double* w, const double* d1, const double* d2, const double* t,
+                     const double* p1, const double* p2, const int64_t* ci, const int64_t* cj) {
+  IMC_REQUIRE(c && n >= 0, kErrArg, "imc2d_put_census: bad arguments");
+  IMC_HIP(hipSetDevice(c->device));
+  int rc = c->A.grow(n, 0, c->stream);
+  if (rc) return rc;
+  c->n = 0;
+  if (n == 0) return kOk;
+  const void* src[8] = {w, d1, d2, t, p1, p2, ci, cj};
+  size_t esz[8] = {8, 8, 8, 8, 8, 8, 8, 8};
+  void* tmp[8];
+  for (int a = 0; a < 8; ++a) {
+    IMC_REQUIRE(src[a], kErrArg, "imc2d_put_census: null array");
+    IMC_HIP(hipMalloc(&tmp[a], esz[a] * (size_t)n));
+    IMC_HIP(hipMemcpyAsync(tmp[a], src[a], esz[a] * (size_t)n, hipMemcpyHostToDevice, c->stream));
+  }
+  int g = launch_blocks(n, 256);
+  if (c->precision == IMC_FP64)
+    k_put<double><<<g, 256, 0, c->stream>>>(c->A.view<double>(), 0, n, (double*)tmp[0], (double*)tmp[1],
+                                            (double*)tmp[2], (double*)tmp[3], (double*)tmp[4], (double*)tmp[5],
+                                            (int64_t*)tmp[6], (int64_t*)tmp[7]);
+  else
+    k_put<float><<<g, 256, 0, c->stream>>>(c->A.view<float>(), 0, n, (double*)tmp[0], (double*)tmp[1],
+                                           (double*)tmp[2], (double*)tmp[3], (double*)tmp[4], (double*)tmp[5],
+                                           (int64_t*)tmp[6], (int64_t*)tmp[7]);
+  IMC_HIP(hipGetLastError());
+  IMC_HIP(hipStreamSynchronize(c->stream));
+  for (int a = 0; a < 8; ++a) IMC_HIP(hipFree(tmp[a]));
+  c->n = n;
+  return kOk;
+}
+
+int imc2d_get_census(imc2d_ctx* c, int64_t n, double* w, double* d1, double* d2, double* t, double* p1, double* p2,
+                     int64_t* ci, int64_t* cj) {
+  IMC_REQUIRE(c && n == c->n, kErrArg, "imc2d_get_census: size %lld != census %lld", (long long)n, (long long)(c ? c->n : -1));
+  IMC_HIP(hipSetDevice(c->device));
+  if (n == 0) return kOk;
+  void* dst[8] = {w, d1, d2, t, p1, p2, ci, cj};
+  void* tmp[8];
+  for (int a = 0; a < 8; ++a) {
+    IMC_REQUIRE(dst[a], kErrArg, "imc2d_get_census: null array");
+    IMC_HIP(hipMalloc(&tmp[a], 8 * (size_t)n));
+  }
+  int g = launch_blocks(n, 256);
+  if (c->precision == IMC_FP64)
+    k_get<double><<<g, 256, 0, c->stream>>>(c->A.view<double>(), n, (double*)tmp[0], (double*)tmp[1],
+                                            (double*)tmp[2], (double*)tmp[3], (double*)tmp[4], (double*)tmp[5],
+                                            (int64_t*)tmp[6], (int64_t*)tmp[7]);
+  else
+    k_get<float><<<g, 256, 0, c->stream>>>(c->A.view<float>(), n, (double*)tmp[0], (double*)tmp[1],
+                                           (double*)tmp[2], (double*)tmp[3], (double*)tmp[4], (double*)tmp[5],
+                                           (int64_t*)tmp[6], (int64_t*)tmp[7]);
+  IMC_HIP(hipGetLastError());
+  for (int a = 0; a < 8; ++a)
+    IMC_HIP(hipMemcpyAsync(dst[a], tmp[a], 8 * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+  IMC_HIP(hipStreamSynchronize(c->stream));
+  for (int a = 0; a < 8; ++a) IMC_HIP(hipFree(tmp[a]));
+  return kOk;
+}
+
+// Fleck factor and effective opacities from host T, sigma(T), c_v(T).
+int imc2d_set_material(imc2d_ctx* c, const double* T, const double* sigma_true, const double* cv, double theta,
+                       double dt, int clip, double* sigma_a_out, double* sigma_s_out) {
+  IMC_REQUIRE(c && T && sigma_true && cv, kErrArg, "imc2d_set_material: null argument");
+  IMC_HIP(hipSetDevice(c->device));
+  size_t bytes = sizeof(double) * c->ncell;
+  IMC_HIP(hipMemcpyAsync(c->T.p, T, bytes, hipMemcpyHostToDevice, c->stream));
+  IMC_HIP(hipMemcpyAsync(c->sig.p, sigma_true, bytes, hipMemcpyHostToDevice, c->stream));
+  IMC_HIP(hipMemcpyAsync(c->cv.p, cv, bytes, hipMemcpyHostToDevice, c->stream));
+  IMC_HIP(hipMemsetAsync(c->bad, 0, sizeof(int), c->stream));
+  bool f32 = c->precision == IMC_FP32_TRACK;
+  k_fleck<<<launch_blocks(c->ncell, 256), 256, 0, c->stream>>>(c->ncell, c->T.p, c->sig.p, c->cv.p, theta, dt, clip,
+                                                               c->sa.p, c->ss.p, f32 ? c->saf.p : nullptr,
+                                                               f32 ? c->ssf.p : nullptr, c->bad);
+  IMC_HIP(hipGetLastError());
+  int bad = 0;
+  IMC_HIP(hipMemcpyAsync(&bad, c->bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  if (sigma_a_out) IMC_HIP(hipMemcpyAsync(sigma_a_out, c->sa.p, bytes, hipMemcpyDeviceToHost, c->stream));
+  if (sigma_s_out) IMC_HIP(hipMemcpyAsync(sigma_s_out, c->ss.p, bytes, hipMemcpyDeviceToHost, c->stream));
+  IMC_HIP(hipStreamSynchronize(c->stream));
+  IMC_REQUIRE(!bad, kErrArg, "Fleck factor out of bounds");
+  return kOk;
+}
+
+static int launch_volume(imc2d_ctx* c, int64_t base, int64_t total, const double* QV, int kind, int geom,
+                         double dt, uint32_t tag, uint32_t step) {
+  if (total <= 0) return kOk;
+  int g = launch_blocks(total, 256);
+  Key key = make_key(c->seed);
+  if (c->precision == IMC_FP64)
+    k_sample_volume<double><<<g, 256, 0, c->stream>>>(c->A.view<double>(), base, total, c->offs.p, c->ncell, c->ny,
+                                                      c->e1, c->e2, QV, kind, geom, c->Ar.p, c->Mr.p, c->Pn.p, dt,
+                                                      key, step, tag);
+  else
+    k_sample_volume<float><<<g, 256, 0, c->stream>>>(c->A.view<float>(), base, total, c->offs.p, c->ncell, c->ny,
+                                                     c->e1, c->e2, QV, kind, geom, c->Ar.p, c->Mr.p, c->Pn.p, dt,
+                                                     key, step, tag);
+  IMC_HIP(hipGetLastError());
+  return kOk;
+}
+
+// counts -> offsets -> total (host)
+static int counts_and_offsets(imc2d_ctx* c, const double* E, int64_t N, int min_one, int64_t* total) {
+  int rc = canonical_sum(E, c->ncell, c->scal + 0, c->sum_scr.p, c->stream);
+  if (rc) return rc;
+  k_counts<<<launch_blocks(c->ncell, 256), 256, 0, c->stream>>>(c->ncell, E, N, min_one, c->scal + 0, c->counts.p);
+  IMC_HIP(hipGetLastError());
+  rc = exclusive_scan_i64(c->counts.p, c->ncell, c->offs.p, c->scan_scr.p, c->stream);
+  if (rc) return rc;
+  IMC_HIP(hipMemcpyAsync(total, c->offs.p + c->ncell, sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
+  IMC_HIP(hipStreamSynchronize(c->stream));
+  return kOk;
+}
+
+// Equilibrium census initial condition (IMC/IMC2D.py:1110-1188): replaces the pool.
+int imc2d_equilibrium_sample(imc2d_ctx* c, int64_t N, const double* Tr) {
+  IMC_REQUIRE(c && Tr && N >= 0, kErrArg, "imc2d_equilibrium_sample: bad arguments");
+  IMC_HIP(hipSetDevice(c->device));
+  IMC_HIP(hipMemcpyAsync(c->E.p, Tr, sizeof(double) * c->ncell, hipMemcpyHostToDevice, c->stream));
+  k_equilibrium_energy<<<launch_blocks(c->ncell, 256), 256, 0, c->stream>>>(c->ncell, c->E.p, c->vol, c->volsrc.p);
+  IMC_HIP(hipGetLastError());
+  int64_t total = 0;
+  int rc = counts_and_offsets(c, c->volsrc.p, N, 1, &total);
+  if (rc) return rc;
+  c->n = 0;
+  if ((rc = pool_ensure(c, total))) return rc;
+  if ((rc = launch_volume(c, 0, total, c->volsrc.p, 1, c->geometry == IMC_GEOM_XY ? 0 : 1, 0.0, kTagIC, 0)))
+    return rc;
+  IMC_HIP(hipStreamSynchronize(c->stream));
+  c->n = total;
+  return kOk;
+}
+
+// Boundary source on one side; appends to the pool.  prof_cum (host, nullable):
+// running sum of per-face-cell emission.
+int imc2d_sample_boundary(imc2d_ctx* c, int side, int64_t n, double E_total, double dt, const double* prof_cum,
+                          int nface) {
+  IMC_REQUIRE(c && side >= 0 && side < 4 && n >= 0, kErrArg, "imc2d_sample_boundary: bad arguments");
+  if (n == 0) return kOk;
+  IMC_HIP(hipSetDevice(c->device));
+  int expect = side < 2 ? c->ny : c->nx;
+  const double* prof = nullptr;
+  if (prof_cum) {
+    IMC_REQUIRE(nface == expect, kErrArg, "imc2d_sample_boundary: profile length %d != %d", nface, expect);
+    IMC_HIP(hipMemcpyAsync(c->prof.p, prof_cum, sizeof(double) * nface, hipMemcpyHostToDevice, c->stream));
+    prof = c->prof.p;
+  }
+  int rc = pool_ensure(c, c->n + n);
+  if (rc) return rc;
+  int g = launch_blocks(n, 256);
+  Key key = make_key(c->seed);
+  if (c->precision == IMC_FP64)
+    k_sample_boundary<double><<<g, 256, 0, c->stream>>>(c->A.view<double>(), c->n, n, c->geometry, side, E_total, dt,
+                                                        c->e1, c->nx, c->e2, c->ny, prof, key, c->step);
+  else
+    k_sample_boundary<float><<<g, 256, 0, c->stream>>>(c->A.view<float>(), c->n, n, c->geometry, side, E_total, dt,
+                                                       c->e1, c->nx, c->e2, c->ny, prof, key, c->step);
+  IMC_HIP(hipGetLastError());
+  if (prof) IMC_HIP(hipStreamSynchronize(c->stream));
+  c->n += n;
+  return kOk;
+}
+
+// Volume source: kind IMC_SRC_EMISSION uses the device emission energies
+// a c T^4 sigma_a dt V from the last imc2d_set_material; IMC_SRC_FIXED takes
+// the per-cell energy QV (host).  rz_linear selects the Box 9.4 tilt in rz.
+int imc2d_sample_volume(imc2d_ctx* c, int kind, int64_t N, const double* QV_host, double dt, int rz_linear,
+                        double* E_out, double* total_out, int64_t* count_out) {
+  IMC_REQUIRE(c && N >= 0, kErrArg, "imc2d_sample_volume: bad arguments");
+  IMC_REQUIRE(kind == IMC_SRC_EMISSION || (kind == IMC_SRC_FIXED && QV_host), kErrArg, "imc2d_sample_volume: bad kind");
+  IMC_HIP(hipSetDevice(c->device));
+  const double* QV;
+  uint32_t tag;
+  if (kind == IMC_SRC_EMISSION) {
+    k_emission_energy<<<launch_blocks(c->ncell, 256), 256, 0, c->stream>>>(c->ncell, c->T.p, c->sa.p, c->vol, dt,
+                                                                           c->E.p);
+    IMC_HIP(hipGetLastError());
+    QV = c->E.p;
+    tag = kTagEmission;
+  } else {
+    IMC_HIP(hipMemcpyAsync(c->volsrc.p, QV_host, sizeof(double) * c->ncell, hipMemcpyHostToDevice, c->stream));
+    QV = c->volsrc.p;
+    tag = kTagSource;
+  }
+  int64_t total = 0;
+  int rc = counts_and_offsets(c, QV, N, 0, &total);
+  if (rc) return rc;
+  double Etot = 0.0;
+  IMC_HIP(hipMemcpyAsync(&Etot, c->scal + 0, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (E_out) IMC_HIP(hipMemcpyAsync(E_out, QV, sizeof(double) * c->ncell, hipMemcpyDeviceToHost, c->stream));
+  IMC_HIP(hipStreamSynchronize(c->stream));
+  if (total_out) *total_out = Etot;
+  if (count_out) *count_out = (Etot > 0.0 && N > 0) ? total : 0;
+  if (!(Etot > 0.0) || N <= 0 || total <= 0) return kOk;
+  int geom = c->geometry == IMC_GEOM_XY ? 0 : (rz_linear ? 2 : 1);
+  if (geom == 2) {
+    k_rz_params<<<launch_blocks(c->ncell, 256), 256, 0, c->stream>>>(c->nx, c->ny, c->T.p, c->e1, c->e2, c->Ar.p,
+                                                                     c->Mr.p, c->Pn.p);
+    IMC_HIP(hipGetLastError());
+  }
+  if ((rc = pool_ensure(c, c->n + total))) return rc;
+  if ((rc = launch_volume(c, c->n, total, QV, 0, geom, dt, tag, c->step))) return rc;
+  c->n += total;
+  return kOk;
+}
+
+// Transport the whole pool (IMC/IMC2D.py:938-1107).  weight_floor < 0 selects
+// 1e-10 * sum(w) / N computed on the device (IMC/IMC2D.py:1771).
+int imc2d_transport(imc2d_ctx* c, double dt, const int32_t* reflect, int64_t max_events, double weight_floor,
+                    double* dep_out, double* si_out, double* bl_out, int64_t* stats_out) {
+  IMC_REQUIRE(c && reflect && max_events >= 0, kErrArg, "imc2d_transport: bad arguments");
+  IMC_HIP(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  int64_t n = c->n;
+  IMC_HIP(hipMemsetAsync(c->dep_raw.p, 0, sizeof(double) * c->ncell, s));
+  IMC_HIP(hipMemsetAsync(c->si_raw.p, 0, sizeof(double) * c->ncell, s));
+  IMC_HIP(hipMemsetAsync(c->stats, 0, sizeof(unsigned long long) * 8, s));
+  IMC_HIP(hipMemsetAsync(c->scal + 3, 0, sizeof(double), s));
+  IMC_HIP(hipMemsetAsync(c->next, 0, sizeof(unsigned long long), s));
+  int rc;
+  if (weight_floor < 0.0) {
+    if ((rc = ensure_sum_scratch(c, n))) return rc;
+    if ((rc = canonical_sum(c->A.w, n, c->scal + 1, c->sum_scr.p, s))) return rc;
+    k_weight_floor<<<1, 1, 0, s>>>(c->scal + 1, n, c->scal + 2);
+    IMC_HIP(hipGetLastError());
+  } else {
+    IMC_HIP(hipMemcpyAsync(c->scal + 2, &weight_floor, sizeof(double), hipMemcpyHostToDevice, s));
+  }
+  int refl = 0;
+  for (int a = 0; a < 4; ++a) refl |= (reflect[a] ? 1 : 0) << a;
+  int block = 256;
+  int grid;
+  if (c->grid_override > 0) {
+    grid = c->grid_override;
+  } else {
+    int per_cu = 0;
+    if (c->precision == IMC_FP64)
+      IMC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+          &per_cu, c->geometry == IMC_GEOM_XY ? (const void*)k_transport<double, 0> : (const void*)k_transport<double, 1>,
+          block, 0));
+    else
+      IMC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+          &per_cu, c->geometry == IMC_GEOM_XY ? (const void*)k_transport<float, 0> : (const void*)k_transport<float, 1>,
+          block, 0));
+    if (per_cu < 1) per_cu = 1;
+    grid = per_cu * c->n_cu;
+  }
+  int64_t need_blocks = (n + block - 1) / block;
+  if (need_blocks < grid) grid = (int)std::max<int64_t>(need_blocks, 1);
+  Key key = make_key(c->seed);
+  IMC_HIP(hipEventRecord(c->ev0, s));
+  if (n > 0) {
+    if (c->precision == IMC_FP64) {
+      TransportArgs<double> A{c->A.view<double>(), n, c->e1, c->e2, c->nx, c->ny, c->sa.p, c->ss.p, dt, refl,
+                              max_events, c->scal + 2, key, c->step, c->next, c->batch, c->dep_raw.p, c->si_raw.p,
+                              c->scal + 3, c->stats};
+      if (c->geometry == IMC_GEOM_XY)
+        k_transport<double, 0><<<grid, block, 0, s>>>(A);
+      else
+        k_transport<double, 1><<<grid, block, 0, s>>>(A);
+    } else {
+      TransportArgs<float> A{c->A.view<float>(), n, c->e1f, c->e2f, c->nx, c->ny, c->saf.p, c->ssf.p, dt, refl,
+                             max_events, c->scal + 2, key, c->step, c->next, c->batch, c->dep_raw.p, c->si_raw.p,
+                             c->scal + 3, c->stats};
+      if (c->geometry == IMC_GEOM_XY)
+        k_transport<float, 0><<<grid, block, 0, s>>>(A);
+      else
+        k_transport<float, 1><<<grid, block, 0, s>>>(A);
+    }
+    IMC_HIP(hipGetLastError());
+  }
+  IMC_HIP(hipEventRecord(c->ev1, s));
+  k_finalize_tallies<<<launch_blocks(c->ncell, 256), 256, 0, s>>>(c->ncell, c->dep_raw.p, c->si_raw.p, c->vol, dt,
+                                                                  c->dep.p, c->si.p);
+  IMC_HIP(hipGetLastError());
+  size_t bytes = sizeof(double) * c->ncell;
+  if (dep_out) IMC_HIP(hipMemcpyAsync(dep_out, c->dep.p, bytes, hipMemcpyDeviceToHost, s));
+  if (si_out) IMC_HIP(hipMemcpyAsync(si_out, c->si.p, bytes, hipMemcpyDeviceToHost, s));
+  double bl = 0.0;
+  unsigned long long st[8];
+  IMC_HIP(hipMemcpyAsync(&bl, c->scal + 3, sizeof(double), hipMemcpyDeviceToHost, s));
+  IMC_HIP(hipMemcpyAsync(st, c->stats, sizeof(st), hipMemcpyDeviceToHost, s));
+  IMC_HIP(hipStreamSynchronize(s));
+  IMC_HIP(hipEventElapsedTime(&c->last_transport_ms, c->ev0, c->ev1));
+  if (bl_out) *bl_out = bl;
+  if (stats_out)
+    for (int a = 0; a < 8; ++a) stats_out[a] = (int64_t)st[a];
+  return kOk;
+}
+
+float imc2d_last_transport_ms(imc2d_ctx* c) { return c ? c->last_transport_ms : -1.0f; }
+
+// Census comb (IMC/IMC2D.py:1448-1511).  ecen_out: pre-comb census energy per
+// cell (also the census-bincount radiation estimate), disc_out: ecen - ecen_after.
+int imc2d_comb(imc2d_ctx* c, int64_t Nmax, double* ecen_out, double* disc_out, double* total_rad_out) {
+  IMC_REQUIRE(c && Nmax >= 0, kErrArg, "imc2d_comb: bad arguments");
+  IMC_HIP(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  int64_t n = c->n;
+  int rc;
+  IMC_HIP(hipMemsetAsync(c->ecen.p, 0, sizeof(double) * c->ncell, s));
+  IMC_HIP(hipMemsetAsync(c->cellcount.p, 0, sizeof(unsigned long long) * c->ncell, s));
+  int g = launch_blocks(n, 256);
+  if (n > 0) {
+    if (c->precision == IMC_FP64)
+      k_comb_ecen<double><<<g, 256, 0, s>>>(c->A.view<double>(), n, c->nx, c->ny, c->ecen.p);
+    else
+      k_comb_ecen<float><<<g, 256, 0, s>>>(c->A.view<float>(), n, c->nx, c->ny, c->ecen.p);
+    IMC_HIP(hipGetLastError());
+  }
+  if ((rc = canonical_sum(c->ecen.p, c->ncell, c->scal + 4, c->sum_scr.p, s))) return rc;
+  double E = 0.0;
+  IMC_HIP(hipMemcpyAsync(&E, c->scal + 4, sizeof(double), hipMemcpyDeviceToHost, s));
+  IMC_HIP(hipStreamSynchronize(s));
+  int passthrough = !(E > 0.0);
+  k_comb_ew<<<launch_blocks(c->ncell, 256), 256, 0, s>>>(c->ncell, c->ecen.p, c->scal + 4, Nmax, c->ew.p);
+  IMC_HIP(hipGetLastError());
+  if ((rc = c->copies.reserve(n + 1))) return rc;
+  if ((rc = c->counts.reserve(n + 2))) return rc;  // reused as copy offsets
+  if ((rc = c->scan_scr.reserve(scan_scratch(n) + 8))) return rc;
+  Key key = make_key(c->seed);
+  if (n > 0) {
+    if (c->precision == IMC_FP64)
+      k_comb_copies<double><<<g, 256, 0, s>>>(c->A.view<double>(), n, c->nx, c->ny, c->ew.p, passthrough, key, c->step,
+                                              c->copies.p);
+    else
+      k_comb_copies<float><<<g, 256, 0, s>>>(c->A.view<float>(), n, c->nx, c->ny, c->ew.p, passthrough, key, c->step,
+                                             c->copies.p);
+    IMC_HIP(hipGetLastError());
+  }
+  if ((rc = exclusive_scan_i32_to_i64(c->copies.p, n, c->counts.p, c->scan_scr.p, s))) return rc;
+  int64_t total = 0;
+  IMC_HIP(hipMemcpyAsync(&total, c->counts.p + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  IMC_HIP(hipStreamSynchronize(s));
+  if ((rc = c->B.grow(std::max<int64_t>(total, 1), 0, s))) return rc;
+  if (n > 0) {
+    if (c->precision == IMC_FP64)
+      k_comb_write<double><<<g, 256, 0, s>>>(c->A.view<double>(), c->B.view<double>(), n, c->ny, c->copies.p,
+                                             c->counts.p, c->ew.p, passthrough, c->cellcount.p);
+    else
+      k_comb_write<float><<<g, 256, 0, s>>>(c->A.view<float>(), c->B.view<float>(), n, c->ny, c->copies.p,
+                                            c->counts.p, c->ew.p, passthrough, c->cellcount.p);
+    IMC_HIP(hipGetLastError());
+  }
+  k_comb_disc<<<launch_blocks(c->ncell, 256), 256, 0, s>>>(c->ncell, c->ecen.p, c->ew.p, c->cellcount.p, passthrough,
+                                                           c->disc.p);
+  IMC_HIP(hipGetLastError());
+  std::swap(c->A, c->B);
+  c->n = total;
+  if ((rc = ensure_sum_scratch(c, total))) return rc;
+  if ((rc = canonical_sum(c->A.w, total, c->scal + 5, c->sum_scr.p, s))) return rc;
+  size_t bytes = sizeof(double) * c->ncell;
+  if (ecen_out) IMC_HIP(hipMemcpyAsync(ecen_out, c->ecen.p, bytes, hipMemcpyDeviceToHost, s));
+  if (disc_out) IMC_HIP(hipMemcpyAsync(disc_out, c->disc.p, bytes, hipMemcpyDeviceToHost, s));
+  double tr = 0.0;
+  IMC_HIP(hipMemcpyAsync(&tr, c->scal + 5, sizeof(double), hipMemcpyDeviceToHost, s));
+  IMC_HIP(hipStreamSynchronize(s));
+  if (total_rad_out) *total_rad_out = tr;
+  return kOk;
+}
+
+// Census energy per cell (bincount of the pool weights, w > 0).
+int imc2d_census_by_cell(imc2d_ctx* c, double* out) {
+  IMC_REQUIRE(c && out, kErrArg, "imc2d_census_by_cell: bad arguments");
+  IMC_HIP(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  IMC_HIP(hipMemsetAsync(c->ecen.p, 0, sizeof(double) * c->ncell, s));
+  if (c->n > 0) {
+    int g = launch_blocks(c->n, 256);
+    if (c->precision == IMC_FP64)
+      k_comb_ecen<double><<<g, 256, 0, s>>>(c->A.view<double>(), c->n, c->nx, c->ny, c->ecen.p);
+    else
+      k_comb_ecen<float><<<g, 256, 0, s>>>(c->A.view<float>(), c->n, c->nx, c->ny, c->ecen.p);
+    IMC_HIP(hipGetLastError());
+  }
+  IMC_HIP(hipMemcpyAsync(out, c->ecen.p, sizeof(double) * c->ncell, hipMemcpyDeviceToHost, s));
+  IMC_HIP(hipStreamSynchronize(s));
+  return kOk;
+}
+
+// Sum of the census weights currently in the pool (canonical order).
+int imc2d_census_energy(imc2d_ctx* c, double* out) {
+  IMC_REQUIRE(c && out, kErrArg, "imc2d_census_energy: bad arguments");
+  IMC_HIP(hipSetDevice(c->device));
+  int rc;
+  if ((rc = ensure_sum_scratch(c, c->n))) return rc;
+  if ((rc = canonical_sum(c->A.w, c->n, c->scal + 5, c->sum_scr.p, c->stream))) return rc;
+  IMC_HIP(hipMemcpyAsync(out, c->scal + 5, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  IMC_HIP(hipStreamSynchronize(c->stream));
+  return kOk;
+}
+
+}  // extern "C"

@@ -1,0 +1,339 @@
+// imc2d_kernels.hpp -- per-segment physics of the gray 2-D IMC tracker.
+//
+// One call = one segment ("event") of one history: distance to the cell faces,
+// to the next effective scatter and to census; the particle moves by the
+// smallest; continuous absorption deposits w(1 - e^{-sigma_a s}).
+//
+// Two instantiations:
+//  * double: restates IMC/IMC2D.py:711-935 (_distance_to_radial_boundary_rz_jit,
+//    _move_particle_xy_jit, _move_particle_rz_jit) operation for operation,
+//    including the 1e-11 face-tie tolerance and the 1e-12 radial nudges.
+//  * float: FP32 geometry for the same physics.  Face ties are decided by
+//    argmin (boundary wins ties, then scatter, as in the reference), non-face
+//    moves are clamped into the cell, the cylinder roots use the
+//    cancellation-free forms, and the absorption split keeps
+//    dep + w_new == w in FP64 so energy bookkeeping stays exact.
+#pragma once
+#include "imc_device.hpp"
+
+namespace imc {
+
+enum Crossing : int { kCrossNone = 0, kCrossIPlus = 1, kCrossIMinus = 2, kCrossJPlus = 3, kCrossJMinus = 4 };
+enum Event : int { kEvtCensus = 0, kEvtBoundary = 1, kEvtScatter = 2 };
+
+template <typename R>
+struct MoveOut {
+  double w, dep_w, dep_i;
+  R d1, d2, p1, p2, dcen;
+  int crossed, evt;
+};
+
+// ---------------------------------------------------------------------------
+// FP64: reference restatement
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void absorb_f64(MoveOut<double>& o, double w, double sa, double s) {
+  double wf = exp(-sa * s);
+  double di = (sa > 1e-12) ? w * (1.0 - wf) / sa : w * s;
+  o.dep_i = di;
+  o.dep_w = di * sa;
+  o.w = w * wf;
+}
+
+// s_scatter with the opacity cap of IMC/IMC2D.py:756-765 (slots 4ev, 4ev+1)
+__device__ __forceinline__ double scatter_distance_f64(Ctr c, Key k, uint32_t ev, double ss, double cl) {
+  double s = 1e30;
+  if (ss > 1e-12) {
+    Pair u = block_d(c, k, 2u * ev);
+    s = -log(1.0 - u.u0) / ss;
+    if (ss * cl > 10000.0) s = -log(1.0 - u.u1) / (10000.0 / cl);
+  }
+  return s;
+}
+
+__device__ __forceinline__ void move_xy(Ctr ctr, Key key, uint32_t ev, double w, double ux, double uy, double x,
+                                        double y, double xl, double xr, double yl, double yr, double sa, double ss,
+                                        double dcen, MoveOut<double>& o) {
+  double sx = 1e30, sy = 1e30;
+  if (ux > 0.0)
+    sx = (xr - x) / ux;
+  else if (ux < 0.0)
+    sx = (xl - x) / ux;
+  if (uy > 0.0)
+    sy = (yr - y) / uy;
+  else if (uy < 0.0)
+    sy = (yl - y) / uy;
+  double cl = fmin(xr - xl, yr - yl);
+  double s_scat = scatter_distance_f64(ctr, key, ev, ss, cl);
+  double sb = sx < sy ? sx : sy;
+  double se = sb;
+  if (s_scat < se) se = s_scat;
+  if (dcen < se) se = dcen;
+  o.d1 = ux; o.d2 = uy; o.p1 = x; o.p2 = y; o.crossed = kCrossNone;
+  if (se <= 0.0) {
+    o.w = w; o.dep_w = 0.0; o.dep_i = 0.0; o.dcen = 0.0; o.evt = kEvtCensus;
+    return;
+  }
+  double xn = x + ux * se, yn = y + uy * se;
+  int evt = kEvtCensus;
+  if (fabs(se - sb) < 1e-11) {
+    evt = kEvtBoundary;
+    if (sx <= sy) {
+      if (ux > 0.0) { xn = xr; o.crossed = kCrossIPlus; } else { xn = xl; o.crossed = kCrossIMinus; }
+    } else {
+      if (uy > 0.0) { yn = yr; o.crossed = kCrossJPlus; } else { yn = yl; o.crossed = kCrossJMinus; }
+    }
+  } else if (se == s_scat) {
+    evt = kEvtScatter;
+    Pair u = block_d(ctr, key, 2u * ev + 1u);
+    double uz = 2.0 * u.u0 - 1.0;
+    double phi = 2.0 * kPi * u.u1;
+    double rxy = sqrt(fmax(0.0, 1.0 - uz * uz));
+    double sp, cp;
+    sincos(phi, &sp, &cp);
+    o.d1 = rxy * cp;
+    o.d2 = rxy * sp;
+  }
+  o.p1 = xn; o.p2 = yn; o.evt = evt;
+  o.dcen = dcen - se;
+  absorb_f64(o, w, sa, se);
+}
+
+__device__ __forceinline__ double dist_radial_f64(double r, double mu, double eta, double R) {
+  double ome2 = 1.0 - eta * eta;
+  if (ome2 < 1e-15) return 1e30;
+  double b2 = r * r * (1.0 - mu * mu);
+  double disc = R * R - b2;
+  if (disc <= 0.0) return 1e30;
+  double root = sqrt(disc);
+  double lp = -r * mu + root, lm = -r * mu - root;
+  double den = sqrt(ome2);
+  double smin = 1e30;
+  if (lp > 1e-15) smin = fmin(smin, lp / den);
+  if (lm > 1e-15) smin = fmin(smin, lm / den);
+  return smin;
+}
+
+__device__ __forceinline__ void move_rz(Ctr ctr, Key key, uint32_t ev, double w, double mu, double eta, double r,
+                                        double z, double rin, double rout, double zlo, double zhi, double sa,
+                                        double ss, double dcen, MoveOut<double>& o) {
+  double s_rin = rin > 0.0 ? dist_radial_f64(r, mu, eta, rin) : 1e30;
+  double s_rout = dist_radial_f64(r, mu, eta, rout);
+  double s_zlo, s_zhi;
+  if (eta > 1e-15) {
+    s_zlo = 1e30; s_zhi = (zhi - z) / eta;
+  } else if (eta < -1e-15) {
+    s_zlo = (zlo - z) / eta; s_zhi = 1e30;
+  } else {
+    s_zlo = 1e30; s_zhi = 1e30;
+  }
+  double cl = fmin(rout - rin, zhi - zlo);
+  double s_scat = scatter_distance_f64(ctr, key, ev, ss, cl);
+  double sb = s_rin;
+  if (s_rout < sb) sb = s_rout;
+  if (s_zlo < sb) sb = s_zlo;
+  if (s_zhi < sb) sb = s_zhi;
+  double se = sb;
+  if (s_scat < se) se = s_scat;
+  if (dcen < se) se = dcen;
+  o.d1 = mu; o.d2 = eta; o.p1 = r; o.p2 = z; o.crossed = kCrossNone;
+  if (se <= 0.0) {
+    o.w = w; o.dep_w = 0.0; o.dep_i = 0.0; o.dcen = 0.0; o.evt = kEvtCensus;
+    return;
+  }
+  double ome2 = 1.0 - eta * eta;
+  if (ome2 < 0.0) ome2 = 0.0;
+  double l = se * sqrt(ome2);
+  double rn2 = r * r + 2.0 * r * mu * l + l * l;
+  if (rn2 < 0.0) rn2 = 0.0;
+  double rn = sqrt(rn2);
+  double zn = z + se * eta;
+  double mun = mu;
+  if (rn > 1e-15 && l > 0.0) mun = fmin(1.0, fmax(-1.0, (r * mu + l) / rn));
+  double etan = eta;
+  int evt = kEvtCensus;
+  if (fabs(se - sb) < 1e-11) {
+    evt = kEvtBoundary;
+    if (sb == s_rin) {
+      rn = rin > 0.0 ? rin * (1.0 - 1e-12) : 1e-12;
+      o.crossed = kCrossIMinus;
+    } else if (sb == s_rout) {
+      rn = rout * (1.0 + 1e-12);
+      o.crossed = kCrossIPlus;
+    } else if (sb == s_zlo) {
+      zn = zlo;
+      o.crossed = kCrossJMinus;
+    } else {
+      zn = zhi;
+      o.crossed = kCrossJPlus;
+    }
+  } else if (se == s_scat) {
+    evt = kEvtScatter;
+    Pair u = block_d(ctr, key, 2u * ev + 1u);
+    etan = 2.0 * u.u0 - 1.0;
+    mun = cos(2.0 * kPi * u.u1);
+  }
+  o.d1 = mun; o.d2 = etan; o.p1 = rn; o.p2 = zn; o.evt = evt;
+  o.dcen = dcen - se;
+  absorb_f64(o, w, sa, se);
+}
+
+// ---------------------------------------------------------------------------
+// FP32 geometry, FP64 weights
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void absorb_f32(MoveOut<float>& o, double w, float sa, float s) {
+  float x = sa * s;
+  double dw, wn;
+  if (x < 0.5f) {
+    float om = -expm1f(-x);
+    dw = w * (double)om;
+    wn = w - dw;
+  } else {
+    float wf = __expf(-x);
+    wn = w * (double)wf;
+    dw = w - wn;
+  }
+  o.dep_w = dw;
+  o.dep_i = (sa > 1e-12f) ? dw / (double)sa : w * (double)s;
+  o.w = wn;
+}
+
+__device__ __forceinline__ float scatter_distance_f32(Ctr c, Key k, uint32_t ev, float ss, float cl) {
+  float s = 1e30f;
+  if (ss > 1e-12f) {
+    PairF u = block_f(c, k, 2u * ev);
+    s = -__logf(1.0f - u.u0) / ss;
+    if (ss * cl > 10000.0f) s = -__logf(1.0f - u.u1) * (cl * 1.0e-4f);
+  }
+  return s;
+}
+
+__device__ __forceinline__ void move_xy(Ctr ctr, Key key, uint32_t ev, double w, float ux, float uy, float x, float y,
+                                        float xl, float xr, float yl, float yr, float sa, float ss, float dcen,
+                                        MoveOut<float>& o) {
+  float sx = 1e30f, sy = 1e30f;
+  if (ux > 0.0f)
+    sx = fmaxf((xr - x) / ux, 0.0f);
+  else if (ux < 0.0f)
+    sx = fmaxf((xl - x) / ux, 0.0f);
+  if (uy > 0.0f)
+    sy = fmaxf((yr - y) / uy, 0.0f);
+  else if (uy < 0.0f)
+    sy = fmaxf((yl - y) / uy, 0.0f);
+  float cl = fminf(xr - xl, yr - yl);
+  float s_scat = scatter_distance_f32(ctr, key, ev, ss, cl);
+  float sb = fminf(sx, sy);
+  o.crossed = kCrossNone;
+  o.d1 = ux; o.d2 = uy;
+  float se;
+  if (sb <= s_scat && sb <= dcen) {
+    se = sb;
+    o.evt = kEvtBoundary;
+    float xn = x + ux * se, yn = y + uy * se;
+    if (sx <= sy) {
+      if (ux > 0.0f) { xn = xr; o.crossed = kCrossIPlus; } else { xn = xl; o.crossed = kCrossIMinus; }
+      yn = fminf(fmaxf(yn, yl), yr);
+    } else {
+      if (uy > 0.0f) { yn = yr; o.crossed = kCrossJPlus; } else { yn = yl; o.crossed = kCrossJMinus; }
+      xn = fminf(fmaxf(xn, xl), xr);
+    }
+    o.p1 = xn; o.p2 = yn;
+  } else {
+    se = fminf(s_scat, dcen);
+    o.p1 = fminf(fmaxf(x + ux * se, xl), xr);
+    o.p2 = fminf(fmaxf(y + uy * se, yl), yr);
+    if (s_scat <= dcen) {
+      o.evt = kEvtScatter;
+      PairF u = block_f(ctr, key, 2u * ev + 1u);
+      float uz = 2.0f * u.u0 - 1.0f;
+      float rxy = sqrtf(fmaxf(0.0f, 1.0f - uz * uz));
+      o.d1 = rxy * __builtin_amdgcn_cosf(u.u1);
+      o.d2 = rxy * __builtin_amdgcn_sinf(u.u1);
+    } else {
+      o.evt = kEvtCensus;
+    }
+  }
+  o.dcen = dcen - se;
+  absorb_f32(o, w, sa, se);
+}
+
+// Distance to the cylinder of radius R along the transverse chord, divided by
+// sqrt(1 - eta^2).  outer: the particle is inside (r <= R); inner: outside.
+__device__ __forceinline__ float dist_outer_f32(float r, float mu, float R, float den) {
+  float ome = (1.0f - mu) * (1.0f + mu);
+  float disc = fmaxf(R * R - r * r * ome, 0.0f);
+  float root = sqrtf(disc);
+  float l = (mu > 0.0f) ? fmaxf((R - r) * (R + r), 0.0f) / (root + r * mu) : root - r * mu;
+  if (!(l >= 0.0f)) l = 0.0f;  // r == R, mu == 1 degenerate: 0/0
+  return l / den;
+}
+__device__ __forceinline__ float dist_inner_f32(float r, float mu, float R, float den) {
+  if (!(mu < 0.0f)) return 1e30f;
+  float ome = (1.0f - mu) * (1.0f + mu);
+  float disc = R * R - r * r * ome;
+  if (disc <= 0.0f) return 1e30f;
+  float root = sqrtf(disc);
+  float l = fmaxf((r - R) * (r + R), 0.0f) / (root - r * mu);
+  return l / den;
+}
+
+__device__ __forceinline__ void move_rz(Ctr ctr, Key key, uint32_t ev, double w, float mu, float eta, float r,
+                                        float z, float rin, float rout, float zlo, float zhi, float sa, float ss,
+                                        float dcen, MoveOut<float>& o) {
+  float ome2 = fmaxf((1.0f - eta) * (1.0f + eta), 0.0f);
+  float den = sqrtf(ome2);
+  float s_rin = 1e30f, s_rout = 1e30f;
+  if (den > 0.0f) {
+    if (rin > 0.0f) s_rin = dist_inner_f32(r, mu, rin, den);
+    s_rout = dist_outer_f32(r, mu, rout, den);
+  }
+  float s_zlo = 1e30f, s_zhi = 1e30f;
+  if (eta > 0.0f)
+    s_zhi = fmaxf((zhi - z) / eta, 0.0f);
+  else if (eta < 0.0f)
+    s_zlo = fmaxf((zlo - z) / eta, 0.0f);
+  float cl = fminf(rout - rin, zhi - zlo);
+  float s_scat = scatter_distance_f32(ctr, key, ev, ss, cl);
+  float sr = fminf(s_rin, s_rout), sz = fminf(s_zlo, s_zhi);
+  float sb = fminf(sr, sz);
+  o.crossed = kCrossNone;
+  float se;
+  bool bnd = (sb <= s_scat && sb <= dcen);
+  if (bnd)
+    se = sb;
+  else
+    se = fminf(s_scat, dcen);
+  // stream along the chord
+  float l = se * den;
+  float a = r + mu * l;
+  float rn = sqrtf(a * a + l * l * fmaxf((1.0f - mu) * (1.0f + mu), 0.0f));
+  float zn = z + se * eta;
+  float mun = mu;
+  if (rn > 0.0f && l > 0.0f) mun = fminf(1.0f, fmaxf(-1.0f, a / rn));
+  float etan = eta;
+  if (bnd) {
+    o.evt = kEvtBoundary;
+    if (sr <= sz) {
+      if (s_rin <= s_rout) { rn = rin; o.crossed = kCrossIMinus; } else { rn = rout; o.crossed = kCrossIPlus; }
+      zn = fminf(fmaxf(zn, zlo), zhi);
+    } else {
+      if (s_zlo <= s_zhi) { zn = zlo; o.crossed = kCrossJMinus; } else { zn = zhi; o.crossed = kCrossJPlus; }
+      rn = fminf(fmaxf(rn, rin), rout);
+    }
+  } else {
+    rn = fminf(fmaxf(rn, rin), rout);
+    zn = fminf(fmaxf(zn, zlo), zhi);
+    if (s_scat <= dcen) {
+      o.evt = kEvtScatter;
+      PairF u = block_f(ctr, key, 2u * ev + 1u);
+      etan = 2.0f * u.u0 - 1.0f;
+      mun = __builtin_amdgcn_cosf(u.u1);
+    } else {
+      o.evt = kEvtCensus;
+    }
+  }
+  o.d1 = mun; o.d2 = etan; o.p1 = rn; o.p2 = zn;
+  o.dcen = dcen - se;
+  absorb_f32(o, w, sa, se);
+}
+
+}  // namespace imc

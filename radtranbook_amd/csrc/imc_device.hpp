@@ -1,0 +1,116 @@
+// imc_device.hpp -- device-side building blocks shared by the IMC kernels:
+// counter-based Philox4x32-10 streams, uniform conversions, constants and
+// wave64 helpers for CDNA (gfx950).
+//
+// Random-stream layout (replaces the reference's hidden per-thread MT19937 of
+// IMC/IMC2D.py:697-707,756-765 with a counter-based generator so any history
+// can be replayed from its counter):
+//   key = { lo32(seed), hi32(seed) }
+//   ctr = { lo32(idx), (hi32(idx) & 0xFFFFFF) | tag << 24, step, block }
+// A block yields 4 words = 2 slots (a,b); slot q is block q/2, half q%2.
+//   FP64 uniform: ((a >> 5) * 2^26 + (b >> 6)) * 2^-53 in [0,1)
+//   FP32 uniform: (a >> 8) * 2^-24                     in [0,1)
+// Philox4x32-10 is the SC'11 generator of Salmon et al.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace imc {
+
+constexpr double kC = 29.98;     // cm/ns (IMC/IMC2D.py:48)
+constexpr double kA = 0.01372;   // GJ/(cm^3 keV^4) (IMC/IMC2D.py:49)
+constexpr double kPi = 3.14159265358979323846;
+
+enum Tag : uint32_t {
+  kTagTransport = 1,
+  kTagEmission = 2,
+  kTagSource = 3,
+  kTagComb = 4,
+  kTagIC = 5,
+  kTagBoundary0 = 8,  // + side
+};
+
+struct Key {
+  uint32_t k0, k1;
+};
+
+__host__ __device__ inline Key make_key(uint64_t seed) { return Key{(uint32_t)seed, (uint32_t)(seed >> 32)}; }
+
+struct Ctr {
+  uint32_t c0, c1, c2;  // c3 is the block number
+};
+
+__host__ __device__ inline Ctr make_ctr(uint32_t tag, uint64_t idx, uint32_t step) {
+  return Ctr{(uint32_t)idx, ((uint32_t)(idx >> 32) & 0x00FFFFFFu) | (tag << 24), step};
+}
+
+struct U4 {
+  uint32_t x, y, z, w;
+};
+
+__device__ __forceinline__ U4 philox10(Ctr c, uint32_t block, Key k) {
+  uint32_t x0 = c.c0, x1 = c.c1, x2 = c.c2, x3 = block;
+  uint32_t k0 = k.k0, k1 = k.k1;
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    if (r > 0) {
+      k0 += 0x9E3779B9u;
+      k1 += 0xBB67AE85u;
+    }
+    uint32_t lo0 = 0xD2511F53u * x0;
+    uint32_t hi0 = __umulhi(0xD2511F53u, x0);
+    uint32_t lo1 = 0xCD9E8D57u * x2;
+    uint32_t hi1 = __umulhi(0xCD9E8D57u, x2);
+    uint32_t n0 = hi1 ^ x1 ^ k0;
+    uint32_t n2 = hi0 ^ x3 ^ k1;
+    x0 = n0;
+    x1 = lo1;
+    x2 = n2;
+    x3 = lo0;
+  }
+  return U4{x0, x1, x2, x3};
+}
+
+__device__ __forceinline__ double u01_d(uint32_t a, uint32_t b) {
+  return ((double)(a >> 5) * 67108864.0 + (double)(b >> 6)) * (1.0 / 9007199254740992.0);
+}
+__device__ __forceinline__ float u01_f(uint32_t a) { return (float)(a >> 8) * (1.0f / 16777216.0f); }
+
+// Uniform from slot q of a stream (one Philox evaluation).
+__device__ __forceinline__ double slot_d(Ctr c, Key k, uint32_t q) {
+  U4 u = philox10(c, q >> 1, k);
+  return (q & 1u) ? u01_d(u.z, u.w) : u01_d(u.x, u.y);
+}
+
+// Both slots of block b.
+struct Pair {
+  double u0, u1;
+};
+__device__ __forceinline__ Pair block_d(Ctr c, Key k, uint32_t b) {
+  U4 u = philox10(c, b, k);
+  return Pair{u01_d(u.x, u.y), u01_d(u.z, u.w)};
+}
+struct PairF {
+  float u0, u1;
+};
+__device__ __forceinline__ PairF block_f(Ctr c, Key k, uint32_t b) {
+  U4 u = philox10(c, b, k);
+  return PairF{u01_f(u.x), u01_f(u.z)};
+}
+
+// wave64 helpers
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+__device__ __forceinline__ uint64_t lanemask_lt() {
+  return (__lane_id() == 0) ? 0ull : ((~0ull) >> (64 - __lane_id()));
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__device__ __forceinline__ void atomic_add_f64(double* p, double v) { unsafeAtomicAdd(p, v); }
+
+}  // namespace imc

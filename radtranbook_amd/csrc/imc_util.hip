@@ -1,0 +1,170 @@
+// imc_util.hip -- error reporting, deterministic reductions and scans.
+#include <cstdarg>
+#include <cstdio>
+
+#include "imc_util.hpp"
+
+namespace imc {
+
+static thread_local char g_err[1024] = "";
+
+void set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+const char* get_error() { return g_err; }
+
+int launch_blocks(int64_t n, int block) {
+  int64_t b = (n + block - 1) / block;
+  if (b < 1) b = 1;
+  if (b > 0x7fffffff) b = 0x7fffffff;
+  return (int)b;
+}
+
+// ---------------------------------------------------------------------------
+// canonical sum
+// ---------------------------------------------------------------------------
+__global__ void k_chunk_sum(const double* __restrict__ x, int64_t n, double* __restrict__ out) {
+  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t lo = c * 1024;
+  if (lo >= n) return;
+  int64_t hi = lo + 1024 < n ? lo + 1024 : n;
+  double s = 0.0;
+  for (int64_t i = lo; i < hi; ++i) s += x[i];
+  out[c] = s;
+}
+
+int64_t canonical_sum_scratch(int64_t n) {
+  int64_t tot = 2;
+  while (n > 1) {
+    n = (n + 1023) / 1024;
+    tot += n;
+  }
+  return tot;
+}
+
+int canonical_sum(const double* x, int64_t n, double* out, double* scratch, hipStream_t s) {
+  if (n <= 0) {
+    IMC_HIP(hipMemsetAsync(out, 0, sizeof(double), s));
+    return kOk;
+  }
+  const double* src = x;
+  double* dst = scratch;
+  int64_t m = n;
+  while (true) {
+    int64_t nc = (m + 1023) / 1024;
+    double* target = (nc == 1) ? out : dst;
+    k_chunk_sum<<<launch_blocks(nc, 64), 64, 0, s>>>(src, m, target);
+    IMC_HIP(hipGetLastError());
+    if (nc == 1) break;
+    src = dst;
+    dst += nc;
+    m = nc;
+  }
+  return kOk;
+}
+
+// ---------------------------------------------------------------------------
+// exclusive scan (int64), 256 threads x 8 items per block
+// ---------------------------------------------------------------------------
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+template <typename TIn>
+__global__ __launch_bounds__(kScanThreads) void k_scan_tile(const TIn* __restrict__ in, int64_t n,
+                                                            int64_t* __restrict__ out, int64_t* __restrict__ sums) {
+  __shared__ int64_t wsum[kScanThreads / 64];
+  int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+  int64_t v[kScanItems];
+  int64_t run = 0;
+#pragma unroll
+  for (int q = 0; q < kScanItems; ++q) {
+    int64_t idx = base + q;
+    int64_t x = idx < n ? (int64_t)in[idx] : 0;
+    v[q] = run;
+    run += x;
+  }
+  // inclusive wave scan of per-thread totals
+  int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  int64_t inc = run;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    int64_t y = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += y;
+  }
+  if (lane == 63) wsum[wid] = inc;
+  __syncthreads();
+  int64_t wofs = 0;
+  for (int w = 0; w < wid; ++w) wofs += wsum[w];
+  int64_t excl = wofs + inc - run;
+#pragma unroll
+  for (int q = 0; q < kScanItems; ++q) {
+    int64_t idx = base + q;
+    if (idx < n) out[idx] = excl + v[q];
+  }
+  if (threadIdx.x == kScanThreads - 1 && sums) sums[blockIdx.x] = wofs + inc;
+}
+
+__global__ void k_scan_add(int64_t* __restrict__ out, int64_t n, const int64_t* __restrict__ sums) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[i] += sums[i / kScanTile];
+}
+
+__global__ void k_scan_total(const int64_t* __restrict__ excl, const int64_t* __restrict__ last_in_i64,
+                             const int32_t* __restrict__ last_in_i32, int64_t n, int64_t* __restrict__ out_total) {
+  int64_t last = n > 0 ? (last_in_i64 ? last_in_i64[n - 1] : (int64_t)last_in_i32[n - 1]) : 0;
+  *out_total = (n > 0 ? excl[n - 1] : 0) + last;
+}
+
+int64_t scan_scratch(int64_t n) {
+  int64_t tot = 1;
+  while (n > kScanTile) {
+    int64_t nb = (n + kScanTile - 1) / kScanTile;
+    tot += 2 * nb + 1;
+    n = nb;
+  }
+  return tot;
+}
+
+template <typename TIn>
+static int scan_impl(const TIn* in, int64_t n, int64_t* out, int64_t* scratch, hipStream_t s) {
+  if (n <= 0) {
+    IMC_HIP(hipMemsetAsync(out, 0, sizeof(int64_t), s));
+    return kOk;
+  }
+  int64_t nb = (n + kScanTile - 1) / kScanTile;
+  int64_t* sums = nb > 1 ? scratch : nullptr;
+  k_scan_tile<TIn><<<(unsigned)nb, kScanThreads, 0, s>>>(in, n, out, sums);
+  IMC_HIP(hipGetLastError());
+  if (nb > 1) {
+    int64_t* sums_scan = scratch + nb;
+    int rc = scan_impl<int64_t>(sums, nb, sums_scan, scratch + 2 * nb + 1, s);
+    if (rc) return rc;
+    k_scan_add<<<launch_blocks(n, 256), 256, 0, s>>>(out, n, sums_scan);
+    IMC_HIP(hipGetLastError());
+  }
+  return kOk;
+}
+
+int exclusive_scan_i64(const int64_t* in, int64_t n, int64_t* out, int64_t* scratch, hipStream_t s) {
+  int rc = scan_impl<int64_t>(in, n, out, scratch, s);
+  if (rc) return rc;
+  k_scan_total<<<1, 1, 0, s>>>(out, in, nullptr, n, out + n);
+  IMC_HIP(hipGetLastError());
+  return kOk;
+}
+
+int exclusive_scan_i32_to_i64(const int32_t* in, int64_t n, int64_t* out, int64_t* scratch, hipStream_t s) {
+  int rc = scan_impl<int32_t>(in, n, out, scratch, s);
+  if (rc) return rc;
+  k_scan_total<<<1, 1, 0, s>>>(out, nullptr, in, n, out + n);
+  IMC_HIP(hipGetLastError());
+  return kOk;
+}
+
+}  // namespace imc

@@ -1,0 +1,88 @@
+// imc_util.hpp -- host-side helpers shared by the IMC C-ABI translation units:
+// error reporting, device buffers, deterministic reductions and scans.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <cstdio>
+#include <string>
+
+namespace imc {
+
+// Status codes returned across the C-ABI (0 = ok).
+enum Status : int {
+  kOk = 0,
+  kErrHip = -1,
+  kErrArg = -2,
+  kErrCapacity = -3,
+  kErrState = -4,
+};
+
+void set_error(const char* fmt, ...);
+const char* get_error();
+
+#define IMC_HIP(expr)                                                                  \
+  do {                                                                                 \
+    hipError_t _e = (expr);                                                            \
+    if (_e != hipSuccess) {                                                            \
+      ::imc::set_error("%s:%d %s -> %s", __FILE__, __LINE__, #expr, hipGetErrorString(_e)); \
+      return ::imc::kErrHip;                                                           \
+    }                                                                                  \
+  } while (0)
+
+#define IMC_REQUIRE(cond, code, ...)  \
+  do {                                \
+    if (!(cond)) {                    \
+      ::imc::set_error(__VA_ARGS__);  \
+      return (code);                  \
+    }                                 \
+  } while (0)
+
+// Device allocation that remembers its size; grows by reallocation.
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  int64_t cap = 0;
+  int reserve(int64_t n) {
+    if (n <= cap) return kOk;
+    T* q = nullptr;
+    IMC_HIP(hipMalloc(&q, sizeof(T) * (size_t)(n > 0 ? n : 1)));
+    if (p) IMC_HIP(hipFree(p));
+    p = q;
+    cap = n;
+    return kOk;
+  }
+  // grow keeping the first `keep` elements
+  int grow(int64_t n, int64_t keep, hipStream_t s) {
+    if (n <= cap) return kOk;
+    T* q = nullptr;
+    IMC_HIP(hipMalloc(&q, sizeof(T) * (size_t)n));
+    if (p && keep > 0) IMC_HIP(hipMemcpyAsync(q, p, sizeof(T) * (size_t)keep, hipMemcpyDeviceToDevice, s));
+    IMC_HIP(hipStreamSynchronize(s));
+    if (p) IMC_HIP(hipFree(p));
+    p = q;
+    cap = n;
+    return kOk;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
+// Deterministic "canonical" sum: 1024-element chunks summed left to right,
+// repeated on the chunk sums until one value is left.  The result lands in
+// *out (device).  `scratch` must hold ceil(n/1024) + ceil(n/2^20) + 2 doubles.
+int canonical_sum(const double* x, int64_t n, double* out, double* scratch, hipStream_t s);
+int64_t canonical_sum_scratch(int64_t n);
+
+// Exclusive scan of int64 counts; writes n+1 offsets (last = total).
+// `scratch` must hold scan_scratch(n) int64.
+int exclusive_scan_i64(const int64_t* in, int64_t n, int64_t* out, int64_t* scratch, hipStream_t s);
+int64_t scan_scratch(int64_t n);
+int exclusive_scan_i32_to_i64(const int32_t* in, int64_t n, int64_t* out, int64_t* scratch, hipStream_t s);
+
+int launch_blocks(int64_t n, int block);
+
+}  // namespace imc
