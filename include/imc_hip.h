@@ -56,8 +56,9 @@ void imc2d_destroy(imc2d_ctx* ctx);
 /* Step number used as the Philox counter word of every draw that follows. */
 int imc2d_set_step(imc2d_ctx* ctx, uint32_t step);
 /* Transport launch shape: grid 0 = occupancy-derived persistent grid;
- * batch = particles a wave claims per queue visit. */
-int imc2d_set_launch(imc2d_ctx* ctx, int grid, int batch);
+ * batch = particles a wave claims per queue visit; tally_copies = private
+ * copies of the per-cell tallies (power of two, 1..64). */
+int imc2d_set_launch(imc2d_ctx* ctx, int grid, int batch, int tally_copies);
 
 /* Census access: the particle arrays of SimulationState2D
  * (weights, dir1, dir2, times, pos1, pos2, cell_i, cell_j; IMC/IMC2D.py:68-75). */
@@ -113,9 +114,12 @@ int imc2d_transport(imc2d_ctx* ctx, double dt, const int32_t* reflect, int64_t m
 float imc2d_last_transport_ms(imc2d_ctx* ctx);
 
 /* Per-cell stochastic comb to about Nmax particles: _comb (IMC/IMC2D.py:1448-1511).
- * ecen_out: pre-comb census energy per cell; disc_out: ecen - ecen_after;
- * total_rad_out: census energy after the comb.  Times reset to 0. */
-int imc2d_comb(imc2d_ctx* ctx, int64_t Nmax, double* ecen_out, double* disc_out, double* total_rad_out);
+ * Optional outputs: ecen_out, pre-comb census energy per cell; disc_out,
+ * ecen - ecen_after per cell (the comb discrepancy of IMC/IMC2D.py:1511);
+ * ecen_total_out, pre-comb census energy; total_rad_out, census energy
+ * after the comb.  Times reset to 0 (IMC/IMC2D.py:1831). */
+int imc2d_comb(imc2d_ctx* ctx, int64_t Nmax, double* ecen_out, double* disc_out, double* ecen_total_out,
+               double* total_rad_out);
 
 #ifdef __cplusplus
 }

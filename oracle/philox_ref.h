@@ -74,6 +74,7 @@ static inline void orc_stream_init(orc_stream* s, uint64_t seed, int tag, uint64
   s->ctr[2] = step;
   s->ctr[3] = 0;
   s->cached_block = -1;
+  s->buf[0] = s->buf[1] = s->buf[2] = s->buf[3] = 0;
 }
 
 static inline void orc_stream_block(orc_stream* s, uint32_t block) {

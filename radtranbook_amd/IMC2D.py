@@ -207,7 +207,8 @@ def step(state, Ntarget, Nboundary, Nsource, Nmax, T_boundary, dt, edges1, edges
     internal_energy = state.internal_energy + dep_cell - E_emit_cells / volumes
     temperature = inv_eos(internal_energy)
 
-    ecen, comb_disc, total_rad = eng.comb(Nmax)
+    ecen, comb_disc, ecen_total, total_rad = eng.comb(Nmax, want_ecen=not use_scalar_intensity_Tr,
+                                                      want_disc=conserve_comb_energy)
     if use_scalar_intensity_Tr:
         radiation_temperature = (si_cell / __a / __c) ** 0.25
     else:
@@ -219,7 +220,8 @@ def step(state, Ntarget, Nboundary, Nsource, Nmax, T_boundary, dt, edges1, edges
     total_internal = float(np.sum(internal_energy * volumes))
     total_energy = total_internal + total_rad
     energy_loss = total_energy - state.previous_total_energy - boundary_emission + boundary_loss - source_emission
-    disc_total = 0.0 if conserve_comb_energy else float(np.sum(comb_disc))
+    # comb discrepancy booked as a sink unless it went back into the material
+    disc_total = 0.0 if conserve_comb_energy else ecen_total - total_rad
 
     state.internal_energy = internal_energy
     state.temperature = temperature

@@ -35,7 +35,7 @@ SIGNATURES = {
     "imc2d_create": (C.c_int, [C.POINTER(_vp), C.c_int, C.c_int, C.c_int, _dp, _dp, C.c_int, C.c_uint64, C.c_int]),
     "imc2d_destroy": (None, [_vp]),
     "imc2d_set_step": (C.c_int, [_vp, C.c_uint32]),
-    "imc2d_set_launch": (C.c_int, [_vp, C.c_int, C.c_int]),
+    "imc2d_set_launch": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
     "imc2d_census_size": (C.c_int64, [_vp]),
     "imc2d_put_census": (C.c_int, [_vp, C.c_int64, _dp, _dp, _dp, _dp, _dp, _dp, _i64p, _i64p]),
     "imc2d_get_census": (C.c_int, [_vp, C.c_int64, _dp, _dp, _dp, _dp, _dp, _dp, _i64p, _i64p]),
@@ -47,7 +47,7 @@ SIGNATURES = {
     "imc2d_sample_volume": (C.c_int, [_vp, C.c_int, C.c_int64, _dp, C.c_double, C.c_int, _dp, _dp, _i64p]),
     "imc2d_transport": (C.c_int, [_vp, C.c_double, _i32p, C.c_int64, C.c_double, _dp, _dp, _dp, _i64p]),
     "imc2d_last_transport_ms": (C.c_float, [_vp]),
-    "imc2d_comb": (C.c_int, [_vp, C.c_int64, _dp, _dp, _dp]),
+    "imc2d_comb": (C.c_int, [_vp, C.c_int64, _dp, _dp, _dp, _dp]),
 }
 
 
@@ -130,8 +130,8 @@ class Engine2D:
     def set_step(self, step):
         check(self._L.imc2d_set_step(self._h, int(step) & 0xFFFFFFFF))
 
-    def set_launch(self, grid=0, batch=256):
-        check(self._L.imc2d_set_launch(self._h, int(grid), int(batch)))
+    def set_launch(self, grid=0, batch=256, tally_copies=8):
+        check(self._L.imc2d_set_launch(self._h, int(grid), int(batch), int(tally_copies)))
 
     # -- census ---------------------------------------------------------------
     @property
@@ -201,9 +201,12 @@ class Engine2D:
     def last_transport_ms(self):
         return float(self._L.imc2d_last_transport_ms(self._h))
 
-    def comb(self, Nmax, want_cells=True):
-        ecen = np.empty(self.shape) if want_cells else None
-        disc = np.empty(self.shape) if want_cells else None
+    def comb(self, Nmax, want_ecen=False, want_disc=False):
+        """Returns (ecen per cell or None, disc per cell or None, pre-comb
+        census energy, post-comb census energy)."""
+        ecen = np.empty(self.shape) if want_ecen else None
+        disc = np.empty(self.shape) if want_disc else None
+        et = C.c_double()
         tr = C.c_double()
-        check(self._L.imc2d_comb(self._h, int(Nmax), _p(ecen), _p(disc), C.byref(tr)))
-        return ecen, disc, tr.value
+        check(self._L.imc2d_comb(self._h, int(Nmax), _p(ecen), _p(disc), C.byref(et), C.byref(tr)))
+        return ecen, disc, et.value, tr.value

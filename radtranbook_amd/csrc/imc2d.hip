@@ -83,7 +83,7 @@ struct PoolBuf {
 __global__ void k_fleck(int64_t n, const double* __restrict__ T, const double* __restrict__ sig,
                         const double* __restrict__ cv, double theta, double dt, int clip, double* __restrict__ sa,
                         double* __restrict__ ss, float* __restrict__ saf, float* __restrict__ ssf,
-                        int* __restrict__ bad) {
+                        double2* __restrict__ sas64, float2* __restrict__ sas32, int* __restrict__ bad) {
 #pragma clang fp contract(off)
   int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
@@ -100,9 +100,11 @@ __global__ void k_fleck(int64_t n, const double* __restrict__ T, const double* _
   double s_a = sig[c] * f;
   sa[c] = s_a;
   ss[c] = s_s;
+  sas64[c] = make_double2(s_a, s_s);
   if (saf) {
     saf[c] = (float)s_a;
     ssf[c] = (float)s_s;
+    sas32[c] = make_float2((float)s_a, (float)s_s);
   }
 }
 
@@ -377,14 +379,25 @@ __global__ void k_sample_boundary(PoolView<R> P, int64_t base, int64_t n, int ge
 // accumulation flushed with FP64 atomics.  Follows IMC/IMC2D.py:938-1107.
 // ---------------------------------------------------------------------------
 template <typename R>
+struct Vec2;
+template <>
+struct Vec2<float> {
+  using type = float2;
+};
+template <>
+struct Vec2<double> {
+  using type = double2;
+};
+
+template <typename R>
 struct TransportArgs {
+  using R2 = typename Vec2<R>::type;
   PoolView<R> P;
   int64_t n;
-  const R* e1;
-  const R* e2;
+  const R2* e1p;  // (e1[i], e1[i+1])
+  const R2* e2p;  // (e2[j], e2[j+1])
   int nx, ny;
-  const R* sa;
-  const R* ss;
+  const R2* sas;  // (sigma_a, sigma_s) per cell
   double dt;
   int reflect;  // bit s set -> side s reflects
   int64_t max_events;
@@ -393,15 +406,26 @@ struct TransportArgs {
   uint32_t step;
   unsigned long long* next;
   int batch;
-  double* dep_raw;  // sum of dep_w per cell
-  double* si_raw;   // sum of dep_i per cell
+  double* dep_raw;  // copies x ncell sums of dep_w
+  double* si_raw;   // copies x ncell sums of w*s in cells with sigma_a <= 1e-12
+  int64_t ncell;
+  int copy_mask;  // number of tally copies - 1 (power of two)
   double* bl;
   unsigned long long* stats;  // [8]
 };
 
 template <typename R, int GEOM>
 __global__ __launch_bounds__(256) void k_transport(TransportArgs<R> A) {
+  using R2 = typename Vec2<R>::type;
   const double wfloor = *A.floor_ptr;
+  // per-workgroup tally copy: spreads same-cell atomics over copies and XCDs
+  const int64_t copy_off = (int64_t)(blockIdx.x & A.copy_mask) * A.ncell;
+  double* const dep_t = A.dep_raw + copy_off;
+  double* const si_t = A.si_raw + copy_off;
+  __shared__ int lkey[kLdsSlots];
+  __shared__ double lval[kLdsSlots];
+  lds_tally_init(lkey, lval);
+  __syncthreads();
   const int ny = A.ny, nx = A.nx;
   int64_t qbase = 0, qend = 0;
   bool exhausted = false;
@@ -467,28 +491,30 @@ __global__ __launch_bounds__(256) void k_transport(TransportArgs<R> A) {
       finish = true;
     } else {
       int c = i * ny + j;
-      R sa = A.sa[c], ss = A.ss[c];
+      R2 mat = A.sas[c];
+      R2 ex = A.e1p[i];
+      R2 ey = A.e2p[j];
       MoveOut<R> o;
       Ctr ctr = make_ctr(kTagTransport, (uint64_t)k, A.step);
       if (GEOM == 0)
-        move_xy(ctr, A.key, ev - 1u, w, d1, d2, p1, p2, A.e1[i], A.e1[i + 1], A.e2[j], A.e2[j + 1], sa, ss, dcen, o);
+        move_xy(ctr, A.key, ev - 1u, w, d1, d2, p1, p2, ex.x, ex.y, ey.x, ey.y, mat.x, mat.y, dcen, o);
       else
-        move_rz(ctr, A.key, ev - 1u, w, d1, d2, p1, p2, A.e1[i], A.e1[i + 1], A.e2[j], A.e2[j + 1], sa, ss, dcen, o);
+        move_rz(ctr, A.key, ev - 1u, w, d1, d2, p1, p2, ex.x, ex.y, ey.x, ey.y, mat.x, mat.y, dcen, o);
       dcen = o.dcen;
       st_bnd += o.evt == kEvtBoundary;
       st_sc += o.evt == kEvtScatter;
       st_cen += o.evt == kEvtCensus;
       if (c != acc_cell) {
         if (acc_cell >= 0) {
-          atomic_add_f64(A.dep_raw + acc_cell, acc_dep);
-          atomic_add_f64(A.si_raw + acc_cell, acc_si);
+          if (!lds_tally_add(lkey, lval, acc_cell, acc_dep)) atomic_add_f64(dep_t + acc_cell, acc_dep);
+          if (acc_si != 0.0) atomic_add_f64(si_t + acc_cell, acc_si);
         }
         acc_cell = c;
         acc_dep = 0.0;
         acc_si = 0.0;
       }
       acc_dep += o.dep_w;
-      acc_si += o.dep_i;
+      if (!(mat.x > (R)1e-12)) acc_si += o.dep_i;
       w = o.w;
       if (o.w <= wfloor) {
         ++st_floor;
@@ -536,9 +562,11 @@ __global__ __launch_bounds__(256) void k_transport(TransportArgs<R> A) {
     }
   }
   if (acc_cell >= 0) {
-    atomic_add_f64(A.dep_raw + acc_cell, acc_dep);
-    atomic_add_f64(A.si_raw + acc_cell, acc_si);
+    if (!lds_tally_add(lkey, lval, acc_cell, acc_dep)) atomic_add_f64(dep_t + acc_cell, acc_dep);
+    if (acc_si != 0.0) atomic_add_f64(si_t + acc_cell, acc_si);
   }
+  __syncthreads();
+  lds_tally_flush(lkey, lval, dep_t);
   // wave reductions of counters
   unsigned long long s0 = wave_sum<unsigned long long>(st_ev);
   unsigned long long s1 = wave_sum<unsigned long long>(st_bnd);
@@ -565,13 +593,33 @@ __global__ void k_weight_floor(const double* __restrict__ sumw, int64_t n, doubl
   *out = 1e-10 * (*sumw) / (double)(n > 1 ? n : 1);
 }
 
-__global__ void k_finalize_tallies(int64_t n, const double* __restrict__ dep_raw, const double* __restrict__ si_raw,
-                                   const double* __restrict__ vol, double dt, double* __restrict__ dep,
-                                   double* __restrict__ si) {
+// dep = sum(dep_w)/V; si = sum(dep_w)/(sigma_a dt V) where sigma_a > 1e-12,
+// else sum(w s)/(dt V) (the two branches of IMC/IMC2D.py:817-821).
+__global__ void k_finalize_tallies(int64_t n, int copies, const double* __restrict__ dep_raw,
+                                   const double* __restrict__ si_raw, const double* __restrict__ sa64,
+                                   const float* __restrict__ sa32, const double* __restrict__ vol, double dt,
+                                   double* __restrict__ dep, double* __restrict__ si) {
   int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
-  dep[c] = dep_raw[c] / vol[c];
-  si[c] = si_raw[c] / (dt * vol[c]);
+  double a = 0.0, b = 0.0;
+  for (int k = 0; k < copies; ++k) {
+    a += dep_raw[(int64_t)k * n + c];
+    b += si_raw[(int64_t)k * n + c];
+  }
+  bool thick = sa32 ? (sa32[c] > 1e-12f) : (sa64[c] > 1e-12);
+  double sa = sa32 ? (double)sa32[c] : sa64[c];
+  dep[c] = a / vol[c];
+  si[c] = thick ? a / sa / (dt * vol[c]) : b / (dt * vol[c]);
+}
+
+// Sum the private copies of a per-cell histogram.
+template <typename T, typename U>
+__global__ void k_reduce_copies(int64_t n, int copies, const T* __restrict__ part, U* __restrict__ out) {
+  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  U a = 0;
+  for (int k = 0; k < copies; ++k) a += (U)part[(int64_t)k * n + c];
+  out[c] = a;
 }
 
 // ---------------------------------------------------------------------------
@@ -584,11 +632,36 @@ __device__ __forceinline__ bool comb_alive(const PoolView<R>& P, int64_t k, int 
   return P.w[k] > 0.0 && ci >= 0 && ci < nx && cj >= 0 && cj < ny;
 }
 
+// Census energy per cell: a grid-stride pass over the pool; runs of equal
+// cells among consecutive lanes are summed in registers, the run totals go to
+// the workgroup's LDS tally cache, and the cache is flushed to this
+// workgroup's private copy of the histogram at the end.
 template <typename R>
-__global__ void k_comb_ecen(PoolView<R> P, int64_t n, int nx, int ny, double* __restrict__ ecen) {
-  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
-  if (comb_alive(P, k, nx, ny)) atomic_add_f64(ecen + (int64_t)P.ci[k] * ny + P.cj[k], P.w[k]);
+__global__ __launch_bounds__(256) void k_comb_ecen(PoolView<R> P, int64_t n, int nx, int ny, int copy_mask,
+                                                   double* __restrict__ ecen_part) {
+  __shared__ int lkey[kLdsSlots];
+  __shared__ double lval[kLdsSlots];
+  lds_tally_init(lkey, lval);
+  __syncthreads();
+  const int64_t ncell = (int64_t)nx * ny;
+  double* const dst = ecen_part + (int64_t)(blockIdx.x & copy_mask) * ncell;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  // every lane runs the same number of trips so the wave-wide run sum sees
+  // all 64 lanes
+  const int64_t trips = (n + stride - 1) / stride;
+  for (int64_t t = 0; t < trips; ++t) {
+    int64_t k = t * stride + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int key = -1;
+    double v = 0.0;
+    if (k < n && comb_alive(P, k, nx, ny)) {
+      key = P.ci[k] * ny + P.cj[k];
+      v = P.w[k];
+    }
+    bool head = wave_run_sum(key, v);
+    if (head && key >= 0 && !lds_tally_add(lkey, lval, key, v)) atomic_add_f64(dst + key, v);
+  }
+  __syncthreads();
+  lds_tally_flush(lkey, lval, dst);
 }
 
 __global__ void k_comb_ew(int64_t ncell, const double* __restrict__ ecen, const double* __restrict__ Etot,
@@ -629,30 +702,54 @@ __global__ void k_comb_copies(PoolView<R> P, int64_t n, int nx, int ny, const do
   copies[k] = m;
 }
 
+// Comb output: workgroup b expands particles [256 b, 256 b + 256).  Their
+// copy offsets are staged in LDS and the workgroup's threads stride over its
+// contiguous output range, each finding its parent by binary search in LDS:
+// coalesced writes, and a parent with many copies is shared by the whole
+// workgroup instead of one lane.
 template <typename R>
-__global__ void k_comb_write(PoolView<R> S, PoolView<R> D, int64_t n, int ny, const int32_t* __restrict__ copies,
-                             const int64_t* __restrict__ offs, const double* __restrict__ ew, int passthrough,
-                             unsigned long long* __restrict__ cellcount) {
-  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
-  int32_t m = copies[k];
-  if (m <= 0) return;
-  int32_t ci = S.ci[k], cj = S.cj[k];
-  int64_t c = (int64_t)ci * ny + cj;
-  double w = passthrough ? S.w[k] : ew[c];
-  R d1 = S.d1[k], d2 = S.d2[k], p1 = S.p1[k], p2 = S.p2[k];
-  int64_t o = offs[k];
-  for (int32_t r = 0; r < m; ++r) {
-    D.w[o + r] = w;
-    D.d1[o + r] = d1;
-    D.d2[o + r] = d2;
-    D.p1[o + r] = p1;
-    D.p2[o + r] = p2;
-    D.t[o + r] = (R)0;  // times reset after the comb (IMC/IMC2D.py:1831)
-    D.ci[o + r] = ci;
-    D.cj[o + r] = cj;
+__global__ __launch_bounds__(256) void k_comb_write(PoolView<R> S, PoolView<R> D, int64_t n, int nx, int ny,
+                                                    const int32_t* __restrict__ copies,
+                                                    const int64_t* __restrict__ offs, const double* __restrict__ ew,
+                                                    int passthrough, int copy_mask,
+                                                    unsigned int* __restrict__ count_part) {
+  __shared__ int64_t so[257];
+  const int t = threadIdx.x;
+  const int64_t k0 = (int64_t)blockIdx.x * 256;
+  const int64_t k = k0 + t;
+  int32_t m = k < n ? copies[k] : 0;
+  if (count_part) {
+    int key = m > 0 ? S.ci[k] * ny + S.cj[k] : -1;
+    unsigned int cnt = m > 0 ? (unsigned int)m : 0u;
+    bool head = wave_run_sum(key, cnt);
+    if (head && key >= 0)
+      atomicAdd(count_part + (int64_t)(blockIdx.x & copy_mask) * ((int64_t)nx * ny) + key, cnt);
   }
-  if (!passthrough) atomicAdd(cellcount + c, (unsigned long long)m);
+  const int cnt = (n - k0 < 256) ? (int)(n - k0) : 256;
+  if (t < cnt) so[t] = offs[k0 + t];
+  if (t == 0) so[cnt] = offs[k0 + cnt];  // offs holds n + 1 entries
+  __syncthreads();
+  const int64_t lo = so[0], hi = so[cnt];
+  for (int64_t o = lo + t; o < hi; o += 256) {
+    int a = 0, b = cnt - 1;  // last parent p with so[p] <= o
+    while (a < b) {
+      int mid = (a + b + 1) >> 1;
+      if (so[mid] <= o)
+        a = mid;
+      else
+        b = mid - 1;
+    }
+    const int64_t src = k0 + a;
+    const int32_t ci = S.ci[src], cj = S.cj[src];
+    D.w[o] = passthrough ? S.w[src] : ew[(int64_t)ci * ny + cj];
+    D.d1[o] = S.d1[src];
+    D.d2[o] = S.d2[src];
+    D.p1[o] = S.p1[src];
+    D.p2[o] = S.p2[src];
+    D.t[o] = (R)0;  // times reset after the comb (IMC/IMC2D.py:1831)
+    D.ci[o] = ci;
+    D.cj[o] = cj;
+  }
 }
 
 __global__ void k_comb_disc(int64_t ncell, const double* __restrict__ ecen, const double* __restrict__ ew,
@@ -722,9 +819,14 @@ struct Imc2dCtx {
   DevBuf<double> T, sig, cv, sa, ss, E, Ar, Mr, Pn, dep_raw, si_raw, dep, si, ecen, ew, disc, prof, sum_scr, wbuf,
       volsrc;
   DevBuf<float> saf, ssf;
+  DevBuf<float2> sas32, e1p32, e2p32;
+  DevBuf<double2> sas64, e1p64, e2p64;
+  int tally_copies = kTallyCopies;
   DevBuf<int64_t> counts, offs, scan_scr;
   DevBuf<int32_t> copies;
   DevBuf<unsigned long long> cellcount;
+  DevBuf<double> ecen_part;
+  DevBuf<unsigned int> count_part;
   double* scal = nullptr;  // [0] Etot  [1] weight sum  [2] floor  [3] bl  [4] comb E  [5] rad total
   unsigned long long* stats = nullptr;
   unsigned long long* next = nullptr;
@@ -813,6 +915,32 @@ int imc2d_create(imc2d_ctx** out, int geometry, int nx, int ny, const double* ed
                                 &c->dep_raw, &c->si_raw, &c->dep, &c->si, &c->ecen, &c->ew, &c->disc, &c->volsrc};
   for (auto* b : cellbufs)
     if ((rc = b->reserve(nc))) return rc;
+  if ((rc = c->dep_raw.reserve(nc * kMaxTallyCopies))) return rc;
+  if ((rc = c->si_raw.reserve(nc * kMaxTallyCopies))) return rc;
+  if ((rc = c->ecen_part.reserve(nc * kMaxTallyCopies))) return rc;
+  if ((rc = c->count_part.reserve(nc * kMaxTallyCopies))) return rc;
+  if ((rc = c->sas32.reserve(nc))) return rc;
+  if ((rc = c->sas64.reserve(nc))) return rc;
+  if ((rc = c->e1p32.reserve(nx))) return rc;
+  if ((rc = c->e2p32.reserve(ny))) return rc;
+  if ((rc = c->e1p64.reserve(nx))) return rc;
+  if ((rc = c->e2p64.reserve(ny))) return rc;
+  {
+    std::vector<float2> a32(nx), b32(ny);
+    std::vector<double2> a64(nx), b64(ny);
+    for (int q = 0; q < nx; ++q) {
+      a32[q] = make_float2((float)edges1[q], (float)edges1[q + 1]);
+      a64[q] = make_double2(edges1[q], edges1[q + 1]);
+    }
+    for (int q = 0; q < ny; ++q) {
+      b32[q] = make_float2((float)edges2[q], (float)edges2[q + 1]);
+      b64[q] = make_double2(edges2[q], edges2[q + 1]);
+    }
+    IMC_HIP(hipMemcpy(c->e1p32.p, a32.data(), sizeof(float2) * nx, hipMemcpyHostToDevice));
+    IMC_HIP(hipMemcpy(c->e2p32.p, b32.data(), sizeof(float2) * ny, hipMemcpyHostToDevice));
+    IMC_HIP(hipMemcpy(c->e1p64.p, a64.data(), sizeof(double2) * nx, hipMemcpyHostToDevice));
+    IMC_HIP(hipMemcpy(c->e2p64.p, b64.data(), sizeof(double2) * ny, hipMemcpyHostToDevice));
+  }
   if ((rc = c->saf.reserve(nc))) return rc;
   if ((rc = c->ssf.reserve(nc))) return rc;
   if ((rc = c->counts.reserve(nc + 1))) return rc;
@@ -841,11 +969,19 @@ void imc2d_destroy(imc2d_ctx* c) {
   for (auto* b : cellbufs) b->release();
   c->saf.release();
   c->ssf.release();
+  c->sas32.release();
+  c->sas64.release();
+  c->e1p32.release();
+  c->e2p32.release();
+  c->e1p64.release();
+  c->e2p64.release();
   c->counts.release();
   c->offs.release();
   c->scan_scr.release();
   c->copies.release();
   c->cellcount.release();
+  c->ecen_part.release();
+  c->count_part.release();
   c->A.release();
   c->B.release();
   void* ptrs[] = {c->e1, c->e2, c->e1f, c->e2f, c->vol, c->scal, c->stats, c->next, c->bad};
@@ -863,11 +999,14 @@ int imc2d_set_step(imc2d_ctx* c, uint32_t step) {
   return kOk;
 }
 
-int imc2d_set_launch(imc2d_ctx* c, int grid, int batch) {
+int imc2d_set_launch(imc2d_ctx* c, int grid, int batch, int tally_copies) {
   IMC_REQUIRE(c, kErrArg, "null ctx");
   IMC_REQUIRE(grid >= 0 && batch > 0, kErrArg, "imc2d_set_launch: bad arguments");
+  IMC_REQUIRE(tally_copies >= 1 && tally_copies <= kMaxTallyCopies && (tally_copies & (tally_copies - 1)) == 0,
+              kErrArg, "imc2d_set_launch: tally_copies must be a power of two in [1, %d]", kMaxTallyCopies);
   c->grid_override = grid;
   c->batch = batch;
+  c->tally_copies = tally_copies;
   return kOk;
 }
 
@@ -946,7 +1085,8 @@ int imc2d_set_material(imc2d_ctx* c, const double* T, const double* sigma_true, 
   bool f32 = c->precision == IMC_FP32_TRACK;
   k_fleck<<<launch_blocks(c->ncell, 256), 256, 0, c->stream>>>(c->ncell, c->T.p, c->sig.p, c->cv.p, theta, dt, clip,
                                                                c->sa.p, c->ss.p, f32 ? c->saf.p : nullptr,
-                                                               f32 ? c->ssf.p : nullptr, c->bad);
+                                                               f32 ? c->ssf.p : nullptr, c->sas64.p, c->sas32.p,
+                                                               c->bad);
   IMC_HIP(hipGetLastError());
   int bad = 0;
   IMC_HIP(hipMemcpyAsync(&bad, c->bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -1087,8 +1227,8 @@ int imc2d_transport(imc2d_ctx* c, double dt, const int32_t* reflect, int64_t max
   IMC_HIP(hipSetDevice(c->device));
   hipStream_t s = c->stream;
   int64_t n = c->n;
-  IMC_HIP(hipMemsetAsync(c->dep_raw.p, 0, sizeof(double) * c->ncell, s));
-  IMC_HIP(hipMemsetAsync(c->si_raw.p, 0, sizeof(double) * c->ncell, s));
+  IMC_HIP(hipMemsetAsync(c->dep_raw.p, 0, sizeof(double) * c->ncell * c->tally_copies, s));
+  IMC_HIP(hipMemsetAsync(c->si_raw.p, 0, sizeof(double) * c->ncell * c->tally_copies, s));
   IMC_HIP(hipMemsetAsync(c->stats, 0, sizeof(unsigned long long) * 8, s));
   IMC_HIP(hipMemsetAsync(c->scal + 3, 0, sizeof(double), s));
   IMC_HIP(hipMemsetAsync(c->next, 0, sizeof(unsigned long long), s));
@@ -1126,17 +1266,17 @@ int imc2d_transport(imc2d_ctx* c, double dt, const int32_t* reflect, int64_t max
   IMC_HIP(hipEventRecord(c->ev0, s));
   if (n > 0) {
     if (c->precision == IMC_FP64) {
-      TransportArgs<double> A{c->A.view<double>(), n, c->e1, c->e2, c->nx, c->ny, c->sa.p, c->ss.p, dt, refl,
+      TransportArgs<double> A{c->A.view<double>(), n, c->e1p64.p, c->e2p64.p, c->nx, c->ny, c->sas64.p, dt, refl,
                               max_events, c->scal + 2, key, c->step, c->next, c->batch, c->dep_raw.p, c->si_raw.p,
-                              c->scal + 3, c->stats};
+                              c->ncell, c->tally_copies - 1, c->scal + 3, c->stats};
       if (c->geometry == IMC_GEOM_XY)
         k_transport<double, 0><<<grid, block, 0, s>>>(A);
       else
         k_transport<double, 1><<<grid, block, 0, s>>>(A);
     } else {
-      TransportArgs<float> A{c->A.view<float>(), n, c->e1f, c->e2f, c->nx, c->ny, c->saf.p, c->ssf.p, dt, refl,
+      TransportArgs<float> A{c->A.view<float>(), n, c->e1p32.p, c->e2p32.p, c->nx, c->ny, c->sas32.p, dt, refl,
                              max_events, c->scal + 2, key, c->step, c->next, c->batch, c->dep_raw.p, c->si_raw.p,
-                             c->scal + 3, c->stats};
+                             c->ncell, c->tally_copies - 1, c->scal + 3, c->stats};
       if (c->geometry == IMC_GEOM_XY)
         k_transport<float, 0><<<grid, block, 0, s>>>(A);
       else
@@ -1145,8 +1285,10 @@ int imc2d_transport(imc2d_ctx* c, double dt, const int32_t* reflect, int64_t max
     IMC_HIP(hipGetLastError());
   }
   IMC_HIP(hipEventRecord(c->ev1, s));
-  k_finalize_tallies<<<launch_blocks(c->ncell, 256), 256, 0, s>>>(c->ncell, c->dep_raw.p, c->si_raw.p, c->vol, dt,
-                                                                  c->dep.p, c->si.p);
+  bool f32 = c->precision == IMC_FP32_TRACK;
+  k_finalize_tallies<<<launch_blocks(c->ncell, 256), 256, 0, s>>>(c->ncell, c->tally_copies, c->dep_raw.p,
+                                                                  c->si_raw.p, c->sa.p, f32 ? c->saf.p : nullptr,
+                                                                  c->vol, dt, c->dep.p, c->si.p);
   IMC_HIP(hipGetLastError());
   size_t bytes = sizeof(double) * c->ncell;
   if (dep_out) IMC_HIP(hipMemcpyAsync(dep_out, c->dep.p, bytes, hipMemcpyDeviceToHost, s));
@@ -1165,42 +1307,59 @@ int imc2d_transport(imc2d_ctx* c, double dt, const int32_t* reflect, int64_t max
 
 float imc2d_last_transport_ms(imc2d_ctx* c) { return c ? c->last_transport_ms : -1.0f; }
 
-// Census comb (IMC/IMC2D.py:1448-1511).  ecen_out: pre-comb census energy per
-// cell (also the census-bincount radiation estimate), disc_out: ecen - ecen_after.
-int imc2d_comb(imc2d_ctx* c, int64_t Nmax, double* ecen_out, double* disc_out, double* total_rad_out) {
+// Census energy per cell of the current pool into c->ecen (device).
+static int census_histogram(imc2d_ctx* c) {
+  hipStream_t s = c->stream;
+  int64_t nc = c->ncell;
+  IMC_HIP(hipMemsetAsync(c->ecen_part.p, 0, sizeof(double) * nc * c->tally_copies, s));
+  if (c->n > 0) {
+    int g = std::min(launch_blocks(c->n, 256), c->n_cu * 4);
+    if (c->precision == IMC_FP64)
+      k_comb_ecen<double><<<g, 256, 0, s>>>(c->A.view<double>(), c->n, c->nx, c->ny, c->tally_copies - 1,
+                                            c->ecen_part.p);
+    else
+      k_comb_ecen<float><<<g, 256, 0, s>>>(c->A.view<float>(), c->n, c->nx, c->ny, c->tally_copies - 1,
+                                           c->ecen_part.p);
+    IMC_HIP(hipGetLastError());
+  }
+  k_reduce_copies<double, double><<<launch_blocks(nc, 256), 256, 0, s>>>(nc, c->tally_copies, c->ecen_part.p,
+                                                                         c->ecen.p);
+  IMC_HIP(hipGetLastError());
+  return kOk;
+}
+
+// Census comb (IMC/IMC2D.py:1448-1511).  Optional outputs: ecen_out, the
+// pre-comb census energy per cell (also the census-bincount radiation
+// estimate); disc_out, ecen - ecen_after per cell; ecen_total_out, the
+// pre-comb census energy; total_rad_out, the census energy after the comb.
+int imc2d_comb(imc2d_ctx* c, int64_t Nmax, double* ecen_out, double* disc_out, double* ecen_total_out,
+               double* total_rad_out) {
   IMC_REQUIRE(c && Nmax >= 0, kErrArg, "imc2d_comb: bad arguments");
   IMC_HIP(hipSetDevice(c->device));
   hipStream_t s = c->stream;
   int64_t n = c->n;
+  int64_t nc = c->ncell;
   int rc;
-  IMC_HIP(hipMemsetAsync(c->ecen.p, 0, sizeof(double) * c->ncell, s));
-  IMC_HIP(hipMemsetAsync(c->cellcount.p, 0, sizeof(unsigned long long) * c->ncell, s));
-  int g = launch_blocks(n, 256);
-  if (n > 0) {
-    if (c->precision == IMC_FP64)
-      k_comb_ecen<double><<<g, 256, 0, s>>>(c->A.view<double>(), n, c->nx, c->ny, c->ecen.p);
-    else
-      k_comb_ecen<float><<<g, 256, 0, s>>>(c->A.view<float>(), n, c->nx, c->ny, c->ecen.p);
-    IMC_HIP(hipGetLastError());
-  }
-  if ((rc = canonical_sum(c->ecen.p, c->ncell, c->scal + 4, c->sum_scr.p, s))) return rc;
+  if ((rc = census_histogram(c))) return rc;
+  if ((rc = canonical_sum(c->ecen.p, nc, c->scal + 4, c->sum_scr.p, s))) return rc;
   double E = 0.0;
   IMC_HIP(hipMemcpyAsync(&E, c->scal + 4, sizeof(double), hipMemcpyDeviceToHost, s));
   IMC_HIP(hipStreamSynchronize(s));
   int passthrough = !(E > 0.0);
-  k_comb_ew<<<launch_blocks(c->ncell, 256), 256, 0, s>>>(c->ncell, c->ecen.p, c->scal + 4, Nmax, c->ew.p);
+  k_comb_ew<<<launch_blocks(nc, 256), 256, 0, s>>>(nc, c->ecen.p, c->scal + 4, Nmax, c->ew.p);
   IMC_HIP(hipGetLastError());
   if ((rc = c->copies.reserve(n + 1))) return rc;
   if ((rc = c->counts.reserve(n + 2))) return rc;  // reused as copy offsets
   if ((rc = c->scan_scr.reserve(scan_scratch(n) + 8))) return rc;
   Key key = make_key(c->seed);
+  int g = launch_blocks(n, 256);
   if (n > 0) {
     if (c->precision == IMC_FP64)
-      k_comb_copies<double><<<g, 256, 0, s>>>(c->A.view<double>(), n, c->nx, c->ny, c->ew.p, passthrough, key, c->step,
-                                              c->copies.p);
+      k_comb_copies<double><<<g, 256, 0, s>>>(c->A.view<double>(), n, c->nx, c->ny, c->ew.p, passthrough, key,
+                                              c->step, c->copies.p);
     else
-      k_comb_copies<float><<<g, 256, 0, s>>>(c->A.view<float>(), n, c->nx, c->ny, c->ew.p, passthrough, key, c->step,
-                                             c->copies.p);
+      k_comb_copies<float><<<g, 256, 0, s>>>(c->A.view<float>(), n, c->nx, c->ny, c->ew.p, passthrough, key,
+                                             c->step, c->copies.p);
     IMC_HIP(hipGetLastError());
   }
   if ((rc = exclusive_scan_i32_to_i64(c->copies.p, n, c->counts.p, c->scan_scr.p, s))) return rc;
@@ -1208,29 +1367,39 @@ int imc2d_comb(imc2d_ctx* c, int64_t Nmax, double* ecen_out, double* disc_out, d
   IMC_HIP(hipMemcpyAsync(&total, c->counts.p + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
   IMC_HIP(hipStreamSynchronize(s));
   if ((rc = c->B.grow(std::max<int64_t>(total, 1), 0, s))) return rc;
+  bool want_counts = disc_out != nullptr && !passthrough;
+  if (want_counts) IMC_HIP(hipMemsetAsync(c->count_part.p, 0, sizeof(unsigned int) * nc * c->tally_copies, s));
+  unsigned int* cp = want_counts ? c->count_part.p : nullptr;
   if (n > 0) {
     if (c->precision == IMC_FP64)
-      k_comb_write<double><<<g, 256, 0, s>>>(c->A.view<double>(), c->B.view<double>(), n, c->ny, c->copies.p,
-                                             c->counts.p, c->ew.p, passthrough, c->cellcount.p);
+      k_comb_write<double><<<g, 256, 0, s>>>(c->A.view<double>(), c->B.view<double>(), n, c->nx, c->ny,
+                                             c->copies.p, c->counts.p, c->ew.p, passthrough, c->tally_copies - 1, cp);
     else
-      k_comb_write<float><<<g, 256, 0, s>>>(c->A.view<float>(), c->B.view<float>(), n, c->ny, c->copies.p,
-                                            c->counts.p, c->ew.p, passthrough, c->cellcount.p);
+      k_comb_write<float><<<g, 256, 0, s>>>(c->A.view<float>(), c->B.view<float>(), n, c->nx, c->ny, c->copies.p,
+                                            c->counts.p, c->ew.p, passthrough, c->tally_copies - 1, cp);
     IMC_HIP(hipGetLastError());
   }
-  k_comb_disc<<<launch_blocks(c->ncell, 256), 256, 0, s>>>(c->ncell, c->ecen.p, c->ew.p, c->cellcount.p, passthrough,
-                                                           c->disc.p);
-  IMC_HIP(hipGetLastError());
+  size_t bytes = sizeof(double) * nc;
+  if (disc_out) {
+    if (want_counts) {
+      k_reduce_copies<unsigned int, unsigned long long><<<launch_blocks(nc, 256), 256, 0, s>>>(
+          nc, c->tally_copies, c->count_part.p, c->cellcount.p);
+      IMC_HIP(hipGetLastError());
+    }
+    k_comb_disc<<<launch_blocks(nc, 256), 256, 0, s>>>(nc, c->ecen.p, c->ew.p, c->cellcount.p, passthrough, c->disc.p);
+    IMC_HIP(hipGetLastError());
+    IMC_HIP(hipMemcpyAsync(disc_out, c->disc.p, bytes, hipMemcpyDeviceToHost, s));
+  }
   std::swap(c->A, c->B);
   c->n = total;
   if ((rc = ensure_sum_scratch(c, total))) return rc;
   if ((rc = canonical_sum(c->A.w, total, c->scal + 5, c->sum_scr.p, s))) return rc;
-  size_t bytes = sizeof(double) * c->ncell;
   if (ecen_out) IMC_HIP(hipMemcpyAsync(ecen_out, c->ecen.p, bytes, hipMemcpyDeviceToHost, s));
-  if (disc_out) IMC_HIP(hipMemcpyAsync(disc_out, c->disc.p, bytes, hipMemcpyDeviceToHost, s));
   double tr = 0.0;
   IMC_HIP(hipMemcpyAsync(&tr, c->scal + 5, sizeof(double), hipMemcpyDeviceToHost, s));
   IMC_HIP(hipStreamSynchronize(s));
   if (total_rad_out) *total_rad_out = tr;
+  if (ecen_total_out) *ecen_total_out = E;
   return kOk;
 }
 
@@ -1238,18 +1407,10 @@ int imc2d_comb(imc2d_ctx* c, int64_t Nmax, double* ecen_out, double* disc_out, d
 int imc2d_census_by_cell(imc2d_ctx* c, double* out) {
   IMC_REQUIRE(c && out, kErrArg, "imc2d_census_by_cell: bad arguments");
   IMC_HIP(hipSetDevice(c->device));
-  hipStream_t s = c->stream;
-  IMC_HIP(hipMemsetAsync(c->ecen.p, 0, sizeof(double) * c->ncell, s));
-  if (c->n > 0) {
-    int g = launch_blocks(c->n, 256);
-    if (c->precision == IMC_FP64)
-      k_comb_ecen<double><<<g, 256, 0, s>>>(c->A.view<double>(), c->n, c->nx, c->ny, c->ecen.p);
-    else
-      k_comb_ecen<float><<<g, 256, 0, s>>>(c->A.view<float>(), c->n, c->nx, c->ny, c->ecen.p);
-    IMC_HIP(hipGetLastError());
-  }
-  IMC_HIP(hipMemcpyAsync(out, c->ecen.p, sizeof(double) * c->ncell, hipMemcpyDeviceToHost, s));
-  IMC_HIP(hipStreamSynchronize(s));
+  int rc = census_histogram(c);
+  if (rc) return rc;
+  IMC_HIP(hipMemcpyAsync(out, c->ecen.p, sizeof(double) * c->ncell, hipMemcpyDeviceToHost, c->stream));
+  IMC_HIP(hipStreamSynchronize(c->stream));
   return kOk;
 }
 

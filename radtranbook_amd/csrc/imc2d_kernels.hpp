@@ -21,6 +21,8 @@ namespace imc {
 enum Crossing : int { kCrossNone = 0, kCrossIPlus = 1, kCrossIMinus = 2, kCrossJPlus = 3, kCrossJMinus = 4 };
 enum Event : int { kEvtCensus = 0, kEvtBoundary = 1, kEvtScatter = 2 };
 
+// dep_w: energy absorbed on the segment.  dep_i: w(1-e^{-sigma_a s})/sigma_a
+// (FP64) or, in FP32, the path weight w*s used only where sigma_a <= 1e-12.
 template <typename R>
 struct MoveOut {
   double w, dep_w, dep_i;
@@ -193,14 +195,17 @@ __device__ __forceinline__ void absorb_f32(MoveOut<float>& o, double w, float sa
     dw = w - wn;
   }
   o.dep_w = dw;
-  o.dep_i = (sa > 1e-12f) ? dw / (double)sa : w * (double)s;
+  // path-length tally w*s is needed only where sigma_a <= 1e-12; elsewhere the
+  // cell total is sum(dep_w) / sigma_a
+  o.dep_i = (sa > 1e-12f) ? 0.0 : w * (double)s;
   o.w = wn;
 }
 
-__device__ __forceinline__ float scatter_distance_f32(Ctr c, Key k, uint32_t ev, float ss, float cl) {
+// FP32 tracking draws four 24-bit uniforms from one Philox block per event:
+// u0 scatter distance, u1 capped redraw, u2 and u3 the new direction.
+__device__ __forceinline__ float scatter_distance_f32(const QuadF& u, float ss, float cl) {
   float s = 1e30f;
   if (ss > 1e-12f) {
-    PairF u = block_f(c, k, 2u * ev);
     s = -__logf(1.0f - u.u0) / ss;
     if (ss * cl > 10000.0f) s = -__logf(1.0f - u.u1) * (cl * 1.0e-4f);
   }
@@ -220,7 +225,8 @@ __device__ __forceinline__ void move_xy(Ctr ctr, Key key, uint32_t ev, double w,
   else if (uy < 0.0f)
     sy = fmaxf((yl - y) / uy, 0.0f);
   float cl = fminf(xr - xl, yr - yl);
-  float s_scat = scatter_distance_f32(ctr, key, ev, ss, cl);
+  QuadF u = (ss > 1e-12f) ? block4_f(ctr, key, ev) : QuadF{0.0f, 0.0f, 0.0f, 0.0f};
+  float s_scat = scatter_distance_f32(u, ss, cl);
   float sb = fminf(sx, sy);
   o.crossed = kCrossNone;
   o.d1 = ux; o.d2 = uy;
@@ -243,11 +249,10 @@ __device__ __forceinline__ void move_xy(Ctr ctr, Key key, uint32_t ev, double w,
     o.p2 = fminf(fmaxf(y + uy * se, yl), yr);
     if (s_scat <= dcen) {
       o.evt = kEvtScatter;
-      PairF u = block_f(ctr, key, 2u * ev + 1u);
-      float uz = 2.0f * u.u0 - 1.0f;
+      float uz = 2.0f * u.u2 - 1.0f;
       float rxy = sqrtf(fmaxf(0.0f, 1.0f - uz * uz));
-      o.d1 = rxy * __builtin_amdgcn_cosf(u.u1);
-      o.d2 = rxy * __builtin_amdgcn_sinf(u.u1);
+      o.d1 = rxy * __builtin_amdgcn_cosf(u.u3);
+      o.d2 = rxy * __builtin_amdgcn_sinf(u.u3);
     } else {
       o.evt = kEvtCensus;
     }
@@ -292,7 +297,8 @@ __device__ __forceinline__ void move_rz(Ctr ctr, Key key, uint32_t ev, double w,
   else if (eta < 0.0f)
     s_zlo = fmaxf((zlo - z) / eta, 0.0f);
   float cl = fminf(rout - rin, zhi - zlo);
-  float s_scat = scatter_distance_f32(ctr, key, ev, ss, cl);
+  QuadF u = (ss > 1e-12f) ? block4_f(ctr, key, ev) : QuadF{0.0f, 0.0f, 0.0f, 0.0f};
+  float s_scat = scatter_distance_f32(u, ss, cl);
   float sr = fminf(s_rin, s_rout), sz = fminf(s_zlo, s_zhi);
   float sb = fminf(sr, sz);
   o.crossed = kCrossNone;
@@ -302,13 +308,14 @@ __device__ __forceinline__ void move_rz(Ctr ctr, Key key, uint32_t ev, double w,
     se = sb;
   else
     se = fminf(s_scat, dcen);
-  // stream along the chord
+  // stream along the chord: r'^2 = (r + mu l)^2 + l^2 (1 - mu^2), and the new
+  // radial cosine is mu' = (r mu + l) / r'
   float l = se * den;
   float a = r + mu * l;
   float rn = sqrtf(a * a + l * l * fmaxf((1.0f - mu) * (1.0f + mu), 0.0f));
   float zn = z + se * eta;
   float mun = mu;
-  if (rn > 0.0f && l > 0.0f) mun = fminf(1.0f, fmaxf(-1.0f, a / rn));
+  if (rn > 0.0f && l > 0.0f) mun = fminf(1.0f, fmaxf(-1.0f, (r * mu + l) / rn));
   float etan = eta;
   if (bnd) {
     o.evt = kEvtBoundary;
@@ -324,9 +331,8 @@ __device__ __forceinline__ void move_rz(Ctr ctr, Key key, uint32_t ev, double w,
     zn = fminf(fmaxf(zn, zlo), zhi);
     if (s_scat <= dcen) {
       o.evt = kEvtScatter;
-      PairF u = block_f(ctr, key, 2u * ev + 1u);
-      etan = 2.0f * u.u0 - 1.0f;
-      mun = __builtin_amdgcn_cosf(u.u1);
+      etan = 2.0f * u.u2 - 1.0f;
+      mun = __builtin_amdgcn_cosf(u.u3);
     } else {
       o.evt = kEvtCensus;
     }

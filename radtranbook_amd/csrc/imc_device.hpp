@@ -90,12 +90,14 @@ __device__ __forceinline__ Pair block_d(Ctr c, Key k, uint32_t b) {
   U4 u = philox10(c, b, k);
   return Pair{u01_d(u.x, u.y), u01_d(u.z, u.w)};
 }
-struct PairF {
-  float u0, u1;
+
+struct QuadF {
+  float u0, u1, u2, u3;
 };
-__device__ __forceinline__ PairF block_f(Ctr c, Key k, uint32_t b) {
+// Four 24-bit uniforms from one block (FP32 tracking stream).
+__device__ __forceinline__ QuadF block4_f(Ctr c, Key k, uint32_t b) {
   U4 u = philox10(c, b, k);
-  return PairF{u01_f(u.x), u01_f(u.z)};
+  return QuadF{u01_f(u.x), u01_f(u.y), u01_f(u.z), u01_f(u.w)};
 }
 
 // wave64 helpers
@@ -112,5 +114,79 @@ __device__ __forceinline__ T wave_sum(T v) {
 }
 
 __device__ __forceinline__ void atomic_add_f64(double* p, double v) { unsafeAtomicAdd(p, v); }
+
+// Default number of private copies of per-cell histograms (tallies, census
+// energy): workgroup b updates copy b % copies, so the global atomics that
+// miss the LDS cache spread over copies (and XCDs).
+constexpr int kTallyCopies = 8;
+constexpr int kMaxTallyCopies = 64;
+
+// Sum of v over each run of equal `key` in consecutive lanes of the wave.
+// Every lane of the wave must call it.  The first lane of a run receives the
+// run total and returns true.
+template <typename T>
+__device__ __forceinline__ bool wave_run_sum(int key, T& v) {
+  const int lane = __lane_id();
+  int prev = __shfl_up(key, 1, 64);
+  bool head = (lane == 0) || (prev != key);
+  uint64_t heads = __ballot(head);
+  uint64_t above = (lane == 63) ? 0ull : (heads & (~0ull << (lane + 1)));
+  int run_end = above ? (int)__builtin_ctzll(above) : 64;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    T t = __shfl_down(v, o, 64);
+    if (lane + o < run_end) v += t;
+  }
+  return head;
+}
+
+// Per-workgroup tally cache in LDS: an open-addressing hash of cell -> FP64
+// sum.  A slot is claimed by the first cell that lands on it and keeps it
+// for the kernel's lifetime; adds to cached cells are LDS atomics, adds that
+// find no slot within kLdsProbes fall back to the caller's global atomic.
+// Global atomics on gfx950 cap near 2.4e10/s device-wide, while the tally
+// traffic of an IMC step concentrates on a few hundred hot cells, so the
+// cache removes most global atomics.
+constexpr int kLdsSlots = 1024;
+constexpr int kLdsProbes = 4;
+
+__device__ __forceinline__ void lds_tally_init(int* key, double* val) {
+  for (int s = threadIdx.x; s < kLdsSlots; s += blockDim.x) {
+    key[s] = -1;
+    val[s] = 0.0;
+  }
+}
+
+// Returns false if the value still has to go to global memory.
+__device__ __forceinline__ bool lds_tally_add(int* key, double* val, int cell, double v) {
+  uint32_t s = ((uint32_t)cell * 2654435761u) >> (32 - 10);  // kLdsSlots = 2^10
+#pragma unroll
+  for (int p = 0; p < kLdsProbes; ++p) {
+    int k = __hip_atomic_load(key + s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (k == -1) {
+      int expected = -1;
+      __hip_atomic_compare_exchange_strong(key + s, &expected, cell, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+      k = (expected == -1) ? cell : expected;
+    }
+    if (k == cell) {
+      __hip_atomic_fetch_add(val + s, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      return true;
+    }
+    s = (s + 1) & (kLdsSlots - 1);
+  }
+  return false;
+}
+
+// After a __syncthreads(): add every cached cell into dst (global).
+__device__ __forceinline__ void lds_tally_flush(const int* key, const double* val, double* dst) {
+  for (int s = threadIdx.x; s < kLdsSlots; s += blockDim.x) {
+    int k = key[s];
+    if (k >= 0) {
+      double v = val[s];
+      if (v != 0.0) atomic_add_f64(dst + k, v);
+    }
+  }
+}
 
 }  // namespace imc

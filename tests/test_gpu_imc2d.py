@@ -41,6 +41,18 @@ def _pool_from(engine):
     return w, d1, d2, t, p1, p2, ci, cj
 
 
+def _assert_pool_equal(got, ref_arrays):
+    """Weights, times, positions and cells bit for bit; directions (cos/sin of
+    the device libm vs glibc) to 2 ulp."""
+    names = ["w", "d1", "d2", "t", "p1", "p2", "ci", "cj"]
+    for nm, a, b in zip(names, got, ref_arrays):
+        assert a.shape == b.shape, nm
+        if nm in ("d1", "d2"):
+            np.testing.assert_allclose(a, b, rtol=0, atol=4.5e-16, err_msg=nm)
+        else:
+            assert np.array_equal(a, b), (nm, np.flatnonzero(a != b)[:5])
+
+
 def _announce(msg):
     print(msg, flush=True)
     sys.stdout.flush()
@@ -66,9 +78,7 @@ def test_sourcing_bitwise(L, oracle, geometry):
     E_ic = A * ((tr * tr) * (tr * tr)) * oracle.volumes_2d(e1, e2, geometry)
     cnt, _ = oracle.counts(E_ic, 5000, True)
     ref = oracle.sample_volume_2d(e1, e2, E_ic, cnt, 1, gcode, None, 0.0, 77, 0, oracle.TAG_IC)
-    got = _pool_from(eng)
-    for a, b in zip(got, ref.arrays()):
-        np.testing.assert_allclose(a, b, rtol=2e-16, atol=1e-300)
+    _assert_pool_equal(_pool_from(eng), ref.arrays())
     # boundary + emission appended (step 3)
     Eb = oracle.boundary_energy(gcode, 2, 0.5, dt, e1, e2)
     eng.sample_boundary(2, 3000, Eb, dt)
@@ -83,14 +93,7 @@ def test_sourcing_bitwise(L, oracle, geometry):
     assert count == totE
     em = oracle.sample_volume_2d(e1, e2, Ec, cE, 0, 0 if geometry == "xy" else 2, T, dt, 77, 3, oracle.TAG_EMISSION)
     ref = ref.concat(b1).concat(b2).concat(em)
-    got = _pool_from(eng)
-    assert len(got[0]) == len(ref)
-    names = ["w", "d1", "d2", "t", "p1", "p2", "ci", "cj"]
-    for nm, a, b in zip(names, got, ref.arrays()):
-        if nm in ("w", "t", "ci", "cj"):
-            assert np.array_equal(a, b), nm
-        else:
-            np.testing.assert_allclose(a, b, rtol=4e-16, atol=4e-16, err_msg=nm)
+    _assert_pool_equal(_pool_from(eng), ref.arrays())
 
 
 def _transport_case(L, oracle, geometry, precision, seed=5, npart=20000):
@@ -159,6 +162,7 @@ def test_transport_fp32_conserves_and_agrees(L, oracle, geometry):
     assert np.sum(got[0]) + E_dep_d + r["bl_d"] == pytest.approx(w0, rel=1e-12)
     # same streams, FP32 geometry: totals agree with the FP64 oracle statistically
     E_dep = np.sum(r["dep"] * r["vol"])
+    _announce(f"  stats fp32 {list(r['st_d'])}\n  stats fp64 {list(r['st'])}")
     assert E_dep_d == pytest.approx(E_dep, rel=2e-2)
     assert r["st_d"][0] == pytest.approx(r["st"][0], rel=5e-2)
 
@@ -198,7 +202,7 @@ def test_comb_parity(L, oracle):
     eng.put_census(*ref.arrays())
     eng.set_step(1)
     n, m = r["e1"].size - 1, r["e2"].size - 1
-    ecen_d, disc_d, tr_d = eng.comb(6000)
+    ecen_d, disc_d, et_d, tr_d = eng.comb(6000, want_ecen=True, want_disc=True)
     newp, disc = oracle.comb_2d(ref, 6000, n, m, eng.seed, 1)
     got = _pool_from(eng)
     assert len(got[0]) == len(newp)
@@ -209,6 +213,11 @@ def test_comb_parity(L, oracle):
             assert np.array_equal(a, b)
     np.testing.assert_allclose(disc_d, disc, rtol=1e-9, atol=1e-14 * np.abs(ecen_d).max())
     assert tr_d == pytest.approx(np.sum(newp.w), rel=1e-13)
+    assert et_d == pytest.approx(np.sum(ecen_d), rel=1e-13)
+    # a second comb without per-cell outputs gives the same census
+    eng.put_census(*ref.arrays())
+    _, _, et2, tr2 = eng.comb(6000)
+    assert tr2 == tr_d and len(eng.get_census()[0]) == len(newp)
 
 
 def _run_both(L, oracle, precision, nsteps, N, seed):
@@ -247,7 +256,7 @@ def test_full_step_fp64_matches_oracle(L, oracle):
     a, b = infos[0], oinfos[0]
     assert a["boundary_emission"] == pytest.approx(b["boundary_emission"], rel=1e-14)
     np.testing.assert_allclose(a["emitted_energies"], b["emitted_energies"], rtol=1e-15)
-    np.testing.assert_allclose(a["deposited"], b["deposited"], rtol=1e-6, atol=1e-9 * np.abs(b["deposited"]).max())
+    np.testing.assert_allclose(a["deposited"], b["dep"], rtol=1e-6, atol=1e-9 * np.abs(b["dep"]).max())
     for a, b in zip(infos, oinfos):
         assert a["total_energy"] == pytest.approx(b["total_energy"], rel=1e-4)
         assert abs(a["energy_residual"]) <= 1e-12 * a["total_energy"]
@@ -285,11 +294,13 @@ def test_fp32_vs_fp64_ensemble(L, oracle):
     e1, e2 = _mesh(12, 12)
     n = 12
     g = np.random.default_rng(1).standard_normal((n, n))
-    sig0 = np.exp(2.3 + 1.0 * (g - g.mean()) / g.std())
-    sf = lambda T: sig0 * np.maximum(T, 1e-3) ** -3
+    # moderately thin, temperature-independent opacity so that radiation
+    # reaches most cells and their tallies are stochastic
+    sig0 = 5.0 * np.exp(0.5 * (g - g.mean()) / g.std())
+    sf = lambda T: sig0 + 0.0 * T
     inv = lambda u: u / 0.3
     cvf = lambda T: 0.3 + 0 * T
-    T0 = np.full((n, n), 1e-3)
+    T0 = np.full((n, n), 0.05)
     gpu, cpu = [], []
     for seed in range(1, 17):
         st = IMC2D.init_simulation(4000, T0, T0, e1, e2, lambda T: 0.3 * T, inv, precision="fp32", seed=seed)
@@ -305,7 +316,15 @@ def test_fp32_vs_fp64_ensemble(L, oracle):
     mg, mc = gpu.mean(0), cpu.mean(0)
     se = np.sqrt(gpu.var(0, ddof=1) / len(gpu) + cpu.var(0, ddof=1) / len(cpu))
     mask = mc > 1e-3 * mc.max()
-    z = np.abs(mg - mc)[mask] / np.maximum(se[mask], 1e-300)
-    _announce(f"  cells {mask.sum()}, max z {z.max():.2f}, frac z>3 {np.mean(z > 3):.3f}")
+    # cells whose deposit is only their own (deterministic) emission have no
+    # variance: there FP32 must agree with FP64 to rounding
+    stoch = mask & (se > 1e-9 * mc)
+    det = mask & ~stoch
+    z = np.abs(mg - mc)[stoch] / se[stoch]
+    _announce(f"  stochastic cells {stoch.sum()}, max z {z.max():.2f}, frac z>3 {np.mean(z > 3):.3f}; "
+              f"deterministic cells {det.sum()}")
+    assert stoch.sum() > 20
     assert np.mean(z > 3) < 0.02
     assert z.max() < 5.5
+    if det.any():
+        np.testing.assert_allclose(mg[det], mc[det], rtol=1e-5)
