@@ -825,7 +825,7 @@ struct Imc2dCtx {
   DevBuf<int64_t> counts, offs, scan_scr;
   DevBuf<int32_t> copies;
   DevBuf<unsigned long long> cellcount;
-  DevBuf<double> ecen_part;
+  DevBuf<double> ecen_part, tree_scr;
   DevBuf<unsigned int> count_part;
   double* scal = nullptr;  // [0] Etot  [1] weight sum  [2] floor  [3] bl  [4] comb E  [5] rad total
   unsigned long long* stats = nullptr;
@@ -919,6 +919,7 @@ int imc2d_create(imc2d_ctx** out, int geometry, int nx, int ny, const double* ed
   if ((rc = c->si_raw.reserve(nc * kMaxTallyCopies))) return rc;
   if ((rc = c->ecen_part.reserve(nc * kMaxTallyCopies))) return rc;
   if ((rc = c->count_part.reserve(nc * kMaxTallyCopies))) return rc;
+  if ((rc = c->tree_scr.reserve(1024))) return rc;
   if ((rc = c->sas32.reserve(nc))) return rc;
   if ((rc = c->sas64.reserve(nc))) return rc;
   if ((rc = c->e1p32.reserve(nx))) return rc;
@@ -981,6 +982,7 @@ void imc2d_destroy(imc2d_ctx* c) {
   c->copies.release();
   c->cellcount.release();
   c->ecen_part.release();
+  c->tree_scr.release();
   c->count_part.release();
   c->A.release();
   c->B.release();
@@ -1234,8 +1236,7 @@ int imc2d_transport(imc2d_ctx* c, double dt, const int32_t* reflect, int64_t max
   IMC_HIP(hipMemsetAsync(c->next, 0, sizeof(unsigned long long), s));
   int rc;
   if (weight_floor < 0.0) {
-    if ((rc = ensure_sum_scratch(c, n))) return rc;
-    if ((rc = canonical_sum(c->A.w, n, c->scal + 1, c->sum_scr.p, s))) return rc;
+    if ((rc = tree_sum(c->A.w, n, c->scal + 1, c->tree_scr.p, s))) return rc;
     k_weight_floor<<<1, 1, 0, s>>>(c->scal + 1, n, c->scal + 2);
     IMC_HIP(hipGetLastError());
   } else {
@@ -1392,8 +1393,7 @@ int imc2d_comb(imc2d_ctx* c, int64_t Nmax, double* ecen_out, double* disc_out, d
   }
   std::swap(c->A, c->B);
   c->n = total;
-  if ((rc = ensure_sum_scratch(c, total))) return rc;
-  if ((rc = canonical_sum(c->A.w, total, c->scal + 5, c->sum_scr.p, s))) return rc;
+  if ((rc = tree_sum(c->A.w, total, c->scal + 5, c->tree_scr.p, s))) return rc;
   if (ecen_out) IMC_HIP(hipMemcpyAsync(ecen_out, c->ecen.p, bytes, hipMemcpyDeviceToHost, s));
   double tr = 0.0;
   IMC_HIP(hipMemcpyAsync(&tr, c->scal + 5, sizeof(double), hipMemcpyDeviceToHost, s));
@@ -1419,8 +1419,7 @@ int imc2d_census_energy(imc2d_ctx* c, double* out) {
   IMC_REQUIRE(c && out, kErrArg, "imc2d_census_energy: bad arguments");
   IMC_HIP(hipSetDevice(c->device));
   int rc;
-  if ((rc = ensure_sum_scratch(c, c->n))) return rc;
-  if ((rc = canonical_sum(c->A.w, c->n, c->scal + 5, c->sum_scr.p, c->stream))) return rc;
+  if ((rc = tree_sum(c->A.w, c->n, c->scal + 5, c->tree_scr.p, c->stream))) return rc;
   IMC_HIP(hipMemcpyAsync(out, c->scal + 5, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   IMC_HIP(hipStreamSynchronize(c->stream));
   return kOk;

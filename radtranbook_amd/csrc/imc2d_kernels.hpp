@@ -182,11 +182,25 @@ __device__ __forceinline__ void move_rz(Ctr ctr, Key key, uint32_t ev, double w,
 // ---------------------------------------------------------------------------
 // FP32 geometry, FP64 weights
 // ---------------------------------------------------------------------------
+// 1 - e^{-x} for 0 <= x < 0.5 by its Taylor series to x^8/8! (remainder
+// below 1e-8 relative): no cancellation for small x.
+__device__ __forceinline__ float one_minus_exp_small(float x) {
+  float p = 1.0f / 40320.0f;
+  p = 1.0f / 5040.0f - x * p;
+  p = 1.0f / 720.0f - x * p;
+  p = 1.0f / 120.0f - x * p;
+  p = 1.0f / 24.0f - x * p;
+  p = 1.0f / 6.0f - x * p;
+  p = 0.5f - x * p;
+  p = 1.0f - x * p;
+  return x * p;
+}
+
 __device__ __forceinline__ void absorb_f32(MoveOut<float>& o, double w, float sa, float s) {
   float x = sa * s;
   double dw, wn;
   if (x < 0.5f) {
-    float om = -expm1f(-x);
+    float om = one_minus_exp_small(x);
     dw = w * (double)om;
     wn = w - dw;
   } else {
@@ -206,7 +220,7 @@ __device__ __forceinline__ void absorb_f32(MoveOut<float>& o, double w, float sa
 __device__ __forceinline__ float scatter_distance_f32(const QuadF& u, float ss, float cl) {
   float s = 1e30f;
   if (ss > 1e-12f) {
-    s = -__logf(1.0f - u.u0) / ss;
+    s = __fdividef(-__logf(1.0f - u.u0), ss);
     if (ss * cl > 10000.0f) s = -__logf(1.0f - u.u1) * (cl * 1.0e-4f);
   }
   return s;
@@ -215,15 +229,9 @@ __device__ __forceinline__ float scatter_distance_f32(const QuadF& u, float ss, 
 __device__ __forceinline__ void move_xy(Ctr ctr, Key key, uint32_t ev, double w, float ux, float uy, float x, float y,
                                         float xl, float xr, float yl, float yr, float sa, float ss, float dcen,
                                         MoveOut<float>& o) {
-  float sx = 1e30f, sy = 1e30f;
-  if (ux > 0.0f)
-    sx = fmaxf((xr - x) / ux, 0.0f);
-  else if (ux < 0.0f)
-    sx = fmaxf((xl - x) / ux, 0.0f);
-  if (uy > 0.0f)
-    sy = fmaxf((yr - y) / uy, 0.0f);
-  else if (uy < 0.0f)
-    sy = fmaxf((yl - y) / uy, 0.0f);
+  // face distances by reciprocal-multiply (about 2 ulp; clamped at 0)
+  float sx = (ux != 0.0f) ? fmaxf(__fdividef((ux > 0.0f ? xr : xl) - x, ux), 0.0f) : 1e30f;
+  float sy = (uy != 0.0f) ? fmaxf(__fdividef((uy > 0.0f ? yr : yl) - y, uy), 0.0f) : 1e30f;
   float cl = fminf(xr - xl, yr - yl);
   QuadF u = (ss > 1e-12f) ? block4_f(ctr, key, ev) : QuadF{0.0f, 0.0f, 0.0f, 0.0f};
   float s_scat = scatter_distance_f32(u, ss, cl);
@@ -250,7 +258,7 @@ __device__ __forceinline__ void move_xy(Ctr ctr, Key key, uint32_t ev, double w,
     if (s_scat <= dcen) {
       o.evt = kEvtScatter;
       float uz = 2.0f * u.u2 - 1.0f;
-      float rxy = sqrtf(fmaxf(0.0f, 1.0f - uz * uz));
+      float rxy = __builtin_amdgcn_sqrtf(fmaxf(0.0f, 1.0f - uz * uz));
       o.d1 = rxy * __builtin_amdgcn_cosf(u.u3);
       o.d2 = rxy * __builtin_amdgcn_sinf(u.u3);
     } else {
@@ -266,26 +274,26 @@ __device__ __forceinline__ void move_xy(Ctr ctr, Key key, uint32_t ev, double w,
 __device__ __forceinline__ float dist_outer_f32(float r, float mu, float R, float den) {
   float ome = (1.0f - mu) * (1.0f + mu);
   float disc = fmaxf(R * R - r * r * ome, 0.0f);
-  float root = sqrtf(disc);
-  float l = (mu > 0.0f) ? fmaxf((R - r) * (R + r), 0.0f) / (root + r * mu) : root - r * mu;
+  float root = __builtin_amdgcn_sqrtf(disc);
+  float l = (mu > 0.0f) ? __fdividef(fmaxf((R - r) * (R + r), 0.0f), root + r * mu) : root - r * mu;
   if (!(l >= 0.0f)) l = 0.0f;  // r == R, mu == 1 degenerate: 0/0
-  return l / den;
+  return __fdividef(l, den);
 }
 __device__ __forceinline__ float dist_inner_f32(float r, float mu, float R, float den) {
   if (!(mu < 0.0f)) return 1e30f;
   float ome = (1.0f - mu) * (1.0f + mu);
   float disc = R * R - r * r * ome;
   if (disc <= 0.0f) return 1e30f;
-  float root = sqrtf(disc);
-  float l = fmaxf((r - R) * (r + R), 0.0f) / (root - r * mu);
-  return l / den;
+  float root = __builtin_amdgcn_sqrtf(disc);
+  float l = __fdividef(fmaxf((r - R) * (r + R), 0.0f), root - r * mu);
+  return __fdividef(l, den);
 }
 
 __device__ __forceinline__ void move_rz(Ctr ctr, Key key, uint32_t ev, double w, float mu, float eta, float r,
                                         float z, float rin, float rout, float zlo, float zhi, float sa, float ss,
                                         float dcen, MoveOut<float>& o) {
   float ome2 = fmaxf((1.0f - eta) * (1.0f + eta), 0.0f);
-  float den = sqrtf(ome2);
+  float den = __builtin_amdgcn_sqrtf(ome2);
   float s_rin = 1e30f, s_rout = 1e30f;
   if (den > 0.0f) {
     if (rin > 0.0f) s_rin = dist_inner_f32(r, mu, rin, den);
@@ -293,9 +301,9 @@ __device__ __forceinline__ void move_rz(Ctr ctr, Key key, uint32_t ev, double w,
   }
   float s_zlo = 1e30f, s_zhi = 1e30f;
   if (eta > 0.0f)
-    s_zhi = fmaxf((zhi - z) / eta, 0.0f);
+    s_zhi = fmaxf(__fdividef(zhi - z, eta), 0.0f);
   else if (eta < 0.0f)
-    s_zlo = fmaxf((zlo - z) / eta, 0.0f);
+    s_zlo = fmaxf(__fdividef(zlo - z, eta), 0.0f);
   float cl = fminf(rout - rin, zhi - zlo);
   QuadF u = (ss > 1e-12f) ? block4_f(ctr, key, ev) : QuadF{0.0f, 0.0f, 0.0f, 0.0f};
   float s_scat = scatter_distance_f32(u, ss, cl);
@@ -312,10 +320,10 @@ __device__ __forceinline__ void move_rz(Ctr ctr, Key key, uint32_t ev, double w,
   // radial cosine is mu' = (r mu + l) / r'
   float l = se * den;
   float a = r + mu * l;
-  float rn = sqrtf(a * a + l * l * fmaxf((1.0f - mu) * (1.0f + mu), 0.0f));
+  float rn = __builtin_amdgcn_sqrtf(a * a + l * l * fmaxf((1.0f - mu) * (1.0f + mu), 0.0f));
   float zn = z + se * eta;
   float mun = mu;
-  if (rn > 0.0f && l > 0.0f) mun = fminf(1.0f, fmaxf(-1.0f, (r * mu + l) / rn));
+  if (rn > 0.0f && l > 0.0f) mun = fminf(1.0f, fmaxf(-1.0f, __fdividef(r * mu + l, rn)));
   float etan = eta;
   if (bnd) {
     o.evt = kEvtBoundary;

@@ -68,6 +68,45 @@ int canonical_sum(const double* x, int64_t n, double* out, double* scratch, hipS
 }
 
 // ---------------------------------------------------------------------------
+// tree sum
+// ---------------------------------------------------------------------------
+constexpr int kTreeBlocks = 1024;
+
+__device__ __forceinline__ double block_sum_256(double v) {
+  __shared__ double ws[4];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double r = (ws[0] + ws[1]) + (ws[2] + ws[3]);
+  return r;
+}
+
+__global__ __launch_bounds__(256) void k_tree_partial(const double* __restrict__ x, int64_t n,
+                                                      double* __restrict__ part) {
+  double v = 0.0;
+  const int64_t stride = (int64_t)kTreeBlocks * 256;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) v += x[i];
+  double r = block_sum_256(v);
+  if (threadIdx.x == 0) part[blockIdx.x] = r;
+}
+
+__global__ __launch_bounds__(256) void k_tree_final(const double* __restrict__ part, double* __restrict__ out) {
+  double v = 0.0;
+  for (int i = threadIdx.x; i < kTreeBlocks; i += 256) v += part[i];
+  double r = block_sum_256(v);
+  if (threadIdx.x == 0) *out = r;
+}
+
+int tree_sum(const double* x, int64_t n, double* out, double* scratch, hipStream_t s) {
+  k_tree_partial<<<kTreeBlocks, 256, 0, s>>>(x, n, scratch);
+  IMC_HIP(hipGetLastError());
+  k_tree_final<<<1, 256, 0, s>>>(scratch, out);
+  IMC_HIP(hipGetLastError());
+  return kOk;
+}
+
+// ---------------------------------------------------------------------------
 // exclusive scan (int64), 256 threads x 8 items per block
 // ---------------------------------------------------------------------------
 constexpr int kScanThreads = 256;

@@ -77,6 +77,13 @@ struct DevBuf {
 int canonical_sum(const double* x, int64_t n, double* out, double* scratch, hipStream_t s);
 int64_t canonical_sum_scratch(int64_t n);
 
+// Fast deterministic sum for long particle arrays: a fixed grid of 1024
+// workgroups accumulates grid-strided partial sums and reduces them in a
+// fixed tree; the result depends only on n, not on scheduling.  Used where
+// the oracle does not need the canonical order (weight floor, census totals).
+// `scratch` must hold 1024 doubles.
+int tree_sum(const double* x, int64_t n, double* out, double* scratch, hipStream_t s);
+
 // Exclusive scan of int64 counts; writes n+1 offsets (last = total).
 // `scratch` must hold scan_scratch(n) int64.
 int exclusive_scan_i64(const int64_t* in, int64_t n, int64_t* out, int64_t* scratch, hipStream_t s);
