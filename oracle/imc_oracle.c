@@ -645,11 +645,12 @@ void orc_transport_2d(int64_t n, double* w, double* d1, double* d2, const double
 /* ------------------------------------------------------------------------- */
 /* a18: per-cell stochastic comb, IMC/IMC2D.py:1448-1511, IMC/IMC1D.py:770-828 */
 /* cell[k] is the flat cell id (any bijection; dead particles have w <= 0).   */
+/* stream_base offsets the Philox index (problem << 32 in batched runs).      */
 /* Fills ecen, ew (per cell) and copies (per particle); returns total copies. */
 /* ------------------------------------------------------------------------- */
 int64_t orc_comb_counts(int64_t n, const double* w, const int64_t* cell, int64_t ncell, int64_t Nmax,
-                        uint64_t seed, uint32_t step, double* ecen, double* ew, int64_t* copies,
-                        int32_t* passthrough) {
+                        uint64_t seed, uint32_t step, uint64_t stream_base, double* ecen, double* ew,
+                        int64_t* copies, int32_t* passthrough) {
   *passthrough = 0;
   for (int64_t c = 0; c < ncell; ++c) ecen[c] = 0.0;
   for (int64_t k = 0; k < n; ++k)
@@ -681,7 +682,7 @@ int64_t orc_comb_counts(int64_t n, const double* w, const int64_t* cell, int64_t
       double et = ew[cell[k]];
       if (et > 0.0) {
         orc_stream s;
-        orc_stream_init(&s, seed, ORC_TAG_COMB, (uint64_t)k, step);
+        orc_stream_init(&s, seed, ORC_TAG_COMB, stream_base + (uint64_t)k, step);
         m = (int64_t)(w[k] / et + orc_u01(&s, 0));
       }
     }

@@ -75,7 +75,7 @@ def lib():
                                        C.c_int, _dp, _dp, _dp, C.c_double, _i32p, C.c_int64, C.c_int32, C.c_double,
                                        C.c_uint64, C.c_uint32, _dp, _dp, _dp, _i64p, C.c_int32]
         L.orc_comb_counts.argtypes = [C.c_int64, _dp, _i64p, C.c_int64, C.c_int64, C.c_uint64, C.c_uint32,
-                                      _dp, _dp, _i64p, _i32p]
+                                      C.c_uint64, _dp, _dp, _i64p, _i32p]
         L.orc_comb_counts.restype = C.c_int64
         _lib = L
     return _lib
@@ -209,11 +209,11 @@ def transport_2d(pool: Pool2D, e1, e2, sigma_a, sigma_s, vol, dt, reflect, max_e
     return dep.reshape(nx, ny), si.reshape(nx, ny), float(bl[0]), st
 
 
-def comb_counts(w, cell, ncell, Nmax, seed, step):
+def comb_counts(w, cell, ncell, Nmax, seed, step, base=0):
     w = _f64(w); cell = np.ascontiguousarray(cell, dtype=np.int64)
     ecen = np.zeros(ncell); ew = np.zeros(ncell); cp = np.zeros(w.size, dtype=np.int64); pt = np.zeros(1, np.int32)
-    tot = lib().orc_comb_counts(w.size, _p(w), _p(cell, _i64p), ncell, int(Nmax), seed, step, _p(ecen), _p(ew),
-                                _p(cp, _i64p), _p(pt, _i32p))
+    tot = lib().orc_comb_counts(w.size, _p(w), _p(cell, _i64p), ncell, int(Nmax), seed, step, base, _p(ecen),
+                                _p(ew), _p(cp, _i64p), _p(pt, _i32p))
     return ecen, ew, cp, int(tot), bool(pt[0])
 
 
@@ -395,4 +395,223 @@ class Oracle2D:
             "dep": dep, "si": si, "emitted_energies": E_c, "stats": stats, "n_transported": n_tr,
             "weight_floor": weight_floor,
         }
+        return state, info
+
+
+# ----------------------------------------------------------------------------
+# gray 1-D (slab / sphere), IMC/IMC1D.py:859-1124
+# ----------------------------------------------------------------------------
+def _lib1d():
+    L = lib()
+    if not hasattr(L, "_o1_ready"):
+        L.o1_sample_volume.argtypes = [C.c_int, C.c_int, _dp, _dp, _i64p, C.c_int32, _dp, C.c_double, C.c_uint64,
+                                       C.c_uint32, C.c_int32, C.c_uint64, _dp, _dp, _dp, _dp, _i32p]
+        L.o1_boundary_energy.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double]
+        L.o1_boundary_energy.restype = C.c_double
+        L.o1_sample_boundary.argtypes = [C.c_int, C.c_int, C.c_int64, C.c_double, C.c_double, C.c_int, _dp,
+                                         C.c_uint64, C.c_uint32, C.c_uint64, _dp, _dp, _dp, _dp, _i32p]
+        L.o1_transport.argtypes = [C.c_int, C.c_int64, _dp, _dp, _dp, _dp, _i32p, C.c_int, _dp, _dp, _dp,
+                                   C.c_double, _i32p, C.c_double, C.c_uint64, C.c_uint32, C.c_uint64, _dp, _dp,
+                                   _dp, _dp, _i64p, C.c_int32]
+        L.o1_counts_ic.argtypes = [C.c_int, _dp, C.c_int64, _i64p]
+        L.o1_counts_ic.restype = C.c_int64
+        L._o1_ready = True
+    return L
+
+
+@dataclass
+class Pool1D:
+    w: np.ndarray
+    mu: np.ndarray
+    t: np.ndarray
+    x: np.ndarray
+    cell: np.ndarray
+
+    @staticmethod
+    def empty(n=0):
+        z = lambda: np.zeros(n, dtype=np.float64)
+        return Pool1D(z(), z(), z(), z(), np.zeros(n, np.int32))
+
+    def __len__(self):
+        return self.w.size
+
+    def arrays(self):
+        return (self.w, self.mu, self.t, self.x, self.cell)
+
+    def concat(self, other):
+        return Pool1D(*[np.concatenate((a, b)) for a, b in zip(self.arrays(), other.arrays())])
+
+    def take(self, idx):
+        return Pool1D(*[a[idx] for a in self.arrays()])
+
+
+def volumes_1d(edges, geometry):
+    e = _f64(edges)
+    if geometry == "slab":
+        return e[1:] - e[:-1]
+    return (4.0 / 3.0) * np.pi * (e[1:] ** 3 - e[:-1] ** 3)
+
+
+def sample_volume_1d(sph, edges, QV, cnt, kind, T, dt, seed, step, tag, base):
+    e = _f64(edges); I = e.size - 1
+    QV = _f64(QV); cnt = np.ascontiguousarray(cnt, dtype=np.int64)
+    Tt = _f64(T) if T is not None else np.ones(I)
+    p = Pool1D.empty(int(cnt.sum()))
+    _lib1d().o1_sample_volume(1 if sph else 0, I, _p(e), _p(QV), _p(cnt, _i64p), kind, _p(Tt), dt, seed, step, tag,
+                              base, _p(p.w), _p(p.mu), _p(p.t), _p(p.x), _p(p.cell, _i32p))
+    return p
+
+
+def sample_boundary_1d(sph, side, n, E, dt, edges, seed, step, base):
+    e = _f64(edges)
+    p = Pool1D.empty(n)
+    _lib1d().o1_sample_boundary(1 if sph else 0, side, n, E, dt, e.size - 1, _p(e), seed, step, base, _p(p.w),
+                                _p(p.mu), _p(p.t), _p(p.x), _p(p.cell, _i32p))
+    return p
+
+
+def transport_1d(sph, pool, edges, sigma_a, sigma_s, dt, reflect, weight_floor, seed, step, base, nthreads=0):
+    """Transport in place (IMC/IMC1D.py:133-200, 439-532).  Returns
+    (dep, si, leak, floor_leak, stats[8]); stats = events, boundary, scatter,
+    census, floor kills, reflections, assertion failures."""
+    e = _f64(edges); I = e.size - 1
+    sa = _f64(sigma_a); ss = _f64(sigma_s)
+    refl = np.array([1 if r else 0 for r in reflect], dtype=np.int32)
+    dep = np.zeros(I); si = np.zeros(I); lk = np.zeros(1); fl = np.zeros(1); st = np.zeros(8, np.int64)
+    _lib1d().o1_transport(1 if sph else 0, len(pool), _p(pool.w), _p(pool.mu), _p(pool.t), _p(pool.x),
+                          _p(pool.cell, _i32p), I, _p(e), _p(sa), _p(ss), dt, _p(refl, _i32p), weight_floor, seed,
+                          step, base, _p(dep), _p(si), _p(lk), _p(fl), _p(st, _i64p), nthreads)
+    return dep, si, float(lk[0]), float(fl[0]), st
+
+
+def comb_1d(pool, Nmax, I, seed, step, base):
+    """IMC/IMC1D.py:770-828 with the comb stream offset by the problem base."""
+    w = pool.w
+    cell = pool.cell.astype(np.int64)
+    ecen, ew, cp, tot, passthrough = comb_counts(w, cell, I, Nmax, seed, step, base)
+    idx = np.repeat(np.arange(len(pool)), cp)
+    newp = pool.take(idx)
+    if not passthrough:
+        newp.w = ew[cell[idx]]
+    ecen_after = np.bincount(cell[idx], weights=newp.w, minlength=I)
+    return newp, ecen - ecen_after
+
+
+@dataclass
+class OracleState1D:
+    pool: Pool1D
+    internal_energy: np.ndarray
+    temperature: np.ndarray
+    radiation_temperature: np.ndarray
+    time: float
+    previous_total_energy: float
+    count: int = 0
+
+
+class Oracle1D:
+    """FP64 CPU restatement of the gray 1-D IMC step (IMC/IMC1D.py).
+
+    ``problem`` selects the Philox index base (problem << 32) so that problem
+    p of a batched device run is reproduced exactly."""
+
+    def __init__(self, mesh, geometry="slab", seed=1, problem=0, nthreads=0):
+        mesh = np.asarray(mesh, dtype=np.float64)
+        self.edges = np.concatenate((mesh[:, 0], mesh[-1:, 1]))
+        self.I = mesh.shape[0]
+        self.geometry = geometry
+        self.sph = geometry != "slab"
+        self.vol = volumes_1d(self.edges, geometry)
+        self.seed = int(seed)
+        self.base = int(problem) << 32
+        self.nthreads = nthreads
+
+    def init_simulation(self, Ntarget, Tinit, Tr_init, eos, inv_eos, Ntarget_ic=None, T_emit_floor=0.0):
+        u = eos(Tinit)
+        assert np.allclose(inv_eos(u), Tinit), "Inverse EOS failed"
+        N_ic = Ntarget_ic if Ntarget_ic is not None else Ntarget
+        tr = np.asarray(Tr_init, dtype=np.float64)
+        E = A_RAD * ((tr * tr) * (tr * tr)) * self.vol
+        if T_emit_floor > 0.0:
+            E = np.where(tr < T_emit_floor, 0.0, E)
+        cnt = np.zeros(self.I, np.int64)
+        _lib1d().o1_counts_ic(self.I, _p(_f64(E)), int(N_ic), _p(cnt, _i64p))
+        pool = sample_volume_1d(self.sph, self.edges, E, cnt, 1, None, 0.0, self.seed, 0, TAG_IC, self.base)
+        Tr = (np.bincount(pool.cell, weights=pool.w, minlength=self.I) / self.vol / A_RAD) ** 0.25
+        tot = float(np.sum(u * self.vol)) + float(np.sum(pool.w))
+        return OracleState1D(pool, u, np.array(Tinit, dtype=np.float64, copy=True), Tr, 0.0, tot, 0)
+
+    def step(self, state, Ntarget, Nboundary, Nsource, NMax, T_boundary, dt, sigma_a_func, inv_eos, cv, source,
+             reflect=(False, False), theta=1.0, use_scalar_intensity_Tr=True, conserve_comb_energy=False,
+             T_emit_floor=0.0):
+        I = self.I
+        sid = state.count
+        T = state.temperature
+        sig = np.asarray(sigma_a_func(T), dtype=np.float64) * np.ones(I)
+        cvv = np.asarray(cv(T), dtype=np.float64) * np.ones(I)
+        sa, ss, _ = fleck(T, sig, cvv, theta, dt, clip=False)
+        pool = state.pool
+        be = 0.0
+        for side in (0, 1):
+            Tb = T_boundary[side]
+            Tb = Tb(state.time) if callable(Tb) else Tb
+            if Tb > 0 and Nboundary > 0:
+                R = self.edges[0] if side == 0 else self.edges[-1]
+                E = _lib1d().o1_boundary_energy(1 if self.sph else 0, Tb, dt, R)
+                b = sample_boundary_1d(self.sph, side, int(Nboundary), E, dt, self.edges, self.seed, sid,
+                                       self.base)
+                pool = pool.concat(b)
+                be += float(np.sum(b.w))
+        src_e = 0.0
+        src = np.asarray(source, dtype=np.float64)
+        if np.max(src) > 0 and Nsource > 0:
+            QV = src * dt * self.vol
+            cnt, tot = counts(QV, Nsource, False)
+            s = sample_volume_1d(self.sph, self.edges, QV, cnt, 2, T, dt, self.seed, sid, TAG_SOURCE, self.base)
+            pool = pool.concat(s)
+            src_e = float(np.sum(s.w))
+        t = np.maximum(T, 0.0)
+        Ei = A_RAD * C_LIGHT * ((t * t) * (t * t)) * sa * dt * self.vol
+        if T_emit_floor > 0.0:
+            Ei = np.where(T < T_emit_floor, 0.0, Ei)
+        cnt, tot = counts(Ei, Ntarget, False)
+        if tot > 0:
+            e = sample_volume_1d(self.sph, self.edges, Ei, cnt, 0, T, dt, self.seed, sid, TAG_EMISSION, self.base)
+            pool = pool.concat(e)
+        n_tr = len(pool)
+        floor = 1e-10 * canonical_sum(pool.w) / max(n_tr, 1)
+        dep, si, leak, fleak, stats = transport_1d(self.sph, pool, self.edges, sa, ss, dt, reflect, floor, self.seed,
+                                                   sid, self.base, self.nthreads)
+        if stats[6] > 0:
+            raise AssertionError("Non-positive distance in move_particle")
+        u = state.internal_energy + dep - Ei / self.vol
+        Tn = inv_eos(u)
+        if use_scalar_intensity_Tr:
+            Tr = (si / A_RAD / C_LIGHT) ** 0.25
+        else:
+            valid = (pool.cell >= 0) & (pool.cell < I) & (pool.w > 0)
+            Tr = (np.bincount(pool.cell[valid], weights=pool.w[valid], minlength=I) / self.vol / A_RAD) ** 0.25
+        alive = (pool.cell >= 0) & (pool.cell < I)
+        pool = pool.take(np.flatnonzero(alive))
+        newp, disc = comb_1d(pool, NMax, I, self.seed, sid, self.base)
+        if conserve_comb_energy:
+            u = u + disc / self.vol
+            Tn = inv_eos(u)
+        newp.t = np.zeros(len(newp))
+        ti = float(np.sum(u * self.vol))
+        tr = float(np.sum(newp.w))
+        total = ti + tr
+        loss = total - state.previous_total_energy - be + leak - src_e
+        state.pool = newp
+        state.internal_energy = u
+        state.temperature = Tn
+        state.radiation_temperature = Tr
+        state.time += dt
+        state.previous_total_energy = total
+        state.count += 1
+        info = {"time": state.time, "temperature": Tn, "radiation_temperature": Tr, "N_particles": len(newp),
+                "total_energy": total, "total_internal_energy": ti, "total_radiation_energy": tr,
+                "boundary_emission": be, "boundary_loss": leak, "source_emission": src_e, "energy_loss": loss,
+                "comb_discrepancy": 0.0 if conserve_comb_energy else float(np.sum(disc)),
+                "floor_kill_leak": fleak, "dep": dep, "si": si, "emitted_energies": Ei, "stats": stats,
+                "n_transported": n_tr, "weight_floor": floor}
         return state, info

@@ -83,3 +83,40 @@ def test_oracle_matches_reference_ensemble(oracle, geometry):
     assert frac3 < 0.02
     # particle counts after the comb: Nmax plus one-per-occupied-cell effects
     assert np.all(np.abs(got["N_mean"] - ref["N_mean"]) < 5 * np.sqrt(ref["N_sem"] ** 2 + got["N_sem"] ** 2) + 2)
+
+
+def _oracle_slab_ensemble(oracle):
+    p = F.SLAB
+    I = p["I"]
+    edges = np.linspace(0, p["L"], I + 1)
+    mesh = np.column_stack([edges[:-1], edges[1:]])
+    cv = p["cv"]
+    sf = lambda T: 300.0 * np.maximum(T, 1e-3) ** -3
+    T0 = np.full(I, p["T0"])
+    rows = []
+    for seed in range(SEEDS):
+        orc = oracle.Oracle1D(mesh, "slab", seed=500 + seed, nthreads=1)
+        st = orc.init_simulation(p["N"], T0, T0, lambda T: cv * T, lambda u: u / cv)
+        out = []
+        for _ in range(p["steps"]):
+            st, info = orc.step(st, p["N"], p["Nb"], 0, p["Nmax"], (p["Tb"], 0.0), p["dt"], sf, lambda u: u / cv,
+                                lambda T: cv + 0 * T, np.zeros(I), reflect=(False, False))
+            out.append((info["temperature"], info["radiation_temperature"], info["total_energy"],
+                        info["total_radiation_energy"], info["boundary_loss"], info["N_particles"]))
+        rows.append(out)
+    res = {}
+    for k, name in enumerate(["T", "Tr", "E_total", "E_rad", "boundary_loss", "N"]):
+        arr = np.array([[r[s][k] for s in range(p["steps"])] for r in rows], dtype=np.float64)
+        res[name + "_mean"] = arr.mean(0)
+        res[name + "_sem"] = arr.std(0, ddof=1) / np.sqrt(arr.shape[0])
+    return res
+
+
+def test_oracle_slab_matches_reference_ensemble(oracle):
+    ref = dict(np.load(os.path.join(HERE, "golden", "ref_imc1d_slab.npz")))
+    got = _oracle_slab_ensemble(oracle)
+    zs = {name: _compare(ref, got, name) for name in ("T", "Tr", "E_total", "E_rad", "boundary_loss")}
+    allz = np.concatenate([z.ravel() for z in zs.values()])
+    print("slab", {k: float(v.max()) for k, v in zs.items()}, "frac z>3", np.mean(allz > 3.0))
+    assert allz.max() < 5.0, {k: float(v.max()) for k, v in zs.items()}
+    assert np.mean(allz > 3.0) < 0.02
