@@ -26,6 +26,9 @@ extern "C" {
 /* geometry codes (IMC/IMC2D.py:51-52) */
 #define IMC_GEOM_XY 0
 #define IMC_GEOM_RZ 1
+/* 1-D geometries (IMC/IMC1D.py:859 "slab" / "spherical") */
+#define IMC_GEOM_SLAB 2
+#define IMC_GEOM_SPHERE 3
 
 /* arithmetic of the tracker: FP64 throughout, or FP32 geometry with FP64
  * weights, tallies and bookkeeping */
@@ -37,6 +40,7 @@ extern "C" {
 #define IMC_SRC_FIXED 1
 
 typedef struct imc2d_ctx imc2d_ctx;
+typedef struct imc1d_ctx imc1d_ctx;
 
 /* Message of the last failed call on this thread. */
 const char* imc_last_error(void);
@@ -119,6 +123,83 @@ float imc2d_last_transport_ms(imc2d_ctx* ctx);
  * ecen_total_out, pre-comb census energy; total_rad_out, census energy
  * after the comb.  Times reset to 0 (IMC/IMC2D.py:1831). */
 int imc2d_comb(imc2d_ctx* ctx, int64_t Nmax, double* ecen_out, double* disc_out, double* ecen_total_out,
+               double* total_rad_out);
+
+/* ---- gray 1-D (slab / sphere), batched ------------------------------------ */
+/* One context holds P independent problems on meshes of ncells[p] cells
+ * (edges: P concatenated arrays of ncells[p]+1 ascending edges).  Particles
+ * of problem p draw Philox index (p << 32) | index-within-problem, so every
+ * problem reproduces a single-problem run.  Per-problem arrays below have P
+ * entries, per-cell arrays sum(ncells) entries in problem order.
+ * Replaces the state set up by IMC1D.init_simulation (IMC/IMC1D.py:859-891)
+ * and SimulationState (IMC/IMC1D.py:54-68). */
+int imc1d_create(imc1d_ctx** out, int geometry, int nprob, const int32_t* ncells, const double* edges,
+                 int precision, uint64_t seed, int device);
+void imc1d_destroy(imc1d_ctx* ctx);
+int imc1d_set_step(imc1d_ctx* ctx, uint32_t step);
+/* Transport launch shape, as imc2d_set_launch. */
+int imc1d_set_launch(imc1d_ctx* ctx, int grid, int batch, int tally_copies);
+
+/* Census particles per problem (nullable output) and in total. */
+int64_t imc1d_census_counts(imc1d_ctx* ctx, int64_t* per_problem);
+/* Census arrays of SimulationState (weights, mus, times, positions,
+ * cell_indices; IMC/IMC1D.py:54-68), grouped by problem: counts[p] of
+ * problem p, in problem order. */
+int imc1d_put_census(imc1d_ctx* ctx, const int64_t* counts, const double* w, const double* mu, const double* t,
+                     const double* x, const int64_t* cell);
+int imc1d_get_census(imc1d_ctx* ctx, int64_t n, double* w, double* mu, double* t, double* x, int64_t* cell,
+                     int32_t* prob);
+
+/* Fleck factor (IMC/IMC1D.py:933-938), fails if f leaves [0,1]; theta and dt
+ * per problem; opacity outputs nullable.  Starts a step (no source blocks). */
+int imc1d_set_material(imc1d_ctx* ctx, const double* T, const double* sigma_true, const double* cv,
+                       const double* theta, const double* dt, double* sigma_a_out, double* sigma_s_out);
+
+/* Equilibrium census at t=0 replacing the pool: equilibrium_sample /
+ * equilibrium_sample_spherical (IMC/IMC1D.py:220-248, 535-575).  N,
+ * T_emit_floor per problem. */
+int imc1d_equilibrium_sample(imc1d_ctx* ctx, const int64_t* N, const double* Tr, const double* T_emit_floor);
+
+/* Face source block: create_boundary_particles / _spherical
+ * (IMC/IMC1D.py:319-329, 744-763, placed as step() 947-975).  side 0 left /
+ * inner (mu = sqrt(xi)), 1 right / outer (mu = -sqrt(xi)); n[p] particles
+ * carrying E[p] in total. */
+int imc1d_sample_boundary(imc1d_ctx* ctx, int side, const int64_t* n, const double* E);
+
+/* Volume source block: emitted_particles / _spherical (IMC/IMC1D.py:251-291,
+ * 629-707) for IMC_SRC_EMISSION (a c T^4 sigma_a dt V from the last
+ * imc1d_set_material, zero below T_emit_floor), or sample_source /
+ * _spherical (IMC/IMC1D.py:294-316, 710-741) for IMC_SRC_FIXED with QV per
+ * cell.  Counts ceil(N E_c / E_p).  E_out per cell and Etot_out per problem
+ * nullable. */
+int imc1d_sample_volume(imc1d_ctx* ctx, int kind, const int64_t* N, const double* QV_host,
+                        const double* T_emit_floor, double* E_out, double* Etot_out);
+
+/* Group census and pending source blocks into the transport pool (census,
+ * then source blocks in call order, per problem: the reference's
+ * concatenation order, IMC/IMC1D.py:947-1002) without tracking it; returns
+ * its size (or a negative status) and fills per_problem (nullable).
+ * imc1d_get_pool downloads that pool, or after imc1d_transport the
+ * transported pool. */
+int64_t imc1d_stage_pool(imc1d_ctx* ctx, int64_t* per_problem);
+int imc1d_get_pool(imc1d_ctx* ctx, int64_t n, double* w, double* mu, double* t, double* x, int64_t* cell,
+                   int32_t* prob);
+
+/* Transport census and this step's source blocks: move_particles /
+ * _move_sph_slice (IMC/IMC1D.py:133-200, 439-532).  reflect: 2 per problem;
+ * weight_floor per problem (< 0: 1e-10 sum(w)/N).  Outputs (nullable): dep,
+ * si per cell; leak and floor_leak per problem; stats[8] = events, boundary
+ * crossings, scatters, census events, floor kills, reflections,
+ * non-positive distances (the reference's AssertionError), particles
+ * transported. */
+int imc1d_transport(imc1d_ctx* ctx, const int32_t* reflect, const double* weight_floor, double* dep_out,
+                    double* si_out, double* leak_out, double* floor_leak_out, int64_t* stats_out);
+float imc1d_last_transport_ms(imc1d_ctx* ctx);
+
+/* Per-problem comb to Nmax[p]: comb (IMC/IMC1D.py:770-828); times reset to 0
+ * (IMC/IMC1D.py:1068).  Outputs (nullable): ecen, disc per cell; ecen_total,
+ * total_rad per problem. */
+int imc1d_comb(imc1d_ctx* ctx, const int64_t* Nmax, double* ecen_out, double* disc_out, double* ecen_total_out,
                double* total_rad_out);
 
 #ifdef __cplusplus

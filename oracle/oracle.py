@@ -449,7 +449,8 @@ def volumes_1d(edges, geometry):
     e = _f64(edges)
     if geometry == "slab":
         return e[1:] - e[:-1]
-    return (4.0 / 3.0) * np.pi * (e[1:] ** 3 - e[:-1] ** 3)
+    # r*r*r as o1_vol in imc_oracle_1d.c (IMC/IMC1D.py:856 writes r**3)
+    return (4.0 / 3.0) * np.pi * (e[1:] * e[1:] * e[1:] - e[:-1] * e[:-1] * e[:-1])
 
 
 def sample_volume_1d(sph, edges, QV, cnt, kind, T, dt, seed, step, tag, base):
@@ -590,8 +591,9 @@ class Oracle1D:
         else:
             valid = (pool.cell >= 0) & (pool.cell < I) & (pool.w > 0)
             Tr = (np.bincount(pool.cell[valid], weights=pool.w[valid], minlength=I) / self.vol / A_RAD) ** 0.25
-        alive = (pool.cell >= 0) & (pool.cell < I)
-        pool = pool.take(np.flatnonzero(alive))
+        # escaped particles carry zero weight after transport and get no comb
+        # copies; combing the whole transported pool keeps the comb stream
+        # index equal to the transport index
         newp, disc = comb_1d(pool, NMax, I, self.seed, sid, self.base)
         if conserve_comb_energy:
             u = u + disc / self.vol

@@ -15,6 +15,8 @@ LIB_PATH = os.path.join(HERE, "lib", "libimc_hip.so")
 
 GEOM_XY = 0
 GEOM_RZ = 1
+GEOM_SLAB = 2
+GEOM_SPHERE = 3
 FP64 = 0
 FP32_TRACK = 1
 SRC_EMISSION = 0
@@ -48,6 +50,22 @@ SIGNATURES = {
     "imc2d_transport": (C.c_int, [_vp, C.c_double, _i32p, C.c_int64, C.c_double, _dp, _dp, _dp, _i64p]),
     "imc2d_last_transport_ms": (C.c_float, [_vp]),
     "imc2d_comb": (C.c_int, [_vp, C.c_int64, _dp, _dp, _dp, _dp]),
+    "imc1d_create": (C.c_int, [C.POINTER(_vp), C.c_int, C.c_int, _i32p, _dp, C.c_int, C.c_uint64, C.c_int]),
+    "imc1d_destroy": (None, [_vp]),
+    "imc1d_set_step": (C.c_int, [_vp, C.c_uint32]),
+    "imc1d_set_launch": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
+    "imc1d_census_counts": (C.c_int64, [_vp, _i64p]),
+    "imc1d_put_census": (C.c_int, [_vp, _i64p, _dp, _dp, _dp, _dp, _i64p]),
+    "imc1d_get_census": (C.c_int, [_vp, C.c_int64, _dp, _dp, _dp, _dp, _i64p, _i32p]),
+    "imc1d_set_material": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
+    "imc1d_equilibrium_sample": (C.c_int, [_vp, _i64p, _dp, _dp]),
+    "imc1d_sample_boundary": (C.c_int, [_vp, C.c_int, _i64p, _dp]),
+    "imc1d_sample_volume": (C.c_int, [_vp, C.c_int, _i64p, _dp, _dp, _dp, _dp]),
+    "imc1d_stage_pool": (C.c_int64, [_vp, _i64p]),
+    "imc1d_get_pool": (C.c_int, [_vp, C.c_int64, _dp, _dp, _dp, _dp, _i64p, _i32p]),
+    "imc1d_transport": (C.c_int, [_vp, _i32p, _dp, _dp, _dp, _dp, _dp, _i64p]),
+    "imc1d_last_transport_ms": (C.c_float, [_vp]),
+    "imc1d_comb": (C.c_int, [_vp, _i64p, _dp, _dp, _dp, _dp]),
 }
 
 
@@ -210,3 +228,143 @@ class Engine2D:
         tr = C.c_double()
         check(self._L.imc2d_comb(self._h, int(Nmax), _p(ecen), _p(disc), C.byref(et), C.byref(tr)))
         return ecen, disc, et.value, tr.value
+
+
+def _per(a, n, dtype=np.float64):
+    return np.ascontiguousarray(np.broadcast_to(np.asarray(a, dtype=dtype), (n,)), dtype=dtype)
+
+
+class Engine1D:
+    """Device context for P independent 1-D problems (wraps ``imc1d_ctx``).
+
+    ``meshes`` is a list of edge arrays (one per problem).  Per-problem
+    arguments accept a scalar (broadcast) or P values; per-cell arguments are
+    the problems' cell arrays concatenated in problem order."""
+
+    def __init__(self, meshes, geometry="slab", precision="fp64", seed=1, device=0):
+        L = load()
+        require_gpu()
+        self.edges = [np.ascontiguousarray(e, dtype=np.float64) for e in meshes]
+        self.P = len(self.edges)
+        self.ncells = np.array([e.size - 1 for e in self.edges], dtype=np.int32)
+        self.coff = np.concatenate(([0], np.cumsum(self.ncells))).astype(np.int64)
+        self.ncell = int(self.coff[-1])
+        self.geometry = geometry
+        self.gcode = {"slab": GEOM_SLAB, "spherical": GEOM_SPHERE, "sphere": GEOM_SPHERE}[geometry]
+        self.precision = {"fp64": FP64, "fp32": FP32_TRACK}[precision]
+        self.seed = int(seed)
+        allE = np.concatenate(self.edges)
+        h = _vp()
+        check(L.imc1d_create(C.byref(h), self.gcode, self.P, _p(self.ncells, _i32p), _p(allE), self.precision,
+                             self.seed, device))
+        self._h = h
+        self._L = L
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._L.imc1d_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def split(self, a):
+        """Per-cell array -> list of per-problem arrays."""
+        return [a[self.coff[p]:self.coff[p + 1]] for p in range(self.P)]
+
+    def set_step(self, step):
+        check(self._L.imc1d_set_step(self._h, int(step) & 0xFFFFFFFF))
+
+    def set_launch(self, grid=0, batch=256, tally_copies=8):
+        check(self._L.imc1d_set_launch(self._h, int(grid), int(batch), int(tally_copies)))
+
+    def census_counts(self):
+        per = np.zeros(self.P, dtype=np.int64)
+        tot = self._L.imc1d_census_counts(self._h, _p(per, _i64p))
+        return per, int(tot)
+
+    def put_census(self, counts, w, mu, t, x, cell):
+        cnt = _per(counts, self.P, np.int64)
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (w, mu, t, x)]
+        ci = np.ascontiguousarray(cell, dtype=np.int64)
+        assert arrs[0].size == int(cnt.sum())
+        check(self._L.imc1d_put_census(self._h, _p(cnt, _i64p), *[_p(a) for a in arrs], _p(ci, _i64p)))
+
+    def get_census(self):
+        """Returns (w, mu, t, x, cell, prob) grouped by problem."""
+        _, n = self.census_counts()
+        f = [np.empty(n) for _ in range(4)]
+        ci = np.empty(n, dtype=np.int64)
+        pr = np.empty(n, dtype=np.int32)
+        check(self._L.imc1d_get_census(self._h, n, *[_p(a) for a in f], _p(ci, _i64p), _p(pr, _i32p)))
+        return tuple(f) + (ci, pr)
+
+    def stage_pool(self):
+        per = np.zeros(self.P, dtype=np.int64)
+        n = int(self._L.imc1d_stage_pool(self._h, _p(per, _i64p)))
+        if n < 0:
+            check(n)
+        return per, n
+
+    def get_pool(self, n):
+        """The staged or transported pool: (w, mu, t, x, cell, prob)."""
+        f = [np.empty(n) for _ in range(4)]
+        ci = np.empty(n, dtype=np.int64)
+        pr = np.empty(n, dtype=np.int32)
+        check(self._L.imc1d_get_pool(self._h, n, *[_p(a) for a in f], _p(ci, _i64p), _p(pr, _i32p)))
+        return tuple(f) + (ci, pr)
+
+    def set_material(self, T, sigma_true, cv, theta, dt, want_opacity=False):
+        T = _per(T, self.ncell); s = _per(sigma_true, self.ncell); c = _per(cv, self.ncell)
+        th = _per(theta, self.P); d = _per(dt, self.P)
+        sa = np.empty(self.ncell) if want_opacity else None
+        ss = np.empty(self.ncell) if want_opacity else None
+        check(self._L.imc1d_set_material(self._h, _p(T), _p(s), _p(c), _p(th), _p(d), _p(sa), _p(ss)))
+        return sa, ss
+
+    def equilibrium_sample(self, N, Tr, T_emit_floor=0.0):
+        n = _per(N, self.P, np.int64); tr = _per(Tr, self.ncell); tf = _per(T_emit_floor, self.P)
+        check(self._L.imc1d_equilibrium_sample(self._h, _p(n, _i64p), _p(tr), _p(tf)))
+
+    def sample_boundary(self, side, n, E):
+        nn = _per(n, self.P, np.int64); e = _per(E, self.P)
+        check(self._L.imc1d_sample_boundary(self._h, int(side), _p(nn, _i64p), _p(e)))
+
+    def sample_volume(self, kind, N, QV=None, T_emit_floor=0.0, want_energy=False):
+        n = _per(N, self.P, np.int64)
+        q = None if QV is None else _per(QV, self.ncell)
+        tf = _per(T_emit_floor, self.P)
+        E = np.empty(self.ncell) if want_energy else None
+        tot = np.zeros(self.P)
+        check(self._L.imc1d_sample_volume(self._h, int(kind), _p(n, _i64p), _p(q), _p(tf), _p(E), _p(tot)))
+        return E, tot
+
+    def transport(self, reflect, weight_floor=-1.0, want_tallies=True):
+        """reflect: (left, right) or P such pairs.  Returns (dep, si, leak[P],
+        floor_leak[P], stats[8])."""
+        r = np.asarray(reflect, dtype=bool)
+        r = np.ascontiguousarray(np.broadcast_to(r.reshape(-1, 2), (self.P, 2)), dtype=np.int32)
+        wf = _per(weight_floor, self.P)
+        dep = np.empty(self.ncell) if want_tallies else None
+        si = np.empty(self.ncell) if want_tallies else None
+        lk = np.zeros(self.P); fl = np.zeros(self.P)
+        st = np.zeros(8, dtype=np.int64)
+        check(self._L.imc1d_transport(self._h, _p(r, _i32p), _p(wf), _p(dep), _p(si), _p(lk), _p(fl),
+                                      _p(st, _i64p)))
+        return dep, si, lk, fl, st
+
+    def last_transport_ms(self):
+        return float(self._L.imc1d_last_transport_ms(self._h))
+
+    def comb(self, Nmax, want_ecen=False, want_disc=False):
+        """Returns (ecen per cell or None, disc per cell or None, pre-comb
+        census energy per problem, post-comb census energy per problem)."""
+        nm = _per(Nmax, self.P, np.int64)
+        ecen = np.empty(self.ncell) if want_ecen else None
+        disc = np.empty(self.ncell) if want_disc else None
+        et = np.zeros(self.P); tr = np.zeros(self.P)
+        check(self._L.imc1d_comb(self._h, _p(nm, _i64p), _p(ecen), _p(disc), _p(et), _p(tr)))
+        return ecen, disc, et, tr

@@ -379,17 +379,6 @@ __global__ void k_sample_boundary(PoolView<R> P, int64_t base, int64_t n, int ge
 // accumulation flushed with FP64 atomics.  Follows IMC/IMC2D.py:938-1107.
 // ---------------------------------------------------------------------------
 template <typename R>
-struct Vec2;
-template <>
-struct Vec2<float> {
-  using type = float2;
-};
-template <>
-struct Vec2<double> {
-  using type = double2;
-};
-
-template <typename R>
 struct TransportArgs {
   using R2 = typename Vec2<R>::type;
   PoolView<R> P;
@@ -610,16 +599,6 @@ __global__ void k_finalize_tallies(int64_t n, int copies, const double* __restri
   double sa = sa32 ? (double)sa32[c] : sa64[c];
   dep[c] = a / vol[c];
   si[c] = thick ? a / sa / (dt * vol[c]) : b / (dt * vol[c]);
-}
-
-// Sum the private copies of a per-cell histogram.
-template <typename T, typename U>
-__global__ void k_reduce_copies(int64_t n, int copies, const T* __restrict__ part, U* __restrict__ out) {
-  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= n) return;
-  U a = 0;
-  for (int k = 0; k < copies; ++k) a += (U)part[(int64_t)k * n + c];
-  out[c] = a;
 }
 
 // ---------------------------------------------------------------------------

@@ -189,4 +189,26 @@ __device__ __forceinline__ void lds_tally_flush(const int* key, const double* va
   }
 }
 
+// FP32 / FP64 pair type of a tracking precision.
+template <typename R>
+struct Vec2;
+template <>
+struct Vec2<float> {
+  using type = float2;
+};
+template <>
+struct Vec2<double> {
+  using type = double2;
+};
+
+// Sum the private copies of a per-cell histogram.
+template <typename T, typename U>
+__global__ void k_reduce_copies(int64_t n, int copies, const T* __restrict__ part, U* __restrict__ out) {
+  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  U a = 0;
+  for (int k = 0; k < copies; ++k) a += (U)part[(int64_t)k * n + c];
+  out[c] = a;
+}
+
 }  // namespace imc
