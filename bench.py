@@ -46,9 +46,14 @@ def run_gpu(prob, precision, steps, warmup, fom_seeds):
     src = np.zeros(shape)
     opac, eos = prob.opacity, prob.eos
 
+    engine = [None]
+
     def one_run(seed, nsteps, timed):
+        # every run after the first reuses the first run's device context
+        # (same buffers, new Philox seed), so no allocation is timed
         st = IMC2D.init_simulation(prob.Ntarget, T0, T0, e1, e2, eos.eos, eos.inv_eos, geometry=prob.geometry,
-                                   precision=precision, seed=seed)
+                                   precision=precision, seed=seed, engine=engine[0])
+        engine[0] = st._engine
         infos = []
         t0 = time.perf_counter()
         for s in range(nsteps):
@@ -62,7 +67,6 @@ def run_gpu(prob, precision, steps, warmup, fom_seeds):
                     f" transport kernel {info['profiling']['transport_kernel_ms']:.1f} ms, segments {te['total']:.3e},"
                     f" histories {te['n_particles_transported']:.3e}")
         wall = time.perf_counter() - t0
-        st._engine.close()
         return infos, wall
 
     if warmup > 0:
@@ -82,6 +86,7 @@ def run_gpu(prob, precision, steps, warmup, fom_seeds):
         inf_s, _ = one_run(s, steps, False)
         deps.append(inf_s[-1]["deposited"])
         temps.append(inf_s[-1]["temperature"])
+    engine[0].close()
     return dict(wall=wall, segments=int(segs), histories=int(hist), kernel_ms=kern_ms, seg_per_launch=seg_per_launch,
                 residual=residual, deps=np.array(deps), temps=np.array(temps))
 
@@ -134,17 +139,166 @@ def run_cpu_baseline(prob, n_particles, nsteps):
     return dict(value=segs / tt if tt > 0 else None, cores=cores, seconds=tt, segments=segs, N=N, steps=nsteps)
 
 
+# ---------------------------------------------------------------------------
+# gray 1-D workloads: the reference's slab case and the batched slabs
+# ---------------------------------------------------------------------------
+BYTES_PER_SEGMENT_1D = {"fp32": 80, "fp64": 116}
+
+
+def _batch_inputs(probs):
+    meshes = [p.mesh for p in probs]
+    T0 = [np.full(p.I, p.T0) for p in probs]
+    eos = [p.eos.eos for p in probs]
+    inv = [p.eos.inv_eos for p in probs]
+    cvf = [p.eos.cv_func for p in probs]
+    sf = [p.opacity for p in probs]
+    N = np.array([p.n_particles for p in probs], dtype=np.int64)
+    Tb = np.array([[p.T_boundary[0], p.T_boundary[1]] for p in probs])
+    return meshes, T0, eos, inv, cvf, sf, N, Tb
+
+
+def run_gpu_batch1d(probs, precision, steps, warmup, tune):
+    from radtranbook_amd.IMC1D import BatchIMC1D
+
+    meshes, T0, eos, inv, cvf, sf, N, Tb = _batch_inputs(probs)
+    Nt, Nb = N // 2, N - N // 2
+    dt = probs[0].dt
+    b = BatchIMC1D(meshes, T0, T0, eos, inv, N, precision=precision, seed=1000)
+    tuned = None
+    for s in range(warmup):
+        out = b.step(Nt, Nb, N, Tb, dt, sf, cvf, tune=(tune and s == warmup - 1))
+        if out["tuned"] is not None:
+            tuned = out["tuned"][0]
+            log(f"[{precision}] autotuned transport launch (grid, batch, copies) = {tuned}")
+    b.reset(T0, T0, eos, inv, N, seed=1)
+    infos = []
+    t0 = time.perf_counter()
+    for s in range(steps):
+        ts = time.perf_counter()
+        out = b.step(Nt, Nb, N, Tb, dt, sf, cvf)
+        infos.append(out)
+        log(f"  [{precision}] step {s + 1}/{steps}: {(time.perf_counter() - ts) * 1e3:.1f} ms, transport kernel "
+            f"{out['transport_kernel_ms']:.2f} ms, segments {out['stats'][0]:.3e}, histories {out['stats'][7]:.3e}")
+    wall = time.perf_counter() - t0
+    b.eng.close()
+    segs = sum(int(i["stats"][0]) for i in infos)
+    hist = sum(int(i["stats"][7]) for i in infos)
+    kern = np.array([i["transport_kernel_ms"] for i in infos])
+    seg_launch = np.array([i["stats"][0] for i in infos], dtype=np.float64)
+    resid = max(float(np.max(np.abs(i["energy_residual"]) / np.abs(i["total_energy"]))) for i in infos)
+    return dict(wall=wall, segments=segs, histories=hist, kernel_ms=kern, seg_per_launch=seg_launch, residual=resid,
+                tuned=tuned, final_T=[t.copy() for t in b.T])
+
+
+def run_cpu_batch1d(probs, nsteps):
+    from oracle import oracle as O
+
+    O.build()
+    cores = len(os.sched_getaffinity(0))
+    segs, tt = 0, 0.0
+    for p in probs:
+        orc = O.Oracle1D(p.mesh, "slab", seed=1, nthreads=cores)
+        T0 = np.full(p.I, p.T0)
+        st = orc.init_simulation(p.n_particles, T0, T0, p.eos.eos, p.eos.inv_eos)
+        for s in range(nsteps):
+            t0 = time.perf_counter()
+            st, info = orc.step(st, p.n_particles // 2, p.n_particles - p.n_particles // 2, 0, p.n_particles,
+                                p.T_boundary, p.dt, p.opacity, p.eos.inv_eos, p.eos.cv_func, np.zeros(p.I))
+            tt += time.perf_counter() - t0
+            segs += int(info["stats"][0])
+    return dict(value=segs / tt if tt > 0 else None, cores=cores, seconds=tt, segments=segs)
+
+
+def main_1d(args, workload):
+    from radtranbook_amd import _lib
+    from radtranbook_amd.problems import batch_slabs, marshak_slab
+
+    _lib.load()
+    _lib.require_gpu()
+    steps = args.steps if args.steps is not None else 10
+    warmup = args.warmup if args.warmup is not None else 2
+    if workload == "batch1d":
+        probs = batch_slabs(P=args.problems, n_steps=steps)
+        desc = (f"batched gray IMC 1D: {args.problems} independent slabs (cells log-U[50,2000], particles/step "
+                f"log-U[1e4,1e6]), dt=0.01 ns x {steps} steps, one launch per stage over all problems")
+        metric = f"IMC segments/s, batched 1D: {args.problems} slab problems in one persistent launch"
+    else:
+        probs = [marshak_slab(n_steps=steps)]
+        desc = f"gray IMC 1D slab: 100 cells on [0,0.2] cm, 1e5 particles/step, dt=0.01 ns x {steps} steps"
+        metric = "IMC segments/s, 1D slab 100 cells 1e5 particles/step"
+    precs = ["fp32", "fp64"] if args.precision == "both" else [args.precision]
+    res = {p: run_gpu_batch1d(probs, p, steps, warmup, args.tune and warmup > 0) for p in precs}
+    head = res[precs[0]]
+    value = head["segments"] / head["wall"]
+    avg_launch_s = head["kernel_ms"].mean() / 1e3
+    achieved = head["seg_per_launch"].mean() * BYTES_PER_SEGMENT_1D[precs[0]] / avg_launch_s / 1e9
+    out = {
+        "metric": metric,
+        "value": value,
+        "unit": "segments/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": head["wall"] / steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 geometry / f64 weights+tallies" if precs[0] == "fp32" else "f64",
+        "data": "synthetic (seeded problem parameters, Philox particle streams)",
+        "config": {"workload": desc, "problems": len(probs), "cells": int(sum(p.I for p in probs)),
+                   "new_particles_per_step": int(sum(p.n_particles for p in probs))},
+        "histories_per_s": head["histories"] / head["wall"],
+        "transport_segments_per_s": head["segments"] / (head["kernel_ms"].sum() / 1e3),
+        "roofline": {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "kernel": f"k1_transport<{'float' if precs[0] == 'fp32' else 'double'},0>",
+            "bytes_per_segment": BYTES_PER_SEGMENT_1D[precs[0]],
+        },
+        "autotuned_launch": head["tuned"],
+    }
+    for p in precs:
+        r = res[p]
+        out[p] = {"segments_per_s": r["segments"] / r["wall"], "ms_per_step": r["wall"] / steps * 1e3,
+                  "transport_kernel_ms_per_step": float(r["kernel_ms"].mean()),
+                  "segments_per_step": float(r["seg_per_launch"].mean()),
+                  "histories_per_step": r["histories"] / steps, "max_energy_residual_rel": r["residual"]}
+    if not args.no_cpu:
+        if workload == "batch1d":
+            sample = batch_slabs(P=args.problems, n_steps=steps, scale=args.cpu_scale)[:args.cpu_problems]
+            nst = args.cpu_steps if args.cpu_steps is not None else 3
+        else:
+            sample, nst = probs, steps
+        cb = run_cpu_batch1d(sample, nst)
+        out["cpu_baseline"] = {
+            "value": cb["value"], "unit": "segments/s", "cores": cb["cores"], "kind": "port",
+            "sample": f"FP64 C oracle (OpenMP, history-based) on {len(sample)} of the problems"
+                      + (f" at {args.cpu_scale:g} x their particle counts" if workload == "batch1d" else "")
+                      + f", {nst} steps ({cb['segments']:.3e} segments in {cb['seconds']:.1f} s)"}
+        if cb["value"]:
+            out["gpu_over_cpu_segments_per_s"] = value / cb["value"]
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", choices=["marshak2d", "slab1d", "batch1d"], default="marshak2d",
+                    help="marshak2d: 2D 200x200 heterogeneous (default, headline); slab1d: the reference's 1D slab "
+                         "case; batch1d: 256 independent slabs")
+    ap.add_argument("--problems", type=int, default=256)
+    ap.add_argument("--tune", action=argparse.BooleanOptionalAction, default=True,
+                    help="autotune the 1D transport launch shape on the first warm-up step")
+    ap.add_argument("--cpu-problems", type=int, default=32)
+    ap.add_argument("--cpu-scale", type=float, default=0.1)
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--particles", type=float, default=1e7)
     ap.add_argument("--cells", type=int, default=200)
     ap.add_argument("--fom-seeds", type=int, default=8)
     ap.add_argument("--precision", choices=["both", "fp32", "fp64"], default="both")
     ap.add_argument("--cpu-particles", type=float, default=4e6)
-    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--cpu-steps", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
 
@@ -153,6 +307,15 @@ def main():
               "(the IMC step shards by particles with one tally all-reduce per step; that path is not built)",
               file=sys.stderr)
         sys.exit(2)
+    if args.workload != "marshak2d":
+        main_1d(args, args.workload)
+        return
+    if args.steps is None:
+        args.steps = 20
+    if args.warmup is None:
+        args.warmup = 2
+    if args.cpu_steps is None:
+        args.cpu_steps = 8
 
     from radtranbook_amd import _lib
     from radtranbook_amd.problems import marshak_2d_heterogeneous

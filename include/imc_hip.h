@@ -59,6 +59,9 @@ void imc2d_destroy(imc2d_ctx* ctx);
 
 /* Step number used as the Philox counter word of every draw that follows. */
 int imc2d_set_step(imc2d_ctx* ctx, uint32_t step);
+/* Philox key of every draw that follows (lets one context, with its device
+ * buffers, run several independent realisations). */
+int imc2d_set_seed(imc2d_ctx* ctx, uint64_t seed);
 /* Transport launch shape: grid 0 = occupancy-derived persistent grid;
  * batch = particles a wave claims per queue visit; tally_copies = private
  * copies of the per-cell tallies (power of two, 1..64). */
@@ -137,6 +140,7 @@ int imc1d_create(imc1d_ctx** out, int geometry, int nprob, const int32_t* ncells
                  int precision, uint64_t seed, int device);
 void imc1d_destroy(imc1d_ctx* ctx);
 int imc1d_set_step(imc1d_ctx* ctx, uint32_t step);
+int imc1d_set_seed(imc1d_ctx* ctx, uint64_t seed);
 /* Transport launch shape, as imc2d_set_launch. */
 int imc1d_set_launch(imc1d_ctx* ctx, int grid, int batch, int tally_copies);
 
@@ -149,6 +153,10 @@ int imc1d_put_census(imc1d_ctx* ctx, const int64_t* counts, const double* w, con
                      const double* x, const int64_t* cell);
 int imc1d_get_census(imc1d_ctx* ctx, int64_t n, double* w, double* mu, double* t, double* x, int64_t* cell,
                      int32_t* prob);
+
+/* Census energy per cell (the np.bincount of IMC/IMC1D.py:884-886) and per
+ * problem (np.sum(weights), IMC/IMC1D.py:889); outputs nullable. */
+int imc1d_census_energy(imc1d_ctx* ctx, double* per_cell, double* per_problem);
 
 /* Fleck factor (IMC/IMC1D.py:933-938), fails if f leaves [0,1]; theta and dt
  * per problem; opacity outputs nullable.  Starts a step (no source blocks). */

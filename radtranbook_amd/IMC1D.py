@@ -100,28 +100,37 @@ class SimulationState:
 
 
 def init_simulation(Ntarget, Tinit, Tr_init, mesh, eos, inv_eos, Ntarget_ic=None, geometry="slab", T_emit_floor=0.0,
-                    precision="fp64", seed=1, device=0):
-    """Initialise material state and the equilibrium census (IMC/IMC1D.py:859-905)."""
+                    precision="fp64", seed=1, device=0, engine=None):
+    """Initialise material state and the equilibrium census (IMC/IMC1D.py:859-905).
+
+    ``engine``: the device context of an earlier state on the same mesh,
+    geometry and precision (``state._engine``) to reuse with its buffers."""
     if geometry not in ("slab", "spherical"):
         raise ValueError(f"Unknown geometry: {geometry}")
     volumes = _cell_volumes(mesh, geometry)
     internal_energy = eos(Tinit)
     temperature = np.array(Tinit, dtype=np.float64, copy=True)
     assert np.allclose(inv_eos(internal_energy), Tinit), "Inverse EOS failed"
-    eng = Engine1D([_edges(mesh)], geometry=geometry, precision=precision, seed=seed, device=device)
+    if engine is not None:
+        eng = engine
+        if (eng.P != 1 or eng.ncell != volumes.size or eng.geometry != geometry
+                or eng.precision != {"fp64": 0, "fp32": 1}[precision]):
+            raise ValueError("init_simulation(): engine was built for another mesh, geometry or precision")
+        eng.set_seed(seed)
+    else:
+        eng = Engine1D([_edges(mesh)], geometry=geometry, precision=precision, seed=seed, device=device)
     N_ic = Ntarget if Ntarget_ic is None else Ntarget_ic
     eng.equilibrium_sample(int(N_ic), Tr_init, T_emit_floor)
-    w, _, _, _, cell, _ = eng.get_census()
-    I = volumes.size
-    radiation_temperature = (np.bincount(cell, weights=w, minlength=I) / volumes / __a) ** 0.25
+    rad_cell, rad_tot = eng.census_energy()
+    radiation_temperature = (rad_cell / volumes / __a) ** 0.25
     total_internal = float(np.sum(internal_energy * volumes))
-    total_rad = float(np.sum(w))
+    total_rad = float(rad_tot[0])
     previous_total = total_internal + total_rad
     if VERBOSE:
         print("Time", "N", "Total Energy", "Total Internal Energy", "Total Radiation Energy", "Boundary Emission",
               "Lost Energy", sep="\t")
         print("=" * 111)
-        print("{:.6f}".format(0.0), w.size, "{:.6f}".format(previous_total), "{:.6f}".format(total_internal),
+        print("{:.6f}".format(0.0), eng.census_counts()[1], "{:.6f}".format(previous_total), "{:.6f}".format(total_internal),
               "{:.6f}".format(total_rad), "{:.6f}".format(0.0), "{:.6f}".format(0.0), sep="\t")
     return SimulationState(eng, internal_energy, temperature, radiation_temperature, 0.0, previous_total, 0)
 
@@ -303,7 +312,13 @@ class BatchIMC1D:
         self.eng = Engine1D([_edges(m) for m in self.meshes], geometry=geometry, precision=precision, seed=seed,
                             device=device)
         self.volumes = [_cell_volumes(m, geometry) for m in self.meshes]
-        self.vol_all = np.concatenate(self.volumes)
+        self.reset(Tinit, Tr_init, eos, inv_eos, Ntarget_ic, T_emit_floor, seed)
+
+    def reset(self, Tinit, Tr_init, eos, inv_eos, Ntarget_ic, T_emit_floor=0.0, seed=None):
+        """Start again from t = 0 (equilibrium census) on the same device
+        buffers, optionally with another Philox seed."""
+        if seed is not None:
+            self.eng.set_seed(seed)
         self.T = [np.array(t, dtype=np.float64, copy=True) for t in Tinit]
         self.u = [self._fn(eos, p)(self.T[p]) for p in range(self.P)]
         self.eng.equilibrium_sample(self._per(Ntarget_ic, np.int64), np.concatenate(Tr_init),
@@ -312,8 +327,7 @@ class BatchIMC1D:
         self.time = np.zeros(self.P)
         self.count = 0
         self._inv = inv_eos
-        w, _, _, _, cell, prob = self.eng.get_census()
-        rad = np.array([np.sum(w[prob == p]) for p in range(self.P)])
+        _, rad = self.eng.census_energy()
         self.previous_total = np.array([float(np.sum(self.u[p] * self.volumes[p])) for p in range(self.P)]) + rad
 
     def _per(self, v, dtype=np.float64):
@@ -324,8 +338,10 @@ class BatchIMC1D:
         return f[p] if isinstance(f, (list, tuple)) else f
 
     def step(self, Ntarget, Nboundary, NMax, T_boundary, dt, sigma_a_func, cv, reflect=(False, False), theta=1.0,
-             T_emit_floor=0.0):
+             T_emit_floor=0.0, tune=False):
         """One step of every problem.  T_boundary: (P, 2) array or pair.
+        ``tune``: time the transport launch shapes on this step first and keep
+        the fastest (results do not depend on the shape).
         Returns a dict of per-problem energy diagnostics and device timings."""
         eng = self.eng
         P = self.P
@@ -349,6 +365,11 @@ class BatchIMC1D:
                 be += np.where(n > 0, E, 0.0)
         Ec, _ = eng.sample_volume(SRC_EMISSION, self._per(Ntarget, np.int64), None, self._per(T_emit_floor),
                                   want_energy=True)
+        tuned = None
+        if tune:
+            from .autotune import tune_engine1d
+
+            tuned = tune_engine1d(eng, reflect)
         dep, si, leak, fleak, stats = eng.transport(reflect, -1.0)
         ms = eng.last_transport_ms()
         if stats[6] > 0:
@@ -368,4 +389,4 @@ class BatchIMC1D:
         self.count += 1
         return {"total_energy": total, "boundary_emission": be, "boundary_loss": leak, "energy_loss": loss,
                 "comb_discrepancy": disc, "floor_kill_leak": fleak, "energy_residual": loss + disc + fleak,
-                "stats": stats, "transport_kernel_ms": ms, "N_particles": eng.census_counts()[0]}
+                "stats": stats, "transport_kernel_ms": ms, "N_particles": eng.census_counts()[0], "tuned": tuned}

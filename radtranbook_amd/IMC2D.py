@@ -123,14 +123,23 @@ def _face_profile(geometry, side, e1, e2, dt, bfunc):
 
 
 def init_simulation(Ntarget, Tinit, Tr_init, edges1, edges2, eos, inv_eos, Ntarget_ic=None, geometry="xy",
-                    precision="fp64", seed=1, device=0):
-    """Initialise material state and the equilibrium census (IMC/IMC2D.py:1514-1588)."""
+                    precision="fp64", seed=1, device=0, engine=None):
+    """Initialise material state and the equilibrium census (IMC/IMC2D.py:1514-1588).
+
+    ``engine``: the device context of an earlier state on the same mesh,
+    geometry and precision (``state._engine``) to reuse with its buffers."""
     nx, ny = _shape_from_edges(edges1, edges2)
     volumes = _cell_volumes(edges1, edges2, geometry)
     internal_energy = eos(Tinit)
     temperature = np.array(Tinit, dtype=np.float64, copy=True)
     assert np.allclose(inv_eos(internal_energy), Tinit), "Inverse EOS failed"
-    eng = Engine2D(edges1, edges2, geometry=geometry, precision=precision, seed=seed, device=device)
+    if engine is not None:
+        eng = engine
+        if eng.shape != (nx, ny) or eng.geometry != geometry or eng.precision != {"fp64": 0, "fp32": 1}[precision]:
+            raise ValueError("init_simulation(): engine was built for another mesh, geometry or precision")
+        eng.set_seed(seed)
+    else:
+        eng = Engine2D(edges1, edges2, geometry=geometry, precision=precision, seed=seed, device=device)
     N_ic = Ntarget if Ntarget_ic is None else Ntarget_ic
     eng.equilibrium_sample(N_ic, Tr_init)
     rad_cell = eng.census_by_cell()

@@ -37,6 +37,7 @@ SIGNATURES = {
     "imc2d_create": (C.c_int, [C.POINTER(_vp), C.c_int, C.c_int, C.c_int, _dp, _dp, C.c_int, C.c_uint64, C.c_int]),
     "imc2d_destroy": (None, [_vp]),
     "imc2d_set_step": (C.c_int, [_vp, C.c_uint32]),
+    "imc2d_set_seed": (C.c_int, [_vp, C.c_uint64]),
     "imc2d_set_launch": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
     "imc2d_census_size": (C.c_int64, [_vp]),
     "imc2d_put_census": (C.c_int, [_vp, C.c_int64, _dp, _dp, _dp, _dp, _dp, _dp, _i64p, _i64p]),
@@ -53,8 +54,10 @@ SIGNATURES = {
     "imc1d_create": (C.c_int, [C.POINTER(_vp), C.c_int, C.c_int, _i32p, _dp, C.c_int, C.c_uint64, C.c_int]),
     "imc1d_destroy": (None, [_vp]),
     "imc1d_set_step": (C.c_int, [_vp, C.c_uint32]),
+    "imc1d_set_seed": (C.c_int, [_vp, C.c_uint64]),
     "imc1d_set_launch": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
     "imc1d_census_counts": (C.c_int64, [_vp, _i64p]),
+    "imc1d_census_energy": (C.c_int, [_vp, _dp, _dp]),
     "imc1d_put_census": (C.c_int, [_vp, _i64p, _dp, _dp, _dp, _dp, _i64p]),
     "imc1d_get_census": (C.c_int, [_vp, C.c_int64, _dp, _dp, _dp, _dp, _i64p, _i32p]),
     "imc1d_set_material": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
@@ -147,6 +150,10 @@ class Engine2D:
     # -- configuration -------------------------------------------------------
     def set_step(self, step):
         check(self._L.imc2d_set_step(self._h, int(step) & 0xFFFFFFFF))
+
+    def set_seed(self, seed):
+        self.seed = int(seed)
+        check(self._L.imc2d_set_seed(self._h, self.seed))
 
     def set_launch(self, grid=0, batch=256, tally_copies=8):
         check(self._L.imc2d_set_launch(self._h, int(grid), int(batch), int(tally_copies)))
@@ -278,6 +285,10 @@ class Engine1D:
     def set_step(self, step):
         check(self._L.imc1d_set_step(self._h, int(step) & 0xFFFFFFFF))
 
+    def set_seed(self, seed):
+        self.seed = int(seed)
+        check(self._L.imc1d_set_seed(self._h, self.seed))
+
     def set_launch(self, grid=0, batch=256, tally_copies=8):
         check(self._L.imc1d_set_launch(self._h, int(grid), int(batch), int(tally_copies)))
 
@@ -285,6 +296,13 @@ class Engine1D:
         per = np.zeros(self.P, dtype=np.int64)
         tot = self._L.imc1d_census_counts(self._h, _p(per, _i64p))
         return per, int(tot)
+
+    def census_energy(self):
+        """(energy per cell, energy per problem) of the census."""
+        pc = np.empty(self.ncell)
+        pp = np.empty(self.P)
+        check(self._L.imc1d_census_energy(self._h, _p(pc), _p(pp)))
+        return pc, pp
 
     def put_census(self, counts, w, mu, t, x, cell):
         cnt = _per(counts, self.P, np.int64)

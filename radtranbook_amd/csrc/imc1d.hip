@@ -912,6 +912,12 @@ int imc1d_set_step(imc1d_ctx* c, uint32_t step) {
   return kOk;
 }
 
+int imc1d_set_seed(imc1d_ctx* c, uint64_t seed) {
+  IMC_REQUIRE(c, kErrArg, "null ctx");
+  c->seed = seed;
+  return kOk;
+}
+
 int imc1d_set_launch(imc1d_ctx* c, int grid, int batch, int tally_copies) {
   IMC_REQUIRE(c && grid >= 0 && batch > 0, kErrArg, "imc1d_set_launch: bad arguments");
   IMC_REQUIRE(tally_copies >= 1 && tally_copies <= kMaxTallyCopies && (tally_copies & (tally_copies - 1)) == 0,
@@ -1000,6 +1006,50 @@ int imc1d_get_census(imc1d_ctx* c, int64_t n, double* w, double* mu, double* t, 
     IMC_HIP(hipMemcpyAsync(dst[a], tmp[a], esz[a] * (size_t)n, hipMemcpyDeviceToHost, c->stream));
   IMC_HIP(hipStreamSynchronize(c->stream));
   for (int a = 0; a < 6; ++a) IMC_HIP(hipFree(tmp[a]));
+  return kOk;
+}
+
+// Census energy per cell (np.bincount of the weights by cell,
+// IMC/IMC1D.py:884-886) and per problem (np.sum(weights), IMC/IMC1D.py:889);
+// both outputs nullable.
+int imc1d_census_energy(imc1d_ctx* c, double* per_cell, double* per_problem) {
+  IMC_REQUIRE(c, kErrArg, "null ctx");
+  IMC_HIP(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  int P = c->P;
+  int64_t n = c->nA, nc = c->ncell;
+  if (per_cell) {
+    IMC_HIP(hipMemsetAsync(c->ecen_part.p, 0, sizeof(double) * nc * c->tally_copies, s));
+    if (n > 0) {
+      // the census carries each particle's problem; pool offsets are not needed
+      int g = std::min(launch_blocks(n, 256), c->n_cu * 4);
+      if (c->precision == IMC_FP64)
+        k1_comb_ecen<double><<<g, 256, 0, s>>>(c->A.view<double>(), n, c->coff.p, nc, c->tally_copies - 1,
+                                               c->ecen_part.p);
+      else
+        k1_comb_ecen<float><<<g, 256, 0, s>>>(c->A.view<float>(), n, c->coff.p, nc, c->tally_copies - 1,
+                                              c->ecen_part.p);
+      IMC_HIP(hipGetLastError());
+    }
+    k_reduce_copies<double, double><<<launch_blocks(nc, 256), 256, 0, s>>>(nc, c->tally_copies, c->ecen_part.p,
+                                                                          c->ecen.p);
+    IMC_HIP(hipGetLastError());
+    IMC_HIP(hipMemcpyAsync(per_cell, c->ecen.p, sizeof(double) * nc, hipMemcpyDeviceToHost, s));
+  }
+  if (per_problem) {
+    std::vector<int64_t> off(P + 1, 0);
+    for (int p = 0; p < P; ++p) off[p + 1] = off[p] + c->h_ncen[p];
+    IMC_HIP(hipMemcpyAsync(c->taken.p, off.data(), sizeof(int64_t) * (P + 1), hipMemcpyHostToDevice, s));
+    if (n > 0) {
+      k1_segment_sums<<<P, 256, 0, s>>>(c->A.w, c->taken.p, c->psum.p);
+      IMC_HIP(hipGetLastError());
+    } else {
+      IMC_HIP(hipMemsetAsync(c->psum.p, 0, sizeof(double) * P, s));
+    }
+    IMC_HIP(hipMemcpyAsync(per_problem, c->psum.p, sizeof(double) * P, hipMemcpyDeviceToHost, s));
+    IMC_HIP(hipStreamSynchronize(s));  // `off` is a pageable source
+  }
+  IMC_HIP(hipStreamSynchronize(s));
   return kOk;
 }
 

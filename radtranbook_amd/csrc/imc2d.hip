@@ -980,6 +980,12 @@ int imc2d_set_step(imc2d_ctx* c, uint32_t step) {
   return kOk;
 }
 
+int imc2d_set_seed(imc2d_ctx* c, uint64_t seed) {
+  IMC_REQUIRE(c, kErrArg, "null ctx");
+  c->seed = seed;
+  return kOk;
+}
+
 int imc2d_set_launch(imc2d_ctx* c, int grid, int batch, int tally_copies) {
   IMC_REQUIRE(c, kErrArg, "null ctx");
   IMC_REQUIRE(grid >= 0 && batch > 0, kErrArg, "imc2d_set_launch: bad arguments");

@@ -6,7 +6,14 @@
   width), sigma_a = sigma0 max(T,1e-3)^-3, c_v = 0.3, T0 = 1e-3 keV, left face
   at 0.5 keV, dt = 0.005 ns x 20 steps, 1e7 new particles per step split
   evenly between emission and the hot face, census combed to 1e7.
-* ``marshak_slab`` -- gray 1-D slab, 100 cells on [0,0.2] cm, sigma = 300 T^-3.
+* ``marshak_slab`` -- gray 1-D slab, 100 uniform cells on [0,0.2] cm,
+  sigma_a = 300 max(T,1e-3)^-3, c_v = 0.3, T0 = 1e-3 keV, left face 1 keV,
+  dt = 0.01 ns x 10 steps, 1e5 new particles per step, census combed to 1e5.
+* ``batch_slabs`` -- 256 independent gray slabs of length 0.2 cm: cells
+  floor(exp U(ln 50, ln 2000)), particles per step floor(exp U(ln 1e4, ln 1e6))
+  (half emission, half from the hot face, census combed to that count),
+  sigma0 ~ U(10, 1000) with sigma_a = sigma0 max(T,1e-3)^-3, c_v ~ U(0.05, 0.5),
+  T_b ~ U(0.1, 1) keV on the left face, T0 = 1e-3 keV, dt = 0.01 ns x 10.
 """
 from __future__ import annotations
 
@@ -99,3 +106,63 @@ def marshak_2d_heterogeneous(n=200, n_particles=10_000_000, n_steps=20, seed=1):
         T_boundary=(0.5, 0.0, 0.0, 0.0), reflect=(False, False, False, False),
         dt=0.005, n_steps=n_steps, n_particles=int(n_particles),
     )
+
+
+@dataclass
+class SlabProblem:
+    name: str
+    mesh: np.ndarray  # (I, 2) cell edges
+    opacity: PowerLawOpacity
+    eos: LinearEOS
+    T0: float
+    T_boundary: tuple
+    reflect: tuple
+    dt: float
+    n_steps: int
+    n_particles: int
+
+    @property
+    def I(self):
+        return self.mesh.shape[0]
+
+    @property
+    def Ntarget(self):
+        return self.n_particles // 2
+
+    @property
+    def Nboundary(self):
+        return self.n_particles - self.n_particles // 2
+
+    @property
+    def Nmax(self):
+        return self.n_particles
+
+
+def _slab_mesh(I, L):
+    e = np.linspace(0.0, L, I + 1)
+    return np.column_stack([e[:-1], e[1:]])
+
+
+def marshak_slab(I=100, n_particles=100_000, n_steps=10):
+    """The reference's own CPU-runnable slab case (1e5 new particles per step)."""
+    return SlabProblem(
+        name=f"marshak_slab_{I}", mesh=_slab_mesh(I, 0.2),
+        opacity=PowerLawOpacity(np.full(I, 300.0), 3.0, 1e-3), eos=LinearEOS(0.3), T0=1e-3,
+        T_boundary=(1.0, 0.0), reflect=(False, False), dt=0.01, n_steps=n_steps, n_particles=int(n_particles))
+
+
+def batch_slabs(P=256, seed=1, n_steps=10, scale=1.0):
+    """P independent slabs with seeded, log-uniform cell and particle counts.
+    ``scale`` multiplies every particle count (for bounded samples)."""
+    rng = np.random.default_rng(seed)
+    cells = np.floor(np.exp(rng.uniform(np.log(50), np.log(2000), P))).astype(int)
+    parts = np.floor(np.exp(rng.uniform(np.log(1e4), np.log(1e6), P))).astype(np.int64)
+    sig0 = rng.uniform(10.0, 1000.0, P)
+    cv = rng.uniform(0.05, 0.5, P)
+    Tb = rng.uniform(0.1, 1.0, P)
+    parts = np.maximum((parts * scale).astype(np.int64), 2)
+    return [SlabProblem(
+        name=f"batch_slab_{p}", mesh=_slab_mesh(int(cells[p]), 0.2),
+        opacity=PowerLawOpacity(np.full(int(cells[p]), sig0[p]), 3.0, 1e-3), eos=LinearEOS(float(cv[p])), T0=1e-3,
+        T_boundary=(float(Tb[p]), 0.0), reflect=(False, False), dt=0.01, n_steps=n_steps,
+        n_particles=int(parts[p])) for p in range(P)]
