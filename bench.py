@@ -142,7 +142,10 @@ def run_cpu_baseline(prob, n_particles, nsteps):
 # ---------------------------------------------------------------------------
 # gray 1-D workloads: the reference's slab case and the batched slabs
 # ---------------------------------------------------------------------------
-BYTES_PER_SEGMENT_1D = {"fp32": 80, "fp64": 116}
+# event-based model, as the 2-D figures: state (x, mu, d_census, cell, history
+# id, event counter: 4 B each in FP32 / 8,8,8,4,4,4 in FP64; weight 8 B) loaded
+# and stored once per segment, (sigma_a, sigma_s) gathered, two FP64 tallies
+BYTES_PER_SEGMENT_1D = {"fp32": 2 * 32 + 8 + 16, "fp64": 2 * 44 + 16 + 16}
 
 
 def _batch_inputs(probs):
