@@ -41,6 +41,7 @@ extern "C" {
 
 typedef struct imc2d_ctx imc2d_ctx;
 typedef struct imc1d_ctx imc1d_ctx;
+typedef struct mgimc_ctx mgimc_ctx;
 
 /* Message of the last failed call on this thread. */
 const char* imc_last_error(void);
@@ -209,6 +210,73 @@ float imc1d_last_transport_ms(imc1d_ctx* ctx);
  * total_rad per problem. */
 int imc1d_comb(imc1d_ctx* ctx, const int64_t* Nmax, double* ecen_out, double* disc_out, double* ecen_total_out,
                double* total_rad_out);
+
+/* ---- multigroup 2-D (xy / rz) ------------------------------------------- */
+/* Per-(group, cell) arrays are (G, ncell) with cell c = i*ny + j, the layout
+ * of the reference's (n_groups, nx, ny) arrays.  Replaces the state set up by
+ * MG_IMC2D.init_simulation (MG_IMC/MG_IMC2D.py:1431-1543) and
+ * SimulationState2DMG (MG_IMC/MG_IMC2D.py:116-136). */
+int mgimc_create(mgimc_ctx** out, int geometry, int nx, int ny, const double* edges1, const double* edges2, int G,
+                 const double* energy_edges, int precision, uint64_t seed, int device);
+void mgimc_destroy(mgimc_ctx* ctx);
+int mgimc_set_step(mgimc_ctx* ctx, uint32_t step);
+int mgimc_set_seed(mgimc_ctx* ctx, uint64_t seed);
+int mgimc_set_launch(mgimc_ctx* ctx, int grid, int batch, int tally_copies);
+
+/* Census: weights, dir1, dir2, times, pos1, pos2, cell_i, cell_j, groups
+ * (MG_IMC2D.py:119-127). */
+int64_t mgimc_census_size(mgimc_ctx* ctx);
+int mgimc_put_census(mgimc_ctx* ctx, int64_t n, const double* w, const double* d1, const double* d2, const double* t,
+                     const double* p1, const double* p2, const int64_t* ci, const int64_t* cj, const int32_t* g);
+int mgimc_get_census(mgimc_ctx* ctx, int64_t n, double* w, double* d1, double* d2, double* t, double* p1,
+                     double* p2, int64_t* ci, int64_t* cj, int32_t* g);
+/* Census energy per (group, cell) (the bincounts of MG_IMC2D.py:1500-1506,
+ * 2004-2011) and in total; outputs nullable. */
+int mgimc_census_by_bin(mgimc_ctx* ctx, double* out, double* total_out);
+
+/* Planck-mean Fleck factor and effective opacities (MG_IMC2D.py:1693-1719),
+ * group fractions b*_g normalised over the groups or the caller's
+ * emission_fractions (1866-1885), emission energies sigma_a,g b*_g a c T^4 dt V
+ * with cells below T_emit_floor silent (1888-1894), and the cell CDF the
+ * emission sampler draws from.  sigma: (G, ncell).  Outputs nullable:
+ * sa, ss, E (G, ncell); Etot; Eplanck (per cell, Planck b*_g, no floor: the
+ * particle-budget estimate of 1741-1755). */
+int mgimc_set_material(mgimc_ctx* ctx, const double* T, const double* sigma, const double* cv, double theta,
+                       double dt, const double* emission_fractions, double T_emit_floor, double* sa_out,
+                       double* ss_out, double* E_out, double* Etot_out, double* Eplanck_out);
+
+/* Equilibrium census at t=0 replacing the pool: _equilibrium_sample_xy_mg /
+ * _rz_mg (MG_IMC2D.py:992-1095): cells by a Tr^4 V, groups from the Planck
+ * spectrum at the cell's Tr. */
+int mgimc_equilibrium_sample(mgimc_ctx* ctx, int64_t N, const double* Tr, double T_emit_floor);
+
+/* Face source appended to the pool: _sample_boundary_xy / _rz
+ * (MG_IMC2D.py:392-577), groups from the Planck spectrum at the face
+ * temperature (Tb, or Tface[m] with the profile prof_cum of a position-
+ * dependent face).  E_by_group_out (nullable, G): energy by group. */
+int mgimc_sample_boundary(mgimc_ctx* ctx, int side, int64_t n, double E_total, double dt, double Tb,
+                          const double* prof_cum, const double* Tface, int nface, double* E_by_group_out);
+
+/* Fixed source appended to the pool: _sample_source_xy_mg / _rz_mg
+ * (MG_IMC2D.py:1098-1234).  S: (G, ncell) energy (source x V x dt). */
+int mgimc_sample_source(mgimc_ctx* ctx, int64_t N, const double* S, double dt, double* total_out);
+
+/* Material emission appended to the pool (MG_IMC2D.py:1896-1943): N
+ * particles of weight E/N from the energies of the last mgimc_set_material. */
+int mgimc_sample_emission(mgimc_ctx* ctx, int64_t N, double dt, double* Etot_out);
+
+/* Transport the pool: _transport_particles_2d_mg (MG_IMC2D.py:774-989).
+ * weight_floor < 0: 1e-10 sum(w)/N (2048).  Outputs nullable: dep, si
+ * (G, ncell); bl[1 + G + 4] = boundary loss total, by group, by side;
+ * stats[8] as the reference's. */
+int mgimc_transport(mgimc_ctx* ctx, double dt, const int32_t* reflect, int64_t max_events, double weight_floor,
+                    double* dep_out, double* si_out, double* bl_out, int64_t* stats_out);
+float mgimc_last_transport_ms(mgimc_ctx* ctx);
+
+/* (group, cell) comb: _comb_mg (MG_IMC2D.py:1314-1378); times reset to 0.
+ * Outputs nullable: ecen, disc, post (G, ncell); ecen_total, total_rad. */
+int mgimc_comb(mgimc_ctx* ctx, int64_t Nmax, double* ecen_out, double* disc_out, double* post_out,
+               double* ecen_total_out, double* total_rad_out);
 
 #ifdef __cplusplus
 }

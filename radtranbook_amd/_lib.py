@@ -69,6 +69,26 @@ SIGNATURES = {
     "imc1d_transport": (C.c_int, [_vp, _i32p, _dp, _dp, _dp, _dp, _dp, _i64p]),
     "imc1d_last_transport_ms": (C.c_float, [_vp]),
     "imc1d_comb": (C.c_int, [_vp, _i64p, _dp, _dp, _dp, _dp]),
+    "mgimc_create": (C.c_int, [C.POINTER(_vp), C.c_int, C.c_int, C.c_int, _dp, _dp, C.c_int, _dp, C.c_int,
+                               C.c_uint64, C.c_int]),
+    "mgimc_destroy": (None, [_vp]),
+    "mgimc_set_step": (C.c_int, [_vp, C.c_uint32]),
+    "mgimc_set_seed": (C.c_int, [_vp, C.c_uint64]),
+    "mgimc_set_launch": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
+    "mgimc_census_size": (C.c_int64, [_vp]),
+    "mgimc_put_census": (C.c_int, [_vp, C.c_int64, _dp, _dp, _dp, _dp, _dp, _dp, _i64p, _i64p, _i32p]),
+    "mgimc_get_census": (C.c_int, [_vp, C.c_int64, _dp, _dp, _dp, _dp, _dp, _dp, _i64p, _i64p, _i32p]),
+    "mgimc_census_by_bin": (C.c_int, [_vp, _dp, _dp]),
+    "mgimc_set_material": (C.c_int, [_vp, _dp, _dp, _dp, C.c_double, C.c_double, _dp, C.c_double, _dp, _dp, _dp,
+                                     _dp, _dp]),
+    "mgimc_equilibrium_sample": (C.c_int, [_vp, C.c_int64, _dp, C.c_double]),
+    "mgimc_sample_boundary": (C.c_int, [_vp, C.c_int, C.c_int64, C.c_double, C.c_double, C.c_double, _dp, _dp,
+                                        C.c_int, _dp]),
+    "mgimc_sample_source": (C.c_int, [_vp, C.c_int64, _dp, C.c_double, _dp]),
+    "mgimc_sample_emission": (C.c_int, [_vp, C.c_int64, C.c_double, _dp]),
+    "mgimc_transport": (C.c_int, [_vp, C.c_double, _i32p, C.c_int64, C.c_double, _dp, _dp, _dp, _i64p]),
+    "mgimc_last_transport_ms": (C.c_float, [_vp]),
+    "mgimc_comb": (C.c_int, [_vp, C.c_int64, _dp, _dp, _dp, _dp, _dp]),
 }
 
 
@@ -386,3 +406,134 @@ class Engine1D:
         et = np.zeros(self.P); tr = np.zeros(self.P)
         check(self._L.imc1d_comb(self._h, _p(nm, _i64p), _p(ecen), _p(disc), _p(et), _p(tr)))
         return ecen, disc, et, tr
+
+
+class EngineMG:
+    """Device context for one multigroup 2-D mesh (wraps ``mgimc_ctx``).
+    Per-(group, cell) arrays are (G, nx, ny)."""
+
+    def __init__(self, edges1, edges2, energy_edges, geometry="xy", precision="fp64", seed=1, device=0):
+        L = load()
+        require_gpu()
+        self.e1 = np.ascontiguousarray(edges1, dtype=np.float64)
+        self.e2 = np.ascontiguousarray(edges2, dtype=np.float64)
+        self.edges = np.ascontiguousarray(energy_edges, dtype=np.float64)
+        self.nx, self.ny = self.e1.size - 1, self.e2.size - 1
+        self.G = self.edges.size - 1
+        self.shape = (self.nx, self.ny)
+        self.gshape = (self.G, self.nx, self.ny)
+        self.geometry = geometry
+        self.gcode = {"xy": GEOM_XY, "rz": GEOM_RZ}[geometry]
+        self.precision = {"fp64": FP64, "fp32": FP32_TRACK}[precision]
+        self.seed = int(seed)
+        h = _vp()
+        check(L.mgimc_create(C.byref(h), self.gcode, self.nx, self.ny, _p(self.e1), _p(self.e2), self.G,
+                             _p(self.edges), self.precision, self.seed, device))
+        self._h = h
+        self._L = L
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._L.mgimc_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_step(self, step):
+        check(self._L.mgimc_set_step(self._h, int(step) & 0xFFFFFFFF))
+
+    def set_seed(self, seed):
+        self.seed = int(seed)
+        check(self._L.mgimc_set_seed(self._h, self.seed))
+
+    def set_launch(self, grid=0, batch=256, tally_copies=8):
+        check(self._L.mgimc_set_launch(self._h, int(grid), int(batch), int(tally_copies)))
+
+    @property
+    def n(self):
+        return int(self._L.mgimc_census_size(self._h))
+
+    def put_census(self, w, d1, d2, t, p1, p2, ci, cj, g):
+        f = [np.ascontiguousarray(a, dtype=np.float64) for a in (w, d1, d2, t, p1, p2)]
+        ii = [np.ascontiguousarray(a, dtype=np.int64) for a in (ci, cj)]
+        gg = np.ascontiguousarray(g, dtype=np.int32)
+        check(self._L.mgimc_put_census(self._h, f[0].size, *[_p(a) for a in f], *[_p(a, _i64p) for a in ii],
+                                       _p(gg, _i32p)))
+
+    def get_census(self):
+        n = self.n
+        f = [np.empty(n) for _ in range(6)]
+        ii = [np.empty(n, dtype=np.int64) for _ in range(2)]
+        gg = np.empty(n, dtype=np.int32)
+        check(self._L.mgimc_get_census(self._h, n, *[_p(a) for a in f], *[_p(a, _i64p) for a in ii],
+                                       _p(gg, _i32p)))
+        return tuple(f) + tuple(ii) + (gg,)
+
+    def census_by_bin(self):
+        out = np.empty(self.gshape)
+        tot = C.c_double()
+        check(self._L.mgimc_census_by_bin(self._h, _p(out), C.byref(tot)))
+        return out, tot.value
+
+    def set_material(self, T, sigma, cv, theta, dt, emission_fractions=None, T_emit_floor=0.0, want=()):
+        """sigma: (G, nx, ny).  want: any of 'sa', 'ss', 'E', 'Eplanck'.
+        Returns (dict of requested arrays, emission energy total)."""
+        T = _cell(T, self.shape)
+        s = np.ascontiguousarray(np.broadcast_to(np.asarray(sigma, dtype=np.float64), self.gshape))
+        c = _cell(cv, self.shape)
+        fr = None if emission_fractions is None else np.ascontiguousarray(emission_fractions, dtype=np.float64)
+        out = {k: np.empty(self.gshape if k != "Eplanck" else self.shape) for k in want}
+        et = C.c_double()
+        check(self._L.mgimc_set_material(self._h, _p(T), _p(s), _p(c), float(theta), float(dt), _p(fr),
+                                         float(T_emit_floor), _p(out.get("sa")), _p(out.get("ss")), _p(out.get("E")),
+                                         C.byref(et), _p(out.get("Eplanck"))))
+        return out, et.value
+
+    def equilibrium_sample(self, N, Tr, T_emit_floor=0.0):
+        check(self._L.mgimc_equilibrium_sample(self._h, int(N), _p(_cell(Tr, self.shape)), float(T_emit_floor)))
+
+    def sample_boundary(self, side, n, E_total, dt, Tb=0.0, prof_cum=None, Tface=None, want_groups=False):
+        pc = None if prof_cum is None else np.ascontiguousarray(prof_cum, dtype=np.float64)
+        tf = None if Tface is None else np.ascontiguousarray(Tface, dtype=np.float64)
+        nface = 0 if pc is None else pc.size
+        eg = np.zeros(self.G) if want_groups else None
+        check(self._L.mgimc_sample_boundary(self._h, int(side), int(n), float(E_total), float(dt), float(Tb), _p(pc),
+                                            _p(tf), nface, _p(eg)))
+        return eg
+
+    def sample_source(self, N, S, dt):
+        S = np.ascontiguousarray(np.broadcast_to(np.asarray(S, dtype=np.float64), self.gshape))
+        tot = C.c_double()
+        check(self._L.mgimc_sample_source(self._h, int(N), _p(S), float(dt), C.byref(tot)))
+        return tot.value
+
+    def sample_emission(self, N, dt):
+        et = C.c_double()
+        check(self._L.mgimc_sample_emission(self._h, int(N), float(dt), C.byref(et)))
+        return et.value
+
+    def transport(self, dt, reflect, max_events, weight_floor=-1.0, want_tallies=True):
+        refl = np.array([1 if r else 0 for r in reflect], dtype=np.int32)
+        dep = np.empty(self.gshape) if want_tallies else None
+        si = np.empty(self.gshape) if want_tallies else None
+        bl = np.zeros(1 + self.G + 4)
+        st = np.zeros(8, dtype=np.int64)
+        check(self._L.mgimc_transport(self._h, float(dt), _p(refl, _i32p), int(max_events), float(weight_floor),
+                                      _p(dep), _p(si), _p(bl), _p(st, _i64p)))
+        return dep, si, bl, st
+
+    def last_transport_ms(self):
+        return float(self._L.mgimc_last_transport_ms(self._h))
+
+    def comb(self, Nmax, want_ecen=False, want_disc=False, want_post=False):
+        ecen = np.empty(self.gshape) if want_ecen else None
+        disc = np.empty(self.gshape) if want_disc else None
+        post = np.empty(self.gshape) if want_post else None
+        et = C.c_double()
+        tr = C.c_double()
+        check(self._L.mgimc_comb(self._h, int(Nmax), _p(ecen), _p(disc), _p(post), C.byref(et), C.byref(tr)))
+        return ecen, disc, post, et.value, tr.value

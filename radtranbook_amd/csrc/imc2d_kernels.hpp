@@ -226,13 +226,16 @@ __device__ __forceinline__ float scatter_distance_f32(const QuadF& u, float ss, 
   return s;
 }
 
+// CAP: apply the gray opacity cap (sigma_s * min cell width > 1e4); the
+// multigroup tracker has none.
+template <bool CAP = true>
 __device__ __forceinline__ void move_xy(Ctr ctr, Key key, uint32_t ev, double w, float ux, float uy, float x, float y,
                                         float xl, float xr, float yl, float yr, float sa, float ss, float dcen,
                                         MoveOut<float>& o) {
   // face distances by reciprocal-multiply (about 2 ulp; clamped at 0)
   float sx = (ux != 0.0f) ? fmaxf(__fdividef((ux > 0.0f ? xr : xl) - x, ux), 0.0f) : 1e30f;
   float sy = (uy != 0.0f) ? fmaxf(__fdividef((uy > 0.0f ? yr : yl) - y, uy), 0.0f) : 1e30f;
-  float cl = fminf(xr - xl, yr - yl);
+  float cl = CAP ? fminf(xr - xl, yr - yl) : 0.0f;
   QuadF u = (ss > 1e-12f) ? block4_f(ctr, key, ev) : QuadF{0.0f, 0.0f, 0.0f, 0.0f};
   float s_scat = scatter_distance_f32(u, ss, cl);
   float sb = fminf(sx, sy);
@@ -289,6 +292,7 @@ __device__ __forceinline__ float dist_inner_f32(float r, float mu, float R, floa
   return __fdividef(l, den);
 }
 
+template <bool CAP = true>
 __device__ __forceinline__ void move_rz(Ctr ctr, Key key, uint32_t ev, double w, float mu, float eta, float r,
                                         float z, float rin, float rout, float zlo, float zhi, float sa, float ss,
                                         float dcen, MoveOut<float>& o) {
@@ -304,7 +308,7 @@ __device__ __forceinline__ void move_rz(Ctr ctr, Key key, uint32_t ev, double w,
     s_zhi = fmaxf(__fdividef(zhi - z, eta), 0.0f);
   else if (eta < 0.0f)
     s_zlo = fmaxf(__fdividef(zlo - z, eta), 0.0f);
-  float cl = fminf(rout - rin, zhi - zlo);
+  float cl = CAP ? fminf(rout - rin, zhi - zlo) : 0.0f;
   QuadF u = (ss > 1e-12f) ? block4_f(ctr, key, ev) : QuadF{0.0f, 0.0f, 0.0f, 0.0f};
   float s_scat = scatter_distance_f32(u, ss, cl);
   float sr = fminf(s_rin, s_rout), sz = fminf(s_zlo, s_zhi);
