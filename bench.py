@@ -283,12 +283,135 @@ def main_1d(args, workload):
     print(json.dumps(out), flush=True)
 
 
+# ---------------------------------------------------------------------------
+# multigroup 2-D workload: the crooked pipe
+# ---------------------------------------------------------------------------
+BYTES_PER_SEGMENT_MG = {"fp32": 2 * 44 + 8 + 16, "fp64": 2 * 64 + 16 + 16}
+
+
+def run_gpu_mg(prob, precision, steps, warmup, n_particles):
+    from radtranbook_amd import MG_IMC2D
+
+    MG_IMC2D.VERBOSE = False
+    T0 = np.full(prob.shape, prob.T0)
+    zeros = np.zeros(prob.shape)
+    sf = prob.sigma_funcs()
+    engine = [None]
+
+    def one_run(seed, nsteps, timed):
+        st = MG_IMC2D.init_simulation(n_particles // 2, T0, T0, prob.edges1, prob.edges2, prob.energy_edges, prob.eos,
+                                      prob.inv_eos, geometry=prob.geometry, precision=precision, seed=seed,
+                                      engine=engine[0])
+        engine[0] = st._engine
+        infos = []
+        t0 = time.perf_counter()
+        for s in range(nsteps):
+            ts = time.perf_counter()
+            st, info = MG_IMC2D.step(st, 0, 0, 0, n_particles, (0.0, 0.0, 0.0, 0.0), prob.dt, prob.edges1, prob.edges2,
+                                     prob.energy_edges, sf, prob.inv_eos, prob.cv_func, zeros,
+                                     boundary_source_func=prob.boundary_source_func, Ntotal=n_particles,
+                                     particle_budget_fmin=0.1)
+            infos.append(info)
+            if timed:
+                te = info["profiling"]["transport_events"]
+                log(f"  [{precision}] step {s + 1}/{nsteps}: {(time.perf_counter() - ts) * 1e3:.1f} ms, transport "
+                    f"kernel {info['profiling']['transport_kernel_ms']:.2f} ms, segments {te['total']:.3e}, "
+                    f"histories {te['n_particles_transported']:.3e}, split {info['N_boundary']}/{info['N_target']}")
+        return infos, time.perf_counter() - t0
+
+    if warmup > 0:
+        one_run(1000, warmup, False)
+    infos, wall = one_run(1, steps, True)
+    engine[0].close()
+    segs = sum(i["profiling"]["transport_events"]["total"] for i in infos)
+    hist = sum(i["profiling"]["transport_events"]["n_particles_transported"] for i in infos)
+    kern = np.array([i["profiling"]["transport_kernel_ms"] for i in infos])
+    seg_launch = np.array([i["profiling"]["transport_events"]["total"] for i in infos], dtype=np.float64)
+    resid = max(abs(i["balance_residual"]) / i["total_energy"] for i in infos)
+    return dict(wall=wall, segments=segs, histories=hist, kernel_ms=kern, seg_per_launch=seg_launch, residual=resid,
+                final_T=infos[-1]["temperature"])
+
+
+def run_cpu_mg(prob, n_particles, nsteps):
+    from oracle import oracle_mg as OM
+
+    OM.O.build()
+    orc = OM.OracleMG2D(prob.edges1, prob.edges2, prob.energy_edges, prob.geometry, seed=1)
+    T0 = np.full(prob.shape, prob.T0)
+    st = orc.init_simulation(n_particles // 2, T0, T0, prob.eos, prob.inv_eos)
+    sf = prob.sigma_funcs()
+    segs, tt = 0, 0.0
+    for s in range(nsteps):
+        t0 = time.perf_counter()
+        st, info = orc.step(st, 0, 0, 0, n_particles, (0.0, 0.0, 0.0, 0.0), prob.dt, sf, prob.inv_eos, prob.cv_func,
+                            np.zeros(prob.shape), boundary_source_func=prob.boundary_source_func,
+                            Ntotal=n_particles, particle_budget_fmin=0.1)
+        tt += time.perf_counter() - t0
+        segs += int(info["stats"][0])
+        log(f"  [cpu] step {s + 1}/{nsteps}: {time.perf_counter() - t0:.2f} s, segments {int(info['stats'][0]):.3e}")
+    return dict(value=segs / tt if tt > 0 else None, cores=1, seconds=tt, segments=segs)
+
+
+def main_mg(args):
+    from radtranbook_amd import _lib
+    from radtranbook_amd.problems import crooked_pipe_mg
+
+    _lib.load()
+    _lib.require_gpu()
+    steps = args.steps if args.steps is not None else 20
+    warmup = args.warmup if args.warmup is not None else 2
+    npart = int(args.particles) if args.particles is not None else 20_000_000
+    prob = crooked_pipe_mg(n_particles=npart, n_steps=steps)
+    precs = ["fp32", "fp64"] if args.precision == "both" else [args.precision]
+    res = {p: run_gpu_mg(prob, p, steps, warmup, npart) for p in precs}
+    head = res[precs[0]]
+    value = head["segments"] / head["wall"]
+    achieved = head["seg_per_launch"].mean() * BYTES_PER_SEGMENT_MG[precs[0]] / (head["kernel_ms"].mean() / 1e3) / 1e9
+    out = {
+        "metric": "IMC segments/s, multigroup 2D crooked pipe 240x240 x 10 groups, 2e7 particles/step",
+        "value": value, "unit": "segments/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
+        "ms_per_step": head["wall"] / steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 geometry / f64 weights+tallies" if precs[0] == "fp32" else "f64",
+        "data": "synthetic (seeded channel polyline, Philox particle streams)",
+        "config": {"workload": f"multigroup gray-body IMC 2D xy crooked pipe, 240x240 cells on [0,2]^2, 10 groups "
+                               f"logspace(-4,1), {npart:.0e} particles/step (energy-proportional split, fmin 0.1), "
+                               f"dt=0.002 ns x {steps} steps", "cells": 240 * 240, "groups": 10,
+                   "particles_per_step": npart},
+        "histories_per_s": head["histories"] / head["wall"],
+        "transport_segments_per_s": head["segments"] / (head["kernel_ms"].sum() / 1e3),
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": f"k_mg_transport<{'float' if precs[0] == 'fp32' else 'double'},xy>",
+                     "bytes_per_segment": BYTES_PER_SEGMENT_MG[precs[0]]},
+    }
+    for p in precs:
+        r = res[p]
+        out[p] = {"segments_per_s": r["segments"] / r["wall"], "ms_per_step": r["wall"] / steps * 1e3,
+                  "transport_kernel_ms_per_step": float(r["kernel_ms"].mean()),
+                  "segments_per_step": float(r["seg_per_launch"].mean()), "histories_per_step": r["histories"] / steps,
+                  "max_balance_residual_rel": r["residual"]}
+    if "fp32" in res and "fp64" in res:
+        a, b = res["fp32"]["final_T"], res["fp64"]["final_T"]
+        out["fp32_vs_fp64_final_T_max_rel"] = float(np.max(np.abs(a - b) / b))
+    if not args.no_cpu:
+        ncpu = int(args.cpu_particles) if args.cpu_particles is not None else 2_000_000
+        nst = args.cpu_steps if args.cpu_steps is not None else 3
+        cb = run_cpu_mg(prob, ncpu, nst)
+        out["cpu_baseline"] = {"value": cb["value"], "unit": "segments/s", "cores": 1, "kind": "port",
+                               "sample": f"FP64 C oracle (serial, like the reference's MG tracker) on the same "
+                                         f"problem at {ncpu:.0e} particles/step, {nst} steps "
+                                         f"({cb['segments']:.3e} segments in {cb['seconds']:.1f} s)"}
+        if cb["value"]:
+            out["gpu_over_cpu_segments_per_s"] = value / cb["value"]
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=["marshak2d", "slab1d", "batch1d"], default="marshak2d",
+    ap.add_argument("--workload", choices=["marshak2d", "slab1d", "batch1d", "mg2d"], default="marshak2d",
                     help="marshak2d: 2D 200x200 heterogeneous (default, headline); slab1d: the reference's 1D slab "
-                         "case; batch1d: 256 independent slabs")
+                         "case; batch1d: 256 independent slabs; mg2d: multigroup crooked pipe")
     ap.add_argument("--problems", type=int, default=256)
     ap.add_argument("--tune", action=argparse.BooleanOptionalAction, default=True,
                     help="autotune the 1D transport launch shape on the first warm-up step")
@@ -296,11 +419,11 @@ def main():
     ap.add_argument("--cpu-scale", type=float, default=0.1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
-    ap.add_argument("--particles", type=float, default=1e7)
+    ap.add_argument("--particles", type=float, default=None)
     ap.add_argument("--cells", type=int, default=200)
     ap.add_argument("--fom-seeds", type=int, default=8)
     ap.add_argument("--precision", choices=["both", "fp32", "fp64"], default="both")
-    ap.add_argument("--cpu-particles", type=float, default=4e6)
+    ap.add_argument("--cpu-particles", type=float, default=None)
     ap.add_argument("--cpu-steps", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -310,6 +433,9 @@ def main():
               "(the IMC step shards by particles with one tally all-reduce per step; that path is not built)",
               file=sys.stderr)
         sys.exit(2)
+    if args.workload == "mg2d":
+        main_mg(args)
+        return
     if args.workload != "marshak2d":
         main_1d(args, args.workload)
         return
@@ -319,6 +445,10 @@ def main():
         args.warmup = 2
     if args.cpu_steps is None:
         args.cpu_steps = 8
+    if args.particles is None:
+        args.particles = 1e7
+    if args.cpu_particles is None:
+        args.cpu_particles = 4e6
 
     from radtranbook_amd import _lib
     from radtranbook_amd.problems import marshak_2d_heterogeneous

@@ -124,12 +124,13 @@ def _face_profile(geometry, side, dt, e1, e2, bfunc):
     return np.cumsum(em), Tc
 
 
-def _estimate_boundary_emission_energy(geometry, e1, e2, Tb4, dt, bfunc):
-    """MG_IMC2D.py:1546-1632, including its xy left/right area of 2 (z1 - z0)."""
+def _estimate_boundary_emission_energy(geometry, e1, e2, Tb4, dt, bfunc, profiles=None):
+    """MG_IMC2D.py:1546-1632, including its xy left/right area of 2 (z1 - z0).
+    profiles: the four _face_profile results when already evaluated."""
     E = 0.0
     for side in range(4):
         if bfunc is not None:
-            prof = _face_profile(geometry, side, dt, e1, e2, bfunc)
+            prof = profiles[side] if profiles is not None else _face_profile(geometry, side, dt, e1, e2, bfunc)
             if prof is None:
                 continue
             e = float(prof[0][-1])
@@ -202,10 +203,13 @@ def step(state, Ntarget, Nboundary, Nsource, Nmax, T_boundary, dt, edges1, edges
     emission_by_group = mat["E"]
 
     Tb4 = [_boundary_temperature_value(T_boundary[s], state.time) for s in range(4)]
+    profiles = None
+    if boundary_source_func is not None:
+        profiles = [_face_profile(geometry, side, dt, e1, e2, boundary_source_func) for side in range(4)]
     E_bc_est = float("nan")
     E_mat_est = float("nan")
     if Ntotal > 0:
-        E_bc_est = _estimate_boundary_emission_energy(geometry, e1, e2, Tb4, dt, boundary_source_func)
+        E_bc_est = _estimate_boundary_emission_energy(geometry, e1, e2, Tb4, dt, boundary_source_func, profiles)
         cell_em = mat["Eplanck"]
         if Ntotal_T_floor > 0.0:
             cell_em = np.where(temperature > Ntotal_T_floor, cell_em, 0.0)
@@ -225,7 +229,7 @@ def step(state, Ntarget, Nboundary, Nsource, Nmax, T_boundary, dt, edges1, edges
         for side in range(4):
             prof = Tface = None
             if boundary_source_func is not None:
-                pr = _face_profile(geometry, side, dt, e1, e2, boundary_source_func)
+                pr = profiles[side]
                 if pr is None:
                     continue
                 prof, Tface = pr

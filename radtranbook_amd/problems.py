@@ -14,6 +14,14 @@
   (half emission, half from the hot face, census combed to that count),
   sigma0 ~ U(10, 1000) with sigma_a = sigma0 max(T,1e-3)^-3, c_v ~ U(0.05, 0.5),
   T_b ~ U(0.1, 1) keV on the left face, T0 = 1e-3 keV, dt = 0.01 ns x 10.
+* ``crooked_pipe_mg`` -- multigroup 2-D xy, 240x240 cells on [0,2]^2, 10 groups
+  with edges logspace(-4, 1, 11) keV and nu_g = sqrt(lo hi).  The thin
+  channel is every cell within 0.05 cm of a polyline from (0, y0) to (2, y5)
+  through x = 0.4, 0.8, 1.2, 1.6 with all six y ~ U(0.3, 1.7) (seed 1):
+  sigma_g = 0.2 cm^-1, c_v = 0.01.  Elsewhere sigma_g = 200 (nu_g/T)^-3 T^-0.5,
+  c_v = 0.3.  Inlet: the left-face cells inside the channel at 0.5 keV;
+  T0 = 0.05 keV; dt = 0.002 ns x 20; 2e7 particles per step split between the
+  inlet and the material in proportion to their energies (fmin 0.1).
 """
 from __future__ import annotations
 
@@ -166,3 +174,88 @@ def batch_slabs(P=256, seed=1, n_steps=10, scale=1.0):
         opacity=PowerLawOpacity(np.full(int(cells[p]), sig0[p]), 3.0, 1e-3), eos=LinearEOS(float(cv[p])), T0=1e-3,
         T_boundary=(float(Tb[p]), 0.0), reflect=(False, False), dt=0.01, n_steps=n_steps,
         n_particles=int(parts[p])) for p in range(P)]
+
+
+@dataclass
+class MGProblem:
+    name: str
+    edges1: np.ndarray
+    edges2: np.ndarray
+    energy_edges: np.ndarray
+    geometry: str
+    thin: np.ndarray          # (nx, ny) bool
+    cv: np.ndarray            # (nx, ny)
+    T0: float
+    dt: float
+    n_steps: int
+    n_particles: int
+    inlet: tuple              # (y_lo, y_hi) of the left-face inlet
+    T_inlet: float
+
+    @property
+    def shape(self):
+        return (self.edges1.size - 1, self.edges2.size - 1)
+
+    @property
+    def G(self):
+        return self.energy_edges.size - 1
+
+    def sigma_funcs(self):
+        """One callable per group.  200 (nu/T)^-3 T^-0.5 = 200 nu^-3 T^2.5; the
+        T^2.5 factor is shared by the groups of one call sequence."""
+        nu = np.sqrt(self.energy_edges[:-1] * self.energy_edges[1:])
+        thin = self.thin
+        cache = {}
+
+        def t25(T):
+            if cache.get("T") is not T:
+                Tc = np.maximum(T, 1e-3)
+                cache["T"] = T
+                cache["v"] = Tc * Tc * np.sqrt(Tc)
+            return cache["v"]
+
+        def make(g):
+            c = 200.0 * nu[g] ** -3
+
+            def f(T):
+                return np.where(thin, 0.2, c * t25(T))
+            return f
+        return [make(g) for g in range(self.G)]
+
+    def eos(self, T):
+        return self.cv * T
+
+    def inv_eos(self, u):
+        return u / self.cv
+
+    def cv_func(self, T):
+        return self.cv + 0.0 * T
+
+    def boundary_source_func(self, x, y, side):
+        lo, hi = self.inlet
+        return self.T_inlet if (side == "left" and lo <= y <= hi) else 0.0
+
+
+def _dist_to_polyline(px, py, vx, vy):
+    d = np.full(px.shape, np.inf)
+    for k in range(len(vx) - 1):
+        ax, ay, bx, by = vx[k], vy[k], vx[k + 1], vy[k + 1]
+        ex, ey = bx - ax, by - ay
+        t = np.clip(((px - ax) * ex + (py - ay) * ey) / (ex * ex + ey * ey), 0.0, 1.0)
+        d = np.minimum(d, np.hypot(px - (ax + t * ex), py - (ay + t * ey)))
+    return d
+
+
+def crooked_pipe_mg(n=240, n_particles=20_000_000, n_steps=20, seed=1):
+    e = np.linspace(0.0, 2.0, n + 1)
+    c = 0.5 * (e[:-1] + e[1:])
+    rng = np.random.default_rng(seed)
+    vy = rng.uniform(0.3, 1.7, 6)
+    vx = np.array([0.0, 0.4, 0.8, 1.2, 1.6, 2.0])
+    X, Y = np.meshgrid(c, c, indexing="ij")
+    thin = _dist_to_polyline(X, Y, vx, vy) <= 0.05
+    cv = np.where(thin, 0.01, 0.3)
+    return MGProblem(name=f"crooked_pipe_mg_{n}x{n}", edges1=e, edges2=e.copy(),
+                     energy_edges=np.logspace(-4, 1, 11), geometry="xy", thin=thin, cv=cv, T0=0.05, dt=0.002,
+                     n_steps=n_steps, n_particles=int(n_particles), inlet=(vy[0] - 0.05, vy[0] + 0.05),
+                     T_inlet=0.5)
