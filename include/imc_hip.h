@@ -129,6 +129,26 @@ float imc2d_last_transport_ms(imc2d_ctx* ctx);
 int imc2d_comb(imc2d_ctx* ctx, int64_t Nmax, double* ecen_out, double* disc_out, double* ecen_total_out,
                double* total_rad_out);
 
+/* ---- differentiable gray 2-D step ----------------------------------------- */
+/* No counterpart in the reference (its tracker is not differentiable): the
+ * adjoint of the transport tallies with respect to the cell opacities, by
+ * replaying every history of the transport's input pool from its Philox
+ * counter.  imc2d_enable_adjoint(1) makes each imc2d_transport keep its
+ * input pool; imc2d_adjoint then maps lam = dL/dE_abs (per cell, E_abs the
+ * absorbed energy of the cell) to dL/dsigma_a and dL/dsigma_s per cell
+ * (pathwise absorption terms, score-function terms of the scatter sampling,
+ * emission weights proportional to sigma_a).  Same dt, reflect and
+ * max_events as the transport; no imc2d_set_material in between. */
+int imc2d_enable_adjoint(imc2d_ctx* ctx, int enable);
+int imc2d_adjoint(imc2d_ctx* ctx, double dt, const int32_t* reflect, int64_t max_events, const double* lam,
+                  double* gA_out, double* gS_out);
+/* Emission counts per cell: out (nullable) receives those of the last
+ * emission sample; fix (nullable) pins them for later samples (common random
+ * numbers across perturbed runs); both null returns to the ceil rule. */
+int imc2d_emission_counts(imc2d_ctx* ctx, int64_t* out, const int64_t* fix);
+/* Copy the census into a device snapshot (save != 0) or back from it. */
+int imc2d_snapshot(imc2d_ctx* ctx, int save);
+
 /* ---- gray 1-D (slab / sphere), batched ------------------------------------ */
 /* One context holds P independent problems on meshes of ncells[p] cells
  * (edges: P concatenated arrays of ncells[p]+1 ascending edges).  Particles

@@ -397,7 +397,7 @@ static double orc_dist_radial(double r, double mu, double eta, double R) {
 }
 
 typedef struct {
-  double w, d1, d2, p1, p2, dep_w, dep_i, dcen;
+  double w, d1, d2, p1, p2, dep_w, dep_i, dcen, s;
   int crossed, evt;
 } orc_move_out;
 
@@ -438,11 +438,12 @@ static void orc_move_xy(orc_stream* st, uint32_t ev, double w, double ux, double
   double se = sb;
   if (s_scat < se) se = s_scat;
   if (dcen < se) se = dcen;
-  o->d1 = ux; o->d2 = uy; o->p1 = x; o->p2 = y; o->crossed = X_NONE;
+  o->d1 = ux; o->d2 = uy; o->p1 = x; o->p2 = y; o->crossed = X_NONE; o->s = 0.0;
   if (se <= 0.0) {
     o->w = w; o->dep_w = 0.0; o->dep_i = 0.0; o->dcen = 0.0; o->evt = E_CENSUS;
     return;
   }
+  o->s = se;
   double xn = x + ux * se, yn = y + uy * se;
   int evt = E_CENSUS;
   if (fabs(se - sb) < 1e-11) {
@@ -488,11 +489,12 @@ static void orc_move_rz(orc_stream* st, uint32_t ev, double w, double mu, double
   double se = sb;
   if (s_scat < se) se = s_scat;
   if (dcen < se) se = dcen;
-  o->d1 = mu; o->d2 = eta; o->p1 = r; o->p2 = z; o->crossed = X_NONE;
+  o->d1 = mu; o->d2 = eta; o->p1 = r; o->p2 = z; o->crossed = X_NONE; o->s = 0.0;
   if (se <= 0.0) {
     o->w = w; o->dep_w = 0.0; o->dep_i = 0.0; o->dcen = 0.0; o->evt = E_CENSUS;
     return;
   }
+  o->s = se;
   double ome2 = 1.0 - eta * eta;
   if (ome2 < 0.0) ome2 = 0.0;
   double l = se * sqrt(ome2);
@@ -640,6 +642,126 @@ void orc_transport_2d(int64_t n, double* w, double* d1, double* d2, const double
   free(siT);
   free(blT);
   free(stT);
+}
+
+/* ------------------------------------------------------------------------- */
+/* Adjoint of the transport tallies (the differentiable step).                */
+/* For adjoint weights lam (per cell, dL/dE_abs) it returns the derivatives   */
+/* of  Lambda = sum over segments lam[cell] * dep  with respect to sigma_a    */
+/* and sigma_s of every cell, by replaying each history twice from its input  */
+/* state with the same streams:                                               */
+/*  - continuous absorption (pathwise): segment j in cell c with length s_j   */
+/*    gives gA[c] += -s_j (Lambda - P_j) + lam[c] w_{j+1} s_j, P_j = deposits */
+/*    of segments <= j weighted by lam;                                       */
+/*  - scatter sampling (score function): segment j gives                      */
+/*    gS[c] += ([scatter] / sigma_s - s_j) (Lambda - P_{j-1}), zero where     */
+/*    sigma_s <= 1e-12 or the opacity cap fixes the sampling rate;            */
+/*  - emission weights proportional to sigma_a of the birth cell: particles   */
+/*    k in [emit_lo, emit_hi) add Lambda / sigma_a[birth] to gA[birth].       */
+/* The input arrays are the transport's input pool (unchanged).               */
+/* ------------------------------------------------------------------------- */
+static double orc_replay(int pass, int64_t k, double w0, double u1, double u2, const double* times, double x,
+                         double y, int i, int j, const double* e1, const double* e2, int nx, int ny,
+                         const double* sigma_a, const double* sigma_s, double dt, const int32_t* reflect,
+                         int64_t max_events, int32_t geometry, double weight_floor, uint64_t seed, uint32_t step,
+                         const double* lam, double Lam, double* gA, double* gS) {
+  orc_stream st;
+  orc_stream_init(&st, seed, ORC_TAG_TRANSPORT, (uint64_t)k, step);
+  double dcen = (dt - times[k]) * ORC_C;
+  double w = w0, P = 0.0;
+  int64_t events = 0;
+  while (dcen > 0.0) {
+    events += 1;
+    if (events > max_events) break;
+    if (i < 0 || i >= nx || j < 0 || j >= ny) break;
+    int64_t c = (int64_t)i * ny + j;
+    double sa = sigma_a[c], ss = sigma_s[c];
+    orc_move_out o;
+    uint32_t ev = (uint32_t)(events - 1);
+    if (geometry == 0)
+      orc_move_xy(&st, ev, w, u1, u2, x, y, i, j, e1, e2, sa, ss, dcen, &o);
+    else
+      orc_move_rz(&st, ev, w, u1, u2, x, y, i, j, e1, e2, sa, ss, dcen, &o);
+    dcen = o.dcen;
+    double Pprev = P;
+    P += lam[c] * o.dep_w;
+    if (pass == 1) {
+      double xl, xr, yl, yr;
+      xl = e1[i]; xr = e1[i + 1]; yl = e2[j]; yr = e2[j + 1];
+      double cl = (xr - xl) < (yr - yl) ? (xr - xl) : (yr - yl);
+      gA[c] += -o.s * (Lam - P) + lam[c] * o.w * o.s;
+      if (ss > 1e-12 && !(ss * cl > 10000.0))
+        gS[c] += ((o.evt == E_SCATTER ? 1.0 / ss : 0.0) - o.s) * (Lam - Pprev);
+    }
+    w = o.w;
+    if (o.w <= weight_floor) break;
+    u1 = o.d1; u2 = o.d2; x = o.p1; y = o.p2;
+    if (o.crossed != X_NONE) {
+      if (o.crossed == X_IP) i += 1;
+      else if (o.crossed == X_IM) i -= 1;
+      else if (o.crossed == X_JP) j += 1;
+      else j -= 1;
+      if (i < 0) {
+        if (reflect[0]) { u1 = -u1; i = 0; } else { w = 0.0; dcen = 0.0; }
+      } else if (i >= nx) {
+        if (reflect[1]) { u1 = -u1; i = nx - 1; } else { w = 0.0; dcen = 0.0; }
+      } else if (j < 0) {
+        if (reflect[2]) { u2 = -u2; j = 0; } else { w = 0.0; dcen = 0.0; }
+      } else if (j >= ny) {
+        if (reflect[3]) { u2 = -u2; j = ny - 1; } else { w = 0.0; dcen = 0.0; }
+      }
+    }
+    if (w <= 1e-300) {
+      w = 0.0;
+      dcen = 0.0;
+    }
+  }
+  return P;
+}
+
+void orc_transport_2d_adjoint(int64_t n, const double* w, const double* d1, const double* d2, const double* times,
+                              const double* p1, const double* p2, const int32_t* ci, const int32_t* cj,
+                              const double* e1, const double* e2, int nx, int ny, const double* sigma_a,
+                              const double* sigma_s, double dt, const int32_t* reflect, int64_t max_events,
+                              int32_t geometry, double weight_floor, uint64_t seed, uint32_t step, const double* lam,
+                              int64_t emit_lo, int64_t emit_hi, double* gA, double* gS, double* Lam_out,
+                              int32_t nthreads) {
+  int nt = 1;
+#ifdef _OPENMP
+  nt = nthreads > 0 ? nthreads : omp_get_max_threads();
+#endif
+  int64_t ncell = (int64_t)nx * ny;
+  double* aT = (double*)calloc((size_t)nt * (size_t)ncell, sizeof(double));
+  double* sT = (double*)calloc((size_t)nt * (size_t)ncell, sizeof(double));
+#pragma omp parallel for schedule(dynamic, 256) num_threads(nt)
+  for (int64_t k = 0; k < n; ++k) {
+    int tid = 0;
+#ifdef _OPENMP
+    tid = omp_get_thread_num();
+#endif
+    double* ga = aT + (size_t)tid * (size_t)ncell;
+    double* gs = sT + (size_t)tid * (size_t)ncell;
+    double Lam = orc_replay(0, k, w[k], d1[k], d2[k], times, p1[k], p2[k], ci[k], cj[k], e1, e2, nx, ny, sigma_a,
+                            sigma_s, dt, reflect, max_events, geometry, weight_floor, seed, step, lam, 0.0, ga, gs);
+    orc_replay(1, k, w[k], d1[k], d2[k], times, p1[k], p2[k], ci[k], cj[k], e1, e2, nx, ny, sigma_a, sigma_s, dt,
+               reflect, max_events, geometry, weight_floor, seed, step, lam, Lam, ga, gs);
+    if (k >= emit_lo && k < emit_hi && ci[k] >= 0 && ci[k] < nx && cj[k] >= 0 && cj[k] < ny) {
+      int64_t c = (int64_t)ci[k] * ny + cj[k];
+      if (sigma_a[c] > 0.0) ga[c] += Lam / sigma_a[c];
+    }
+    if (Lam_out) Lam_out[k] = Lam;
+  }
+  for (int64_t c = 0; c < ncell; ++c) {
+    double a = 0.0, b = 0.0;
+    for (int th = 0; th < nt; ++th) {
+      a += aT[(size_t)th * ncell + c];
+      b += sT[(size_t)th * ncell + c];
+    }
+    gA[c] = a;
+    gS[c] = b;
+  }
+  free(aT);
+  free(sT);
 }
 
 /* ------------------------------------------------------------------------- */

@@ -398,6 +398,32 @@ class Oracle2D:
         return state, info
 
 
+def transport_2d_adjoint(pool, e1, e2, geometry, sigma_a, sigma_s, dt, reflect, max_events, weight_floor, seed, step,
+                         lam, emit_lo=0, emit_hi=0, nthreads=0):
+    """Replay adjoint of transport_2d on the same input pool (unchanged):
+    returns (dL/dsigma_a, dL/dsigma_s) per cell (nx, ny) and Lambda per
+    history, for lam = dL/dE_abs per cell."""
+    L = lib()
+    if not hasattr(L, "_adj_ready"):
+        L.orc_transport_2d_adjoint.argtypes = [C.c_int64, _dp, _dp, _dp, _dp, _dp, _dp, _i32p, _i32p, _dp, _dp,
+                                               C.c_int, C.c_int, _dp, _dp, C.c_double, _i32p, C.c_int64, C.c_int32,
+                                               C.c_double, C.c_uint64, C.c_uint32, _dp, C.c_int64, C.c_int64, _dp,
+                                               _dp, _dp, C.c_int32]
+        L._adj_ready = True
+    e1 = _f64(e1); e2 = _f64(e2); nx, ny = e1.size - 1, e2.size - 1
+    gcode = 0 if geometry in ("xy", 0) else 1
+    ci = np.ascontiguousarray(pool.ci, dtype=np.int32); cj = np.ascontiguousarray(pool.cj, dtype=np.int32)
+    gA = np.zeros(nx * ny); gS = np.zeros(nx * ny); Lam = np.zeros(len(pool))
+    refl = np.array([1 if r else 0 for r in reflect], dtype=np.int32)
+    L.orc_transport_2d_adjoint(len(pool), _p(_f64(pool.w)), _p(_f64(pool.d1)), _p(_f64(pool.d2)), _p(_f64(pool.t)),
+                               _p(_f64(pool.p1)), _p(_f64(pool.p2)), _p(ci, _i32p), _p(cj, _i32p), _p(e1), _p(e2),
+                               nx, ny, _p(_f64(sigma_a).ravel()), _p(_f64(sigma_s).ravel()), float(dt),
+                               _p(refl, _i32p), int(max_events), gcode, float(weight_floor), int(seed), int(step),
+                               _p(_f64(lam).ravel()), int(emit_lo), int(emit_hi), _p(gA), _p(gS), _p(Lam),
+                               int(nthreads))
+    return gA.reshape(nx, ny), gS.reshape(nx, ny), Lam
+
+
 # ----------------------------------------------------------------------------
 # gray 1-D (slab / sphere), IMC/IMC1D.py:859-1124
 # ----------------------------------------------------------------------------

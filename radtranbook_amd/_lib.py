@@ -51,6 +51,10 @@ SIGNATURES = {
     "imc2d_transport": (C.c_int, [_vp, C.c_double, _i32p, C.c_int64, C.c_double, _dp, _dp, _dp, _i64p]),
     "imc2d_last_transport_ms": (C.c_float, [_vp]),
     "imc2d_comb": (C.c_int, [_vp, C.c_int64, _dp, _dp, _dp, _dp]),
+    "imc2d_enable_adjoint": (C.c_int, [_vp, C.c_int]),
+    "imc2d_adjoint": (C.c_int, [_vp, C.c_double, _i32p, C.c_int64, _dp, _dp, _dp]),
+    "imc2d_emission_counts": (C.c_int, [_vp, _i64p, _i64p]),
+    "imc2d_snapshot": (C.c_int, [_vp, C.c_int]),
     "imc1d_create": (C.c_int, [C.POINTER(_vp), C.c_int, C.c_int, _i32p, _dp, C.c_int, C.c_uint64, C.c_int]),
     "imc1d_destroy": (None, [_vp]),
     "imc1d_set_step": (C.c_int, [_vp, C.c_uint32]),
@@ -255,6 +259,36 @@ class Engine2D:
         tr = C.c_double()
         check(self._L.imc2d_comb(self._h, int(Nmax), _p(ecen), _p(disc), C.byref(et), C.byref(tr)))
         return ecen, disc, et.value, tr.value
+
+    # -- differentiable step ----------------------------------------------------
+    def enable_adjoint(self, on=True):
+        check(self._L.imc2d_enable_adjoint(self._h, 1 if on else 0))
+
+    def adjoint(self, dt, reflect, max_events, lam):
+        """dL/dsigma_a, dL/dsigma_s per cell for lam = dL/dE_abs per cell."""
+        refl = np.array([1 if r else 0 for r in reflect], dtype=np.int32)
+        lam = _cell(lam, self.shape)
+        gA = np.empty(self.shape)
+        gS = np.empty(self.shape)
+        check(self._L.imc2d_adjoint(self._h, float(dt), _p(refl, _i32p), int(max_events), _p(lam), _p(gA), _p(gS)))
+        return gA, gS
+
+    def emission_counts(self, fix=None):
+        """Counts of the last emission sample; fix: pin counts (array) or
+        unpin (False)."""
+        if fix is False:
+            check(self._L.imc2d_emission_counts(self._h, None, None))
+            return None
+        if fix is not None:
+            f = np.ascontiguousarray(np.broadcast_to(np.asarray(fix, dtype=np.int64), self.shape))
+            check(self._L.imc2d_emission_counts(self._h, None, _p(f, _i64p)))
+            return None
+        out = np.empty(self.shape, dtype=np.int64)
+        check(self._L.imc2d_emission_counts(self._h, _p(out, _i64p), None))
+        return out
+
+    def snapshot(self, save=True):
+        check(self._L.imc2d_snapshot(self._h, 1 if save else 0))
 
 
 def _per(a, n, dtype=np.float64):

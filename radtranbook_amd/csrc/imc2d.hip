@@ -602,6 +602,216 @@ __global__ void k_finalize_tallies(int64_t n, int copies, const double* __restri
 }
 
 // ---------------------------------------------------------------------------
+// adjoint of the transport tallies: every history of the transport's input
+// pool is replayed twice with its own Philox stream (recompute, no stored
+// paths).  Pass 0 accumulates Lambda = sum_j lam[c_j] dep_j; pass 1 adds, per
+// segment j in cell c of length s_j,
+//   gA[c] += -s_j (Lambda - P_j) + lam[c] w_{j+1} s_j        (absorption)
+//   gS[c] += ([scatter]/sigma_s - s_j) (Lambda - P_{j-1})   (scatter sampling)
+// with P_j the lam-weighted deposits of segments <= j; the score term is zero
+// where sigma_s <= 1e-12 or the opacity cap fixes the sampling rate.
+// Emission-born particles (pool index in [elo, ehi)) add Lambda / sigma_a of
+// their birth cell to gA (their weight is proportional to it).
+// gA, gS are derivatives of sum_c lam_c E_abs,c with respect to the cell
+// opacities sigma_a, sigma_s.
+// ---------------------------------------------------------------------------
+template <typename R>
+struct AdjointArgs {
+  using R2 = typename Vec2<R>::type;
+  PoolView<R> S;  // transport input pool (read only)
+  int64_t n;
+  const R2* e1p;
+  const R2* e2p;
+  int nx, ny;
+  const R2* sas;
+  const double* sa64;  // emission weights use the FP64 sigma_a
+  double dt;
+  int reflect;
+  int64_t max_events;
+  const double* floor_ptr;
+  Key key;
+  uint32_t step;
+  unsigned long long* next;
+  int batch;
+  const double* lam;
+  int64_t elo, ehi;
+  double* g_raw;  // copies x 2 ncell: gA then gS
+  int64_t ncell;
+  int copy_mask;
+};
+
+template <typename R, int GEOM>
+__global__ __launch_bounds__(256) void k_transport_adjoint(AdjointArgs<R> A) {
+  using R2 = typename Vec2<R>::type;
+  const double wfloor = *A.floor_ptr;
+  double* const g_t = A.g_raw + (int64_t)(blockIdx.x & A.copy_mask) * 2 * A.ncell;
+  __shared__ int lkey[kLdsSlots];
+  __shared__ double lval[kLdsSlots];
+  lds_tally_init(lkey, lval);
+  __syncthreads();
+  const int ny = A.ny, nx = A.nx;
+  const int ncell = (int)A.ncell;
+  int64_t qbase = 0, qend = 0;
+  bool exhausted = false, active = false;
+  int64_t k = 0;
+  int phase = 0;
+  double w = 0.0, Lam = 0.0, P = 0.0;
+  R d1 = 0, d2 = 0, p1 = 0, p2 = 0, dcen = 0;
+  int i = 0, j = 0;
+  uint32_t ev = 0;
+  double acc_a = 0.0, acc_s = 0.0;
+  int acc_cell = -1;
+  auto load = [&](int64_t q) {
+    w = A.S.w[q];
+    d1 = A.S.d1[q];
+    d2 = A.S.d2[q];
+    p1 = A.S.p1[q];
+    p2 = A.S.p2[q];
+    i = A.S.ci[q];
+    j = A.S.cj[q];
+    dcen = (R)((A.dt - (double)A.S.t[q]) * kC);
+    ev = 0;
+    P = 0.0;
+  };
+  auto flush = [&]() {
+    if (acc_cell >= 0) {
+      if (acc_a != 0.0 && !lds_tally_add(lkey, lval, acc_cell, acc_a)) atomic_add_f64(g_t + acc_cell, acc_a);
+      if (acc_s != 0.0 && !lds_tally_add(lkey, lval, ncell + acc_cell, acc_s))
+        atomic_add_f64(g_t + ncell + acc_cell, acc_s);
+    }
+    acc_a = 0.0;
+    acc_s = 0.0;
+  };
+  while (true) {
+    uint64_t need = __ballot(!active);
+    if (need) {
+      if (qbase >= qend && !exhausted) {
+        unsigned long long b = 0;
+        if (__lane_id() == 0) b = atomicAdd(A.next, (unsigned long long)A.batch);
+        b = __shfl(b, 0, 64);
+        if ((int64_t)b >= A.n) {
+          exhausted = true;
+        } else {
+          qbase = (int64_t)b;
+          qend = ((int64_t)b + A.batch < A.n) ? (int64_t)b + A.batch : A.n;
+        }
+      }
+      int64_t avail = qend - qbase;
+      if (avail > 0) {
+        int64_t rank = __popcll(need & lanemask_lt());
+        if (!active && rank < avail) {
+          k = qbase + rank;
+          load(k);
+          phase = 0;
+          active = dcen > (R)0;
+        }
+        int64_t take = __popcll(need);
+        qbase += take < avail ? take : avail;
+      }
+    }
+    if (__ballot(active) == 0) {
+      if (exhausted) break;
+      continue;
+    }
+    if (!active) continue;
+    bool finish = false;
+    ++ev;
+    if ((int64_t)ev > A.max_events || i < 0 || i >= nx || j < 0 || j >= ny) {
+      finish = true;
+    } else {
+      int c = i * ny + j;
+      R2 mat = A.sas[c];
+      R2 ex = A.e1p[i];
+      R2 ey = A.e2p[j];
+      MoveOut<R> o;
+      Ctr ctr = make_ctr(kTagTransport, (uint64_t)k, A.step);
+      if (GEOM == 0)
+        move_xy(ctr, A.key, ev - 1u, w, d1, d2, p1, p2, ex.x, ex.y, ey.x, ey.y, mat.x, mat.y, dcen, o);
+      else
+        move_rz(ctr, A.key, ev - 1u, w, d1, d2, p1, p2, ex.x, ex.y, ey.x, ey.y, mat.x, mat.y, dcen, o);
+      dcen = o.dcen;
+      const double lc = A.lam[c];
+      const double Pprev = P;
+      P += lc * o.dep_w;
+      if (phase == 1) {
+        if (c != acc_cell) {
+          flush();
+          acc_cell = c;
+        }
+        const double sseg = (double)o.s;
+        acc_a += -sseg * (Lam - P) + lc * o.w * sseg;
+        R cl = fmin(ex.y - ex.x, ey.y - ey.x);
+        if (mat.y > (R)1e-12 && !(mat.y * cl > (R)10000))
+          acc_s += ((o.evt == kEvtScatter ? 1.0 / (double)mat.y : 0.0) - sseg) * (Lam - Pprev);
+      }
+      w = o.w;
+      if (o.w <= wfloor) {
+        finish = true;
+      } else {
+        d1 = o.d1; d2 = o.d2; p1 = o.p1; p2 = o.p2;
+        if (o.crossed != kCrossNone) {
+          if (o.crossed == kCrossIPlus) ++i;
+          else if (o.crossed == kCrossIMinus) --i;
+          else if (o.crossed == kCrossJPlus) ++j;
+          else --j;
+          int side = -1;
+          if (i < 0) side = 0;
+          else if (i >= nx) side = 1;
+          else if (j < 0) side = 2;
+          else if (j >= ny) side = 3;
+          if (side >= 0) {
+            if ((A.reflect >> side) & 1) {
+              if (side < 2) { d1 = -d1; i = side == 0 ? 0 : nx - 1; }
+              else { d2 = -d2; j = side == 2 ? 0 : ny - 1; }
+            } else {
+              w = 0.0;
+              dcen = 0;
+            }
+          }
+        }
+        if (w <= 1e-300) {
+          w = 0.0;
+          dcen = 0;
+        }
+        if (!(dcen > (R)0)) finish = true;
+      }
+    }
+    if (finish) {
+      if (phase == 0) {
+        Lam = P;
+        if (k >= A.elo && k < A.ehi) {
+          int bi = A.S.ci[k], bj = A.S.cj[k];
+          if (bi >= 0 && bi < nx && bj >= 0 && bj < ny) {
+            int bc = bi * ny + bj;
+            double sab = A.sa64[bc];
+            if (sab > 0.0) {
+              double v = Lam / sab;
+              if (!lds_tally_add(lkey, lval, bc, v)) atomic_add_f64(g_t + bc, v);
+            }
+          }
+        }
+        load(k);
+        phase = 1;
+        active = dcen > (R)0;
+      } else {
+        active = false;
+      }
+    }
+  }
+  flush();
+  __syncthreads();
+  lds_tally_flush(lkey, lval, g_t);
+}
+
+__global__ void k_adjoint_reduce(int64_t n2, int copies, const double* __restrict__ g_raw, double* __restrict__ g) {
+  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n2) return;
+  double a = 0.0;
+  for (int k = 0; k < copies; ++k) a += g_raw[(int64_t)k * n2 + c];
+  g[c] = a;
+}
+
+// ---------------------------------------------------------------------------
 // comb (IMC/IMC2D.py:1448-1511)
 // ---------------------------------------------------------------------------
 // Particles outside the mesh (escaped) or with non-positive weight are dead.
@@ -812,6 +1022,14 @@ struct Imc2dCtx {
   int* bad = nullptr;
   PoolBuf A, B;
   int64_t n = 0;
+  // differentiable step: the transport's input pool, its size, the emission
+  // range of the step's pool, optional fixed emission counts, a snapshot
+  PoolBuf S, snap;
+  bool adjoint = false;
+  int64_t n_saved = 0, n_snap = 0, emit_lo = 0, emit_hi = 0;
+  bool fixed_counts = false;
+  DevBuf<int64_t> fcounts;
+  DevBuf<double> g_raw, g;
   int grid_override = 0;
   int batch = 256;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -934,7 +1152,7 @@ int imc2d_create(imc2d_ctx** out, int geometry, int nx, int ny, const double* ed
   IMC_HIP(hipMalloc(&c->stats, sizeof(unsigned long long) * 8));
   IMC_HIP(hipMalloc(&c->next, sizeof(unsigned long long)));
   IMC_HIP(hipMalloc(&c->bad, sizeof(int)));
-  c->A.rsz = c->B.rsz = (precision == IMC_FP64) ? 8 : 4;
+  c->A.rsz = c->B.rsz = c->S.rsz = c->snap.rsz = (precision == IMC_FP64) ? 8 : 4;
   *out = c;
   return kOk;
 }
@@ -965,6 +1183,11 @@ void imc2d_destroy(imc2d_ctx* c) {
   c->count_part.release();
   c->A.release();
   c->B.release();
+  c->S.release();
+  c->snap.release();
+  c->fcounts.release();
+  c->g_raw.release();
+  c->g.release();
   void* ptrs[] = {c->e1, c->e2, c->e1f, c->e2f, c->vol, c->scal, c->stats, c->next, c->bad};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -1081,6 +1304,7 @@ int imc2d_set_material(imc2d_ctx* c, const double* T, const double* sigma_true, 
   if (sigma_s_out) IMC_HIP(hipMemcpyAsync(sigma_s_out, c->ss.p, bytes, hipMemcpyDeviceToHost, c->stream));
   IMC_HIP(hipStreamSynchronize(c->stream));
   IMC_REQUIRE(!bad, kErrArg, "Fleck factor out of bounds");
+  c->emit_lo = c->emit_hi = 0;
   return kOk;
 }
 
@@ -1102,11 +1326,18 @@ static int launch_volume(imc2d_ctx* c, int64_t base, int64_t total, const double
 }
 
 // counts -> offsets -> total (host)
-static int counts_and_offsets(imc2d_ctx* c, const double* E, int64_t N, int min_one, int64_t* total) {
+static int counts_and_offsets(imc2d_ctx* c, const double* E, int64_t N, int min_one, int64_t* total,
+                              bool use_fixed = false) {
   int rc = canonical_sum(E, c->ncell, c->scal + 0, c->sum_scr.p, c->stream);
   if (rc) return rc;
-  k_counts<<<launch_blocks(c->ncell, 256), 256, 0, c->stream>>>(c->ncell, E, N, min_one, c->scal + 0, c->counts.p);
-  IMC_HIP(hipGetLastError());
+  if (use_fixed) {
+    IMC_HIP(hipMemcpyAsync(c->counts.p, c->fcounts.p, sizeof(int64_t) * c->ncell, hipMemcpyDeviceToDevice,
+                           c->stream));
+  } else {
+    k_counts<<<launch_blocks(c->ncell, 256), 256, 0, c->stream>>>(c->ncell, E, N, min_one, c->scal + 0,
+                                                                  c->counts.p);
+    IMC_HIP(hipGetLastError());
+  }
   rc = exclusive_scan_i64(c->counts.p, c->ncell, c->offs.p, c->scan_scr.p, c->stream);
   if (rc) return rc;
   IMC_HIP(hipMemcpyAsync(total, c->offs.p + c->ncell, sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
@@ -1185,8 +1416,12 @@ int imc2d_sample_volume(imc2d_ctx* c, int kind, int64_t N, const double* QV_host
     tag = kTagSource;
   }
   int64_t total = 0;
-  int rc = counts_and_offsets(c, QV, N, 0, &total);
+  bool fixed = c->fixed_counts && kind == IMC_SRC_EMISSION;
+  int rc = counts_and_offsets(c, QV, N, 0, &total, fixed);
   if (rc) return rc;
+  if (kind == IMC_SRC_EMISSION && c->fcounts.reserve(c->ncell) == kOk && !fixed)
+    IMC_HIP(hipMemcpyAsync(c->fcounts.p, c->counts.p, sizeof(int64_t) * c->ncell, hipMemcpyDeviceToDevice,
+                           c->stream));  // the counts used, for imc2d_emission_counts
   double Etot = 0.0;
   IMC_HIP(hipMemcpyAsync(&Etot, c->scal + 0, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   if (E_out) IMC_HIP(hipMemcpyAsync(E_out, QV, sizeof(double) * c->ncell, hipMemcpyDeviceToHost, c->stream));
@@ -1194,6 +1429,10 @@ int imc2d_sample_volume(imc2d_ctx* c, int kind, int64_t N, const double* QV_host
   if (total_out) *total_out = Etot;
   if (count_out) *count_out = (Etot > 0.0 && N > 0) ? total : 0;
   if (!(Etot > 0.0) || N <= 0 || total <= 0) return kOk;
+  if (kind == IMC_SRC_EMISSION) {
+    c->emit_lo = c->n;
+    c->emit_hi = c->n + total;
+  }
   int geom = c->geometry == IMC_GEOM_XY ? 0 : (rz_linear ? 2 : 1);
   if (geom == 2) {
     k_rz_params<<<launch_blocks(c->ncell, 256), 256, 0, c->stream>>>(c->nx, c->ny, c->T.p, c->e1, c->e2, c->Ar.p,
@@ -1249,6 +1488,17 @@ int imc2d_transport(imc2d_ctx* c, double dt, const int32_t* reflect, int64_t max
   int64_t need_blocks = (n + block - 1) / block;
   if (need_blocks < grid) grid = (int)std::max<int64_t>(need_blocks, 1);
   Key key = make_key(c->seed);
+  if (c->adjoint) {
+    // keep the transport's input pool for the adjoint replay
+    if ((rc = c->S.grow(std::max<int64_t>(n, 1), 0, s))) return rc;
+    int re = c->precision == IMC_FP64 ? 8 : 4;
+    void* src[8] = {c->A.w, c->A.d1, c->A.d2, c->A.t, c->A.p1, c->A.p2, c->A.ci, c->A.cj};
+    void* dst[8] = {c->S.w, c->S.d1, c->S.d2, c->S.t, c->S.p1, c->S.p2, c->S.ci, c->S.cj};
+    size_t esz[8] = {8, (size_t)re, (size_t)re, (size_t)re, (size_t)re, (size_t)re, 4, 4};
+    for (int a = 0; a < 8; ++a)
+      if (n > 0) IMC_HIP(hipMemcpyAsync(dst[a], src[a], esz[a] * (size_t)n, hipMemcpyDeviceToDevice, s));
+    c->n_saved = n;
+  }
   IMC_HIP(hipEventRecord(c->ev0, s));
   if (n > 0) {
     if (c->precision == IMC_FP64) {
@@ -1407,6 +1657,110 @@ int imc2d_census_energy(imc2d_ctx* c, double* out) {
   if ((rc = tree_sum(c->A.w, c->n, c->scal + 5, c->tree_scr.p, c->stream))) return rc;
   IMC_HIP(hipMemcpyAsync(out, c->scal + 5, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   IMC_HIP(hipStreamSynchronize(c->stream));
+  return kOk;
+}
+
+// ---- differentiable step -------------------------------------------------
+// Keep each transport's input pool for imc2d_adjoint (enable != 0).
+int imc2d_enable_adjoint(imc2d_ctx* c, int enable) {
+  IMC_REQUIRE(c, kErrArg, "null ctx");
+  c->adjoint = enable != 0;
+  if (!c->adjoint) c->n_saved = 0;
+  return kOk;
+}
+
+// Emission counts per cell: get the counts of the last emission sample, or
+// fix them (counts non-null) for the following samples (null: ceil rule).
+int imc2d_emission_counts(imc2d_ctx* c, int64_t* out, const int64_t* fix) {
+  IMC_REQUIRE(c, kErrArg, "null ctx");
+  IMC_HIP(hipSetDevice(c->device));
+  int rc = c->fcounts.reserve(c->ncell);
+  if (rc) return rc;
+  if (out) {
+    IMC_HIP(hipMemcpyAsync(out, c->fcounts.p, sizeof(int64_t) * c->ncell, hipMemcpyDeviceToHost, c->stream));
+    IMC_HIP(hipStreamSynchronize(c->stream));
+  }
+  if (fix) {
+    IMC_HIP(hipMemcpyAsync(c->fcounts.p, fix, sizeof(int64_t) * c->ncell, hipMemcpyHostToDevice, c->stream));
+    IMC_HIP(hipStreamSynchronize(c->stream));
+    c->fixed_counts = true;
+  } else if (!out) {
+    c->fixed_counts = false;
+  }
+  return kOk;
+}
+
+// Copy the census to a snapshot (save != 0) or restore it from the snapshot.
+int imc2d_snapshot(imc2d_ctx* c, int save) {
+  IMC_REQUIRE(c, kErrArg, "null ctx");
+  IMC_HIP(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  PoolBuf& from = save ? c->A : c->snap;
+  PoolBuf& to = save ? c->snap : c->A;
+  int64_t n = save ? c->n : c->n_snap;
+  int rc = to.grow(std::max<int64_t>(n, 1), save ? 0 : 0, s);
+  if (rc) return rc;
+  int re = c->precision == IMC_FP64 ? 8 : 4;
+  void* src[8] = {from.w, from.d1, from.d2, from.t, from.p1, from.p2, from.ci, from.cj};
+  void* dst[8] = {to.w, to.d1, to.d2, to.t, to.p1, to.p2, to.ci, to.cj};
+  size_t esz[8] = {8, (size_t)re, (size_t)re, (size_t)re, (size_t)re, (size_t)re, 4, 4};
+  for (int a = 0; a < 8; ++a)
+    if (n > 0) IMC_HIP(hipMemcpyAsync(dst[a], src[a], esz[a] * (size_t)n, hipMemcpyDeviceToDevice, s));
+  IMC_HIP(hipStreamSynchronize(s));
+  if (save)
+    c->n_snap = n;
+  else
+    c->n = n;
+  return kOk;
+}
+
+// Adjoint of the last transport's tallies (see k_transport_adjoint): lam per
+// cell (dL/dE_abs, energy per cell); gA_out, gS_out per cell: dL/dsigma_a,
+// dL/dsigma_s.  Same dt, reflect and max_events as that transport; the
+// material must not have changed since.
+int imc2d_adjoint(imc2d_ctx* c, double dt, const int32_t* reflect, int64_t max_events, const double* lam,
+                  double* gA_out, double* gS_out) {
+  IMC_REQUIRE(c && reflect && lam && gA_out && gS_out, kErrArg, "imc2d_adjoint: bad arguments");
+  IMC_REQUIRE(c->adjoint, kErrState, "imc2d_adjoint: enable the adjoint before the transport");
+  IMC_HIP(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  int64_t nc = c->ncell, n = c->n_saved;
+  int rc;
+  if ((rc = c->g_raw.reserve(2 * nc * kMaxTallyCopies)) || (rc = c->g.reserve(2 * nc))) return rc;
+  IMC_HIP(hipMemcpyAsync(c->volsrc.p, lam, sizeof(double) * nc, hipMemcpyHostToDevice, s));
+  IMC_HIP(hipMemsetAsync(c->g_raw.p, 0, sizeof(double) * 2 * nc * c->tally_copies, s));
+  IMC_HIP(hipMemsetAsync(c->next, 0, sizeof(unsigned long long), s));
+  int refl = 0;
+  for (int a = 0; a < 4; ++a) refl |= (reflect[a] ? 1 : 0) << a;
+  int block = 256;
+  int grid = c->grid_override > 0 ? c->grid_override : c->n_cu * 4;
+  grid = (int)std::max<int64_t>(std::min<int64_t>(grid, (n + block - 1) / block), 1);
+  Key key = make_key(c->seed);
+  if (n > 0) {
+    if (c->precision == IMC_FP64) {
+      AdjointArgs<double> A{c->S.view<double>(), n, c->e1p64.p, c->e2p64.p, c->nx, c->ny, c->sas64.p, c->sa.p, dt,
+                            refl, max_events, c->scal + 2, key, c->step, c->next, c->batch, c->volsrc.p, c->emit_lo,
+                            c->emit_hi, c->g_raw.p, nc, c->tally_copies - 1};
+      if (c->geometry == IMC_GEOM_XY)
+        k_transport_adjoint<double, 0><<<grid, block, 0, s>>>(A);
+      else
+        k_transport_adjoint<double, 1><<<grid, block, 0, s>>>(A);
+    } else {
+      AdjointArgs<float> A{c->S.view<float>(), n, c->e1p32.p, c->e2p32.p, c->nx, c->ny, c->sas32.p, c->sa.p, dt,
+                           refl, max_events, c->scal + 2, key, c->step, c->next, c->batch, c->volsrc.p, c->emit_lo,
+                           c->emit_hi, c->g_raw.p, nc, c->tally_copies - 1};
+      if (c->geometry == IMC_GEOM_XY)
+        k_transport_adjoint<float, 0><<<grid, block, 0, s>>>(A);
+      else
+        k_transport_adjoint<float, 1><<<grid, block, 0, s>>>(A);
+    }
+    IMC_HIP(hipGetLastError());
+  }
+  k_adjoint_reduce<<<launch_blocks(2 * nc, 256), 256, 0, s>>>(2 * nc, c->tally_copies, c->g_raw.p, c->g.p);
+  IMC_HIP(hipGetLastError());
+  IMC_HIP(hipMemcpyAsync(gA_out, c->g.p, sizeof(double) * nc, hipMemcpyDeviceToHost, s));
+  IMC_HIP(hipMemcpyAsync(gS_out, c->g.p + nc, sizeof(double) * nc, hipMemcpyDeviceToHost, s));
+  IMC_HIP(hipStreamSynchronize(s));
   return kOk;
 }
 

@@ -23,10 +23,11 @@ enum Event : int { kEvtCensus = 0, kEvtBoundary = 1, kEvtScatter = 2 };
 
 // dep_w: energy absorbed on the segment.  dep_i: w(1-e^{-sigma_a s})/sigma_a
 // (FP64) or, in FP32, the path weight w*s used only where sigma_a <= 1e-12.
+// s: the segment length (the adjoint replay needs it).
 template <typename R>
 struct MoveOut {
   double w, dep_w, dep_i;
-  R d1, d2, p1, p2, dcen;
+  R d1, d2, p1, p2, dcen, s;
   int crossed, evt;
 };
 
@@ -70,11 +71,12 @@ __device__ __forceinline__ void move_xy(Ctr ctr, Key key, uint32_t ev, double w,
   double se = sb;
   if (s_scat < se) se = s_scat;
   if (dcen < se) se = dcen;
-  o.d1 = ux; o.d2 = uy; o.p1 = x; o.p2 = y; o.crossed = kCrossNone;
+  o.d1 = ux; o.d2 = uy; o.p1 = x; o.p2 = y; o.crossed = kCrossNone; o.s = 0.0;
   if (se <= 0.0) {
     o.w = w; o.dep_w = 0.0; o.dep_i = 0.0; o.dcen = 0.0; o.evt = kEvtCensus;
     return;
   }
+  o.s = se;
   double xn = x + ux * se, yn = y + uy * se;
   int evt = kEvtCensus;
   if (fabs(se - sb) < 1e-11) {
@@ -137,11 +139,12 @@ __device__ __forceinline__ void move_rz(Ctr ctr, Key key, uint32_t ev, double w,
   double se = sb;
   if (s_scat < se) se = s_scat;
   if (dcen < se) se = dcen;
-  o.d1 = mu; o.d2 = eta; o.p1 = r; o.p2 = z; o.crossed = kCrossNone;
+  o.d1 = mu; o.d2 = eta; o.p1 = r; o.p2 = z; o.crossed = kCrossNone; o.s = 0.0;
   if (se <= 0.0) {
     o.w = w; o.dep_w = 0.0; o.dep_i = 0.0; o.dcen = 0.0; o.evt = kEvtCensus;
     return;
   }
+  o.s = se;
   double ome2 = 1.0 - eta * eta;
   if (ome2 < 0.0) ome2 = 0.0;
   double l = se * sqrt(ome2);
@@ -269,6 +272,7 @@ __device__ __forceinline__ void move_xy(Ctr ctr, Key key, uint32_t ev, double w,
     }
   }
   o.dcen = dcen - se;
+  o.s = se;
   absorb_f32(o, w, sa, se);
 }
 
@@ -351,6 +355,7 @@ __device__ __forceinline__ void move_rz(Ctr ctr, Key key, uint32_t ev, double w,
   }
   o.d1 = mun; o.d2 = etan; o.p1 = rn; o.p2 = zn;
   o.dcen = dcen - se;
+  o.s = se;
   absorb_f32(o, w, sa, se);
 }
 
