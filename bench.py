@@ -406,12 +406,92 @@ def main_mg(args):
     print(json.dumps(out), flush=True)
 
 
+# ---------------------------------------------------------------------------
+# differentiable gray rz step (opacity-scale gradients by replay adjoint)
+# ---------------------------------------------------------------------------
+def main_diff(args):
+    import torch
+
+    from radtranbook_amd import IMC2D, _lib
+    from radtranbook_amd.diff import AbsorptionStep
+
+    _lib.load()
+    _lib.require_gpu()
+    IMC2D.VERBOSE = False
+    steps = args.steps if args.steps is not None else 5
+    N = int(args.particles) if args.particles is not None else 4_000_000
+    prec = "fp64" if args.precision == "fp64" else "fp32"
+    nr, nz = 128, 256
+    e1 = np.linspace(0.0, 0.5, nr + 1)
+    e2 = np.linspace(0.0, 1.0, nz + 1)
+    rng = np.random.default_rng(1)
+    theta = (1.0 + 0.1 * rng.standard_normal((nr, nz))).astype(np.float32)
+    sig = lambda T: 100.0 * np.maximum(T, 1e-3) ** -3
+    sf = lambda T: theta.astype(np.float64) * sig(T)
+    cv = 0.3
+    T0 = np.full((nr, nz), 1e-3)
+    Tb = (0.0, 0.0, 0.3, 0.0)
+    st = IMC2D.init_simulation(N // 2, T0, T0, e1, e2, lambda T: cv * T, lambda u: u / cv, geometry="rz",
+                               precision=prec, seed=1)
+    t0 = time.perf_counter()
+    for s in range(steps - 1):
+        st, info = IMC2D.step(st, N // 2, N // 2, 0, N, Tb, 0.01, e1, e2, sf, lambda u: u / cv, lambda T: cv + 0 * T,
+                              np.zeros((nr, nz)), geometry="rz")
+        log(f"  step {s + 1}: {info['profiling']['phase_times_s']['total'] * 1e3:.1f} ms, "
+            f"transport kernel {info['profiling']['transport_kernel_ms']:.2f} ms")
+    t_pre = time.perf_counter() - t0
+    step = AbsorptionStep(st, N // 2, N // 2, Tb, 0.01, sig, lambda T: cv + 0 * T)
+    g2 = np.random.default_rng(2)
+    E0 = step.forward_np(theta.astype(np.float64), 1.0)
+    Estar = E0 * g2.uniform(0.8, 1.2, E0.shape)
+    w = g2.uniform(0.5, 1.5, E0.shape) / max(np.sum(E0 ** 2), 1e-300)
+    scale = torch.tensor(theta, requires_grad=True)
+    cvs = torch.ones((nr, nz), dtype=torch.float32, requires_grad=True)
+    t0 = time.perf_counter()
+    E = step(scale, cvs)
+    t_fwd = time.perf_counter() - t0
+    fwd_kernel_ms = step.last_forward_ms
+    loss = torch.sum(torch.from_numpy(w) * (E - torch.from_numpy(Estar)) ** 2)
+    t0 = time.perf_counter()
+    loss.backward()
+    t_bwd = time.perf_counter() - t0
+    g = scale.grad.double().numpy()
+    # directional finite differences with common random numbers (fixed emission counts)
+    step.fixed_emission_counts = step.last_counts
+    v = np.random.default_rng(3).standard_normal((nr, nz))
+    s0 = theta.astype(np.float64)
+    fds = {}
+    for h in (1e-3, 3e-2):
+        lp = float(np.sum(w * (step.forward_np(s0 + h * v, 1.0) - Estar) ** 2))
+        lm = float(np.sum(w * (step.forward_np(s0 - h * v, 1.0) - Estar) ** 2))
+        fds[h] = (lp - lm) / (2 * h)
+    ad = float(np.sum(g * v))
+    out = {
+        "metric": "differentiable IMC: replay-adjoint gradient of a tally loss, 2D rz 128x256, 4e6 particles/step",
+        "value": float(np.asarray(step.last_stats)[0]) / (t_fwd + t_bwd), "unit": "forward segments/s (fwd+bwd)",
+        "n_gpus": 1, "steps": steps, "warmup": 0, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 geometry / f64 weights+tallies" if prec == "fp32" else "f64",
+        "data": "synthetic (seeded opacity scales, loss weights and targets; Philox particle streams)",
+        "config": {"workload": f"gray IMC rz r in [0,0.5], z in [0,1], 128x256, sigma_a = theta 100 T^-3, "
+                               f"c_v 0.3, zmin 0.3 keV, dt 0.01 x {steps} (gradient at the last step), "
+                               f"{N:.0e} particles/step", "cells": nr * nz},
+        "forward_step_s": t_fwd, "backward_s": t_bwd, "forward_transport_kernel_ms": fwd_kernel_ms,
+        "segments_last_step": int(np.asarray(step.last_stats)[0]), "steps_before_s": t_pre,
+        "loss": float(loss.detach()), "grad_dot_v": ad,
+        "fd_dot_v": {f"{h:g}": fds[h] for h in fds},
+        "grad_cv_scale_finite": bool(np.all(np.isfinite(cvs.grad.numpy()))),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=["marshak2d", "slab1d", "batch1d", "mg2d"], default="marshak2d",
+    ap.add_argument("--workload", choices=["marshak2d", "slab1d", "batch1d", "mg2d", "diff_rz"],
+                    default="marshak2d",
                     help="marshak2d: 2D 200x200 heterogeneous (default, headline); slab1d: the reference's 1D slab "
-                         "case; batch1d: 256 independent slabs; mg2d: multigroup crooked pipe")
+                         "case; batch1d: 256 independent slabs; mg2d: multigroup crooked pipe; diff_rz: "
+                         "differentiable rz step")
     ap.add_argument("--problems", type=int, default=256)
     ap.add_argument("--tune", action=argparse.BooleanOptionalAction, default=True,
                     help="autotune the 1D transport launch shape on the first warm-up step")
@@ -435,6 +515,9 @@ def main():
         sys.exit(2)
     if args.workload == "mg2d":
         main_mg(args)
+        return
+    if args.workload == "diff_rz":
+        main_diff(args)
         return
     if args.workload != "marshak2d":
         main_1d(args, args.workload)
