@@ -127,25 +127,25 @@ __device__ __forceinline__ void move_1d(Ctr ctr, Key key, uint32_t ev, double w,
   float db;
   int hit_outer = 1;
   if (!SPH) {
-    db = (mu != 0.0f) ? fmaxf(__fdividef((mu > 0.0f ? r1 : r0) - x, mu), 0.0f) : 1e30f;
+    db = (mu != 0.0f) ? fmaxf(fdiv_fast((mu > 0.0f ? r1 : r0) - x, mu), 0.0f) : 1e30f;
   } else {
     float ome = fmaxf((1.0f - mu) * (1.0f + mu), 0.0f);
     float rootout = __builtin_amdgcn_sqrtf(fmaxf(r1 * r1 - x * x * ome, 0.0f));
-    float dout = (mu > 0.0f) ? __fdividef(fmaxf((r1 - x) * (r1 + x), 0.0f), rootout + x * mu) : rootout - x * mu;
+    float dout = (mu > 0.0f) ? fdiv_fast(fmaxf((r1 - x) * (r1 + x), 0.0f), rootout + x * mu) : rootout - x * mu;
     if (!(dout >= 0.0f)) dout = 0.0f;
     float din = 1e30f;
     float discin = r0 * r0 - x * x * ome;
     if (mu < 0.0f && discin > 0.0f) {
       float rootin = __builtin_amdgcn_sqrtf(discin);
-      din = __fdividef(fmaxf((x - r0) * (x + r0), 0.0f), rootin - x * mu);
+      din = fdiv_fast(fmaxf((x - r0) * (x + r0), 0.0f), rootin - x * mu);
     }
     db = fminf(dout, din);
     hit_outer = (dout <= din);
   }
   QuadF u = (ss > 1e-10f || ss * dxc > 10000.0f) ? block4_f(ctr, key, ev) : QuadF{0.0f, 0.0f, 0.0f, 0.0f};
   float dsc = 1e30f;
-  if (ss > 1e-10f) dsc = __fdividef(-__logf(1.0f - u.u0), ss);
-  if (ss * dxc > 10000.0f) dsc = -__logf(1.0f - u.u1) * (dxc * 1.0e-4f);
+  if (ss > 1e-10f) dsc = fdiv_fast(neg_ln_fast(1.0f - u.u0), ss);
+  if (ss * dxc > 10000.0f) dsc = neg_ln_fast(1.0f - u.u1) * (dxc * 1.0e-4f);
   o.bad = false;
   o.newloc = 0;
   float s, rn, mn = mu;
@@ -157,7 +157,7 @@ __device__ __forceinline__ void move_1d(Ctr ctr, Key key, uint32_t ev, double w,
     } else {
       float a = x + mu * s;
       float rr = __builtin_amdgcn_sqrtf(a * a + s * s * fmaxf((1.0f - mu) * (1.0f + mu), 0.0f));
-      mn = rr > 0.0f ? fminf(1.0f, fmaxf(-1.0f, __fdividef(x * mu + s, rr))) : mu;
+      mn = rr > 0.0f ? fminf(1.0f, fmaxf(-1.0f, fdiv_fast(x * mu + s, rr))) : mu;
       if (hit_outer) { rn = r1; o.newloc = 1; } else { rn = r0; o.newloc = -1; }
     }
   } else {
@@ -167,7 +167,7 @@ __device__ __forceinline__ void move_1d(Ctr ctr, Key key, uint32_t ev, double w,
     } else {
       float a = x + mu * s;
       float rr = __builtin_amdgcn_sqrtf(a * a + s * s * fmaxf((1.0f - mu) * (1.0f + mu), 0.0f));
-      mn = rr > 0.0f ? fminf(1.0f, fmaxf(-1.0f, __fdividef(x * mu + s, rr))) : 1.0f;
+      mn = rr > 0.0f ? fminf(1.0f, fmaxf(-1.0f, fdiv_fast(x * mu + s, rr))) : 1.0f;
       rn = fminf(fmaxf(rr, r0), r1);
     }
     if (dsc <= dcen) {

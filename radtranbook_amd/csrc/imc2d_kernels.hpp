@@ -223,8 +223,8 @@ __device__ __forceinline__ void absorb_f32(MoveOut<float>& o, double w, float sa
 __device__ __forceinline__ float scatter_distance_f32(const QuadF& u, float ss, float cl) {
   float s = 1e30f;
   if (ss > 1e-12f) {
-    s = __fdividef(-__logf(1.0f - u.u0), ss);
-    if (ss * cl > 10000.0f) s = -__logf(1.0f - u.u1) * (cl * 1.0e-4f);
+    s = fdiv_fast(neg_ln_fast(1.0f - u.u0), ss);
+    if (ss * cl > 10000.0f) s = neg_ln_fast(1.0f - u.u1) * (cl * 1.0e-4f);
   }
   return s;
 }
@@ -236,8 +236,8 @@ __device__ __forceinline__ void move_xy(Ctr ctr, Key key, uint32_t ev, double w,
                                         float xl, float xr, float yl, float yr, float sa, float ss, float dcen,
                                         MoveOut<float>& o) {
   // face distances by reciprocal-multiply (about 2 ulp; clamped at 0)
-  float sx = (ux != 0.0f) ? fmaxf(__fdividef((ux > 0.0f ? xr : xl) - x, ux), 0.0f) : 1e30f;
-  float sy = (uy != 0.0f) ? fmaxf(__fdividef((uy > 0.0f ? yr : yl) - y, uy), 0.0f) : 1e30f;
+  float sx = (ux != 0.0f) ? fmaxf(fdiv_fast((ux > 0.0f ? xr : xl) - x, ux), 0.0f) : 1e30f;
+  float sy = (uy != 0.0f) ? fmaxf(fdiv_fast((uy > 0.0f ? yr : yl) - y, uy), 0.0f) : 1e30f;
   float cl = CAP ? fminf(xr - xl, yr - yl) : 0.0f;
   QuadF u = (ss > 1e-12f) ? block4_f(ctr, key, ev) : QuadF{0.0f, 0.0f, 0.0f, 0.0f};
   float s_scat = scatter_distance_f32(u, ss, cl);
@@ -282,9 +282,9 @@ __device__ __forceinline__ float dist_outer_f32(float r, float mu, float R, floa
   float ome = (1.0f - mu) * (1.0f + mu);
   float disc = fmaxf(R * R - r * r * ome, 0.0f);
   float root = __builtin_amdgcn_sqrtf(disc);
-  float l = (mu > 0.0f) ? __fdividef(fmaxf((R - r) * (R + r), 0.0f), root + r * mu) : root - r * mu;
+  float l = (mu > 0.0f) ? fdiv_fast(fmaxf((R - r) * (R + r), 0.0f), root + r * mu) : root - r * mu;
   if (!(l >= 0.0f)) l = 0.0f;  // r == R, mu == 1 degenerate: 0/0
-  return __fdividef(l, den);
+  return fdiv_fast(l, den);
 }
 __device__ __forceinline__ float dist_inner_f32(float r, float mu, float R, float den) {
   if (!(mu < 0.0f)) return 1e30f;
@@ -292,8 +292,8 @@ __device__ __forceinline__ float dist_inner_f32(float r, float mu, float R, floa
   float disc = R * R - r * r * ome;
   if (disc <= 0.0f) return 1e30f;
   float root = __builtin_amdgcn_sqrtf(disc);
-  float l = __fdividef(fmaxf((r - R) * (r + R), 0.0f), root - r * mu);
-  return __fdividef(l, den);
+  float l = fdiv_fast(fmaxf((r - R) * (r + R), 0.0f), root - r * mu);
+  return fdiv_fast(l, den);
 }
 
 template <bool CAP = true>
@@ -309,9 +309,9 @@ __device__ __forceinline__ void move_rz(Ctr ctr, Key key, uint32_t ev, double w,
   }
   float s_zlo = 1e30f, s_zhi = 1e30f;
   if (eta > 0.0f)
-    s_zhi = fmaxf(__fdividef(zhi - z, eta), 0.0f);
+    s_zhi = fmaxf(fdiv_fast(zhi - z, eta), 0.0f);
   else if (eta < 0.0f)
-    s_zlo = fmaxf(__fdividef(zlo - z, eta), 0.0f);
+    s_zlo = fmaxf(fdiv_fast(zlo - z, eta), 0.0f);
   float cl = CAP ? fminf(rout - rin, zhi - zlo) : 0.0f;
   QuadF u = (ss > 1e-12f) ? block4_f(ctr, key, ev) : QuadF{0.0f, 0.0f, 0.0f, 0.0f};
   float s_scat = scatter_distance_f32(u, ss, cl);
@@ -331,7 +331,7 @@ __device__ __forceinline__ void move_rz(Ctr ctr, Key key, uint32_t ev, double w,
   float rn = __builtin_amdgcn_sqrtf(a * a + l * l * fmaxf((1.0f - mu) * (1.0f + mu), 0.0f));
   float zn = z + se * eta;
   float mun = mu;
-  if (rn > 0.0f && l > 0.0f) mun = fminf(1.0f, fmaxf(-1.0f, __fdividef(r * mu + l, rn)));
+  if (rn > 0.0f && l > 0.0f) mun = fminf(1.0f, fmaxf(-1.0f, fdiv_fast(r * mu + l, rn)));
   float etan = eta;
   if (bnd) {
     o.evt = kEvtBoundary;

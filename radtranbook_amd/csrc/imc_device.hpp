@@ -57,16 +57,16 @@ __device__ __forceinline__ U4 philox10(Ctr c, uint32_t block, Key k) {
       k0 += 0x9E3779B9u;
       k1 += 0xBB67AE85u;
     }
-    uint32_t lo0 = 0xD2511F53u * x0;
-    uint32_t hi0 = __umulhi(0xD2511F53u, x0);
-    uint32_t lo1 = 0xCD9E8D57u * x2;
-    uint32_t hi1 = __umulhi(0xCD9E8D57u, x2);
-    uint32_t n0 = hi1 ^ x1 ^ k0;
-    uint32_t n2 = hi0 ^ x3 ^ k1;
+    // full 32x32 -> 64 products: one v_mad_u64_u32 each instead of a
+    // mul_lo/mul_hi pair
+    uint64_t p0 = (uint64_t)0xD2511F53u * x0;
+    uint64_t p1 = (uint64_t)0xCD9E8D57u * x2;
+    uint32_t n0 = (uint32_t)(p1 >> 32) ^ x1 ^ k0;
+    uint32_t n2 = (uint32_t)(p0 >> 32) ^ x3 ^ k1;
     x0 = n0;
-    x1 = lo1;
+    x1 = (uint32_t)p1;
     x2 = n2;
-    x3 = lo0;
+    x3 = (uint32_t)p0;
   }
   return U4{x0, x1, x2, x3};
 }
@@ -99,6 +99,12 @@ __device__ __forceinline__ QuadF block4_f(Ctr c, Key k, uint32_t b) {
   U4 u = philox10(c, b, k);
   return QuadF{u01_f(u.x), u01_f(u.y), u01_f(u.z), u01_f(u.w)};
 }
+
+// FP32 tracking arithmetic on the hardware transcendental units (about 1 ulp
+// each): a/b as a times the reciprocal, and -ln(x) from log2.  (HIP's
+// __fdividef is an IEEE division and __logf the full-range logf.)
+__device__ __forceinline__ float fdiv_fast(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ float neg_ln_fast(float x) { return -0.693147180559945f * __builtin_amdgcn_logf(x); }
 
 // wave64 helpers
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
