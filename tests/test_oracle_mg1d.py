@@ -1,0 +1,199 @@
+"""Pins of the multigroup 1-D spherical oracle (MG_IMC/MG_IMC1D.py) against
+what the mathematics fixes: the rounding allocation rule, exact bin energies,
+Planck group statistics of the census and face sources (chi^2 against the
+Planck group fractions), volume-uniform radii (KS), the face-source energy
+4 pi R^2 a c T^4 / 4 dt, 0-D reflecting-sphere closed forms with
+group-dependent opacities, straight-line attenuation by group, local floor
+kills, and the step's energy balance in each comb mode."""
+import numpy as np
+import pytest
+from scipy import stats
+
+C = 29.98
+A = 0.01372
+EDGES = np.array([1e-4, 0.03, 0.1, 0.3, 1.0, 10.0])
+
+
+@pytest.fixture(scope="module")
+def M1():
+    from oracle import oracle_mg1d
+
+    return oracle_mg1d
+
+
+def test_round_counts(M1):
+    E = np.array([0.0, 1.0, 2.5, 0.5, 1e-9, 6.0])
+    # N E / Et = 10 E / 10 = E: round half to even, at least 1 where E > 0
+    cnt, tot = M1.counts(E, 10, 1)
+    assert list(cnt) == [0, 1, 2, 1, 1, 6] and tot == 11
+    cnt0, _ = M1.counts(E, 10, 0)
+    assert list(cnt0) == [0, 1, 2, 0, 0, 6]
+
+
+def _planck_probs(M1, T):
+    from oracle import oracle_mg as OM
+
+    f = np.array([OM.planck_fraction(EDGES[g] / T, EDGES[g + 1] / T) for g in range(len(EDGES) - 1)])
+    # the sampler puts nu below the first edge in group 0 and above the last in G-1
+    f[0] += OM.planck_fraction(0.0, EDGES[0] / T)
+    f[-1] += 1.0 - OM.planck_fraction(0.0, EDGES[-1] / T)
+    return f / f.sum()
+
+
+def test_equilibrium_census(M1):
+    edges = np.array([0.0, 0.1, 0.25, 0.3])
+    vol = M1.volumes(edges)
+    np.testing.assert_allclose(vol, 4 / 3 * np.pi * np.diff(edges ** 3), rtol=1e-14)
+    Tr = np.array([0.3, 0.5, 0.2])
+    E = M1.equilibrium_energy(Tr, vol)
+    np.testing.assert_allclose(E, A * Tr ** 4 * vol, rtol=1e-14)
+    cnt, tot = M1.counts(E, 60000, 1)
+    p = M1.sample_ic(edges, Tr, E, cnt, EDGES, 3)
+    for i in range(3):
+        sel = p.cell == i
+        assert np.sum(p.w[sel]) == pytest.approx(E[i], rel=1e-12)
+        r = p.x[sel]
+        u = (r ** 3 - edges[i] ** 3) / (edges[i + 1] ** 3 - edges[i] ** 3)
+        assert stats.kstest(u, "uniform").pvalue > 1e-3
+        obs = np.bincount(p.g[sel], minlength=5)
+        exp = _planck_probs(M1, Tr[i]) * sel.sum()
+        keep = exp > 5
+        chi2 = np.sum((obs[keep] - exp[keep]) ** 2 / exp[keep])
+        assert stats.chi2.sf(chi2, keep.sum() - 1) > 1e-4, (i, obs, exp)
+    assert np.all(p.t == 0.0)
+    assert stats.kstest((p.mu + 1) / 2, "uniform").pvalue > 1e-3
+
+
+def test_face_source(M1):
+    E = M1.face_energy(0.7, 0.01, 0.4)
+    assert E == pytest.approx(A * C * 0.7 ** 4 / 4 * 0.01 * 4 * np.pi * 0.16, rel=1e-14)
+    for side in (0, 1):
+        p = M1.sample_face(side, 50000, E, 0.7, 0.01, 0.4, 6, EDGES, 5, 2)
+        assert np.sum(p.w) == pytest.approx(E, rel=1e-12)
+        assert stats.kstest(p.mu ** 2, "uniform").pvalue > 1e-3
+        assert np.all(p.mu > 0) if side == 0 else np.all(p.mu < 0)
+        assert np.all(p.cell == (0 if side == 0 else 5)) and np.all(p.x == 0.4)
+        obs = np.bincount(p.g, minlength=5)
+        exp = _planck_probs(M1, 0.7) * 50000
+        keep = exp > 5
+        assert stats.chi2.sf(np.sum((obs[keep] - exp[keep]) ** 2 / exp[keep]), keep.sum() - 1) > 1e-4
+
+
+def test_bin_sources(M1):
+    edges = np.array([0.0, 0.2, 0.5])
+    G = 3
+    E = np.array([[1.0, 0.0], [0.5, 2.0], [0.0, 0.25]])
+    cnt, tot = M1.counts(E, 1000, 1)
+    p = M1.sample_bins(edges, G, E, cnt, 0.01, 7, 1, M1.TAG_EMISSION)
+    assert len(p) == tot
+    for g in range(G):
+        for i in range(2):
+            sel = (p.g == g) & (p.cell == i)
+            assert sel.sum() == cnt[g * 2 + i]
+            if sel.any():
+                assert np.sum(p.w[sel]) == pytest.approx(E[g, i], rel=1e-12)
+                assert np.all((p.x[sel] >= edges[i]) & (p.x[sel] <= edges[i + 1]))
+    # bins in the reference's loop order: group outer, cell inner
+    b = p.g * 2 + p.cell
+    assert np.all(np.diff(b) >= 0)
+    assert np.all((p.t >= 0) & (p.t < 0.01))
+
+
+def _iso_pool(M1, n, R, G, rng):
+    p = M1.Pool1MG.empty(n)
+    p.w[:] = rng.uniform(0.5, 1.5, n)
+    p.mu[:] = rng.uniform(-1, 1, n)
+    p.t[:] = rng.uniform(0, 0.01, n)
+    p.x[:] = R * rng.uniform(0, 1, n) ** (1 / 3)
+    p.g[:] = rng.integers(0, G, n)
+    return p
+
+
+def test_zero_d_reflecting_sphere(M1):
+    """One reflecting sphere: every history reaches census, so each group's
+    deposit is sum w (1 - exp(-sigma_a,g c (dt - t))), scattering or not."""
+    G = 3
+    edges = np.array([0.0, 0.3])
+    rng = np.random.default_rng(1)
+    p = _iso_pool(M1, 4000, 0.3, G, rng)
+    sa = np.array([[2.0], [9.0], [40.0]])
+    for ss in (np.zeros((G, 1)), np.array([[5.0], [50.0], [0.0]])):
+        q = M1.Pool1MG(*[a.copy() for a in p.arrays()])
+        dep, path, bl, st = M1.transport(q, edges, G, sa, ss, 0.01, (False, True), 0.0, 4, 1)
+        for g in range(G):
+            sel = p.g == g
+            exact = np.sum(p.w[sel] * (1 - np.exp(-sa[g, 0] * C * (0.01 - p.t[sel]))))
+            assert dep[g, 0] == pytest.approx(exact, rel=1e-11)
+        assert np.all(bl == 0.0)
+        assert np.sum(q.w) + dep.sum() == pytest.approx(np.sum(p.w), rel=1e-13)
+        if ss.any():
+            assert st[2] > 0
+
+
+def test_attenuation_by_group_and_floor(M1):
+    """Pure absorbers, vacuum faces: group g leaks w exp(-sigma_g l) along
+    the radial chord; a floor above the surviving weight deposits it where
+    the particle stands."""
+    G = 2
+    edges = np.array([0.0, 0.1, 0.3])
+    sa = np.array([[3.0, 1.0], [20.0, 8.0]])
+    p = M1.Pool1MG.empty(2)
+    p.w[:] = 1.0; p.mu[:] = 1.0; p.t[:] = 0.0; p.x[:] = 0.05; p.cell[:] = 0; p.g[:] = [0, 1]
+    q = M1.Pool1MG(*[a.copy() for a in p.arrays()])
+    dep, path, bl, st = M1.transport(q, edges, G, sa, np.zeros((G, 2)), 1.0, (False, False), 0.0, 1, 1)
+    for g in range(G):
+        assert bl[g] == pytest.approx(np.exp(-sa[g, 0] * 0.05 - sa[g, 1] * 0.2), rel=1e-12)
+    assert np.all(q.w == 0.0)
+    # floor: kill after the first segment, deposit locally in cell 1
+    q = M1.Pool1MG(*[a.copy() for a in p.arrays()])
+    dep2, _, bl2, st2 = M1.transport(q, edges, G, sa, np.zeros((G, 2)), 1.0, (False, False), 0.9, 1, 1)
+    assert st2[4] == 2 and np.all(bl2 == 0.0)
+    for g in range(G):
+        assert dep2[g, 0] + dep2[g, 1] == pytest.approx(1.0, rel=1e-14)
+        assert dep2[g, 1] == pytest.approx(np.exp(-sa[g, 0] * 0.05), rel=1e-12)
+
+
+def _problem(M1, seed=1):
+    I = 12
+    r = np.linspace(0.0, 0.6, I + 1)
+    mesh = np.stack([r[:-1], r[1:]], axis=1)
+    O = M1.OracleMG1D(mesh, EDGES, seed=seed, nthreads=2)
+    cvv = 0.2
+    eos = lambda T: cvv * T
+    inv = lambda u: u / cvv
+    T0 = np.full(I, 0.05)
+    funcs = [lambda T, g=g: 30.0 * (1 + g) * np.maximum(T, 1e-3) ** -1.0 for g in range(5)]
+    return O, T0, eos, inv, funcs, (lambda T: cvv + 0 * T)
+
+
+@pytest.mark.parametrize("mode", [True, "radiation", False])
+def test_step_energy_balance(M1, mode):
+    O, T0, eos, inv, funcs, cv = _problem(M1)
+    st = O.init_simulation(3000, T0, T0, eos, inv)
+    for k in range(3):
+        st, info = O.step(st, 2000, 2000, 0, 3000, (0.0, 0.8), 0.005, funcs, inv, cv, np.zeros(O.I),
+                          conserve_comb_energy=mode)
+        scale = info["total_energy"]
+        if mode is False:
+            # the discarded comb discrepancy is the only unbooked sink
+            assert info["energy_residual"] == pytest.approx(-info["comb_discrepancy"].sum(), abs=1e-12 * scale)
+        else:
+            assert abs(info["energy_residual"]) < 1e-12 * scale
+            if mode == "radiation":
+                assert np.all(info["comb_discrepancy"] == 0.0)
+    assert info["boundary_emission"] > 0 and info["N_particles"] > 0
+
+
+def test_budget_split_and_threshold(M1):
+    O, T0, eos, inv, funcs, cv = _problem(M1)
+    st = O.init_simulation(500, T0, T0, eos, inv)
+    st, info = O.step(st, 10000, 0, 0, -10 ** 9, (0.0, 0.8), 0.005, funcs, inv, cv, np.zeros(O.I),
+                      particle_budget_fmin=0.1)
+    N_in, N_out, N_src, N_emit = info["budget"]
+    # the cold material's share is below fmin: it gets the floor fmin N, the
+    # hot face the rest of its proportional share
+    assert N_in == 0 and N_src == 0 and N_emit == 1000 and 9000 < N_out <= 10000
+    # threshold far above the census: no comb, survivors kept as they are
+    assert np.all(info["comb_discrepancy"] == 0.0)
+    assert info["N_particles"] == int(np.sum(st.pool.w > 0))
+    assert abs(info["energy_residual"]) < 1e-12 * info["total_energy"]
