@@ -298,6 +298,57 @@ float mgimc_last_transport_ms(mgimc_ctx* ctx);
 int mgimc_comb(mgimc_ctx* ctx, int64_t Nmax, double* ecen_out, double* disc_out, double* post_out,
                double* ecen_total_out, double* total_rad_out);
 
+/* ---- multigroup 1-D spherical -------------------------------------------- */
+/* A mgimc_ctx over I radial shells (edges: I+1 radii), the state of
+ * MG_IMC1D.init_simulation (MG_IMC/MG_IMC1D.py:850-914) and
+ * SimulationState1DMG (116-138).  Per-(group, cell) arrays are (G, I).  The
+ * census calls above apply with d1 = mu, p1 = r, ci = cell (d2, p2, cj
+ * unused); mgimc_set_material is the Planck-mean Fleck factor of
+ * MG_IMC1D.py:987-1002 and the emission energies of 600-612 (pass
+ * emission_fractions = NULL); mgimc_destroy, mgimc_set_step/seed/launch and
+ * mgimc_last_transport_ms apply unchanged. */
+int mgimc_create_sphere(mgimc_ctx** out, int I, const double* edges, int G, const double* energy_edges,
+                        int precision, uint64_t seed, int device);
+
+/* Equilibrium census at t=0 replacing the pool: _equilibrium_sample_spherical_mg
+ * (MG_IMC1D.py:514-566): max(1, round(N E_i / E)) particles per emitting
+ * cell, groups from the Planck spectrum at Tr_i. */
+int mgimc_sphere_equilibrium(mgimc_ctx* ctx, int64_t N, const double* Tr, double T_emit_floor);
+
+/* Half-Lambertian source on the inner (side 0, mu > 0, cell 0) or outer
+ * (side 1, mu < 0, cell I-1) sphere appended to the pool:
+ * _create_boundary_spherical_mg (MG_IMC1D.py:653-676) as placed by step()
+ * (1061-1088).  E_out: a c T^4 / 4 dt 4 pi R^2. */
+int mgimc_sphere_face(mgimc_ctx* ctx, int side, int64_t n, double T, double dt, double* E_out);
+
+/* Volume source by (group, cell) bin appended to the pool:
+ * _emitted_particles_spherical_mg (MG_IMC1D.py:569-650, kind IMC_SRC_EMISSION,
+ * energies of the last mgimc_set_material, max(1, round(.)) per emitting bin)
+ * or _sample_source_spherical_mg (679-743, kind IMC_SRC_FIXED, QV (G, I)
+ * source energies this step, round(.) per bin).  total_out: energy added. */
+int mgimc_sphere_volume(mgimc_ctx* ctx, int kind, int64_t N, const double* QV, double dt, double* total_out);
+
+/* Transport the pool: _move_particles_spherical_mg (MG_IMC1D.py:368-507).
+ * weight_floor < 0: 1e-14 sum(w)/N (1138).  Outputs nullable: dep, si
+ * (G, I); bl (G) escaping weight by group; stats[8] = events, boundary,
+ * scatter, census, 0, floor kills, reflections, non-positive distances. */
+int mgimc_sphere_transport(mgimc_ctx* ctx, double dt, const int32_t* reflect, double weight_floor, double* dep_out,
+                           double* si_out, double* bl_out, int64_t* stats_out);
+
+/* Census after transport (MG_IMC1D.py:1147-1215 with _comb_mg_1d 746-843):
+ * Nmax > 0 combs per (cell, group) bin to about Nmax, Nmax < 0 combs to
+ * |Nmax| once at least |Nmax| particles live, Nmax == 0 only drops dead
+ * particles; radiation != 0 rescales each bin's survivors to its pre-comb
+ * energy.  Outputs nullable: disc (G, I) = pre - post comb energy per bin
+ * (radiation mode: the energy of bins left without copies, which the
+ * reference drops unbooked); n_alive; combed. */
+int mgimc_sphere_comb(mgimc_ctx* ctx, int64_t Nmax, int radiation, double* disc_out, int64_t* n_alive_out,
+                      int* combed_out);
+
+/* Census energy and sum w mu per (group, cell) bin (MG_IMC1D.py:1262-1276,
+ * before the c / V scaling of the flux) and the total energy. */
+int mgimc_sphere_census(mgimc_ctx* ctx, double* E_out, double* wmu_out, double* total_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -161,7 +161,10 @@ def tallies(dep_raw, path_raw, sa, vol, dt):
 def comb(pool, Nmax, I, G, seed, step, radiation):
     """(cell, group) comb (MG_IMC1D.py:746-843) over the transported pool (the
     comb stream of a particle is its pool position; zero weights get no
-    copies).  radiation: rescale survivors per bin to the pre-comb energy."""
+    copies).  radiation: rescale survivors per bin to the pre-comb energy.
+    Returns (new pool, pre - post energy per bin): in radiation mode that is
+    the energy of bins whose copies all vanished, which the reference
+    reports as zero and drops unbooked."""
     b = pool.g.astype(np.int64) * I + np.clip(pool.cell, 0, I - 1)
     ecen, ew, cp, tot, passthrough = O.comb_counts(pool.w, b, G * I, Nmax, seed, step)
     idx = np.repeat(np.arange(len(pool)), cp)
@@ -174,7 +177,7 @@ def comb(pool, Nmax, I, G, seed, step, radiation):
         post = np.bincount(nbid, weights=newp.w, minlength=G * I)
         scale = np.where(post > 0.0, ecen / (post + 1e-300), 1.0)
         newp.w = newp.w * scale[nbid]
-        after = ecen
+        after = np.where(post > 0.0, ecen, 0.0)
     else:
         after = np.bincount(nbid, weights=newp.w, minlength=G * I) if len(newp) else np.zeros(G * I)
     return newp, (ecen - after).reshape(G, I)
@@ -307,6 +310,7 @@ class OracleMG1D:
         if do_comb:
             newp, disc = comb(pool, target, I, G, self.seed, sid, conserve_comb_energy == "radiation")
             if (conserve_comb_energy is True or conserve_comb_energy == "material") and np.any(disc != 0.0):
+                # the discrepancy goes into the material (MG_IMC1D.py:1217-1221)
                 u = u + disc.sum(axis=0) / vol
                 Tn = inv_eos(u)
         else:

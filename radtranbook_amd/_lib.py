@@ -93,6 +93,13 @@ SIGNATURES = {
     "mgimc_transport": (C.c_int, [_vp, C.c_double, _i32p, C.c_int64, C.c_double, _dp, _dp, _dp, _i64p]),
     "mgimc_last_transport_ms": (C.c_float, [_vp]),
     "mgimc_comb": (C.c_int, [_vp, C.c_int64, _dp, _dp, _dp, _dp, _dp]),
+    "mgimc_create_sphere": (C.c_int, [C.POINTER(_vp), C.c_int, _dp, C.c_int, _dp, C.c_int, C.c_uint64, C.c_int]),
+    "mgimc_sphere_equilibrium": (C.c_int, [_vp, C.c_int64, _dp, C.c_double]),
+    "mgimc_sphere_face": (C.c_int, [_vp, C.c_int, C.c_int64, C.c_double, C.c_double, _dp]),
+    "mgimc_sphere_volume": (C.c_int, [_vp, C.c_int, C.c_int64, _dp, C.c_double, _dp]),
+    "mgimc_sphere_transport": (C.c_int, [_vp, C.c_double, _i32p, C.c_double, _dp, _dp, _dp, _i64p]),
+    "mgimc_sphere_comb": (C.c_int, [_vp, C.c_int64, C.c_int, _dp, _i64p, C.POINTER(C.c_int)]),
+    "mgimc_sphere_census": (C.c_int, [_vp, _dp, _dp, _dp]),
 }
 
 
@@ -571,3 +578,129 @@ class EngineMG:
         tr = C.c_double()
         check(self._L.mgimc_comb(self._h, int(Nmax), _p(ecen), _p(disc), _p(post), C.byref(et), C.byref(tr)))
         return ecen, disc, post, et.value, tr.value
+
+
+class EngineMG1D:
+    """Device context for a multigroup 1-D spherical mesh (``mgimc_ctx`` made
+    by ``mgimc_create_sphere``).  ``edges``: I+1 radii; per-(group, cell)
+    arrays are (G, I)."""
+
+    def __init__(self, edges, energy_edges, precision="fp64", seed=1, device=0):
+        L = load()
+        require_gpu()
+        self.e = np.ascontiguousarray(edges, dtype=np.float64)
+        self.edges = np.ascontiguousarray(energy_edges, dtype=np.float64)
+        self.I = self.e.size - 1
+        self.G = self.edges.size - 1
+        self.gshape = (self.G, self.I)
+        self.precision = {"fp64": FP64, "fp32": FP32_TRACK}[precision]
+        self.seed = int(seed)
+        h = _vp()
+        check(L.mgimc_create_sphere(C.byref(h), self.I, _p(self.e), self.G, _p(self.edges), self.precision, self.seed,
+                                    device))
+        self._h = h
+        self._L = L
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._L.mgimc_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_step(self, step):
+        check(self._L.mgimc_set_step(self._h, int(step) & 0xFFFFFFFF))
+
+    def set_seed(self, seed):
+        self.seed = int(seed)
+        check(self._L.mgimc_set_seed(self._h, self.seed))
+
+    def set_launch(self, grid=0, batch=256, tally_copies=8):
+        check(self._L.mgimc_set_launch(self._h, int(grid), int(batch), int(tally_copies)))
+
+    @property
+    def n(self):
+        return int(self._L.mgimc_census_size(self._h))
+
+    def put_census(self, w, mu, t, x, cell, g):
+        n = len(w)
+        f = [np.ascontiguousarray(a, dtype=np.float64) for a in (w, mu, t, x)]
+        z = np.zeros(n)
+        ci = np.ascontiguousarray(cell, dtype=np.int64)
+        cj = np.zeros(n, dtype=np.int64)
+        gg = np.ascontiguousarray(g, dtype=np.int32)
+        check(self._L.mgimc_put_census(self._h, n, _p(f[0]), _p(f[1]), _p(z), _p(f[2]), _p(f[3]), _p(z),
+                                       _p(ci, _i64p), _p(cj, _i64p), _p(gg, _i32p)))
+
+    def get_census(self):
+        """(w, mu, t, x, cell int64, group int32)."""
+        n = self.n
+        f = [np.empty(n) for _ in range(6)]
+        ii = [np.empty(n, dtype=np.int64) for _ in range(2)]
+        gg = np.empty(n, dtype=np.int32)
+        check(self._L.mgimc_get_census(self._h, n, *[_p(a) for a in f], *[_p(a, _i64p) for a in ii],
+                                       _p(gg, _i32p)))
+        return f[0], f[1], f[3], f[4], ii[0], gg
+
+    def set_material(self, T, sigma, cv, theta, dt, T_emit_floor=0.0, want=()):
+        """sigma: (G, I) true opacities.  want: any of 'sa', 'ss', 'E'.
+        Returns (dict of requested (G, I) arrays, emission energy total)."""
+        T = np.ascontiguousarray(np.broadcast_to(np.asarray(T, dtype=np.float64), (self.I,)))
+        s = np.ascontiguousarray(np.broadcast_to(np.asarray(sigma, dtype=np.float64), self.gshape))
+        c = np.ascontiguousarray(np.broadcast_to(np.asarray(cv, dtype=np.float64), (self.I,)))
+        out = {k: np.empty(self.gshape) for k in want}
+        et = C.c_double()
+        check(self._L.mgimc_set_material(self._h, _p(T), _p(s), _p(c), float(theta), float(dt), None,
+                                         float(T_emit_floor), _p(out.get("sa")), _p(out.get("ss")), _p(out.get("E")),
+                                         C.byref(et), None))
+        return out, et.value
+
+    def equilibrium_sample(self, N, Tr, T_emit_floor=0.0):
+        Tr = np.ascontiguousarray(np.broadcast_to(np.asarray(Tr, dtype=np.float64), (self.I,)))
+        check(self._L.mgimc_sphere_equilibrium(self._h, int(N), _p(Tr), float(T_emit_floor)))
+
+    def sample_face(self, side, n, T, dt):
+        E = C.c_double()
+        check(self._L.mgimc_sphere_face(self._h, int(side), int(n), float(T), float(dt), C.byref(E)))
+        return E.value
+
+    def sample_volume(self, kind, N, dt, QV=None):
+        q = None if QV is None else np.ascontiguousarray(np.broadcast_to(np.asarray(QV, dtype=np.float64),
+                                                                          self.gshape))
+        tot = C.c_double()
+        check(self._L.mgimc_sphere_volume(self._h, int(kind), int(N), _p(q), float(dt), C.byref(tot)))
+        return tot.value
+
+    def transport(self, dt, reflect, weight_floor=-1.0):
+        refl = np.array([1 if r else 0 for r in reflect], dtype=np.int32)
+        dep = np.empty(self.gshape)
+        si = np.empty(self.gshape)
+        bl = np.zeros(self.G)
+        st = np.zeros(8, dtype=np.int64)
+        check(self._L.mgimc_sphere_transport(self._h, float(dt), _p(refl, _i32p), float(weight_floor), _p(dep), _p(si),
+                                             _p(bl), _p(st, _i64p)))
+        return dep, si, bl, st
+
+    def last_transport_ms(self):
+        return float(self._L.mgimc_last_transport_ms(self._h))
+
+    def comb(self, Nmax, radiation=False):
+        """Returns (disc (G, I), n_alive, combed)."""
+        disc = np.empty(self.gshape)
+        alive = C.c_int64()
+        combed = C.c_int()
+        check(self._L.mgimc_sphere_comb(self._h, int(Nmax), 1 if radiation else 0, _p(disc), C.byref(alive),
+                                        C.byref(combed)))
+        return disc, alive.value, bool(combed.value)
+
+    def census(self):
+        """(energy (G, I), sum w mu (G, I), total energy) of the pool."""
+        E = np.empty(self.gshape)
+        F = np.empty(self.gshape)
+        tot = C.c_double()
+        check(self._L.mgimc_sphere_census(self._h, _p(E), _p(F), C.byref(tot)))
+        return E, F, tot.value

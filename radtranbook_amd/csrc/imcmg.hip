@@ -15,6 +15,7 @@
 #include "../../include/imc_hip.h"
 #include "imc_device.hpp"
 #include "imc_util.hpp"
+#include "imc1d_kernels.hpp"
 #include "imcmg_kernels.hpp"
 
 namespace imc {
@@ -750,6 +751,356 @@ __global__ void k_mg_get(MgView<R> P, int64_t n, double* __restrict__ w, double*
 }
 
 // ---------------------------------------------------------------------------
+// spherical 1-D multigroup (MG_IMC/MG_IMC1D.py): pool d1 = mu, p1 = r,
+// ci = cell; bins b = g I + i (the (group, cell) layout with ny = 1)
+// ---------------------------------------------------------------------------
+// a Tr^4 V per cell, zero below T_emit_floor (MG_IMC1D.py:526-528)
+__global__ void k_m1_eq_energy(int I, const double* __restrict__ Tr, const double* __restrict__ vol,
+                               double T_emit_floor, double* __restrict__ E) {
+#pragma clang fp contract(off)
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= I) return;
+  double t = Tr[i];
+  double e = kA * ((t * t) * (t * t)) * vol[i];
+  if (T_emit_floor > 0.0 && t < T_emit_floor) e = 0.0;
+  E[i] = e;
+}
+
+// Counts by rounding half to even: E_b > 0 gets max(min_one, round(N E_b /
+// E_total)) (MG_IMC1D.py:536-539, 626-628, 712-714).
+__global__ void k_m1_counts(int64_t nb, const double* __restrict__ E, int64_t N, int min_one,
+                            const double* __restrict__ Etot, int64_t* __restrict__ cnt) {
+#pragma clang fp contract(off)
+  int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  double e = E[b], et = *Etot;
+  int64_t m = 0;
+  if (e > 0.0 && et > 0.0) {
+    double d = rint((double)N * e / et);
+    if (d < (double)min_one) d = (double)min_one;
+    m = (int64_t)d;
+  }
+  cnt[b] = m;
+}
+
+__device__ __forceinline__ int64_t m1_bin_of(const int64_t* __restrict__ offs, int64_t nb, int64_t k) {
+  int64_t lo = 0, hi = nb - 1;  // last b with offs[b] <= k
+  while (lo < hi) {
+    int64_t mid = (lo + hi + 1) >> 1;
+    if (offs[mid] <= k)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ double m1_radius(double r0, double r1, double u) {
+#pragma clang fp contract(off)
+  return pow(r0 * r0 * r0 + (r1 * r1 * r1 - r0 * r0 * r0) * u, 1.0 / 3.0);
+}
+
+template <typename R>
+__device__ __forceinline__ void m1_store(MgView<R> P, int64_t o, double w, double mu, double t, double r, int i,
+                                         int g) {
+  P.w[o] = w;
+  P.d1[o] = (R)mu;
+  P.d2[o] = (R)0;
+  P.t[o] = (R)t;
+  P.p1[o] = (R)r;
+  P.p2[o] = (R)0;
+  P.ci[o] = i;
+  P.cj[o] = 0;
+  P.g[o] = g;
+}
+
+// Equilibrium census (MG_IMC1D.py:514-566).  Slots: 0 radius, 1 mu, 2..6
+// Planck group at the cell's Tr.  offs: exclusive offsets over cells.
+template <typename R>
+__global__ void k_m1_sample_ic(MgView<R> P, int64_t total, const int64_t* __restrict__ offs, int I,
+                               const double* __restrict__ e, const double* __restrict__ Tr,
+                               const double* __restrict__ E, int G, const double* __restrict__ edges,
+                               const double* __restrict__ pcdf, Key key) {
+#pragma clang fp contract(off)
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= total) return;
+  int i = (int)m1_bin_of(offs, I, k);
+  int64_t n = offs[i + 1] - offs[i];
+  Ctr ctr = make_ctr(kTagIC, (uint64_t)k, 0);
+  Pair b0 = block_d(ctr, key, 0);
+  int g = planck_group(ctr, key, 2, Tr[i], G, edges, pcdf);
+  m1_store(P, k, E[i] / (double)n, 2.0 * b0.u1 - 1.0, 0.0, m1_radius(e[i], e[i + 1], b0.u0), i, g);
+}
+
+// Volume source by (group, cell) bin (emission MG_IMC1D.py:569-650, fixed
+// source 679-743).  Slots: 0 radius, 1 mu, 2 time.
+template <typename R>
+__global__ void k_m1_sample_bins(MgView<R> P, int64_t base, int64_t total, const int64_t* __restrict__ offs,
+                                 int64_t nb, int I, const double* __restrict__ e, const double* __restrict__ E,
+                                 double dt, Key key, uint32_t step, uint32_t tag) {
+#pragma clang fp contract(off)
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= total) return;
+  int64_t b = m1_bin_of(offs, nb, k);
+  int64_t n = offs[b + 1] - offs[b];
+  int i = (int)(b % I), g = (int)(b / I);
+  Ctr ctr = make_ctr(tag, (uint64_t)k, step);
+  Pair b0 = block_d(ctr, key, 0), b1 = block_d(ctr, key, 1);
+  m1_store(P, base + k, E[b] / (double)n, 2.0 * b0.u1 - 1.0, dt * b1.u0, m1_radius(e[i], e[i + 1], b0.u0), i, g);
+}
+
+// Half-Lambertian sphere source (MG_IMC1D.py:653-676, placed by step()
+// 1061-1088).  Slots: 0 |mu| = sqrt(u), 1 time, 2..6 Planck group at T.
+template <typename R>
+__global__ void k_m1_sample_face(MgView<R> P, int64_t base, int64_t n, int side, double E, double T, double dt,
+                                 double Rf, int I, int G, const double* __restrict__ edges,
+                                 const double* __restrict__ pcdf, Key key, uint32_t step) {
+#pragma clang fp contract(off)
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  Ctr ctr = make_ctr(kTagBoundary0 + side, (uint64_t)k, step);
+  Pair b0 = block_d(ctr, key, 0);
+  double m = sqrt(b0.u0);
+  int g = planck_group(ctr, key, 2, T, G, edges, pcdf);
+  m1_store(P, base + k, E / (double)n, side == 0 ? m : -m, dt * b0.u1, Rf, side == 0 ? 0 : I - 1, g);
+}
+
+// Transport (MG_IMC1D.py:277-461): the gray spherical move (opacity cap
+// included) with group opacities; histories run while dcen > 1e-15; an
+// escape books the weight by group (tally key nb + g); a weight below the
+// floor is deposited where the particle stands.  stats: events, boundary,
+// scatter, census, 0, floor kills, reflections, non-positive distances.
+template <typename R>
+__global__ __launch_bounds__(256) void k_m1_transport(MgArgs<R> A) {
+  using R2 = typename Vec2<R>::type;
+  const double wfloor = *A.floor_ptr;
+  const int64_t stride = A.nb + A.G + 4;
+  const int copy = blockIdx.x & A.copy_mask;
+  double* const dep_t = A.dep_raw + (int64_t)copy * stride;
+  double* const path_t = A.path_raw + (int64_t)copy * A.nb;
+  __shared__ int lkey[kLdsSlots];
+  __shared__ double lval[kLdsSlots];
+  lds_tally_init(lkey, lval);
+  __syncthreads();
+  const int I = A.nx;
+  int64_t qbase = 0, qend = 0;
+  bool exhausted = false, active = false;
+  int64_t k = 0;
+  double w = 0.0;
+  R mu = 0, x = 0, dcen = 0;
+  int cell = 0, g = 0;
+  uint32_t ev = 0;
+  double acc_dep = 0.0, acc_path = 0.0;
+  int acc_bin = -1;
+  uint32_t st_ev = 0, st_bnd = 0, st_sc = 0, st_cen = 0, st_floor = 0, st_refl = 0, st_bad = 0;
+  while (true) {
+    uint64_t need = __ballot(!active);
+    if (need) {
+      if (qbase >= qend && !exhausted) {
+        unsigned long long b = 0;
+        if (__lane_id() == 0) b = atomicAdd(A.next, (unsigned long long)A.batch);
+        b = __shfl(b, 0, 64);
+        if ((int64_t)b >= A.n) {
+          exhausted = true;
+        } else {
+          qbase = (int64_t)b;
+          qend = ((int64_t)b + A.batch < A.n) ? (int64_t)b + A.batch : A.n;
+        }
+      }
+      int64_t avail = qend - qbase;
+      if (avail > 0) {
+        int64_t rank = __popcll(need & lanemask_lt());
+        if (!active && rank < avail) {
+          k = qbase + rank;
+          w = A.P.w[k];
+          mu = A.P.d1[k];
+          x = A.P.p1[k];
+          cell = A.P.ci[k];
+          g = A.P.g[k];
+          dcen = (R)((A.dt - (double)A.P.t[k]) * kC);
+          ev = 0;
+          active = dcen > (R)1e-15 && cell >= 0 && cell < I;
+        }
+        int64_t take = __popcll(need);
+        qbase += take < avail ? take : avail;
+      }
+    }
+    if (__ballot(active) == 0) {
+      if (exhausted) break;
+      continue;
+    }
+    if (!active) continue;
+    bool finish = false;
+    ++st_ev;
+    const int bin = g * I + cell;
+    R2 edg = A.e1p[cell];
+    R2 mat = A.sas[bin];
+    Move1D<R> o;
+    Ctr ctr = make_ctr(kTagTransport, (uint64_t)k, A.step);
+    move_1d<1>(ctr, A.key, ev, w, mu, x, edg.x, edg.y, mat.x, mat.y, dcen, o);
+    ++ev;
+    if (o.bad) {
+      ++st_bad;
+      finish = true;
+    } else {
+      if (bin != acc_bin) {
+        if (acc_bin >= 0) {
+          if (!lds_tally_add(lkey, lval, acc_bin, acc_dep)) atomic_add_f64(dep_t + acc_bin, acc_dep);
+          if (acc_path != 0.0) atomic_add_f64(path_t + acc_bin, acc_path);
+        }
+        acc_bin = bin;
+        acc_dep = 0.0;
+        acc_path = 0.0;
+      }
+      acc_dep += o.dep;
+      if (!(mat.x > (R)1e-10)) acc_path += o.path;
+      w = o.w;
+      mu = o.mu;
+      x = o.x;
+      cell += o.newloc;
+      dcen = o.dcen;
+      st_bnd += o.code == 1;
+      st_sc += o.code == 2;
+      st_cen += o.code == 3;
+      bool gone = false;
+      if (cell >= I || cell < 0) {
+        int face = cell < 0 ? 0 : 1;
+        if ((A.reflect >> face) & 1) {
+          mu = -mu;
+          cell = face == 0 ? 0 : I - 1;
+          ++st_refl;
+        } else {
+          int kg = (int)(A.nb + g);
+          if (!lds_tally_add(lkey, lval, kg, w)) atomic_add_f64(dep_t + kg, w);
+          w = 0.0;
+          gone = true;
+        }
+      }
+      if (gone) {
+        finish = true;
+      } else if (w < wfloor && w > 0.0) {
+        ++st_floor;
+        const int b2 = g * I + cell;
+        if (!lds_tally_add(lkey, lval, b2, w)) atomic_add_f64(dep_t + b2, w);
+        w = 0.0;
+        finish = true;
+      } else if (!(dcen > (R)1e-15)) {
+        finish = true;
+      }
+    }
+    if (finish) {
+      A.P.w[k] = w;
+      A.P.d1[k] = mu;
+      A.P.p1[k] = x;
+      A.P.ci[k] = cell;
+      active = false;
+    }
+  }
+  if (acc_bin >= 0) {
+    if (!lds_tally_add(lkey, lval, acc_bin, acc_dep)) atomic_add_f64(dep_t + acc_bin, acc_dep);
+    if (acc_path != 0.0) atomic_add_f64(path_t + acc_bin, acc_path);
+  }
+  __syncthreads();
+  lds_tally_flush(lkey, lval, dep_t);
+  unsigned long long s0 = wave_sum<unsigned long long>(st_ev);
+  unsigned long long s1 = wave_sum<unsigned long long>(st_bnd);
+  unsigned long long s2 = wave_sum<unsigned long long>(st_sc);
+  unsigned long long s3 = wave_sum<unsigned long long>(st_cen);
+  unsigned long long s5 = wave_sum<unsigned long long>(st_floor);
+  unsigned long long s6 = wave_sum<unsigned long long>(st_refl);
+  unsigned long long s7 = wave_sum<unsigned long long>(st_bad);
+  if (__lane_id() == 0) {
+    atomicAdd(A.stats + 0, s0);
+    atomicAdd(A.stats + 1, s1);
+    atomicAdd(A.stats + 2, s2);
+    atomicAdd(A.stats + 3, s3);
+    atomicAdd(A.stats + 5, s5);
+    atomicAdd(A.stats + 6, s6);
+    atomicAdd(A.stats + 7, s7);
+  }
+}
+
+__global__ void k_m1_floor(const double* __restrict__ sumw, int64_t n, double* __restrict__ out) {
+#pragma clang fp contract(off)
+  *out = 1e-14 * (*sumw) / (double)(n > 1 ? n : 1);  // MG_IMC1D.py:1138
+}
+
+// dep = sum dep / V; si = sum dep / (sigma_a dt V) where sigma_a > 1e-10,
+// else sum w s / (dt V) (MG_IMC1D.py:341-347, 413-414, 455-458); loss by group.
+__global__ void k_m1_finalize(int64_t nb, int I, int G, int copies, const double* __restrict__ dep_raw,
+                              const double* __restrict__ path_raw, const double* __restrict__ sa64,
+                              const float2* __restrict__ sas32, int f32, const double* __restrict__ vol, double dt,
+                              double* __restrict__ dep, double* __restrict__ si, double* __restrict__ bl) {
+  int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t stride = nb + G + 4;
+  if (b < nb) {
+    double a = 0.0, p = 0.0;
+    for (int k = 0; k < copies; ++k) {
+      a += dep_raw[(int64_t)k * stride + b];
+      p += path_raw[(int64_t)k * nb + b];
+    }
+    int64_t c = b % I;
+    double sa = f32 ? (double)sas32[b].x : sa64[b];
+    bool thick = f32 ? (sas32[b].x > 1e-10f) : (sa64[b] > 1e-10);
+    dep[b] = a / vol[c];
+    si[b] = thick ? a / sa / (dt * vol[c]) : p / (dt * vol[c]);
+  } else if (b < nb + G) {
+    double a = 0.0;
+    for (int k = 0; k < copies; ++k) a += dep_raw[(int64_t)k * stride + b];
+    bl[b - nb] = a;
+  }
+}
+
+// Energy of bins whose combed copies all vanished: the one loss the
+// "radiation" mode cannot rescale away.
+__global__ void k_m1_lost(int64_t nb, const double* __restrict__ ecen, const double* __restrict__ post,
+                          double* __restrict__ lost) {
+  int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < nb) lost[b] = post[b] > 0.0 ? 0.0 : ecen[b];
+}
+
+// Histogram of w mu per bin over live particles (the radial flux tally,
+// MG_IMC1D.py:1270-1276).
+template <typename R>
+__global__ __launch_bounds__(256) void k_m1_flux(MgView<R> P, int64_t n, int I, int64_t nb, int copy_mask,
+                                                 double* __restrict__ part) {
+  __shared__ int lkey[kLdsSlots];
+  __shared__ double lval[kLdsSlots];
+  lds_tally_init(lkey, lval);
+  __syncthreads();
+  double* const dst = part + (int64_t)(blockIdx.x & copy_mask) * nb;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t trips = (n + stride - 1) / stride;
+  for (int64_t tr = 0; tr < trips; ++tr) {
+    int64_t k = tr * stride + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int key = -1;
+    double v = 0.0;
+    if (k < n) {
+      key = mg_bin(P, k, I, 1);
+      if (key >= 0) v = P.w[k] * (double)P.d1[k];
+    }
+    bool head = wave_run_sum(key, v);
+    if (head && key >= 0 && !lds_tally_add(lkey, lval, key, v)) atomic_add_f64(dst + key, v);
+  }
+  __syncthreads();
+  lds_tally_flush(lkey, lval, dst);
+}
+
+// "radiation" comb mode: survivors of each bin rescaled to the bin's
+// pre-comb energy (MG_IMC1D.py:811-823).
+template <typename R>
+__global__ void k_m1_rescale(MgView<R> P, int64_t n, int I, const double* __restrict__ ecen,
+                             const double* __restrict__ post) {
+#pragma clang fp contract(off)
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  int b = mg_bin(P, k, I, 1);
+  if (b < 0) return;
+  double q = post[b];
+  if (q > 0.0) P.w[k] = P.w[k] * (ecen[b] / (q + 1e-300));
+}
+
+// ---------------------------------------------------------------------------
 // context
 // ---------------------------------------------------------------------------
 struct MgCtx {
@@ -815,10 +1166,11 @@ extern "C" {
 
 struct mgimc_ctx : public MgCtx {};
 
-int mgimc_create(mgimc_ctx** out, int geometry, int nx, int ny, const double* edges1, const double* edges2, int G,
-                 const double* energy_edges, int precision, uint64_t seed, int device) {
+static int mg_create(mgimc_ctx** out, int geometry, int nx, int ny, const double* edges1, const double* edges2, int G,
+                     const double* energy_edges, int precision, uint64_t seed, int device) {
   IMC_REQUIRE(out && edges1 && edges2 && energy_edges, kErrArg, "mgimc_create: null argument");
-  IMC_REQUIRE(geometry == IMC_GEOM_XY || geometry == IMC_GEOM_RZ, kErrArg, "mgimc_create: bad geometry");
+  IMC_REQUIRE(geometry == IMC_GEOM_XY || geometry == IMC_GEOM_RZ || geometry == IMC_GEOM_SPHERE, kErrArg,
+              "mgimc_create: bad geometry");
   IMC_REQUIRE(nx > 0 && ny > 0 && G > 0 && G <= kMaxGroups, kErrArg, "mgimc_create: bad sizes");
   IMC_REQUIRE(precision == IMC_FP64 || precision == IMC_FP32_TRACK, kErrArg, "mgimc_create: bad precision");
   for (int a = 0; a < nx; ++a) IMC_REQUIRE(edges1[a + 1] > edges1[a], kErrArg, "mgimc_create: edges1 not increasing");
@@ -826,7 +1178,7 @@ int mgimc_create(mgimc_ctx** out, int geometry, int nx, int ny, const double* ed
   for (int a = 0; a < G; ++a)
     IMC_REQUIRE(energy_edges[a + 1] > energy_edges[a] && energy_edges[a] >= 0.0, kErrArg,
                 "mgimc_create: energy edges not increasing");
-  if (geometry == IMC_GEOM_RZ) IMC_REQUIRE(edges1[0] >= 0.0, kErrArg, "mgimc_create: negative radius");
+  if (geometry != IMC_GEOM_XY) IMC_REQUIRE(edges1[0] >= 0.0, kErrArg, "mgimc_create: negative radius");
   IMC_REQUIRE((int64_t)nx * ny * G + G + 4 < (int64_t)0x7fffffff, kErrArg, "mgimc_create: too many bins");
   IMC_HIP(hipSetDevice(device));
   mgimc_ctx* c = new mgimc_ctx();
@@ -874,15 +1226,19 @@ int mgimc_create(mgimc_ctx** out, int geometry, int nx, int ny, const double* ed
   IMC_HIP(hipMemcpy(c->e2.p, edges2, sizeof(double) * (ny + 1), hipMemcpyHostToDevice));
   IMC_HIP(hipMemcpy(c->edges.p, energy_edges, sizeof(double) * (G + 1), hipMemcpyHostToDevice));
   {
-    // cell volumes (MG_IMC2D.py:146-157)
+    // cell volumes (MG_IMC2D.py:146-157; sphere 4/3 pi (r1^3 - r0^3),
+    // MG_IMC1D.py:920)
     std::vector<double> vol((size_t)nc);
     for (int a = 0; a < nx; ++a)
       for (int b = 0; b < ny; ++b) {
         double dy = edges2[b + 1] - edges2[b];
+        double r0 = edges1[a], r1 = edges1[a + 1];
         if (geometry == IMC_GEOM_XY)
-          vol[(size_t)a * ny + b] = (edges1[a + 1] - edges1[a]) * dy;
+          vol[(size_t)a * ny + b] = (r1 - r0) * dy;
+        else if (geometry == IMC_GEOM_RZ)
+          vol[(size_t)a * ny + b] = M_PI * (r1 * r1 - r0 * r0) * dy;
         else
-          vol[(size_t)a * ny + b] = M_PI * (edges1[a + 1] * edges1[a + 1] - edges1[a] * edges1[a]) * dy;
+          vol[(size_t)a * ny + b] = (4.0 / 3.0) * M_PI * (r1 * r1 * r1 - r0 * r0 * r0);
       }
     IMC_HIP(hipMemcpy(c->vol.p, vol.data(), sizeof(double) * nc, hipMemcpyHostToDevice));
     // Planck mixture CDF over n = 1..200 (MG_IMC2D.py:99-101), sequential sums
@@ -917,6 +1273,20 @@ int mgimc_create(mgimc_ctx** out, int geometry, int nx, int ny, const double* ed
   c->A.rsz = c->B.rsz = precision == IMC_FP64 ? 8 : 4;
   *out = c;
   return kOk;
+}
+
+int mgimc_create(mgimc_ctx** out, int geometry, int nx, int ny, const double* edges1, const double* edges2, int G,
+                 const double* energy_edges, int precision, uint64_t seed, int device) {
+  IMC_REQUIRE(geometry == IMC_GEOM_XY || geometry == IMC_GEOM_RZ, kErrArg, "mgimc_create: bad geometry");
+  return mg_create(out, geometry, nx, ny, edges1, edges2, G, energy_edges, precision, seed, device);
+}
+
+// Spherical 1-D multigroup context (MG_IMC/MG_IMC1D.py): I radial cells, the
+// pool's d1 is mu, p1 the radius, ci the cell; d2, p2, cj stay unused.
+int mgimc_create_sphere(mgimc_ctx** out, int I, const double* edges, int G, const double* energy_edges,
+                        int precision, uint64_t seed, int device) {
+  const double unit[2] = {0.0, 1.0};
+  return mg_create(out, IMC_GEOM_SPHERE, I, 1, edges, unit, G, energy_edges, precision, seed, device);
 }
 
 void mgimc_destroy(mgimc_ctx* c) {
@@ -1399,4 +1769,351 @@ int mgimc_comb(mgimc_ctx* c, int64_t Nmax, double* ecen_out, double* disc_out, d
   return kOk;
 }
 
+// ---------------------------------------------------------------------------
+// spherical 1-D multigroup entry points (MG_IMC/MG_IMC1D.py)
+// ---------------------------------------------------------------------------
+// Round-rule counts of E (nb entries, device) into c->counts, exclusive
+// offsets into c->copies-free scratch c->scan: returns the total.
+static int m1_counts_offsets(mgimc_ctx* c, const double* E, int64_t nb, int64_t N, int min_one, int64_t* total) {
+  hipStream_t s = c->stream;
+  int rc;
+  if ((rc = c->sum_scr.reserve(canonical_sum_scratch(nb) + 8)) || (rc = c->counts.reserve(2 * (nb + 2))) ||
+      (rc = c->scan_scr.reserve(scan_scratch(nb) + 8)))
+    return rc;
+  if ((rc = canonical_sum(E, nb, c->scal.p + 6, c->sum_scr.p, s))) return rc;
+  int64_t* cnt = c->counts.p;
+  int64_t* offs = c->counts.p + nb + 2;
+  k_m1_counts<<<launch_blocks(nb, 256), 256, 0, s>>>(nb, E, N, min_one, c->scal.p + 6, cnt);
+  IMC_HIP(hipGetLastError());
+  if ((rc = exclusive_scan_i64(cnt, nb, offs, c->scan_scr.p, s))) return rc;
+  IMC_HIP(hipMemcpyAsync(total, offs + nb, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  IMC_HIP(hipStreamSynchronize(s));
+  return kOk;
+}
+
+static int require_sphere(mgimc_ctx* c, const char* fn) {
+  IMC_REQUIRE(c && c->geometry == IMC_GEOM_SPHERE, kErrArg, "%s: needs a context from mgimc_create_sphere", fn);
+  return kOk;
+}
+
+// Equilibrium census at t = 0 replacing the pool (MG_IMC1D.py:514-566).
+int mgimc_sphere_equilibrium(mgimc_ctx* c, int64_t N, const double* Tr, double T_emit_floor) {
+  int rc;
+  if ((rc = require_sphere(c, "mgimc_sphere_equilibrium"))) return rc;
+  IMC_REQUIRE(Tr && N >= 0, kErrArg, "mgimc_sphere_equilibrium: bad arguments");
+  IMC_HIP(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  const int I = c->nx;
+  IMC_HIP(hipMemcpyAsync(c->T.p, Tr, sizeof(double) * I, hipMemcpyHostToDevice, s));
+  k_m1_eq_energy<<<launch_blocks(I, 256), 256, 0, s>>>(I, c->T.p, c->vol.p, T_emit_floor, c->Ecell.p);
+  IMC_HIP(hipGetLastError());
+  c->n = 0;
+  c->have_material = false;  // T now holds Tr
+  int64_t total = 0;
+  if ((rc = m1_counts_offsets(c, c->Ecell.p, I, N, 1, &total))) return rc;
+  if (N <= 0 || total <= 0) return kOk;
+  if ((rc = c->A.grow(total, 0, s))) return rc;
+  Key key = make_key(c->seed);
+  const int64_t* offs = c->counts.p + I + 2;
+  int g = launch_blocks(total, 256);
+  if (c->precision == IMC_FP64)
+    k_m1_sample_ic<double><<<g, 256, 0, s>>>(c->A.view<double>(), total, offs, I, c->e1.p, c->T.p, c->Ecell.p,
+                                             c->G, c->edges.p, c->pcdf.p, key);
+  else
+    k_m1_sample_ic<float><<<g, 256, 0, s>>>(c->A.view<float>(), total, offs, I, c->e1.p, c->T.p, c->Ecell.p, c->G,
+                                            c->edges.p, c->pcdf.p, key);
+  IMC_HIP(hipGetLastError());
+  IMC_HIP(hipStreamSynchronize(s));
+  c->n = total;
+  return kOk;
+}
+
+// Half-Lambertian source on the inner (side 0) or outer (side 1) sphere,
+// appended to the pool.  E_out: a c T^4 / 4 dt 4 pi R^2 (MG_IMC1D.py:668).
+int mgimc_sphere_face(mgimc_ctx* c, int side, int64_t n, double T, double dt, double* E_out) {
+  int rc;
+  if ((rc = require_sphere(c, "mgimc_sphere_face"))) return rc;
+  IMC_REQUIRE((side == 0 || side == 1) && n >= 0, kErrArg, "mgimc_sphere_face: bad arguments");
+  const int I = c->nx;
+  double R = side == 0 ? c->h_e1[0] : c->h_e1[I];
+  double t4 = (T * T) * (T * T);
+  double E = kA * kC * t4 / 4.0 * dt * 4.0 * kPi * R * R;
+  if (E_out) *E_out = E;
+  if (n == 0) return kOk;
+  IMC_HIP(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  if ((rc = c->A.grow(c->n + n, c->n, s))) return rc;
+  Key key = make_key(c->seed);
+  int g = launch_blocks(n, 256);
+  if (c->precision == IMC_FP64)
+    k_m1_sample_face<double><<<g, 256, 0, s>>>(c->A.view<double>(), c->n, n, side, E, T, dt, R, I, c->G, c->edges.p,
+                                               c->pcdf.p, key, c->step);
+  else
+    k_m1_sample_face<float><<<g, 256, 0, s>>>(c->A.view<float>(), c->n, n, side, E, T, dt, R, I, c->G, c->edges.p,
+                                              c->pcdf.p, key, c->step);
+  IMC_HIP(hipGetLastError());
+  c->n += n;
+  return kOk;
+}
+
+// Volume source by (group, cell) bin appended to the pool.  kind
+// IMC_SRC_EMISSION: the emission energies of the last mgimc_set_material,
+// at least one particle per emitting bin; IMC_SRC_FIXED: QV_host (G, I)
+// source energies this step, bins rounding to zero get none.  total_out:
+// the energy of the particles added (0 when none are).
+int mgimc_sphere_volume(mgimc_ctx* c, int kind, int64_t N, const double* QV_host, double dt, double* total_out) {
+  int rc;
+  if ((rc = require_sphere(c, "mgimc_sphere_volume"))) return rc;
+  IMC_REQUIRE((kind == IMC_SRC_EMISSION || (kind == IMC_SRC_FIXED && QV_host)) && N >= 0, kErrArg,
+              "mgimc_sphere_volume: bad arguments");
+  IMC_REQUIRE(kind != IMC_SRC_EMISSION || c->have_material, kErrState,
+              "mgimc_sphere_volume: call mgimc_set_material first");
+  IMC_HIP(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  const int64_t nb = c->nb;
+  const double* E = c->E.p;
+  if (kind == IMC_SRC_FIXED) {
+    IMC_HIP(hipMemcpyAsync(c->tmp.p, QV_host, sizeof(double) * nb, hipMemcpyHostToDevice, s));
+    E = c->tmp.p;
+  }
+  if (total_out) *total_out = 0.0;
+  if (N <= 0) return kOk;
+  int64_t total = 0;
+  if ((rc = m1_counts_offsets(c, E, nb, N, kind == IMC_SRC_EMISSION ? 1 : 0, &total))) return rc;
+  if (total <= 0) return kOk;
+  if ((rc = c->A.grow(c->n + total, c->n, s))) return rc;
+  Key key = make_key(c->seed);
+  const int64_t* offs = c->counts.p + nb + 2;
+  uint32_t tag = kind == IMC_SRC_EMISSION ? kTagEmission : kTagSource;
+  int g = launch_blocks(total, 256);
+  if (c->precision == IMC_FP64)
+    k_m1_sample_bins<double><<<g, 256, 0, s>>>(c->A.view<double>(), c->n, total, offs, nb, c->nx, c->e1.p, E, dt,
+                                               key, c->step, tag);
+  else
+    k_m1_sample_bins<float><<<g, 256, 0, s>>>(c->A.view<float>(), c->n, total, offs, nb, c->nx, c->e1.p, E, dt, key,
+                                              c->step, tag);
+  IMC_HIP(hipGetLastError());
+  // energy of the added particles: the bins that received particles
+  if (total_out) {
+    std::vector<double> e(nb);
+    std::vector<int64_t> cnt(nb);
+    IMC_HIP(hipMemcpyAsync(e.data(), E, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
+    IMC_HIP(hipMemcpyAsync(cnt.data(), c->counts.p, sizeof(int64_t) * nb, hipMemcpyDeviceToHost, s));
+    IMC_HIP(hipStreamSynchronize(s));
+    double t = 0.0;
+    for (int64_t b = 0; b < nb; ++b)
+      if (cnt[b] > 0) t += e[b];
+    *total_out = t;
+  } else {
+    IMC_HIP(hipStreamSynchronize(s));
+  }
+  c->n += total;
+  return kOk;
+}
+
+// Transport the pool (MG_IMC1D.py:368-507).  weight_floor < 0: 1e-14 times
+// the mean weight (canonical sum).  Outputs (nullable): dep, si (G, I); bl
+// (G) weight escaping by group; stats[8] = events, boundary crossings,
+// scatters, census, 0, floor kills, reflections, non-positive distances.
+int mgimc_sphere_transport(mgimc_ctx* c, double dt, const int32_t* reflect, double weight_floor, double* dep_out,
+                           double* si_out, double* bl_out, int64_t* stats_out) {
+  int rc;
+  if ((rc = require_sphere(c, "mgimc_sphere_transport"))) return rc;
+  IMC_REQUIRE(reflect, kErrArg, "mgimc_sphere_transport: null reflect");
+  IMC_REQUIRE(c->have_material, kErrState, "mgimc_sphere_transport: call mgimc_set_material first");
+  IMC_HIP(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  int64_t n = c->n, nb = c->nb;
+  int G = c->G;
+  IMC_HIP(hipMemsetAsync(c->dep_raw.p, 0, sizeof(double) * (nb + G + 4) * c->tally_copies, s));
+  IMC_HIP(hipMemsetAsync(c->path_raw.p, 0, sizeof(double) * nb * c->tally_copies, s));
+  IMC_HIP(hipMemsetAsync(c->stats, 0, sizeof(unsigned long long) * 8, s));
+  IMC_HIP(hipMemsetAsync(c->next, 0, sizeof(unsigned long long), s));
+  if (weight_floor < 0.0) {
+    if ((rc = c->sum_scr.reserve(canonical_sum_scratch(n) + 8))) return rc;
+    if ((rc = canonical_sum((const double*)c->A.a[0], n, c->scal.p + 1, c->sum_scr.p, s))) return rc;
+    k_m1_floor<<<1, 1, 0, s>>>(c->scal.p + 1, n, c->scal.p + 2);
+    IMC_HIP(hipGetLastError());
+  } else {
+    IMC_HIP(hipMemcpyAsync(c->scal.p + 2, &weight_floor, sizeof(double), hipMemcpyHostToDevice, s));
+  }
+  int refl = (reflect[0] ? 1 : 0) | (reflect[1] ? 2 : 0);
+  int block = 256, grid;
+  const void* fn =
+      c->precision == IMC_FP64 ? (const void*)k_m1_transport<double> : (const void*)k_m1_transport<float>;
+  if (c->grid_override > 0) {
+    grid = c->grid_override;
+  } else {
+    int per_cu = 0;
+    IMC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, block, 0));
+    grid = std::max(per_cu, 1) * c->n_cu;
+  }
+  grid = (int)std::max<int64_t>(std::min<int64_t>(grid, (n + block - 1) / block), 1);
+  Key key = make_key(c->seed);
+  IMC_HIP(hipEventRecord(c->ev0, s));
+  if (n > 0) {
+    if (c->precision == IMC_FP64) {
+      MgArgs<double> A{c->A.view<double>(), n, c->e1p64.p, c->e2p64.p, c->nx, 1, G, c->sas64.p, dt, refl, 0,
+                       c->scal.p + 2, key, c->step, c->next, c->batch, c->dep_raw.p, c->path_raw.p, nb,
+                       c->tally_copies - 1, c->stats};
+      k_m1_transport<double><<<grid, block, 0, s>>>(A);
+    } else {
+      MgArgs<float> A{c->A.view<float>(), n, c->e1p32.p, c->e2p32.p, c->nx, 1, G, c->sas32.p, dt, refl, 0,
+                      c->scal.p + 2, key, c->step, c->next, c->batch, c->dep_raw.p, c->path_raw.p, nb,
+                      c->tally_copies - 1, c->stats};
+      k_m1_transport<float><<<grid, block, 0, s>>>(A);
+    }
+    IMC_HIP(hipGetLastError());
+  }
+  IMC_HIP(hipEventRecord(c->ev1, s));
+  k_m1_finalize<<<launch_blocks(nb + G, 256), 256, 0, s>>>(nb, c->nx, G, c->tally_copies, c->dep_raw.p,
+                                                            c->path_raw.p, c->sa.p, c->sas32.p,
+                                                            c->precision == IMC_FP32_TRACK, c->vol.p, dt, c->dep.p,
+                                                            c->si.p, c->bl.p);
+  IMC_HIP(hipGetLastError());
+  if (dep_out) IMC_HIP(hipMemcpyAsync(dep_out, c->dep.p, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
+  if (si_out) IMC_HIP(hipMemcpyAsync(si_out, c->si.p, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
+  if (bl_out) IMC_HIP(hipMemcpyAsync(bl_out, c->bl.p, sizeof(double) * G, hipMemcpyDeviceToHost, s));
+  unsigned long long st[8];
+  IMC_HIP(hipMemcpyAsync(st, c->stats, sizeof(st), hipMemcpyDeviceToHost, s));
+  IMC_HIP(hipStreamSynchronize(s));
+  IMC_HIP(hipEventElapsedTime(&c->last_ms, c->ev0, c->ev1));
+  if (stats_out)
+    for (int a = 0; a < 8; ++a) stats_out[a] = (int64_t)st[a];
+  return kOk;
+}
+
+// Post-transport census handling (MG_IMC1D.py:1147-1215): Nmax > 0 combs
+// the live particles per (cell, group) bin to about Nmax; Nmax < 0 combs to
+// |Nmax| only once at least |Nmax| particles live; otherwise dead particles
+// are dropped and the rest kept.  radiation != 0: survivors of each bin are
+// rescaled to the bin's pre-comb energy.  Times reset to 0.  Outputs
+// (nullable): disc (G, I) = pre - post comb energy per bin (radiation mode:
+// the energy of bins left without copies); n_alive, combed.
+int mgimc_sphere_comb(mgimc_ctx* c, int64_t Nmax, int radiation, double* disc_out, int64_t* n_alive_out,
+                      int* combed_out) {
+  int rc;
+  if ((rc = require_sphere(c, "mgimc_sphere_comb"))) return rc;
+  IMC_HIP(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  int64_t n = c->n, nb = c->nb;
+  const int I = c->nx;
+  if ((rc = c->copies.reserve(n + 1)) || (rc = c->counts.reserve(n + 2)) ||
+      (rc = c->scan_scr.reserve(scan_scratch(n) + 8)))
+    return rc;
+  Key key = make_key(c->seed);
+  int g = launch_blocks(n, 256);
+  // live particles (w > 0): pass-through copies
+  if (n > 0) {
+    if (c->precision == IMC_FP64)
+      k_mg_copies<double><<<g, 256, 0, s>>>(c->A.view<double>(), n, I, 1, c->ew.p, 1, key, c->step, c->copies.p);
+    else
+      k_mg_copies<float><<<g, 256, 0, s>>>(c->A.view<float>(), n, I, 1, c->ew.p, 1, key, c->step, c->copies.p);
+    IMC_HIP(hipGetLastError());
+  }
+  if ((rc = exclusive_scan_i32_to_i64(c->copies.p, n, c->counts.p, c->scan_scr.p, s))) return rc;
+  int64_t alive = 0;
+  IMC_HIP(hipMemcpyAsync(&alive, c->counts.p + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  IMC_HIP(hipStreamSynchronize(s));
+  if (n_alive_out) *n_alive_out = alive;
+  int64_t target = Nmax < 0 ? -Nmax : Nmax;
+  bool comb = Nmax > 0 || (Nmax < 0 && alive >= target);
+  if (combed_out) *combed_out = comb ? 1 : 0;
+  if (comb) {
+    if ((rc = mg_histogram(c, c->A, n, c->ecen.p))) return rc;
+    if ((rc = c->sum_scr.reserve(canonical_sum_scratch(nb) + 8))) return rc;
+    if ((rc = canonical_sum(c->ecen.p, nb, c->scal.p + 4, c->sum_scr.p, s))) return rc;
+    double E = 0.0;
+    IMC_HIP(hipMemcpyAsync(&E, c->scal.p + 4, sizeof(double), hipMemcpyDeviceToHost, s));
+    IMC_HIP(hipStreamSynchronize(s));
+    comb = E > 0.0;
+    if (comb) {
+      k_mg_ew<<<launch_blocks(nb, 256), 256, 0, s>>>(nb, c->ecen.p, c->scal.p + 4, target, c->ew.p);
+      IMC_HIP(hipGetLastError());
+      if (n > 0) {
+        if (c->precision == IMC_FP64)
+          k_mg_copies<double><<<g, 256, 0, s>>>(c->A.view<double>(), n, I, 1, c->ew.p, 0, key, c->step,
+                                                c->copies.p);
+        else
+          k_mg_copies<float><<<g, 256, 0, s>>>(c->A.view<float>(), n, I, 1, c->ew.p, 0, key, c->step, c->copies.p);
+        IMC_HIP(hipGetLastError());
+      }
+      if ((rc = exclusive_scan_i32_to_i64(c->copies.p, n, c->counts.p, c->scan_scr.p, s))) return rc;
+    }
+  }
+  int64_t total = 0;
+  IMC_HIP(hipMemcpyAsync(&total, c->counts.p + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  IMC_HIP(hipStreamSynchronize(s));
+  if ((rc = c->B.grow(std::max<int64_t>(total, 1), 0, s))) return rc;
+  if (n > 0) {
+    if (c->precision == IMC_FP64)
+      k_mg_comb_write<double><<<g, 256, 0, s>>>(c->A.view<double>(), c->B.view<double>(), n, I, 1, c->counts.p,
+                                                c->ew.p, comb ? 0 : 1);
+    else
+      k_mg_comb_write<float><<<g, 256, 0, s>>>(c->A.view<float>(), c->B.view<float>(), n, I, 1, c->counts.p,
+                                               c->ew.p, comb ? 0 : 1);
+    IMC_HIP(hipGetLastError());
+  }
+  std::swap(c->A, c->B);
+  c->n = total;
+  if (comb && (radiation || disc_out)) {
+    if ((rc = mg_histogram(c, c->A, total, c->post.p))) return rc;
+    if (radiation) {
+      if (total > 0) {
+        if (c->precision == IMC_FP64)
+          k_m1_rescale<double><<<launch_blocks(total, 256), 256, 0, s>>>(c->A.view<double>(), total, I, c->ecen.p,
+                                                                         c->post.p);
+        else
+          k_m1_rescale<float><<<launch_blocks(total, 256), 256, 0, s>>>(c->A.view<float>(), total, I, c->ecen.p,
+                                                                        c->post.p);
+        IMC_HIP(hipGetLastError());
+      }
+      k_m1_lost<<<launch_blocks(nb, 256), 256, 0, s>>>(nb, c->ecen.p, c->post.p, c->tmp.p);
+      IMC_HIP(hipGetLastError());
+    } else {
+      k_sub<<<launch_blocks(nb, 256), 256, 0, s>>>(nb, c->ecen.p, c->post.p, c->tmp.p);
+      IMC_HIP(hipGetLastError());
+    }
+    if (disc_out) IMC_HIP(hipMemcpyAsync(disc_out, c->tmp.p, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
+  } else if (disc_out) {
+    for (int64_t b = 0; b < nb; ++b) disc_out[b] = 0.0;
+  }
+  IMC_HIP(hipStreamSynchronize(s));
+  return kOk;
+}
+
+// Census diagnostics per (group, cell) bin: energy sum w and radial moment
+// sum w mu (MG_IMC1D.py:1262-1276, before the c / V scaling); total_out: sum w.
+int mgimc_sphere_census(mgimc_ctx* c, double* E_out, double* wmu_out, double* total_out) {
+  int rc;
+  if ((rc = require_sphere(c, "mgimc_sphere_census"))) return rc;
+  IMC_HIP(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  int64_t n = c->n, nb = c->nb;
+  if ((rc = mg_histogram(c, c->A, n, c->post.p))) return rc;
+  if (E_out) IMC_HIP(hipMemcpyAsync(E_out, c->post.p, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
+  if (wmu_out) {
+    IMC_HIP(hipMemsetAsync(c->ecen_part.p, 0, sizeof(double) * nb * c->tally_copies, s));
+    if (n > 0) {
+      int g = std::min(launch_blocks(n, 256), c->n_cu * 4);
+      if (c->precision == IMC_FP64)
+        k_m1_flux<double><<<g, 256, 0, s>>>(c->A.view<double>(), n, c->nx, nb, c->tally_copies - 1, c->ecen_part.p);
+      else
+        k_m1_flux<float><<<g, 256, 0, s>>>(c->A.view<float>(), n, c->nx, nb, c->tally_copies - 1, c->ecen_part.p);
+      IMC_HIP(hipGetLastError());
+    }
+    k_reduce_copies<double, double><<<launch_blocks(nb, 256), 256, 0, s>>>(nb, c->tally_copies, c->ecen_part.p,
+                                                                           c->tmp.p);
+    IMC_HIP(hipGetLastError());
+    IMC_HIP(hipMemcpyAsync(wmu_out, c->tmp.p, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
+  }
+  if ((rc = tree_sum((const double*)c->A.a[0], n, c->scal.p + 5, c->tree_scr.p, s))) return rc;
+  double tr = 0.0;
+  IMC_HIP(hipMemcpyAsync(&tr, c->scal.p + 5, sizeof(double), hipMemcpyDeviceToHost, s));
+  IMC_HIP(hipStreamSynchronize(s));
+  if (total_out) *total_out = tr;
+  return kOk;
+}
+
 }  // extern "C"
+

@@ -174,13 +174,14 @@ def test_step_energy_balance(M1, mode):
         st, info = O.step(st, 2000, 2000, 0, 3000, (0.0, 0.8), 0.005, funcs, inv, cv, np.zeros(O.I),
                           conserve_comb_energy=mode)
         scale = info["total_energy"]
-        if mode is False:
-            # the discarded comb discrepancy is the only unbooked sink
-            assert info["energy_residual"] == pytest.approx(-info["comb_discrepancy"].sum(), abs=1e-12 * scale)
-        else:
+        if mode is True:
             assert abs(info["energy_residual"]) < 1e-12 * scale
+        else:
+            # the discarded comb discrepancy (radiation mode: the energy of
+            # bins left without copies) is the only unbooked sink
+            assert info["energy_residual"] == pytest.approx(-info["comb_discrepancy"].sum(), abs=1e-12 * scale)
             if mode == "radiation":
-                assert np.all(info["comb_discrepancy"] == 0.0)
+                assert info["comb_discrepancy"].sum() <= 1e-3 * scale
     assert info["boundary_emission"] > 0 and info["N_particles"] > 0
 
 
