@@ -20,7 +20,7 @@ import time as _time
 
 import numpy as np
 
-from ._lib import A_RAD, C_LIGHT, SRC_EMISSION, SRC_FIXED, Engine1D, ImcError
+from ._lib import quarter_root, A_RAD, C_LIGHT, SRC_EMISSION, SRC_FIXED, Engine1D, ImcError
 
 __c = C_LIGHT
 __a = A_RAD
@@ -122,7 +122,7 @@ def init_simulation(Ntarget, Tinit, Tr_init, mesh, eos, inv_eos, Ntarget_ic=None
     N_ic = Ntarget if Ntarget_ic is None else Ntarget_ic
     eng.equilibrium_sample(int(N_ic), Tr_init, T_emit_floor)
     rad_cell, rad_tot = eng.census_energy()
-    radiation_temperature = (rad_cell / volumes / __a) ** 0.25
+    radiation_temperature = quarter_root(rad_cell / volumes / __a)
     total_internal = float(np.sum(internal_energy * volumes))
     total_rad = float(rad_tot[0])
     previous_total = total_internal + total_rad
@@ -198,9 +198,9 @@ def step(state, Ntarget, Nboundary, Nsource, NMax, T_boundary, dt, mesh, sigma_a
     ecen, comb_disc, ecen_total, total_rad = eng.comb(int(NMax), want_ecen=not use_scalar_intensity_Tr,
                                                       want_disc=conserve_comb_energy)
     if use_scalar_intensity_Tr:
-        radiation_temperature = (si / __a / __c) ** 0.25
+        radiation_temperature = quarter_root(si / __a / __c)
     else:
-        radiation_temperature = (ecen / volumes / __a) ** 0.25
+        radiation_temperature = quarter_root(ecen / volumes / __a)
     if conserve_comb_energy:
         internal_energy = internal_energy + comb_disc / volumes
         temperature = inv_eos(internal_energy)
@@ -380,7 +380,7 @@ class BatchIMC1D:
         for p in range(P):
             self.u[p] = self.u[p] + dep_p[p] - ec_p[p] / self.volumes[p]
             self.T[p] = self._fn(self._inv, p)(self.u[p])
-            self.Tr[p] = (si_p[p] / A_RAD / C_LIGHT) ** 0.25
+            self.Tr[p] = quarter_root(si_p[p] / A_RAD / C_LIGHT)
             total[p] = float(np.sum(self.u[p] * self.volumes[p])) + total_rad[p]
         loss = total - self.previous_total - be + leak
         disc = ecen_total - total_rad

@@ -17,7 +17,7 @@ import time as _time
 
 import numpy as np
 
-from ._lib import A_RAD, C_LIGHT, SRC_EMISSION, SRC_FIXED, Engine2D
+from ._lib import quarter_root, A_RAD, C_LIGHT, SRC_EMISSION, SRC_FIXED, Engine2D
 
 __c = C_LIGHT
 __a = A_RAD
@@ -143,7 +143,7 @@ def init_simulation(Ntarget, Tinit, Tr_init, edges1, edges2, eos, inv_eos, Ntarg
     N_ic = Ntarget if Ntarget_ic is None else Ntarget_ic
     eng.equilibrium_sample(N_ic, Tr_init)
     rad_cell = eng.census_by_cell()
-    radiation_temperature = (rad_cell / volumes / __a) ** 0.25
+    radiation_temperature = quarter_root(rad_cell / volumes / __a)
     total_internal = float(np.sum(internal_energy * volumes))
     total_rad = eng.census_energy()
     previous_total = total_internal + total_rad
@@ -169,8 +169,10 @@ def step(state, Ntarget, Nboundary, Nsource, Nmax, T_boundary, dt, edges1, edges
         raise ValueError("step(): mesh or geometry differs from init_simulation")
     e1 = eng.e1
     e2 = eng.e2
-    volumes = _cell_volumes(e1, e2, geometry)
     t_step_start = _time.perf_counter()
+    volumes = getattr(eng, "_volumes", None)
+    if volumes is None:
+        volumes = eng._volumes = _cell_volumes(e1, e2, geometry)
     state._invalidate()
     eng.set_step(state.count)
 
@@ -219,9 +221,9 @@ def step(state, Ntarget, Nboundary, Nsource, Nmax, T_boundary, dt, edges1, edges
     ecen, comb_disc, ecen_total, total_rad = eng.comb(Nmax, want_ecen=not use_scalar_intensity_Tr,
                                                       want_disc=conserve_comb_energy)
     if use_scalar_intensity_Tr:
-        radiation_temperature = (si_cell / __a / __c) ** 0.25
+        radiation_temperature = quarter_root(si_cell * (1.0 / (__a * __c)))
     else:
-        radiation_temperature = (ecen / volumes / __a) ** 0.25
+        radiation_temperature = quarter_root(ecen / volumes / __a)
     if conserve_comb_energy:
         internal_energy = internal_energy + comb_disc / volumes
         temperature = inv_eos(internal_energy)

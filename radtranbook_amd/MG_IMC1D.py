@@ -27,7 +27,7 @@ import time as _time
 
 import numpy as np
 
-from ._lib import A_RAD, C_LIGHT, SRC_EMISSION, SRC_FIXED, EngineMG1D
+from ._lib import quarter_root, A_RAD, C_LIGHT, SRC_EMISSION, SRC_FIXED, EngineMG1D
 
 __c = C_LIGHT
 __a = A_RAD
@@ -121,7 +121,7 @@ def init_simulation(Ntarget, Tinit, Tr_init, mesh, energy_edges, eos, inv_eos, N
     N_ic = Ntarget if Ntarget_ic is None else Ntarget_ic
     eng.equilibrium_sample(N_ic, np.asarray(Tr_init, dtype=np.float64), T_emit_floor)
     Eg, _, total_rad = eng.census()
-    radiation_temperature = (Eg.sum(axis=0) / volumes / __a) ** 0.25
+    radiation_temperature = quarter_root(Eg.sum(axis=0) / volumes / __a)
     total_internal = float(np.sum(internal_energy * volumes))
     prev_total = total_internal + total_rad
     if VERBOSE:
@@ -213,10 +213,10 @@ def step(state, Ntarget, Nboundary, Nsource, Nmax, T_boundary, dt, mesh, energy_
     internal_energy = state.internal_energy + dep.sum(axis=0) - emitted_by_group.sum(axis=0) / volumes
     temperature = inv_eos(internal_energy)
     if use_scalar_intensity_Tr:
-        radiation_temperature_new = (si.sum(axis=0) / __a / __c) ** 0.25
+        radiation_temperature_new = quarter_root(si.sum(axis=0) / __a / __c)
     else:
         Eg_live, _, _ = eng.census()
-        radiation_temperature_new = (Eg_live.sum(axis=0) / volumes / __a) ** 0.25
+        radiation_temperature_new = quarter_root(Eg_live.sum(axis=0) / volumes / __a)
 
     radiation_mode = conserve_comb_energy == "radiation"
     comb_disc, _, combed = eng.comb(int(Nmax), radiation=radiation_mode)

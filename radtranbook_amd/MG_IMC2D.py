@@ -26,7 +26,7 @@ import time as _time
 
 import numpy as np
 
-from ._lib import A_RAD, C_LIGHT, EngineMG
+from ._lib import A_RAD, C_LIGHT, EngineMG, quarter_root
 
 __c = C_LIGHT
 __a = A_RAD
@@ -165,7 +165,7 @@ def init_simulation(Ntarget, Tinit, Tr_init, edges1, edges2, energy_edges, eos, 
     eng.equilibrium_sample(int(N_ic), Tr_init, T_emit_floor)
     by_bin, total_rad = eng.census_by_bin()
     erg = by_bin / volumes[None, :, :]
-    radiation_temperature = (np.sum(erg, axis=0) / __a) ** 0.25
+    radiation_temperature = quarter_root(np.sum(erg, axis=0) / __a)
     total_internal = float(np.sum(internal_energy * volumes))
     previous_total = total_internal + total_rad
     if VERBOSE:
@@ -276,7 +276,7 @@ def step(state, Ntarget, Nboundary, Nsource, Nmax, T_boundary, dt, edges1, edges
         internal_energy = internal_energy + comb_disc.sum(0) / volumes
         temperature = inv_eos(internal_energy)
     erg_post = post / volumes[None, :, :]
-    radiation_temperature = (np.sum(erg_pre, axis=0) / __a) ** 0.25
+    radiation_temperature = quarter_root(np.sum(erg_pre, axis=0) / __a)
 
     total_internal = float(np.sum(internal_energy * volumes))
     total_energy = total_internal + total_rad

@@ -103,6 +103,12 @@ SIGNATURES = {
 }
 
 
+def quarter_root(x):
+    """x ** 0.25 for x >= 0 as two square roots (several times faster than
+    the generic power on the host, equal to it within 1 ulp)."""
+    return np.sqrt(np.sqrt(x))
+
+
 class ImcError(RuntimeError):
     pass
 

@@ -119,24 +119,27 @@ __global__ void k1_equilibrium_energy(int64_t n, const int32_t* __restrict__ cpr
 }
 
 // Canonical sum of each problem's cells (chunks of 1024 left to right, then
-// the chunk sums left to right); one thread per problem.
-__global__ void k1_problem_sums(int P, const int64_t* __restrict__ coff, const double* __restrict__ x,
-                                double* __restrict__ out) {
-  int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= P) return;
-  int64_t lo = coff[p], hi = coff[p + 1], n = hi - lo;
+// the chunk sums left to right); one workgroup per problem, each chunk staged
+// through LDS and added in order by one lane.
+__global__ __launch_bounds__(256) void k1_problem_sums(int P, const int64_t* __restrict__ coff,
+                                                       const double* __restrict__ x, double* __restrict__ out) {
+  __shared__ double v[1024];
+  const int p = blockIdx.x;
+  const int64_t lo = coff[p], hi = coff[p + 1];
   double tot = 0.0;
-  if (n <= 1024) {
-    for (int64_t i = lo; i < hi; ++i) tot += x[i];
-  } else {
-    for (int64_t c0 = lo; c0 < hi; c0 += 1024) {
-      int64_t c1 = c0 + 1024 < hi ? c0 + 1024 : hi;
+  for (int64_t c0 = lo; c0 < hi; c0 += 1024) {
+    const int m = (int)((hi - c0) < 1024 ? (hi - c0) : 1024);
+    __syncthreads();
+    for (int i = threadIdx.x; i < m; i += 256) v[i] = x[c0 + i];
+    __syncthreads();
+    if (threadIdx.x == 0) {
       double s = 0.0;
-      for (int64_t i = c0; i < c1; ++i) s += x[i];
+#pragma unroll 16
+      for (int i = 0; i < m; ++i) s += v[i];
       tot += s;
     }
   }
-  out[p] = tot;
+  if (threadIdx.x == 0) out[p] = tot;
 }
 
 // counts: mode 0 ceil(N E_c / E_p), zero when E_p <= 0 (IMC/IMC1D.py:277-281);
@@ -1149,7 +1152,7 @@ static int volume_counts(imc1d_ctx* c, const double* E, const int64_t* N, int mo
   hipStream_t s = c->stream;
   int P = c->P;
   IMC_HIP(hipMemcpyAsync(c->Nvec.p, N, sizeof(int64_t) * P, hipMemcpyHostToDevice, s));
-  k1_problem_sums<<<launch_blocks(P, 64), 64, 0, s>>>(P, c->coff.p, E, c->Ep.p);
+  k1_problem_sums<<<P, 256, 0, s>>>(P, c->coff.p, E, c->Ep.p);
   IMC_HIP(hipGetLastError());
   k1_counts<<<launch_blocks(c->ncell, 256), 256, 0, s>>>(c->ncell, c->cprob.p, E, c->Ep.p, c->Nvec.p, mode,
                                                          c->counts.p);
@@ -1477,7 +1480,7 @@ int imc1d_comb(imc1d_ctx* c, const int64_t* Nmax, double* ecen_out, double* disc
   }
   k_reduce_copies<double, double><<<launch_blocks(nc, 256), 256, 0, s>>>(nc, c->tally_copies, c->ecen_part.p, c->ecen.p);
   IMC_HIP(hipGetLastError());
-  k1_problem_sums<<<launch_blocks(P, 64), 64, 0, s>>>(P, c->coff.p, c->ecen.p, c->Ep.p);
+  k1_problem_sums<<<P, 256, 0, s>>>(P, c->coff.p, c->ecen.p, c->Ep.p);
   IMC_HIP(hipGetLastError());
   IMC_HIP(hipMemcpyAsync(c->Nvec.p, Nmax, sizeof(int64_t) * P, hipMemcpyHostToDevice, s));
   k1_comb_ew<<<launch_blocks(nc, 256), 256, 0, s>>>(nc, c->cprob.p, c->ecen.p, c->Ep.p, c->Nvec.p, c->ew.p);

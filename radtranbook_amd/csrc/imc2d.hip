@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <vector>
 
 #include "../../include/imc_hip.h"
@@ -1035,7 +1036,19 @@ struct Imc2dCtx {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   float last_transport_ms = 0.0f;
   double last_energy_emit = 0.0;
+  // page-locked staging: slots of ncell doubles (0 T, 1 sigma, 2 cv, 3 and 4
+  // per-cell outputs, 5 source energies), then 16 scalars
+  PinnedBuf stage;
 };
+
+static int stage_in(Imc2dCtx* c, int slot, double* dst, const double* src) {
+  double* h = c->stage.slot(c->ncell, slot);
+  std::memcpy(h, src, sizeof(double) * c->ncell);
+  IMC_HIP(hipMemcpyAsync(dst, h, sizeof(double) * c->ncell, hipMemcpyHostToDevice, c->stream));
+  return kOk;
+}
+
+static double* stage_scalars(Imc2dCtx* c) { return c->stage.slot(c->ncell, 6); }
 
 static int ensure_sum_scratch(Imc2dCtx* c, int64_t n) { return c->sum_scr.reserve(canonical_sum_scratch(n) + 8); }
 
@@ -1152,6 +1165,10 @@ int imc2d_create(imc2d_ctx** out, int geometry, int nx, int ny, const double* ed
   IMC_HIP(hipMalloc(&c->stats, sizeof(unsigned long long) * 8));
   IMC_HIP(hipMalloc(&c->next, sizeof(unsigned long long)));
   IMC_HIP(hipMalloc(&c->bad, sizeof(int)));
+  {
+    int rc0 = c->stage.reserve(sizeof(double) * (6 * (size_t)c->ncell + 16));
+    if (rc0) return rc0;
+  }
   c->A.rsz = c->B.rsz = c->S.rsz = c->snap.rsz = (precision == IMC_FP64) ? 8 : 4;
   *out = c;
   return kOk;
@@ -1165,6 +1182,7 @@ void imc2d_destroy(imc2d_ctx* c) {
                                 &c->dep_raw, &c->si_raw, &c->dep, &c->si, &c->ecen, &c->ew, &c->disc, &c->prof,
                                 &c->sum_scr, &c->wbuf, &c->volsrc};
   for (auto* b : cellbufs) b->release();
+  c->stage.release();
   c->saf.release();
   c->ssf.release();
   c->sas32.release();
@@ -1288,9 +1306,10 @@ int imc2d_set_material(imc2d_ctx* c, const double* T, const double* sigma_true, 
   IMC_REQUIRE(c && T && sigma_true && cv, kErrArg, "imc2d_set_material: null argument");
   IMC_HIP(hipSetDevice(c->device));
   size_t bytes = sizeof(double) * c->ncell;
-  IMC_HIP(hipMemcpyAsync(c->T.p, T, bytes, hipMemcpyHostToDevice, c->stream));
-  IMC_HIP(hipMemcpyAsync(c->sig.p, sigma_true, bytes, hipMemcpyHostToDevice, c->stream));
-  IMC_HIP(hipMemcpyAsync(c->cv.p, cv, bytes, hipMemcpyHostToDevice, c->stream));
+  int rc;
+  if ((rc = stage_in(c, 0, c->T.p, T)) || (rc = stage_in(c, 1, c->sig.p, sigma_true)) ||
+      (rc = stage_in(c, 2, c->cv.p, cv)))
+    return rc;
   IMC_HIP(hipMemsetAsync(c->bad, 0, sizeof(int), c->stream));
   bool f32 = c->precision == IMC_FP32_TRACK;
   k_fleck<<<launch_blocks(c->ncell, 256), 256, 0, c->stream>>>(c->ncell, c->T.p, c->sig.p, c->cv.p, theta, dt, clip,
@@ -1298,12 +1317,16 @@ int imc2d_set_material(imc2d_ctx* c, const double* T, const double* sigma_true, 
                                                                f32 ? c->ssf.p : nullptr, c->sas64.p, c->sas32.p,
                                                                c->bad);
   IMC_HIP(hipGetLastError());
-  int bad = 0;
-  IMC_HIP(hipMemcpyAsync(&bad, c->bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-  if (sigma_a_out) IMC_HIP(hipMemcpyAsync(sigma_a_out, c->sa.p, bytes, hipMemcpyDeviceToHost, c->stream));
-  if (sigma_s_out) IMC_HIP(hipMemcpyAsync(sigma_s_out, c->ss.p, bytes, hipMemcpyDeviceToHost, c->stream));
+  int* hbad = (int*)stage_scalars(c);
+  IMC_HIP(hipMemcpyAsync(hbad, c->bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  if (sigma_a_out)
+    IMC_HIP(hipMemcpyAsync(c->stage.slot(c->ncell, 3), c->sa.p, bytes, hipMemcpyDeviceToHost, c->stream));
+  if (sigma_s_out)
+    IMC_HIP(hipMemcpyAsync(c->stage.slot(c->ncell, 4), c->ss.p, bytes, hipMemcpyDeviceToHost, c->stream));
   IMC_HIP(hipStreamSynchronize(c->stream));
-  IMC_REQUIRE(!bad, kErrArg, "Fleck factor out of bounds");
+  if (sigma_a_out) std::memcpy(sigma_a_out, c->stage.slot(c->ncell, 3), bytes);
+  if (sigma_s_out) std::memcpy(sigma_s_out, c->stage.slot(c->ncell, 4), bytes);
+  IMC_REQUIRE(!*hbad, kErrArg, "Fleck factor out of bounds");
   c->emit_lo = c->emit_hi = 0;
   return kOk;
 }
@@ -1411,7 +1434,8 @@ int imc2d_sample_volume(imc2d_ctx* c, int kind, int64_t N, const double* QV_host
     QV = c->E.p;
     tag = kTagEmission;
   } else {
-    IMC_HIP(hipMemcpyAsync(c->volsrc.p, QV_host, sizeof(double) * c->ncell, hipMemcpyHostToDevice, c->stream));
+    int rc0 = stage_in(c, 5, c->volsrc.p, QV_host);
+    if (rc0) return rc0;
     QV = c->volsrc.p;
     tag = kTagSource;
   }
@@ -1422,10 +1446,14 @@ int imc2d_sample_volume(imc2d_ctx* c, int kind, int64_t N, const double* QV_host
   if (kind == IMC_SRC_EMISSION && c->fcounts.reserve(c->ncell) == kOk && !fixed)
     IMC_HIP(hipMemcpyAsync(c->fcounts.p, c->counts.p, sizeof(int64_t) * c->ncell, hipMemcpyDeviceToDevice,
                            c->stream));  // the counts used, for imc2d_emission_counts
-  double Etot = 0.0;
-  IMC_HIP(hipMemcpyAsync(&Etot, c->scal + 0, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  if (E_out) IMC_HIP(hipMemcpyAsync(E_out, QV, sizeof(double) * c->ncell, hipMemcpyDeviceToHost, c->stream));
+  double* hs = stage_scalars(c);
+  IMC_HIP(hipMemcpyAsync(hs, c->scal + 0, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (E_out)
+    IMC_HIP(hipMemcpyAsync(c->stage.slot(c->ncell, 3), QV, sizeof(double) * c->ncell, hipMemcpyDeviceToHost,
+                           c->stream));
   IMC_HIP(hipStreamSynchronize(c->stream));
+  double Etot = hs[0];
+  if (E_out) std::memcpy(E_out, c->stage.slot(c->ncell, 3), sizeof(double) * c->ncell);
   if (total_out) *total_out = Etot;
   if (count_out) *count_out = (Etot > 0.0 && N > 0) ? total : 0;
   if (!(Etot > 0.0) || N <= 0 || total <= 0) return kOk;
@@ -1527,17 +1555,21 @@ int imc2d_transport(imc2d_ctx* c, double dt, const int32_t* reflect, int64_t max
                                                                   c->vol, dt, c->dep.p, c->si.p);
   IMC_HIP(hipGetLastError());
   size_t bytes = sizeof(double) * c->ncell;
-  if (dep_out) IMC_HIP(hipMemcpyAsync(dep_out, c->dep.p, bytes, hipMemcpyDeviceToHost, s));
-  if (si_out) IMC_HIP(hipMemcpyAsync(si_out, c->si.p, bytes, hipMemcpyDeviceToHost, s));
-  double bl = 0.0;
-  unsigned long long st[8];
-  IMC_HIP(hipMemcpyAsync(&bl, c->scal + 3, sizeof(double), hipMemcpyDeviceToHost, s));
-  IMC_HIP(hipMemcpyAsync(st, c->stats, sizeof(st), hipMemcpyDeviceToHost, s));
+  double* h_dep = c->stage.slot(c->ncell, 3);
+  double* h_si = c->stage.slot(c->ncell, 4);
+  double* hs = stage_scalars(c);
+  unsigned long long* hst = (unsigned long long*)(hs + 1);
+  if (dep_out) IMC_HIP(hipMemcpyAsync(h_dep, c->dep.p, bytes, hipMemcpyDeviceToHost, s));
+  if (si_out) IMC_HIP(hipMemcpyAsync(h_si, c->si.p, bytes, hipMemcpyDeviceToHost, s));
+  IMC_HIP(hipMemcpyAsync(hs, c->scal + 3, sizeof(double), hipMemcpyDeviceToHost, s));
+  IMC_HIP(hipMemcpyAsync(hst, c->stats, sizeof(unsigned long long) * 8, hipMemcpyDeviceToHost, s));
   IMC_HIP(hipStreamSynchronize(s));
   IMC_HIP(hipEventElapsedTime(&c->last_transport_ms, c->ev0, c->ev1));
-  if (bl_out) *bl_out = bl;
+  if (dep_out) std::memcpy(dep_out, h_dep, bytes);
+  if (si_out) std::memcpy(si_out, h_si, bytes);
+  if (bl_out) *bl_out = hs[0];
   if (stats_out)
-    for (int a = 0; a < 8; ++a) stats_out[a] = (int64_t)st[a];
+    for (int a = 0; a < 8; ++a) stats_out[a] = (int64_t)hst[a];
   return kOk;
 }
 
@@ -1578,9 +1610,10 @@ int imc2d_comb(imc2d_ctx* c, int64_t Nmax, double* ecen_out, double* disc_out, d
   int rc;
   if ((rc = census_histogram(c))) return rc;
   if ((rc = canonical_sum(c->ecen.p, nc, c->scal + 4, c->sum_scr.p, s))) return rc;
-  double E = 0.0;
-  IMC_HIP(hipMemcpyAsync(&E, c->scal + 4, sizeof(double), hipMemcpyDeviceToHost, s));
+  double* hs = stage_scalars(c);
+  IMC_HIP(hipMemcpyAsync(hs, c->scal + 4, sizeof(double), hipMemcpyDeviceToHost, s));
   IMC_HIP(hipStreamSynchronize(s));
+  double E = hs[0];
   int passthrough = !(E > 0.0);
   k_comb_ew<<<launch_blocks(nc, 256), 256, 0, s>>>(nc, c->ecen.p, c->scal + 4, Nmax, c->ew.p);
   IMC_HIP(hipGetLastError());
@@ -1599,9 +1632,9 @@ int imc2d_comb(imc2d_ctx* c, int64_t Nmax, double* ecen_out, double* disc_out, d
     IMC_HIP(hipGetLastError());
   }
   if ((rc = exclusive_scan_i32_to_i64(c->copies.p, n, c->counts.p, c->scan_scr.p, s))) return rc;
-  int64_t total = 0;
-  IMC_HIP(hipMemcpyAsync(&total, c->counts.p + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  IMC_HIP(hipMemcpyAsync(hs + 1, c->counts.p + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
   IMC_HIP(hipStreamSynchronize(s));
+  int64_t total = *(int64_t*)(hs + 1);
   if ((rc = c->B.grow(std::max<int64_t>(total, 1), 0, s))) return rc;
   bool want_counts = disc_out != nullptr && !passthrough;
   if (want_counts) IMC_HIP(hipMemsetAsync(c->count_part.p, 0, sizeof(unsigned int) * nc * c->tally_copies, s));
@@ -1624,15 +1657,17 @@ int imc2d_comb(imc2d_ctx* c, int64_t Nmax, double* ecen_out, double* disc_out, d
     }
     k_comb_disc<<<launch_blocks(nc, 256), 256, 0, s>>>(nc, c->ecen.p, c->ew.p, c->cellcount.p, passthrough, c->disc.p);
     IMC_HIP(hipGetLastError());
-    IMC_HIP(hipMemcpyAsync(disc_out, c->disc.p, bytes, hipMemcpyDeviceToHost, s));
+    IMC_HIP(hipMemcpyAsync(c->stage.slot(nc, 4), c->disc.p, bytes, hipMemcpyDeviceToHost, s));
   }
   std::swap(c->A, c->B);
   c->n = total;
   if ((rc = tree_sum(c->A.w, total, c->scal + 5, c->tree_scr.p, s))) return rc;
-  if (ecen_out) IMC_HIP(hipMemcpyAsync(ecen_out, c->ecen.p, bytes, hipMemcpyDeviceToHost, s));
-  double tr = 0.0;
-  IMC_HIP(hipMemcpyAsync(&tr, c->scal + 5, sizeof(double), hipMemcpyDeviceToHost, s));
+  if (ecen_out) IMC_HIP(hipMemcpyAsync(c->stage.slot(nc, 3), c->ecen.p, bytes, hipMemcpyDeviceToHost, s));
+  IMC_HIP(hipMemcpyAsync(hs + 2, c->scal + 5, sizeof(double), hipMemcpyDeviceToHost, s));
   IMC_HIP(hipStreamSynchronize(s));
+  if (disc_out) std::memcpy(disc_out, c->stage.slot(nc, 4), bytes);
+  if (ecen_out) std::memcpy(ecen_out, c->stage.slot(nc, 3), bytes);
+  double tr = hs[2];
   if (total_rad_out) *total_rad_out = tr;
   if (ecen_total_out) *ecen_total_out = E;
   return kOk;

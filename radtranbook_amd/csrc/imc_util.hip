@@ -27,14 +27,22 @@ int launch_blocks(int64_t n, int block) {
 // ---------------------------------------------------------------------------
 // canonical sum
 // ---------------------------------------------------------------------------
-__global__ void k_chunk_sum(const double* __restrict__ x, int64_t n, double* __restrict__ out) {
-  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int64_t lo = c * 1024;
-  if (lo >= n) return;
-  int64_t hi = lo + 1024 < n ? lo + 1024 : n;
-  double s = 0.0;
-  for (int64_t i = lo; i < hi; ++i) s += x[i];
-  out[c] = s;
+// One workgroup per 1024-element chunk: the chunk is staged through LDS
+// with coalesced loads, then one lane adds it left to right (the canonical
+// order; a lane walking global memory pays a load latency per element).
+__global__ __launch_bounds__(256) void k_chunk_sum(const double* __restrict__ x, int64_t n,
+                                                   double* __restrict__ out) {
+  __shared__ double v[1024];
+  const int64_t lo = (int64_t)blockIdx.x * 1024;
+  const int m = (int)((n - lo) < 1024 ? (n - lo) : 1024);
+  for (int i = threadIdx.x; i < m; i += 256) v[i] = x[lo + i];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+#pragma unroll 16
+    for (int i = 0; i < m; ++i) s += v[i];
+    out[blockIdx.x] = s;
+  }
 }
 
 int64_t canonical_sum_scratch(int64_t n) {
@@ -57,7 +65,7 @@ int canonical_sum(const double* x, int64_t n, double* out, double* scratch, hipS
   while (true) {
     int64_t nc = (m + 1023) / 1024;
     double* target = (nc == 1) ? out : dst;
-    k_chunk_sum<<<launch_blocks(nc, 64), 64, 0, s>>>(src, m, target);
+    k_chunk_sum<<<(unsigned)nc, 256, 0, s>>>(src, m, target);
     IMC_HIP(hipGetLastError());
     if (nc == 1) break;
     src = dst;

@@ -71,6 +71,31 @@ struct DevBuf {
   }
 };
 
+// Page-locked host staging for per-cell transfers.  hipMemcpyAsync from
+// pageable memory runs through the runtime's own staging with host-side
+// waits (about 0.1 ms per 320 KB array measured); copying into a
+// page-locked buffer first costs one host memcpy and lets the DMA run
+// asynchronously.  A caller must synchronise its stream before the staged
+// region is reused.
+struct PinnedBuf {
+  char* p = nullptr;
+  size_t cap = 0;
+  int reserve(size_t n) {
+    if (n <= cap) return kOk;
+    if (p) IMC_HIP(hipHostFree(p));
+    p = nullptr;
+    IMC_HIP(hipHostMalloc((void**)&p, n, hipHostMallocDefault));
+    cap = n;
+    return kOk;
+  }
+  double* slot(int64_t n, int k) const { return (double*)p + (size_t)k * (size_t)n; }
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
 // Deterministic "canonical" sum: 1024-element chunks summed left to right,
 // repeated on the chunk sums until one value is left.  The result lands in
 // *out (device).  `scratch` must hold ceil(n/1024) + ceil(n/2^20) + 2 doubles.

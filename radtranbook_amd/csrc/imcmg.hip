@@ -125,18 +125,26 @@ __global__ void k_mg_fleck(int64_t ncell, int G, const double* __restrict__ edge
 
 // Canonical inclusive prefix: sequential within chunks of 1024, chunk offsets
 // summed sequentially (one thread), then offset + local prefix.
-__global__ void k_prefix_chunks(const double* __restrict__ x, int64_t n, double* __restrict__ local,
-                                double* __restrict__ chunk_tot) {
-  int64_t ch = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int64_t c0 = ch * 1024;
-  if (c0 >= n) return;
-  int64_t c1 = c0 + 1024 < n ? c0 + 1024 : n;
-  double s = 0.0;
-  for (int64_t i = c0; i < c1; ++i) {
-    s += x[i];
-    local[i] = s;
+// One workgroup per chunk, staged through LDS (coalesced loads and stores);
+// one lane forms the running sum in order.
+__global__ __launch_bounds__(256) void k_prefix_chunks(const double* __restrict__ x, int64_t n,
+                                                       double* __restrict__ local, double* __restrict__ chunk_tot) {
+#pragma clang fp contract(off)
+  __shared__ double v[1024];
+  const int64_t c0 = (int64_t)blockIdx.x * 1024;
+  const int m = (int)((n - c0) < 1024 ? (n - c0) : 1024);
+  for (int i = threadIdx.x; i < m; i += 256) v[i] = x[c0 + i];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int i = 0; i < m; ++i) {
+      s += v[i];
+      v[i] = s;
+    }
+    chunk_tot[blockIdx.x] = s;
   }
-  chunk_tot[ch] = s;
+  __syncthreads();
+  for (int i = threadIdx.x; i < m; i += 256) local[c0 + i] = v[i];
 }
 
 __global__ void k_prefix_offsets(const double* __restrict__ chunk_tot, int64_t nch, double* __restrict__ off) {
@@ -1131,7 +1139,7 @@ struct MgCtx {
 static int prefix_canonical(MgCtx* c, const double* x, int64_t n, double* out) {
   int64_t nch = (n + 1023) / 1024;
   hipStream_t s = c->stream;
-  k_prefix_chunks<<<launch_blocks(nch, 64), 64, 0, s>>>(x, n, c->local.p, c->chunk.p);
+  k_prefix_chunks<<<(unsigned)nch, 256, 0, s>>>(x, n, c->local.p, c->chunk.p);
   IMC_HIP(hipGetLastError());
   k_prefix_offsets<<<1, 1, 0, s>>>(c->chunk.p, nch, c->choff.p);
   IMC_HIP(hipGetLastError());
