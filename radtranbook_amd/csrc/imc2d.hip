@@ -870,26 +870,39 @@ __global__ void k_comb_ew(int64_t ncell, const double* __restrict__ ecen, const 
   }
 }
 
+// Comb decisions, one thread per particle, and each 256-particle block's
+// copy total (the block sums are scanned instead of every particle's count).
+__device__ __forceinline__ int block_sum_256i(int v, int* wsum) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = v;
+  __syncthreads();
+  return wsum[0] + wsum[1] + wsum[2] + wsum[3];
+}
+
 template <typename R>
-__global__ void k_comb_copies(PoolView<R> P, int64_t n, int nx, int ny, const double* __restrict__ ew,
-                              int passthrough, Key key, uint32_t step, int32_t* __restrict__ copies) {
+__global__ __launch_bounds__(256) void k_comb_copies(PoolView<R> P, int64_t n, int nx, int ny,
+                                                     const double* __restrict__ ew, int passthrough, Key key,
+                                                     uint32_t step, int32_t* __restrict__ copies,
+                                                     int64_t* __restrict__ bsum) {
 #pragma clang fp contract(off)
+  __shared__ int wsum[4];
   int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
-  double w = P.w[k];
   int32_t m = 0;
-  if (comb_alive(P, k, nx, ny)) {
+  if (k < n && comb_alive(P, k, nx, ny)) {
     if (passthrough) {
       m = 1;
     } else {
       double et = ew[(int64_t)P.ci[k] * ny + P.cj[k]];
       if (et > 0.0) {
         double u = slot_d(make_ctr(kTagComb, (uint64_t)k, step), key, 0);
-        m = (int32_t)(w / et + u);
+        m = (int32_t)(P.w[k] / et + u);
       }
     }
   }
-  copies[k] = m;
+  if (k < n) copies[k] = m;
+  int tot = block_sum_256i(m, wsum);
+  if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
 }
 
 // Comb output: workgroup b expands particles [256 b, 256 b + 256).  Their
@@ -900,14 +913,33 @@ __global__ void k_comb_copies(PoolView<R> P, int64_t n, int nx, int ny, const do
 template <typename R>
 __global__ __launch_bounds__(256) void k_comb_write(PoolView<R> S, PoolView<R> D, int64_t n, int nx, int ny,
                                                     const int32_t* __restrict__ copies,
-                                                    const int64_t* __restrict__ offs, const double* __restrict__ ew,
+                                                    const int64_t* __restrict__ boff, const double* __restrict__ ew,
                                                     int passthrough, int copy_mask,
                                                     unsigned int* __restrict__ count_part) {
   __shared__ int64_t so[257];
+  __shared__ int wsum[4];
   const int t = threadIdx.x;
   const int64_t k0 = (int64_t)blockIdx.x * 256;
   const int64_t k = k0 + t;
   int32_t m = k < n ? copies[k] : 0;
+  {
+    // this block's copy offsets: block offset + exclusive scan of the counts
+    const int lane = t & 63, wid = t >> 6;
+    int inc = m;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      int y = __shfl_up(inc, o, 64);
+      if (lane >= o) inc += y;
+    }
+    if (lane == 63) wsum[wid] = inc;
+    __syncthreads();
+    int wofs = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) wofs += q < wid ? wsum[q] : 0;
+    const int64_t b0 = boff[blockIdx.x];
+    so[t] = b0 + wofs + inc - m;
+    if (t == 255) so[256] = b0 + wofs + inc;
+  }
   if (count_part) {
     int key = m > 0 ? S.ci[k] * ny + S.cj[k] : -1;
     unsigned int cnt = m > 0 ? (unsigned int)m : 0u;
@@ -916,9 +948,7 @@ __global__ __launch_bounds__(256) void k_comb_write(PoolView<R> S, PoolView<R> D
       atomicAdd(count_part + (int64_t)(blockIdx.x & copy_mask) * ((int64_t)nx * ny) + key, cnt);
   }
   const int cnt = (n - k0 < 256) ? (int)(n - k0) : 256;
-  if (t < cnt) so[t] = offs[k0 + t];
-  if (t == 0) so[cnt] = offs[k0 + cnt];  // offs holds n + 1 entries
-  __syncthreads();
+  __syncthreads();  // so[cnt] is the block's end: threads past n carry zero copies
   const int64_t lo = so[0], hi = so[cnt];
   for (int64_t o = lo + t; o < hi; o += 256) {
     int a = 0, b = cnt - 1;  // last parent p with so[p] <= o
@@ -1617,22 +1647,24 @@ int imc2d_comb(imc2d_ctx* c, int64_t Nmax, double* ecen_out, double* disc_out, d
   int passthrough = !(E > 0.0);
   k_comb_ew<<<launch_blocks(nc, 256), 256, 0, s>>>(nc, c->ecen.p, c->scal + 4, Nmax, c->ew.p);
   IMC_HIP(hipGetLastError());
-  if ((rc = c->copies.reserve(n + 1))) return rc;
-  if ((rc = c->counts.reserve(n + 2))) return rc;  // reused as copy offsets
-  if ((rc = c->scan_scr.reserve(scan_scratch(n) + 8))) return rc;
+  const int64_t nblk = (n + 255) / 256;
+  if ((rc = c->copies.reserve(n + 1)) || (rc = c->counts.reserve(2 * nblk + 4)) ||
+      (rc = c->scan_scr.reserve(scan_scratch(nblk) + 8)))
+    return rc;
+  int64_t* bsum = c->counts.p;             // block copy totals
+  int64_t* boff = c->counts.p + nblk + 1;  // their exclusive scan, then the total
   Key key = make_key(c->seed);
-  int g = launch_blocks(n, 256);
   if (n > 0) {
     if (c->precision == IMC_FP64)
-      k_comb_copies<double><<<g, 256, 0, s>>>(c->A.view<double>(), n, c->nx, c->ny, c->ew.p, passthrough, key,
-                                              c->step, c->copies.p);
+      k_comb_copies<double><<<(unsigned)nblk, 256, 0, s>>>(c->A.view<double>(), n, c->nx, c->ny, c->ew.p,
+                                                           passthrough, key, c->step, c->copies.p, bsum);
     else
-      k_comb_copies<float><<<g, 256, 0, s>>>(c->A.view<float>(), n, c->nx, c->ny, c->ew.p, passthrough, key,
-                                             c->step, c->copies.p);
+      k_comb_copies<float><<<(unsigned)nblk, 256, 0, s>>>(c->A.view<float>(), n, c->nx, c->ny, c->ew.p, passthrough,
+                                                          key, c->step, c->copies.p, bsum);
     IMC_HIP(hipGetLastError());
   }
-  if ((rc = exclusive_scan_i32_to_i64(c->copies.p, n, c->counts.p, c->scan_scr.p, s))) return rc;
-  IMC_HIP(hipMemcpyAsync(hs + 1, c->counts.p + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  if ((rc = exclusive_scan_i64(bsum, nblk, boff, c->scan_scr.p, s))) return rc;
+  IMC_HIP(hipMemcpyAsync(hs + 1, boff + nblk, sizeof(int64_t), hipMemcpyDeviceToHost, s));
   IMC_HIP(hipStreamSynchronize(s));
   int64_t total = *(int64_t*)(hs + 1);
   if ((rc = c->B.grow(std::max<int64_t>(total, 1), 0, s))) return rc;
@@ -1641,11 +1673,13 @@ int imc2d_comb(imc2d_ctx* c, int64_t Nmax, double* ecen_out, double* disc_out, d
   unsigned int* cp = want_counts ? c->count_part.p : nullptr;
   if (n > 0) {
     if (c->precision == IMC_FP64)
-      k_comb_write<double><<<g, 256, 0, s>>>(c->A.view<double>(), c->B.view<double>(), n, c->nx, c->ny,
-                                             c->copies.p, c->counts.p, c->ew.p, passthrough, c->tally_copies - 1, cp);
+      k_comb_write<double><<<(unsigned)nblk, 256, 0, s>>>(c->A.view<double>(), c->B.view<double>(), n, c->nx, c->ny,
+                                                          c->copies.p, boff, c->ew.p, passthrough, c->tally_copies - 1,
+                                                          cp);
     else
-      k_comb_write<float><<<g, 256, 0, s>>>(c->A.view<float>(), c->B.view<float>(), n, c->nx, c->ny, c->copies.p,
-                                            c->counts.p, c->ew.p, passthrough, c->tally_copies - 1, cp);
+      k_comb_write<float><<<(unsigned)nblk, 256, 0, s>>>(c->A.view<float>(), c->B.view<float>(), n, c->nx, c->ny,
+                                                         c->copies.p, boff, c->ew.p, passthrough, c->tally_copies - 1,
+                                                         cp);
     IMC_HIP(hipGetLastError());
   }
   size_t bytes = sizeof(double) * nc;
