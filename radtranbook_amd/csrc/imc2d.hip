@@ -1071,14 +1071,20 @@ struct Imc2dCtx {
   PinnedBuf stage;
 };
 
-static int stage_in(Imc2dCtx* c, int slot, double* dst, const double* src) {
-  double* h = c->stage.slot(c->ncell, slot);
-  std::memcpy(h, src, sizeof(double) * c->ncell);
-  IMC_HIP(hipMemcpyAsync(dst, h, sizeof(double) * c->ncell, hipMemcpyHostToDevice, c->stream));
-  return kOk;
+// Host array -> staging slot (host memcpy); the returned job copies it on to
+// the device buffer inside a kernel.
+static CopyJob stage_in(Imc2dCtx* c, int slot, double* dst, const double* src) {
+  std::memcpy(c->stage.slot(c->ncell, slot), src, sizeof(double) * c->ncell);
+  return CopyJob{c->stage.dslot(c->ncell, slot), dst, (int64_t)(sizeof(double) * c->ncell)};
+}
+
+// Device per-cell array -> staging slot (kernel writes host memory).
+static CopyJob stage_out(Imc2dCtx* c, int slot, const double* src) {
+  return CopyJob{src, c->stage.dslot(c->ncell, slot), (int64_t)(sizeof(double) * c->ncell)};
 }
 
 static double* stage_scalars(Imc2dCtx* c) { return c->stage.slot(c->ncell, 6); }
+static double* stage_scalars_dev(Imc2dCtx* c) { return c->stage.dslot(c->ncell, 6); }
 
 static int ensure_sum_scratch(Imc2dCtx* c, int64_t n) { return c->sum_scr.reserve(canonical_sum_scratch(n) + 8); }
 
@@ -1337,9 +1343,10 @@ int imc2d_set_material(imc2d_ctx* c, const double* T, const double* sigma_true, 
   IMC_HIP(hipSetDevice(c->device));
   size_t bytes = sizeof(double) * c->ncell;
   int rc;
-  if ((rc = stage_in(c, 0, c->T.p, T)) || (rc = stage_in(c, 1, c->sig.p, sigma_true)) ||
-      (rc = stage_in(c, 2, c->cv.p, cv)))
-    return rc;
+  {
+    CopyJob in[3] = {stage_in(c, 0, c->T.p, T), stage_in(c, 1, c->sig.p, sigma_true), stage_in(c, 2, c->cv.p, cv)};
+    if ((rc = copy_jobs(in, 3, c->stream))) return rc;
+  }
   IMC_HIP(hipMemsetAsync(c->bad, 0, sizeof(int), c->stream));
   bool f32 = c->precision == IMC_FP32_TRACK;
   k_fleck<<<launch_blocks(c->ncell, 256), 256, 0, c->stream>>>(c->ncell, c->T.p, c->sig.p, c->cv.p, theta, dt, clip,
@@ -1348,11 +1355,11 @@ int imc2d_set_material(imc2d_ctx* c, const double* T, const double* sigma_true, 
                                                                c->bad);
   IMC_HIP(hipGetLastError());
   int* hbad = (int*)stage_scalars(c);
-  IMC_HIP(hipMemcpyAsync(hbad, c->bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-  if (sigma_a_out)
-    IMC_HIP(hipMemcpyAsync(c->stage.slot(c->ncell, 3), c->sa.p, bytes, hipMemcpyDeviceToHost, c->stream));
-  if (sigma_s_out)
-    IMC_HIP(hipMemcpyAsync(c->stage.slot(c->ncell, 4), c->ss.p, bytes, hipMemcpyDeviceToHost, c->stream));
+  {
+    CopyJob out[3] = {CopyJob{c->bad, stage_scalars_dev(c), 4}, sigma_a_out ? stage_out(c, 3, c->sa.p) : CopyJob{},
+                      sigma_s_out ? stage_out(c, 4, c->ss.p) : CopyJob{}};
+    if ((rc = copy_jobs(out, 3, c->stream))) return rc;
+  }
   IMC_HIP(hipStreamSynchronize(c->stream));
   if (sigma_a_out) std::memcpy(sigma_a_out, c->stage.slot(c->ncell, 3), bytes);
   if (sigma_s_out) std::memcpy(sigma_s_out, c->stage.slot(c->ncell, 4), bytes);
@@ -1393,8 +1400,10 @@ static int counts_and_offsets(imc2d_ctx* c, const double* E, int64_t N, int min_
   }
   rc = exclusive_scan_i64(c->counts.p, c->ncell, c->offs.p, c->scan_scr.p, c->stream);
   if (rc) return rc;
-  IMC_HIP(hipMemcpyAsync(total, c->offs.p + c->ncell, sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
+  CopyJob out{c->offs.p + c->ncell, stage_scalars_dev(c) + 4, 8};
+  if ((rc = copy_jobs(&out, 1, c->stream))) return rc;
   IMC_HIP(hipStreamSynchronize(c->stream));
+  *total = *(int64_t*)(stage_scalars(c) + 4);
   return kOk;
 }
 
@@ -1464,7 +1473,8 @@ int imc2d_sample_volume(imc2d_ctx* c, int kind, int64_t N, const double* QV_host
     QV = c->E.p;
     tag = kTagEmission;
   } else {
-    int rc0 = stage_in(c, 5, c->volsrc.p, QV_host);
+    CopyJob in = stage_in(c, 5, c->volsrc.p, QV_host);
+    int rc0 = copy_jobs(&in, 1, c->stream);
     if (rc0) return rc0;
     QV = c->volsrc.p;
     tag = kTagSource;
@@ -1477,10 +1487,11 @@ int imc2d_sample_volume(imc2d_ctx* c, int kind, int64_t N, const double* QV_host
     IMC_HIP(hipMemcpyAsync(c->fcounts.p, c->counts.p, sizeof(int64_t) * c->ncell, hipMemcpyDeviceToDevice,
                            c->stream));  // the counts used, for imc2d_emission_counts
   double* hs = stage_scalars(c);
-  IMC_HIP(hipMemcpyAsync(hs, c->scal + 0, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  if (E_out)
-    IMC_HIP(hipMemcpyAsync(c->stage.slot(c->ncell, 3), QV, sizeof(double) * c->ncell, hipMemcpyDeviceToHost,
-                           c->stream));
+  {
+    CopyJob out[2] = {CopyJob{c->scal + 0, stage_scalars_dev(c), 8}, E_out ? stage_out(c, 3, QV) : CopyJob{}};
+    int rc1 = copy_jobs(out, 2, c->stream);
+    if (rc1) return rc1;
+  }
   IMC_HIP(hipStreamSynchronize(c->stream));
   double Etot = hs[0];
   if (E_out) std::memcpy(E_out, c->stage.slot(c->ncell, 3), sizeof(double) * c->ncell);
@@ -1589,10 +1600,13 @@ int imc2d_transport(imc2d_ctx* c, double dt, const int32_t* reflect, int64_t max
   double* h_si = c->stage.slot(c->ncell, 4);
   double* hs = stage_scalars(c);
   unsigned long long* hst = (unsigned long long*)(hs + 1);
-  if (dep_out) IMC_HIP(hipMemcpyAsync(h_dep, c->dep.p, bytes, hipMemcpyDeviceToHost, s));
-  if (si_out) IMC_HIP(hipMemcpyAsync(h_si, c->si.p, bytes, hipMemcpyDeviceToHost, s));
-  IMC_HIP(hipMemcpyAsync(hs, c->scal + 3, sizeof(double), hipMemcpyDeviceToHost, s));
-  IMC_HIP(hipMemcpyAsync(hst, c->stats, sizeof(unsigned long long) * 8, hipMemcpyDeviceToHost, s));
+  {
+    double* hsd = stage_scalars_dev(c);
+    CopyJob out[4] = {dep_out ? stage_out(c, 3, c->dep.p) : CopyJob{}, si_out ? stage_out(c, 4, c->si.p) : CopyJob{},
+                      CopyJob{c->scal + 3, hsd, 8}, CopyJob{c->stats, hsd + 1, 64}};
+    int rc1 = copy_jobs(out, 4, s);
+    if (rc1) return rc1;
+  }
   IMC_HIP(hipStreamSynchronize(s));
   IMC_HIP(hipEventElapsedTime(&c->last_transport_ms, c->ev0, c->ev1));
   if (dep_out) std::memcpy(dep_out, h_dep, bytes);
@@ -1641,7 +1655,11 @@ int imc2d_comb(imc2d_ctx* c, int64_t Nmax, double* ecen_out, double* disc_out, d
   if ((rc = census_histogram(c))) return rc;
   if ((rc = canonical_sum(c->ecen.p, nc, c->scal + 4, c->sum_scr.p, s))) return rc;
   double* hs = stage_scalars(c);
-  IMC_HIP(hipMemcpyAsync(hs, c->scal + 4, sizeof(double), hipMemcpyDeviceToHost, s));
+  double* hsd = stage_scalars_dev(c);
+  {
+    CopyJob out{c->scal + 4, hsd, 8};
+    if ((rc = copy_jobs(&out, 1, s))) return rc;
+  }
   IMC_HIP(hipStreamSynchronize(s));
   double E = hs[0];
   int passthrough = !(E > 0.0);
@@ -1664,7 +1682,10 @@ int imc2d_comb(imc2d_ctx* c, int64_t Nmax, double* ecen_out, double* disc_out, d
     IMC_HIP(hipGetLastError());
   }
   if ((rc = exclusive_scan_i64(bsum, nblk, boff, c->scan_scr.p, s))) return rc;
-  IMC_HIP(hipMemcpyAsync(hs + 1, boff + nblk, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  {
+    CopyJob out{boff + nblk, hsd + 1, 8};
+    if ((rc = copy_jobs(&out, 1, s))) return rc;
+  }
   IMC_HIP(hipStreamSynchronize(s));
   int64_t total = *(int64_t*)(hs + 1);
   if ((rc = c->B.grow(std::max<int64_t>(total, 1), 0, s))) return rc;
@@ -1691,13 +1712,15 @@ int imc2d_comb(imc2d_ctx* c, int64_t Nmax, double* ecen_out, double* disc_out, d
     }
     k_comb_disc<<<launch_blocks(nc, 256), 256, 0, s>>>(nc, c->ecen.p, c->ew.p, c->cellcount.p, passthrough, c->disc.p);
     IMC_HIP(hipGetLastError());
-    IMC_HIP(hipMemcpyAsync(c->stage.slot(nc, 4), c->disc.p, bytes, hipMemcpyDeviceToHost, s));
   }
   std::swap(c->A, c->B);
   c->n = total;
   if ((rc = tree_sum(c->A.w, total, c->scal + 5, c->tree_scr.p, s))) return rc;
-  if (ecen_out) IMC_HIP(hipMemcpyAsync(c->stage.slot(nc, 3), c->ecen.p, bytes, hipMemcpyDeviceToHost, s));
-  IMC_HIP(hipMemcpyAsync(hs + 2, c->scal + 5, sizeof(double), hipMemcpyDeviceToHost, s));
+  {
+    CopyJob out[3] = {disc_out ? stage_out(c, 4, c->disc.p) : CopyJob{}, ecen_out ? stage_out(c, 3, c->ecen.p) : CopyJob{},
+                      CopyJob{c->scal + 5, hsd + 2, 8}};
+    if ((rc = copy_jobs(out, 3, s))) return rc;
+  }
   IMC_HIP(hipStreamSynchronize(s));
   if (disc_out) std::memcpy(disc_out, c->stage.slot(nc, 4), bytes);
   if (ecen_out) std::memcpy(ecen_out, c->stage.slot(nc, 3), bytes);

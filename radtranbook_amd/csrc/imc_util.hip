@@ -2,6 +2,8 @@
 #include <cstdarg>
 #include <cstdio>
 
+#include <algorithm>
+
 #include "imc_util.hpp"
 
 namespace imc {
@@ -22,6 +24,42 @@ int launch_blocks(int64_t n, int block) {
   if (b < 1) b = 1;
   if (b > 0x7fffffff) b = 0x7fffffff;
   return (int)b;
+}
+
+// ---------------------------------------------------------------------------
+// kernel copies
+// ---------------------------------------------------------------------------
+struct CopyJobs4 {
+  CopyJob j[4];
+  int n;
+};
+
+__global__ void k_copy_jobs(CopyJobs4 J) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (int q = 0; q < J.n; ++q) {
+    const uint32_t* src = (const uint32_t*)J.j[q].src;
+    uint32_t* dst = (uint32_t*)J.j[q].dst;
+    const int64_t words = J.j[q].bytes >> 2;
+    for (int64_t i = i0; i < words; i += stride) dst[i] = src[i];
+  }
+}
+
+int copy_jobs(const CopyJob* jobs, int njobs, hipStream_t s) {
+  CopyJobs4 J{};
+  int64_t most = 0;
+  J.n = 0;
+  for (int q = 0; q < njobs; ++q) {
+    if (!jobs[q].src || !jobs[q].dst || jobs[q].bytes <= 0) continue;
+    IMC_REQUIRE((jobs[q].bytes & 3) == 0, kErrArg, "copy_jobs: size not a multiple of 4");
+    J.j[J.n++] = jobs[q];
+    most = std::max<int64_t>(most, jobs[q].bytes >> 2);
+  }
+  if (J.n == 0) return kOk;
+  int blocks = (int)std::min<int64_t>((most + 255) / 256, 1024);
+  k_copy_jobs<<<blocks, 256, 0, s>>>(J);
+  IMC_HIP(hipGetLastError());
+  return kOk;
 }
 
 // ---------------------------------------------------------------------------

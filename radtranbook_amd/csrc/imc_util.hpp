@@ -79,22 +79,36 @@ struct DevBuf {
 // region is reused.
 struct PinnedBuf {
   char* p = nullptr;
+  char* dev = nullptr;  // the same memory as seen by kernels
   size_t cap = 0;
   int reserve(size_t n) {
     if (n <= cap) return kOk;
     if (p) IMC_HIP(hipHostFree(p));
-    p = nullptr;
-    IMC_HIP(hipHostMalloc((void**)&p, n, hipHostMallocDefault));
+    p = dev = nullptr;
+    IMC_HIP(hipHostMalloc((void**)&p, n, hipHostMallocMapped));
+    IMC_HIP(hipHostGetDevicePointer((void**)&dev, p, 0));
     cap = n;
     return kOk;
   }
   double* slot(int64_t n, int k) const { return (double*)p + (size_t)k * (size_t)n; }
+  double* dslot(int64_t n, int k) const { return (double*)dev + (size_t)k * (size_t)n; }
   void release() {
     if (p) (void)hipHostFree(p);
-    p = nullptr;
+    p = dev = nullptr;
     cap = 0;
   }
 };
+
+// Up to four copies in one kernel launch (4-byte granularity), between
+// device buffers and the kernel-visible address of page-locked staging: a
+// kernel reads or writes host memory directly over PCIe, which for the
+// per-cell arrays of a step costs less than queueing DMA copies.
+struct CopyJob {
+  const void* src;
+  void* dst;
+  int64_t bytes;
+};
+int copy_jobs(const CopyJob* jobs, int njobs, hipStream_t s);
 
 // Deterministic "canonical" sum: 1024-element chunks summed left to right,
 // repeated on the chunk sums until one value is left.  The result lands in
