@@ -1688,7 +1688,9 @@ int imc2d_comb(imc2d_ctx* c, int64_t Nmax, double* ecen_out, double* disc_out, d
   }
   IMC_HIP(hipStreamSynchronize(s));
   int64_t total = *(int64_t*)(hs + 1);
-  if ((rc = c->B.grow(std::max<int64_t>(total, 1), 0, s))) return rc;
+  // the comb output becomes the next pool: give it the capacity the current
+  // pool reached, so the next step's sources do not reallocate
+  if ((rc = c->B.grow(std::max<int64_t>(std::max<int64_t>(total, 1), c->A.cap), 0, s))) return rc;
   bool want_counts = disc_out != nullptr && !passthrough;
   if (want_counts) IMC_HIP(hipMemsetAsync(c->count_part.p, 0, sizeof(unsigned int) * nc * c->tally_copies, s));
   unsigned int* cp = want_counts ? c->count_part.p : nullptr;

@@ -437,7 +437,10 @@ __global__ __launch_bounds__(256) void k_mg_transport(MgArgs<R> A) {
           i = A.P.ci[k];
           j = A.P.cj[k];
           g = A.P.g[k];
-          dcen = (R)((A.dt - t) * kC);
+          // FP32 times are stored rounded to nearest: a time sampled just below
+          // dt can read back above it, and a negative distance to census would
+          // grow the weight by e^{sigma_a |dcen|}.  Such a particle is at census.
+          dcen = (R)fmax((A.dt - t) * kC, 0.0);
           ev = 0;
           active = w > wfloor;  // below the floor: skipped, unchanged (MG_IMC2D.py:842-843)
         }
@@ -1746,7 +1749,9 @@ int mgimc_comb(mgimc_ctx* c, int64_t Nmax, double* ecen_out, double* disc_out, d
   int64_t total = 0;
   IMC_HIP(hipMemcpyAsync(&total, c->counts.p + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
   IMC_HIP(hipStreamSynchronize(s));
-  if ((rc = c->B.grow(std::max<int64_t>(total, 1), 0, s))) return rc;
+  // the comb output becomes the next pool: give it the capacity the current
+  // pool reached, so the next step's sources do not reallocate
+  if ((rc = c->B.grow(std::max<int64_t>(std::max<int64_t>(total, 1), c->A.cap), 0, s))) return rc;
   if (n > 0) {
     if (c->precision == IMC_FP64)
       k_mg_comb_write<double><<<g, 256, 0, s>>>(c->A.view<double>(), c->B.view<double>(), n, c->nx, c->ny,
@@ -2052,7 +2057,9 @@ int mgimc_sphere_comb(mgimc_ctx* c, int64_t Nmax, int radiation, double* disc_ou
   int64_t total = 0;
   IMC_HIP(hipMemcpyAsync(&total, c->counts.p + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
   IMC_HIP(hipStreamSynchronize(s));
-  if ((rc = c->B.grow(std::max<int64_t>(total, 1), 0, s))) return rc;
+  // the comb output becomes the next pool: give it the capacity the current
+  // pool reached, so the next step's sources do not reallocate
+  if ((rc = c->B.grow(std::max<int64_t>(std::max<int64_t>(total, 1), c->A.cap), 0, s))) return rc;
   if (n > 0) {
     if (c->precision == IMC_FP64)
       k_mg_comb_write<double><<<g, 256, 0, s>>>(c->A.view<double>(), c->B.view<double>(), n, I, 1, c->counts.p,

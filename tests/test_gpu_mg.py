@@ -213,6 +213,40 @@ def test_mg_zero_d_closed_form(L, M, precision, geometry):
     assert np.sum(w1) + np.sum(dep) * vol == pytest.approx(np.sum(w0), rel=1e-12)
 
 
+def test_mg_fp32_time_at_step_end(L, M):
+    """FP32 stores times rounded to nearest, so a time just below dt = 0.002
+    reads back as float(dt) > dt.  Such a particle is at census: its weight is
+    unchanged and it deposits nothing, however large sigma_a is (a negative
+    distance to census would multiply its weight by e^{sigma_a |dcen|})."""
+    _announce("test_mg_fp32_time_at_step_end")
+    e1, e2 = np.array([0.0, 0.5]), np.array([0.0, 0.5])
+    dt = 0.002
+    assert float(np.float32(dt)) > dt
+    eng = L.EngineMG(e1, e2, EDGES, "xy", "fp32", seed=3)
+    T = np.array([[0.05]])
+    sig = np.full((10, 1, 1), 1e9)
+    mat, _ = eng.set_material(T, sig, 0.3, 1.0, dt, want=("sa",))
+    assert mat["sa"].min() > 1e5
+    rng = np.random.default_rng(4)
+    n_late, n_early = 256, 256
+    n = n_late + n_early
+    w0 = rng.uniform(0.5, 1.5, n)
+    mu = rng.uniform(-1.0, 1.0, n)
+    t0 = np.concatenate([np.full(n_late, dt * (1.0 - 2.0 ** -30)), np.full(n_early, 0.999 * dt)])
+    p = rng.uniform(0.1, 0.4, (2, n))
+    g = rng.integers(0, 10, n).astype(np.int32)
+    eng.put_census(w0, mu, np.sqrt(1 - mu * mu), t0, p[0], p[1], np.zeros(n, np.int64), np.zeros(n, np.int64), g)
+    eng.set_step(0)
+    dep, si, bl, st = eng.transport(dt, (True,) * 4, 10**6, 0.0)
+    w1, _, _, t1 = eng.get_census()[:4]
+    late = t0 > 0.9995 * dt
+    np.testing.assert_array_equal(w1[late], w0[late])
+    assert np.all(w1 <= w0)
+    assert np.all(dep >= 0.0)
+    vol = M.volumes(e1, e2, "xy")[0]
+    assert np.sum(w1) + np.sum(dep) * vol == pytest.approx(np.sum(w0), rel=1e-12)
+
+
 def test_mg_comb_parity(L, M):
     _announce("test_mg_comb_parity")
     r = _transport_setup(L, M, "xy", "fp64", npart=6000)
