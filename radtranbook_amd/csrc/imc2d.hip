@@ -325,8 +325,7 @@ __global__ void k_sample_boundary(PoolView<R> P, int64_t base, int64_t n, int ge
     Pair b3 = block_d(ctr, key, 3);
     double tot = prof[nface - 1];
     double u = b3.u0 * tot;
-    int m = 0;
-    while (m < nface - 1 && !(u < prof[m])) ++m;
+    int m = first_above(prof, nface, u);
     while (m > 0 && prof[m] == prof[m - 1]) --m;
     a = fe[m];
     b = fe[m + 1];

@@ -106,6 +106,20 @@ __device__ __forceinline__ QuadF block4_f(Ctr c, Key k, uint32_t b) {
 __device__ __forceinline__ float fdiv_fast(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 __device__ __forceinline__ float neg_ln_fast(float x) { return -0.693147180559945f * __builtin_amdgcn_logf(x); }
 
+// First m in [0, n-1] with u < cum[m] for a non-decreasing running sum cum,
+// n - 1 when there is none (the index a left-to-right scan stops at).
+__device__ __forceinline__ int first_above(const double* __restrict__ cum, int n, double u) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (u < cum[mid])
+      hi = mid;
+    else
+      lo = mid + 1;
+  }
+  return lo;
+}
+
 // wave64 helpers
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 __device__ __forceinline__ uint64_t lanemask_lt() {
@@ -156,16 +170,21 @@ __device__ __forceinline__ bool wave_run_sum(int key, T& v) {
 constexpr int kLdsSlots = 1024;
 constexpr int kLdsProbes = 4;
 
+constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n >> 1); }
+
+template <int SLOTS = kLdsSlots>
 __device__ __forceinline__ void lds_tally_init(int* key, double* val) {
-  for (int s = threadIdx.x; s < kLdsSlots; s += blockDim.x) {
+  static_assert((SLOTS & (SLOTS - 1)) == 0, "slot count must be a power of two");
+  for (int s = threadIdx.x; s < SLOTS; s += blockDim.x) {
     key[s] = -1;
     val[s] = 0.0;
   }
 }
 
 // Returns false if the value still has to go to global memory.
+template <int SLOTS = kLdsSlots>
 __device__ __forceinline__ bool lds_tally_add(int* key, double* val, int cell, double v) {
-  uint32_t s = ((uint32_t)cell * 2654435761u) >> (32 - 10);  // kLdsSlots = 2^10
+  uint32_t s = ((uint32_t)cell * 2654435761u) >> (32 - ilog2(SLOTS));
 #pragma unroll
   for (int p = 0; p < kLdsProbes; ++p) {
     int k = __hip_atomic_load(key + s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -179,14 +198,15 @@ __device__ __forceinline__ bool lds_tally_add(int* key, double* val, int cell, d
       __hip_atomic_fetch_add(val + s, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
       return true;
     }
-    s = (s + 1) & (kLdsSlots - 1);
+    s = (s + 1) & (SLOTS - 1);
   }
   return false;
 }
 
 // After a __syncthreads(): add every cached cell into dst (global).
+template <int SLOTS = kLdsSlots>
 __device__ __forceinline__ void lds_tally_flush(const int* key, const double* val, double* dst) {
-  for (int s = threadIdx.x; s < kLdsSlots; s += blockDim.x) {
+  for (int s = threadIdx.x; s < SLOTS; s += blockDim.x) {
     int k = key[s];
     if (k >= 0) {
       double v = val[s];
@@ -194,6 +214,70 @@ __device__ __forceinline__ void lds_tally_flush(const int* key, const double* va
     }
   }
 }
+
+// Per-wave tally cache in LDS: each wave owns W direct-mapped slots of
+// (cell, FP64 sum, owner lane).  A cell that lands on an occupied slot
+// evicts the holder's sum to global memory and takes the slot, so the cache
+// follows the cells in current use whatever order the histories arrive in.
+// Only the owning wave touches its slots.  Lanes that land on one slot take
+// turns: each round every waiting lane writes its id to `own`, the lane whose
+// id survives updates the slot, and the round ends at a wave-wide ballot, so
+// one round's updates are complete before the next round's claims (the loop
+// exit is uniform; a per-lane exit would let the compiler move the update
+// past the other lanes' claims).  The slots are volatile LDS (address space
+// 3) so each read-back is a real ds_read in program order.
+typedef __attribute__((address_space(3))) volatile int lds_vint;
+typedef __attribute__((address_space(3))) volatile double lds_vdouble;
+
+template <int W>
+struct WaveTally {
+  static_assert((W & (W - 1)) == 0 && W >= 64, "slot count must be a power of two >= 64");
+  static constexpr int kBytes = W * (4 + 8 + 4);
+  lds_vdouble* val;
+  lds_vint* key;
+  lds_vint* own;
+
+  // lds: this wave's kBytes of a __shared__ array (8-byte aligned)
+  __device__ __forceinline__ explicit WaveTally(char* lds)
+      : val((lds_vdouble*)(lds)), key((lds_vint*)(lds + W * 8)), own((lds_vint*)(lds + W * 12)) {
+    for (int s = __lane_id(); s < W; s += 64) {
+      key[s] = -1;
+      val[s] = 0.0;
+    }
+  }
+
+  // Called by any subset of the wave's lanes.
+  __device__ __forceinline__ void add(int cell, double v, double* __restrict__ dst) {
+    const uint32_t s = ((uint32_t)cell * 2654435761u) >> (32 - ilog2(W));
+    const int lane = __lane_id();
+    bool pending = true;
+    do {
+      if (pending) own[s] = lane;
+      if (pending && own[s] == lane) {
+        const int k = key[s];
+        if (k == cell) {
+          val[s] = val[s] + v;
+        } else {
+          if (k >= 0) atomic_add_f64(dst + k, val[s]);
+          key[s] = cell;
+          val[s] = v;
+        }
+        pending = false;
+      }
+    } while (__ballot(pending) != 0);
+  }
+
+  // Every lane of the wave: add the resident sums to dst.
+  __device__ __forceinline__ void flush(double* __restrict__ dst) {
+    for (int s = __lane_id(); s < W; s += 64) {
+      const int k = key[s];
+      if (k >= 0) {
+        const double v = val[s];
+        if (v != 0.0) atomic_add_f64(dst + k, v);
+      }
+    }
+  }
+};
 
 // FP32 / FP64 pair type of a tracking precision.
 template <typename R>

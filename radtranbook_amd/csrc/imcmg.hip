@@ -272,18 +272,15 @@ __device__ __forceinline__ int mg_locate(const double* __restrict__ e, int n, do
 }
 
 // Face source (MG_IMC2D.py:392-577).  Slots: 0 time, 1 mu_n, 2 phi / eta, 3
-// along the face, 4 face cell (profile), 5..9 Planck group.  gcount
-// (nullable): particles per group.
+// along the face, 4 face cell (profile), 5..9 Planck group.  Returns the group.
 template <typename R>
-__global__ void k_mg_sample_boundary(MgView<R> P, int64_t base, int64_t n, int geom, int side, double E_total,
-                                     double dt, double Tb, int nx, int ny, const double* __restrict__ e1,
-                                     const double* __restrict__ e2, int G, const double* __restrict__ edges,
-                                     const double* __restrict__ pcdf, const double* __restrict__ prof,
-                                     const double* __restrict__ Tface, Key key, uint32_t step,
-                                     unsigned long long* __restrict__ gcount) {
+__device__ __forceinline__ int mg_face_particle(MgView<R> P, int64_t base, int64_t k, int64_t n, int geom, int side,
+                                                double E_total, double dt, double Tb, int nx, int ny,
+                                                const double* __restrict__ e1, const double* __restrict__ e2, int G,
+                                                const double* __restrict__ edges, const double* __restrict__ pcdf,
+                                                const double* __restrict__ prof, const double* __restrict__ Tface,
+                                                Key key, uint32_t step) {
 #pragma clang fp contract(off)
-  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
   const int nface = side < 2 ? ny : nx;
   const double* fe = side < 2 ? e2 : e1;
   double x0 = e1[0], x1 = e1[nx], y0 = e2[0], y1 = e2[ny];
@@ -292,8 +289,7 @@ __global__ void k_mg_sample_boundary(MgView<R> P, int64_t base, int64_t n, int g
   double a = side < 2 ? y0 : x0, b = side < 2 ? y1 : x1, Tk = Tb;
   if (prof) {
     double u = slot_d(ctr, key, 4) * prof[nface - 1];
-    int m = 0;
-    while (m < nface - 1 && !(u < prof[m])) ++m;
+    int m = first_above(prof, nface, u);
     while (m > 0 && prof[m] == prof[m - 1]) --m;
     a = fe[m];
     b = fe[m + 1];
@@ -325,7 +321,34 @@ __global__ void k_mg_sample_boundary(MgView<R> P, int64_t base, int64_t n, int g
   }
   int g = planck_group(ctr, key, 5, Tk, G, edges, pcdf);
   mg_store(P, base + k, E_total / (double)n, u1, u2, t, q1, q2, mg_locate(e1, nx, q1), mg_locate(e2, ny, q2), g);
-  if (gcount) atomicAdd(gcount + g, 1ull);
+  return g;
+}
+
+// gcount (nullable): particles per group, counted per workgroup in LDS (G
+// 32-bit counters of dynamic shared memory) and added once per group.
+template <typename R>
+__global__ void k_mg_sample_boundary(MgView<R> P, int64_t base, int64_t n, int geom, int side, double E_total,
+                                     double dt, double Tb, int nx, int ny, const double* __restrict__ e1,
+                                     const double* __restrict__ e2, int G, const double* __restrict__ edges,
+                                     const double* __restrict__ pcdf, const double* __restrict__ prof,
+                                     const double* __restrict__ Tface, Key key, uint32_t step,
+                                     unsigned long long* __restrict__ gcount) {
+  extern __shared__ unsigned int ghist[];
+  if (gcount) {
+    for (int q = threadIdx.x; q < G; q += blockDim.x) ghist[q] = 0u;
+    __syncthreads();
+  }
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) {
+    int g = mg_face_particle<R>(P, base, k, n, geom, side, E_total, dt, Tb, nx, ny, e1, e2, G, edges, pcdf, prof,
+                                Tface, key, step);
+    if (gcount) atomicAdd(ghist + g, 1u);
+  }
+  if (gcount) {
+    __syncthreads();
+    for (int q = threadIdx.x; q < G; q += blockDim.x)
+      if (ghist[q]) atomicAdd(gcount + q, (unsigned long long)ghist[q]);
+  }
 }
 
 // Fixed source (MG_IMC2D.py:1098-1234).  cdfg: (G, ncell) canonical prefixes
@@ -385,6 +408,8 @@ struct MgArgs {
   unsigned long long* stats;
 };
 
+constexpr int kMgWaveSlots = 512;  // tally cache slots per wave of the 2-D tracker
+
 template <typename R, int GEOM>
 __global__ __launch_bounds__(256) void k_mg_transport(MgArgs<R> A) {
   using R2 = typename Vec2<R>::type;
@@ -393,10 +418,9 @@ __global__ __launch_bounds__(256) void k_mg_transport(MgArgs<R> A) {
   const int copy = blockIdx.x & A.copy_mask;
   double* const dep_t = A.dep_raw + (int64_t)copy * stride;
   double* const path_t = A.path_raw + (int64_t)copy * A.nb;
-  __shared__ int lkey[kLdsSlots];
-  __shared__ double lval[kLdsSlots];
-  lds_tally_init(lkey, lval);
-  __syncthreads();
+  using Tally = WaveTally<kMgWaveSlots>;
+  __shared__ __align__(8) char tally_lds[4 * Tally::kBytes];
+  Tally tally(tally_lds + (threadIdx.x >> 6) * Tally::kBytes);
   const int nx = A.nx, ny = A.ny;
   const int64_t ncell = (int64_t)nx * ny;
   int64_t qbase = 0, qend = 0;
@@ -457,8 +481,8 @@ __global__ __launch_bounds__(256) void k_mg_transport(MgArgs<R> A) {
     if (i < 0 || i >= nx || j < 0 || j >= ny) {
       int side = i < 0 ? 0 : (i >= nx ? 1 : (j < 0 ? 2 : 3));
       int kg = (int)(A.nb + g), ks = (int)(A.nb + A.G + side);
-      if (!lds_tally_add(lkey, lval, kg, w)) atomic_add_f64(dep_t + kg, w);
-      if (!lds_tally_add(lkey, lval, ks, w)) atomic_add_f64(dep_t + ks, w);
+      tally.add(kg, w, dep_t);
+      tally.add(ks, w, dep_t);
       ++st_bnd;
       w = 0.0;
       finish = true;
@@ -506,7 +530,7 @@ __global__ __launch_bounds__(256) void k_mg_transport(MgArgs<R> A) {
       ++st_ev;
       if (bin != acc_bin) {
         if (acc_bin >= 0) {
-          if (!lds_tally_add(lkey, lval, acc_bin, acc_dep)) atomic_add_f64(dep_t + acc_bin, acc_dep);
+          tally.add(acc_bin, acc_dep, dep_t);
           if (acc_path != 0.0) atomic_add_f64(path_t + acc_bin, acc_path);
         }
         acc_bin = bin;
@@ -538,8 +562,8 @@ __global__ __launch_bounds__(256) void k_mg_transport(MgArgs<R> A) {
             ++st_refl;
           } else {
             int kg = (int)(A.nb + g), ks = (int)(A.nb + A.G + side);
-            if (!lds_tally_add(lkey, lval, kg, w)) atomic_add_f64(dep_t + kg, w);
-            if (!lds_tally_add(lkey, lval, ks, w)) atomic_add_f64(dep_t + ks, w);
+            tally.add(kg, w, dep_t);
+            tally.add(ks, w, dep_t);
             w = 0.0;
             finish = true;
           }
@@ -563,11 +587,10 @@ __global__ __launch_bounds__(256) void k_mg_transport(MgArgs<R> A) {
     }
   }
   if (acc_bin >= 0) {
-    if (!lds_tally_add(lkey, lval, acc_bin, acc_dep)) atomic_add_f64(dep_t + acc_bin, acc_dep);
+    tally.add(acc_bin, acc_dep, dep_t);
     if (acc_path != 0.0) atomic_add_f64(path_t + acc_bin, acc_path);
   }
-  __syncthreads();
-  lds_tally_flush(lkey, lval, dep_t);
+  tally.flush(dep_t);
   unsigned long long s0 = wave_sum<unsigned long long>(st_ev);
   unsigned long long s1 = wave_sum<unsigned long long>(st_bnd);
   unsigned long long s2 = wave_sum<unsigned long long>(st_sc);
@@ -1534,12 +1557,13 @@ int mgimc_sample_boundary(mgimc_ctx* c, int side, int64_t n, double E_total, dou
   int g = launch_blocks(n, 256);
   Key key = make_key(c->seed);
   unsigned long long* gc = E_by_group_out ? c->gcount.p : nullptr;
+  const size_t lds = gc ? sizeof(unsigned int) * (size_t)c->G : 0;
   if (c->precision == IMC_FP64)
-    k_mg_sample_boundary<double><<<g, 256, 0, s>>>(c->A.view<double>(), c->n, n, c->geometry, side, E_total, dt, Tb,
+    k_mg_sample_boundary<double><<<g, 256, lds, s>>>(c->A.view<double>(), c->n, n, c->geometry, side, E_total, dt, Tb,
                                                    c->nx, c->ny, c->e1.p, c->e2.p, c->G, c->edges.p, c->pcdf.p, prof,
                                                    tf, key, c->step, gc);
   else
-    k_mg_sample_boundary<float><<<g, 256, 0, s>>>(c->A.view<float>(), c->n, n, c->geometry, side, E_total, dt, Tb,
+    k_mg_sample_boundary<float><<<g, 256, lds, s>>>(c->A.view<float>(), c->n, n, c->geometry, side, E_total, dt, Tb,
                                                   c->nx, c->ny, c->e1.p, c->e2.p, c->G, c->edges.p, c->pcdf.p, prof,
                                                   tf, key, c->step, gc);
   IMC_HIP(hipGetLastError());
