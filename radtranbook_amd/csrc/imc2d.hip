@@ -403,7 +403,9 @@ struct TransportArgs {
   unsigned long long* stats;  // [8]
 };
 
-template <typename R, int GEOM>
+// Off: offset type of the particle-array accesses (uint32_t while every
+// byte offset fits 32 bits, else uint64_t).
+template <typename R, int GEOM, typename Off>
 __global__ __launch_bounds__(256) void k_transport(TransportArgs<R> A) {
   using R2 = typename Vec2<R>::type;
   const double wfloor = *A.floor_ptr;
@@ -448,14 +450,15 @@ __global__ __launch_bounds__(256) void k_transport(TransportArgs<R> A) {
         int64_t rank = __popcll(need & lanemask_lt());
         if (!active && rank < avail) {
           k = qbase + rank;
-          w = A.P.w[k];
-          d1 = A.P.d1[k];
-          d2 = A.P.d2[k];
-          p1 = A.P.p1[k];
-          p2 = A.P.p2[k];
-          i = A.P.ci[k];
-          j = A.P.cj[k];
-          dcen = (R)((A.dt - (double)A.P.t[k]) * kC);
+          const Off ko = (Off)k;
+          w = elem(A.P.w, ko);
+          d1 = elem(A.P.d1, ko);
+          d2 = elem(A.P.d2, ko);
+          p1 = elem(A.P.p1, ko);
+          p2 = elem(A.P.p2, ko);
+          i = elem(A.P.ci, ko);
+          j = elem(A.P.cj, ko);
+          dcen = (R)((A.dt - (double)elem(A.P.t, ko)) * kC);
           ev = 0;
           active = dcen > (R)0;
         }
@@ -480,9 +483,9 @@ __global__ __launch_bounds__(256) void k_transport(TransportArgs<R> A) {
       finish = true;
     } else {
       int c = i * ny + j;
-      R2 mat = A.sas[c];
-      R2 ex = A.e1p[i];
-      R2 ey = A.e2p[j];
+      R2 mat = elem(A.sas, (uint32_t)c);
+      R2 ex = elem(A.e1p, (uint32_t)i);
+      R2 ey = elem(A.e2p, (uint32_t)j);
       MoveOut<R> o;
       Ctr ctr = make_ctr(kTagTransport, (uint64_t)k, A.step);
       if (GEOM == 0)
@@ -495,8 +498,8 @@ __global__ __launch_bounds__(256) void k_transport(TransportArgs<R> A) {
       st_cen += o.evt == kEvtCensus;
       if (c != acc_cell) {
         if (acc_cell >= 0) {
-          if (!lds_tally_add(lkey, lval, acc_cell, acc_dep)) atomic_add_f64(dep_t + acc_cell, acc_dep);
-          if (acc_si != 0.0) atomic_add_f64(si_t + acc_cell, acc_si);
+          if (!lds_tally_add(lkey, lval, acc_cell, acc_dep)) atomic_add_f64(&elem(dep_t, (uint32_t)acc_cell), acc_dep);
+          if (acc_si != 0.0) atomic_add_f64(&elem(si_t, (uint32_t)acc_cell), acc_si);
         }
         acc_cell = c;
         acc_dep = 0.0;
@@ -540,19 +543,20 @@ __global__ __launch_bounds__(256) void k_transport(TransportArgs<R> A) {
       }
     }
     if (finish) {
-      A.P.w[k] = w;
-      A.P.d1[k] = d1;
-      A.P.d2[k] = d2;
-      A.P.p1[k] = p1;
-      A.P.p2[k] = p2;
-      A.P.ci[k] = i;
-      A.P.cj[k] = j;
+      const Off ko = (Off)k;
+      elem(A.P.w, ko) = w;
+      elem(A.P.d1, ko) = d1;
+      elem(A.P.d2, ko) = d2;
+      elem(A.P.p1, ko) = p1;
+      elem(A.P.p2, ko) = p2;
+      elem(A.P.ci, ko) = i;
+      elem(A.P.cj, ko) = j;
       active = false;
     }
   }
   if (acc_cell >= 0) {
-    if (!lds_tally_add(lkey, lval, acc_cell, acc_dep)) atomic_add_f64(dep_t + acc_cell, acc_dep);
-    if (acc_si != 0.0) atomic_add_f64(si_t + acc_cell, acc_si);
+    if (!lds_tally_add(lkey, lval, acc_cell, acc_dep)) atomic_add_f64(&elem(dep_t, (uint32_t)acc_cell), acc_dep);
+    if (acc_si != 0.0) atomic_add_f64(&elem(si_t, (uint32_t)acc_cell), acc_si);
   }
   __syncthreads();
   lds_tally_flush(lkey, lval, dep_t);
@@ -1093,6 +1097,14 @@ static int pool_ensure(Imc2dCtx* c, int64_t n_total) { return c->A.grow(n_total,
 
 using namespace imc;
 
+template <typename R, typename Off>
+static void launch_transport(int geometry, int grid, int block, hipStream_t s, const TransportArgs<R>& A) {
+  if (geometry == IMC_GEOM_XY)
+    k_transport<R, 0, Off><<<grid, block, 0, s>>>(A);
+  else
+    k_transport<R, 1, Off><<<grid, block, 0, s>>>(A);
+}
+
 extern "C" {
 
 struct imc2d_ctx : public Imc2dCtx {};
@@ -1542,14 +1554,12 @@ int imc2d_transport(imc2d_ctx* c, double dt, const int32_t* reflect, int64_t max
     grid = c->grid_override;
   } else {
     int per_cu = 0;
-    if (c->precision == IMC_FP64)
-      IMC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-          &per_cu, c->geometry == IMC_GEOM_XY ? (const void*)k_transport<double, 0> : (const void*)k_transport<double, 1>,
-          block, 0));
-    else
-      IMC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-          &per_cu, c->geometry == IMC_GEOM_XY ? (const void*)k_transport<float, 0> : (const void*)k_transport<float, 1>,
-          block, 0));
+    const void* fn = c->precision == IMC_FP64
+                         ? (c->geometry == IMC_GEOM_XY ? (const void*)k_transport<double, 0, uint32_t>
+                                                       : (const void*)k_transport<double, 1, uint32_t>)
+                         : (c->geometry == IMC_GEOM_XY ? (const void*)k_transport<float, 0, uint32_t>
+                                                       : (const void*)k_transport<float, 1, uint32_t>);
+    IMC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, block, 0));
     if (per_cu < 1) per_cu = 1;
     grid = per_cu * c->n_cu;
   }
@@ -1569,22 +1579,23 @@ int imc2d_transport(imc2d_ctx* c, double dt, const int32_t* reflect, int64_t max
   }
   IMC_HIP(hipEventRecord(c->ev0, s));
   if (n > 0) {
+    const bool narrow = n <= kNarrowMax;
     if (c->precision == IMC_FP64) {
       TransportArgs<double> A{c->A.view<double>(), n, c->e1p64.p, c->e2p64.p, c->nx, c->ny, c->sas64.p, dt, refl,
                               max_events, c->scal + 2, key, c->step, c->next, c->batch, c->dep_raw.p, c->si_raw.p,
                               c->ncell, c->tally_copies - 1, c->scal + 3, c->stats};
-      if (c->geometry == IMC_GEOM_XY)
-        k_transport<double, 0><<<grid, block, 0, s>>>(A);
+      if (narrow)
+        launch_transport<double, uint32_t>(c->geometry, grid, block, s, A);
       else
-        k_transport<double, 1><<<grid, block, 0, s>>>(A);
+        launch_transport<double, uint64_t>(c->geometry, grid, block, s, A);
     } else {
       TransportArgs<float> A{c->A.view<float>(), n, c->e1p32.p, c->e2p32.p, c->nx, c->ny, c->sas32.p, dt, refl,
                              max_events, c->scal + 2, key, c->step, c->next, c->batch, c->dep_raw.p, c->si_raw.p,
                              c->ncell, c->tally_copies - 1, c->scal + 3, c->stats};
-      if (c->geometry == IMC_GEOM_XY)
-        k_transport<float, 0><<<grid, block, 0, s>>>(A);
+      if (narrow)
+        launch_transport<float, uint32_t>(c->geometry, grid, block, s, A);
       else
-        k_transport<float, 1><<<grid, block, 0, s>>>(A);
+        launch_transport<float, uint64_t>(c->geometry, grid, block, s, A);
     }
     IMC_HIP(hipGetLastError());
   }

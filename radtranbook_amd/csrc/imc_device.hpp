@@ -15,6 +15,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace imc {
 
 constexpr double kC = 29.98;     // cm/ns (IMC/IMC2D.py:48)
@@ -119,6 +121,19 @@ __device__ __forceinline__ int first_above(const double* __restrict__ cum, int n
   }
   return lo;
 }
+
+// Element idx of an array whose base pointer is uniform across the wave,
+// addressed as base + idx * sizeof(T) computed in the offset type Off.  With
+// a 32-bit Off the load or store uses the scalar-base + 32-bit vector-offset
+// form, with no 64-bit address arithmetic in the vector ALU; the caller
+// guarantees idx * sizeof(T) < 2^32 then.
+template <typename Off, typename T>
+__device__ __forceinline__ T& elem(T* base, Off idx) {
+  using Byte = typename std::conditional<std::is_const<T>::value, const char, char>::type;
+  return *reinterpret_cast<T*>(reinterpret_cast<Byte*>(base) + (Off)(idx * (Off)sizeof(T)));
+}
+// Largest element count whose byte offsets fit 32 bits for 8-byte elements.
+constexpr int64_t kNarrowMax = (int64_t)(0xFFFFFFFFull / 8);
 
 // wave64 helpers
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
