@@ -298,6 +298,15 @@ float mgimc_last_transport_ms(mgimc_ctx* ctx);
 int mgimc_comb(mgimc_ctx* ctx, int64_t Nmax, double* ecen_out, double* disc_out, double* post_out,
                double* ecen_total_out, double* total_rad_out);
 
+/* Per-cell sums over groups of a (G, ncell) device array, accumulated from
+ * group 0 upwards as numpy sums axis 0 (the group sums of MG_IMC2D.py:
+ * 1978-1979,2013: deposition, emission and comb discrepancy per cell)
+ * without moving the (G, ncell) array to the host.  which: 0 emission
+ * energies of the last mgimc_set_material, 1 deposition of the last
+ * mgimc_transport, 2 pre- minus post-comb census energy of the last
+ * mgimc_comb called with disc_out or post_out.  out: ncell doubles. */
+int mgimc_group_sum(mgimc_ctx* ctx, int which, double* out);
+
 /* ---- multigroup 1-D spherical -------------------------------------------- */
 /* A mgimc_ctx over I radial shells (edges: I+1 radii), the state of
  * MG_IMC1D.init_simulation (MG_IMC/MG_IMC1D.py:850-914) and

@@ -14,8 +14,9 @@ reference ignores it there); right/top and rmax/zmax face particles start in
 the last cell (the reference loses or drops them); a reflecting left/rmin
 face resets the cell index.  Added ``info`` keys: ``comb_discrepancy`` and
 ``balance_residual`` (the residual with the comb discrepancy booked),
-``deposited``, ``scalar_intensity``, ``emission_by_group`` and
-``profiling.transport_kernel_ms``.
+``deposited`` (per cell, summed over groups), ``scalar_intensity`` (by
+group), ``profiling.transport_kernel_ms``, and with ``diagnostics=True``
+``deposited_by_group`` and ``emission_by_group``.
 
 Extra keyword arguments: ``precision`` ('fp64' or 'fp32' tracking), ``seed``,
 ``device``.
@@ -181,8 +182,12 @@ def step(state, Ntarget, Nboundary, Nsource, Nmax, T_boundary, dt, edges1, edges
          inv_eos, cv, source, reflect=(False, False, False, False), theta=1.0, use_scalar_intensity_Tr=True,
          conserve_comb_energy=False, geometry="xy", max_events_per_particle=1_000_000, boundary_source_func=None,
          emission_fractions=None, Ntotal=0, Ntotal_T_floor=0.0, particle_budget_fmin=0.1, T_emit_floor=0.0,
-         Nmax_growth=0, Nmax_final=None, _timing=False):
-    """Advance one multigroup 2-D IMC step (MG_IMC2D.py:1635-2122)."""
+         Nmax_growth=0, Nmax_final=None, diagnostics=False, _timing=False):
+    """Advance one multigroup 2-D IMC step (MG_IMC2D.py:1635-2122).
+
+    The per-cell totals over groups (deposition, emission, comb discrepancy)
+    are summed on the device; ``diagnostics=True`` also returns the (G, nx, ny)
+    arrays ``deposited_by_group`` and ``emission_by_group`` in ``info``."""
     eng = state._engine
     nx, ny = len(edges1) - 1, len(edges2) - 1
     G = len(energy_edges) - 1
@@ -198,9 +203,11 @@ def step(state, Ntarget, Nboundary, Nsource, Nmax, T_boundary, dt, edges1, edges
     sigma = np.stack([np.asarray(sigma_a_funcs[g](temperature), dtype=np.float64) * np.ones((nx, ny))
                       for g in range(G)])
     cvv = np.asarray(cv(temperature), dtype=np.float64) * np.ones((nx, ny))
-    want = ("E", "Eplanck") if Ntotal > 0 else ("E",)
+    want = ("Eplanck",) if Ntotal > 0 else ()
+    if diagnostics:
+        want = want + ("E",)
     mat, E_emit = eng.set_material(temperature, sigma, cvv, theta, dt, emission_fractions, T_emit_floor, want)
-    emission_by_group = mat["E"]
+    emission_by_group = mat.get("E")
 
     Tb4 = [_boundary_temperature_value(T_boundary[s], state.time) for s in range(4)]
     profiles = None
@@ -256,24 +263,28 @@ def step(state, Ntarget, Nboundary, Nsource, Nmax, T_boundary, dt, edges1, edges
 
     if Ntarget > 0:
         eng.sample_emission(int(Ntarget), dt)
+        emitted_cell = eng.group_sum("emission")
     else:
-        emission_by_group = np.zeros((G, nx, ny))
+        emitted_cell = np.zeros((nx, ny))
+        if diagnostics:
+            emission_by_group = np.zeros((G, nx, ny))
 
     t_transport_start = _time.perf_counter()
     n_transported = eng.n
-    dep, si, bl, stats = eng.transport(dt, reflect, max_events_per_particle, -1.0)
+    dep, si, bl, stats = eng.transport(dt, reflect, max_events_per_particle, -1.0, want_dep=diagnostics)
     t_post_start = _time.perf_counter()
-    internal_energy = state.internal_energy + dep.sum(0) - emission_by_group.sum(0) / volumes
+    dep_cell = eng.group_sum("dep")
+    internal_energy = state.internal_energy + dep_cell - emitted_cell / volumes
     temperature = inv_eos(internal_energy)
 
-    ecen, comb_disc, post, ecen_total, total_rad = eng.comb(int(Nmax), want_ecen=not use_scalar_intensity_Tr,
-                                                            want_disc=True, want_post=True)
+    ecen, _, post, ecen_total, total_rad = eng.comb(int(Nmax), want_ecen=not use_scalar_intensity_Tr,
+                                                    want_post=True)
     if use_scalar_intensity_Tr:
         erg_pre = si / __c
     else:
         erg_pre = ecen / volumes[None, :, :]
     if conserve_comb_energy:
-        internal_energy = internal_energy + comb_disc.sum(0) / volumes
+        internal_energy = internal_energy + eng.group_sum("disc") / volumes
         temperature = inv_eos(internal_energy)
     erg_post = post / volumes[None, :, :]
     radiation_temperature = quarter_root(np.sum(erg_pre, axis=0) / __a)
@@ -328,9 +339,8 @@ def step(state, Ntarget, Nboundary, Nsource, Nmax, T_boundary, dt, edges1, edges
         "energy_loss": energy_residual,
         "comb_discrepancy": disc_total,
         "balance_residual": energy_residual + disc_total,
-        "deposited": dep,
+        "deposited": dep_cell,
         "scalar_intensity": si,
-        "emission_by_group": emission_by_group,
         "Nmax_next": ((min(Nmax + Nmax_growth, Nmax_final) if Nmax_final is not None else Nmax + Nmax_growth)
                       if Nmax_growth > 0 else Nmax),
         "profiling": {
@@ -355,6 +365,9 @@ def step(state, Ntarget, Nboundary, Nsource, Nmax, T_boundary, dt, edges1, edges
             },
         },
     }
+    if diagnostics:
+        info["deposited_by_group"] = dep
+        info["emission_by_group"] = emission_by_group
     return state, info
 
 

@@ -93,6 +93,7 @@ SIGNATURES = {
     "mgimc_transport": (C.c_int, [_vp, C.c_double, _i32p, C.c_int64, C.c_double, _dp, _dp, _dp, _i64p]),
     "mgimc_last_transport_ms": (C.c_float, [_vp]),
     "mgimc_comb": (C.c_int, [_vp, C.c_int64, _dp, _dp, _dp, _dp, _dp]),
+    "mgimc_group_sum": (C.c_int, [_vp, C.c_int, _dp]),
     "mgimc_create_sphere": (C.c_int, [C.POINTER(_vp), C.c_int, _dp, C.c_int, _dp, C.c_int, C.c_uint64, C.c_int]),
     "mgimc_sphere_equilibrium": (C.c_int, [_vp, C.c_int64, _dp, C.c_double]),
     "mgimc_sphere_face": (C.c_int, [_vp, C.c_int, C.c_int64, C.c_double, C.c_double, _dp]),
@@ -563,9 +564,12 @@ class EngineMG:
         check(self._L.mgimc_sample_emission(self._h, int(N), float(dt), C.byref(et)))
         return et.value
 
-    def transport(self, dt, reflect, max_events, weight_floor=-1.0, want_tallies=True):
+    def transport(self, dt, reflect, max_events, weight_floor=-1.0, want_tallies=True, want_dep=None):
+        """want_dep (default: want_tallies) downloads the (G, nx, ny)
+        deposition; group_sum(1) gives its per-cell total without it."""
         refl = np.array([1 if r else 0 for r in reflect], dtype=np.int32)
-        dep = np.empty(self.gshape) if want_tallies else None
+        want_dep = want_tallies if want_dep is None else want_dep
+        dep = np.empty(self.gshape) if want_dep else None
         si = np.empty(self.gshape) if want_tallies else None
         bl = np.zeros(1 + self.G + 4)
         st = np.zeros(8, dtype=np.int64)
@@ -584,6 +588,16 @@ class EngineMG:
         tr = C.c_double()
         check(self._L.mgimc_comb(self._h, int(Nmax), _p(ecen), _p(disc), _p(post), C.byref(et), C.byref(tr)))
         return ecen, disc, post, et.value, tr.value
+
+    _GROUP_SUM = {"emission": 0, "dep": 1, "disc": 2}
+
+    def group_sum(self, what):
+        """Per-cell sum over groups, (nx, ny), of 'emission' (last
+        set_material), 'dep' (last transport) or 'disc' (last comb with
+        want_disc or want_post), summed on the device in numpy's axis-0 order."""
+        out = np.empty(self.shape)
+        check(self._L.mgimc_group_sum(self._h, self._GROUP_SUM[what], _p(out)))
+        return out
 
 
 class EngineMG1D:

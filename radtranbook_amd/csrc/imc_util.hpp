@@ -2,6 +2,8 @@
 // error reporting, device buffers, deterministic reductions and scans.
 #pragma once
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 #include <stdint.h>
 
 #include <cstdio>
@@ -43,8 +45,12 @@ template <typename T>
 struct DevBuf {
   T* p = nullptr;
   int64_t cap = 0;
+  // A buffer that has to grow again gets 25% headroom, so sizes that creep
+  // up from step to step (particle counts) do not reallocate every step.
+  int64_t next_cap(int64_t n) const { return cap > 0 ? std::max<int64_t>(n, cap + cap / 4) : n; }
   int reserve(int64_t n) {
     if (n <= cap) return kOk;
+    n = next_cap(n);
     T* q = nullptr;
     IMC_HIP(hipMalloc(&q, sizeof(T) * (size_t)(n > 0 ? n : 1)));
     if (p) IMC_HIP(hipFree(p));
@@ -55,6 +61,7 @@ struct DevBuf {
   // grow keeping the first `keep` elements
   int grow(int64_t n, int64_t keep, hipStream_t s) {
     if (n <= cap) return kOk;
+    n = next_cap(n);
     T* q = nullptr;
     IMC_HIP(hipMalloc(&q, sizeof(T) * (size_t)n));
     if (p && keep > 0) IMC_HIP(hipMemcpyAsync(q, p, sizeof(T) * (size_t)keep, hipMemcpyDeviceToDevice, s));

@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <vector>
 
 #include "../../include/imc_hip.h"
@@ -174,6 +175,34 @@ __device__ __forceinline__ int64_t mg_pick(const double* __restrict__ cdf, int64
   return lo;
 }
 
+// Guide table for mg_pick over a running sum cdf[0..n-1] of total E =
+// cdf[n-1]: guide[j] = mg_pick(cdf, n, (j / kGuide) * E), j = 0..kGuide.  For
+// u = x * E with x in [0, 1), j = floor(x * kGuide) is exact and rounding is
+// monotone, so mg_pick(cdf, n, u) lies in [guide[j], guide[j + 1]] and the
+// search over that bracket with the same predicate returns the same index.
+constexpr int kGuide = 65536;
+
+__global__ void k_mg_guide(const double* __restrict__ cdf, int64_t n, int32_t* __restrict__ guide) {
+#pragma clang fp contract(off)
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j > kGuide) return;
+  guide[j] = (int32_t)mg_pick(cdf, n, ((double)j * (1.0 / kGuide)) * cdf[n - 1]);
+}
+
+__device__ __forceinline__ int64_t mg_pick_guided(const double* __restrict__ cdf, const int32_t* __restrict__ guide,
+                                                  double x, double u) {
+  int j = (int)(x * (double)kGuide);
+  int64_t lo = guide[j], hi = guide[j + 1];
+  while (lo < hi) {
+    int64_t mid = (lo + hi) >> 1;
+    if (cdf[mid] > u)
+      hi = mid;
+    else
+      lo = mid + 1;
+  }
+  return lo;
+}
+
 __device__ __forceinline__ void mg_pos(int geom, const double* __restrict__ e1, const double* __restrict__ e2, int i,
                                        int j, double u1, double u2, double& p1, double& p2) {
 #pragma clang fp contract(off)
@@ -205,8 +234,8 @@ __device__ __forceinline__ void mg_store(MgView<R> P, int64_t o, double w, doubl
 template <typename R>
 __global__ void k_mg_sample_emission(MgView<R> P, int64_t base, int64_t N, int geom, int nx, int ny, int G,
                                      const double* __restrict__ e1, const double* __restrict__ e2,
-                                     const double* __restrict__ E, const double* __restrict__ cdf, double dt, Key key,
-                                     uint32_t step) {
+                                     const double* __restrict__ E, const double* __restrict__ cdf,
+                                     const int32_t* __restrict__ guide, double dt, Key key, uint32_t step) {
 #pragma clang fp contract(off)
   int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= N) return;
@@ -214,7 +243,7 @@ __global__ void k_mg_sample_emission(MgView<R> P, int64_t base, int64_t N, int g
   const double Etot = cdf[ncell - 1];
   Ctr ctr = make_ctr(kTagEmission, (uint64_t)k, step);
   Pair b0 = block_d(ctr, key, 0), b1 = block_d(ctr, key, 1), b2 = block_d(ctr, key, 2), b3 = block_d(ctr, key, 3);
-  int64_t c = mg_pick(cdf, ncell, b0.u0 * Etot);
+  int64_t c = mg_pick_guided(cdf, guide, b0.u0, b0.u0 * Etot);
   int i = (int)(c / ny), j = (int)(c % ny);
   double tot = 0.0;
   for (int g = 0; g < G; ++g) tot += E[(int64_t)g * ncell + c];
@@ -659,10 +688,9 @@ __device__ __forceinline__ int mg_bin(const MgView<R>& P, int64_t k, int nx, int
 template <typename R>
 __global__ __launch_bounds__(256) void k_mg_ecen(MgView<R> P, int64_t n, int nx, int ny, int64_t nb, int copy_mask,
                                                  double* __restrict__ part) {
-  __shared__ int lkey[kLdsSlots];
-  __shared__ double lval[kLdsSlots];
-  lds_tally_init(lkey, lval);
-  __syncthreads();
+  using Tally = WaveTally<kMgWaveSlots>;
+  __shared__ __align__(8) char tally_lds[4 * Tally::kBytes];
+  Tally tally(tally_lds + (threadIdx.x >> 6) * Tally::kBytes);
   double* const dst = part + (int64_t)(blockIdx.x & copy_mask) * nb;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   const int64_t trips = (n + stride - 1) / stride;
@@ -675,10 +703,9 @@ __global__ __launch_bounds__(256) void k_mg_ecen(MgView<R> P, int64_t n, int nx,
       if (key >= 0) v = P.w[k];
     }
     bool head = wave_run_sum(key, v);
-    if (head && key >= 0 && !lds_tally_add(lkey, lval, key, v)) atomic_add_f64(dst + key, v);
+    if (head && key >= 0) tally.add(key, v, dst);
   }
-  __syncthreads();
-  lds_tally_flush(lkey, lval, dst);
+  tally.flush(dst);
 }
 
 __global__ void k_mg_ew(int64_t nb, const double* __restrict__ ecen, const double* __restrict__ Etot, int64_t Nmax,
@@ -754,6 +781,21 @@ __global__ __launch_bounds__(256) void k_mg_comb_write(MgView<R> S, MgView<R> D,
 __global__ void k_sub(int64_t n, const double* __restrict__ a, const double* __restrict__ b, double* __restrict__ c) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) c[i] = a[i] - b[i];
+}
+
+// out[c] = sum over g of a[g, c] (or of a[g, c] - b[g, c]), accumulated from
+// g = 0 upwards: the order of numpy's sum over axis 0 of a (G, ...) array.
+__global__ void k_group_sum(int64_t ncell, int G, const double* __restrict__ a, const double* __restrict__ b,
+                            double* __restrict__ out) {
+#pragma clang fp contract(off)
+  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  double acc = b ? a[c] - b[c] : a[c];
+  for (int g = 1; g < G; ++g) {
+    int64_t q = (int64_t)g * ncell + c;
+    acc += b ? a[q] - b[q] : a[q];
+  }
+  out[c] = acc;
 }
 
 template <typename R>
@@ -1152,7 +1194,7 @@ struct MgCtx {
   DevBuf<double2> sas64, e1p64, e2p64;
   DevBuf<float2> sas32, e1p32, e2p32;
   DevBuf<int64_t> counts, scan_scr;
-  DevBuf<int32_t> copies;
+  DevBuf<int32_t> copies, guide;
   DevBuf<unsigned long long> gcount;
   unsigned long long* stats = nullptr;
   unsigned long long* next = nullptr;
@@ -1160,6 +1202,11 @@ struct MgCtx {
   int64_t n = 0;
   int grid_override = 0, batch = 256, tally_copies = kTallyCopies;
   bool have_material = false;
+  // page-locked staging of the (G, ncell) transfers: slot 0 opacities in,
+  // slot 1 scalar intensity out, slot 2 post-comb census energy out
+  PinnedBuf stage;
+  // device arrays that mgimc_group_sum may read
+  bool e_valid = false, dep_valid = false, post_valid = false;
 };
 
 static int prefix_canonical(MgCtx* c, const double* x, int64_t n, double* out) {
@@ -1254,7 +1301,7 @@ static int mg_create(mgimc_ctx** out, int geometry, int nx, int ny, const double
     if ((rc = r.b->reserve(r.n))) return rc;
   if ((rc = c->sas64.reserve(nb)) || (rc = c->sas32.reserve(nb)) || (rc = c->e1p64.reserve(nx)) ||
       (rc = c->e2p64.reserve(ny)) || (rc = c->e1p32.reserve(nx)) || (rc = c->e2p32.reserve(ny)) ||
-      (rc = c->gcount.reserve(G)))
+      (rc = c->gcount.reserve(G)) || (rc = c->stage.reserve(sizeof(double) * 3 * (size_t)nb)))
     return rc;
   IMC_HIP(hipMemcpy(c->e1.p, edges1, sizeof(double) * (nx + 1), hipMemcpyHostToDevice));
   IMC_HIP(hipMemcpy(c->e2.p, edges2, sizeof(double) * (ny + 1), hipMemcpyHostToDevice));
@@ -1342,7 +1389,9 @@ void mgimc_destroy(mgimc_ctx* c) {
   c->counts.release();
   c->scan_scr.release();
   c->copies.release();
+  c->guide.release();
   c->gcount.release();
+  c->stage.release();
   c->A.release();
   c->B.release();
   if (c->stats) (void)hipFree(c->stats);
@@ -1465,7 +1514,11 @@ int mgimc_set_material(mgimc_ctx* c, const double* T, const double* sigma, const
   hipStream_t s = c->stream;
   int64_t nc = c->ncell, nb = c->nb;
   IMC_HIP(hipMemcpyAsync(c->T.p, T, sizeof(double) * nc, hipMemcpyHostToDevice, s));
-  IMC_HIP(hipMemcpyAsync(c->sig.p, sigma, sizeof(double) * nb, hipMemcpyHostToDevice, s));
+  {
+    double* h = c->stage.slot(nb, 0);
+    std::memcpy(h, sigma, sizeof(double) * nb);
+    IMC_HIP(hipMemcpyAsync(c->sig.p, h, sizeof(double) * nb, hipMemcpyHostToDevice, s));
+  }
   IMC_HIP(hipMemcpyAsync(c->cv.p, cv, sizeof(double) * nc, hipMemcpyHostToDevice, s));
   if (emission_fractions)
     IMC_HIP(hipMemcpyAsync(c->frac.p, emission_fractions, sizeof(double) * c->G, hipMemcpyHostToDevice, s));
@@ -1485,6 +1538,7 @@ int mgimc_set_material(mgimc_ctx* c, const double* T, const double* sigma, const
   IMC_HIP(hipStreamSynchronize(s));
   if (Etot_out) *Etot_out = Et;
   c->have_material = true;
+  c->e_valid = true;
   return kOk;
 }
 
@@ -1639,14 +1693,17 @@ int mgimc_sample_emission(mgimc_ctx* c, int64_t N, double dt, double* Etot_out) 
   if (N <= 0 || !(Et > 0.0)) return kOk;
   int rc = c->A.grow(c->n + N, c->n, s);
   if (rc) return rc;
+  if ((rc = c->guide.reserve(kGuide + 1))) return rc;
+  k_mg_guide<<<launch_blocks(kGuide + 1, 256), 256, 0, s>>>(c->cdf.p, c->ncell, c->guide.p);
+  IMC_HIP(hipGetLastError());
   int g = launch_blocks(N, 256);
   Key key = make_key(c->seed);
   if (c->precision == IMC_FP64)
     k_mg_sample_emission<double><<<g, 256, 0, s>>>(c->A.view<double>(), c->n, N, c->geometry, c->nx, c->ny, c->G,
-                                                   c->e1.p, c->e2.p, c->E.p, c->cdf.p, dt, key, c->step);
+                                                   c->e1.p, c->e2.p, c->E.p, c->cdf.p, c->guide.p, dt, key, c->step);
   else
     k_mg_sample_emission<float><<<g, 256, 0, s>>>(c->A.view<float>(), c->n, N, c->geometry, c->nx, c->ny, c->G,
-                                                  c->e1.p, c->e2.p, c->E.p, c->cdf.p, dt, key, c->step);
+                                                  c->e1.p, c->e2.p, c->E.p, c->cdf.p, c->guide.p, dt, key, c->step);
   IMC_HIP(hipGetLastError());
   c->n += N;
   return kOk;
@@ -1722,13 +1779,16 @@ int mgimc_transport(mgimc_ctx* c, double dt, const int32_t* reflect, int64_t max
   IMC_HIP(hipGetLastError());
   k_mg_bl_total<<<1, 1, 0, s>>>(G, c->bl.p);
   IMC_HIP(hipGetLastError());
+  c->dep_valid = true;
   if (dep_out) IMC_HIP(hipMemcpyAsync(dep_out, c->dep.p, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
-  if (si_out) IMC_HIP(hipMemcpyAsync(si_out, c->si.p, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
+  double* h_si = c->stage.slot(nb, 1);
+  if (si_out) IMC_HIP(hipMemcpyAsync(h_si, c->si.p, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
   if (bl_out) IMC_HIP(hipMemcpyAsync(bl_out, c->bl.p, sizeof(double) * (1 + G + 4), hipMemcpyDeviceToHost, s));
   unsigned long long st[8];
   IMC_HIP(hipMemcpyAsync(st, c->stats, sizeof(st), hipMemcpyDeviceToHost, s));
   IMC_HIP(hipStreamSynchronize(s));
   IMC_HIP(hipEventElapsedTime(&c->last_ms, c->ev0, c->ev1));
+  if (si_out) std::memcpy(si_out, h_si, sizeof(double) * nb);
   if (stats_out)
     for (int a = 0; a < 8; ++a) stats_out[a] = (int64_t)st[a];
   return kOk;
@@ -1787,6 +1847,7 @@ int mgimc_comb(mgimc_ctx* c, int64_t Nmax, double* ecen_out, double* disc_out, d
   }
   std::swap(c->A, c->B);
   c->n = total;
+  c->post_valid = disc_out || post_out;
   if (disc_out || post_out) {
     if ((rc = mg_histogram(c, c->A, total, c->post.p))) return rc;
     if (disc_out) {
@@ -1794,15 +1855,45 @@ int mgimc_comb(mgimc_ctx* c, int64_t Nmax, double* ecen_out, double* disc_out, d
       IMC_HIP(hipGetLastError());
       IMC_HIP(hipMemcpyAsync(disc_out, c->tmp.p, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
     }
-    if (post_out) IMC_HIP(hipMemcpyAsync(post_out, c->post.p, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
+    if (post_out) IMC_HIP(hipMemcpyAsync(c->stage.slot(nb, 2), c->post.p, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
   }
   if (ecen_out) IMC_HIP(hipMemcpyAsync(ecen_out, c->ecen.p, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
   if ((rc = tree_sum((const double*)c->A.a[0], total, c->scal.p + 5, c->tree_scr.p, s))) return rc;
   double tr = 0.0;
   IMC_HIP(hipMemcpyAsync(&tr, c->scal.p + 5, sizeof(double), hipMemcpyDeviceToHost, s));
   IMC_HIP(hipStreamSynchronize(s));
+  if (post_out) std::memcpy(post_out, c->stage.slot(nb, 2), sizeof(double) * nb);
   if (total_rad_out) *total_rad_out = tr;
   if (ecen_total_out) *ecen_total_out = E;
+  return kOk;
+}
+
+// Per-cell sums over groups of a (G, ncell) device array, in the order of
+// numpy's axis-0 sum: which = 0 emission energies of the last
+// mgimc_set_material, 1 deposition of the last mgimc_transport, 2 comb
+// discrepancy (pre-comb minus post-comb census energy) of the last
+// mgimc_comb that produced disc or post.  out: ncell doubles.
+int mgimc_group_sum(mgimc_ctx* c, int which, double* out) {
+  IMC_REQUIRE(c && out && which >= 0 && which <= 2, kErrArg, "mgimc_group_sum: bad arguments");
+  const double* a = nullptr;
+  const double* b = nullptr;
+  if (which == 0) {
+    IMC_REQUIRE(c->e_valid, kErrState, "mgimc_group_sum: no emission energies yet (mgimc_set_material)");
+    a = c->E.p;
+  } else if (which == 1) {
+    IMC_REQUIRE(c->dep_valid, kErrState, "mgimc_group_sum: no deposition yet (mgimc_transport)");
+    a = c->dep.p;
+  } else {
+    IMC_REQUIRE(c->post_valid, kErrState, "mgimc_group_sum: the last mgimc_comb produced no post-comb census");
+    a = c->ecen.p;
+    b = c->post.p;
+  }
+  IMC_HIP(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  k_group_sum<<<launch_blocks(c->ncell, 256), 256, 0, s>>>(c->ncell, c->G, a, b, c->tmp.p);
+  IMC_HIP(hipGetLastError());
+  IMC_HIP(hipMemcpyAsync(out, c->tmp.p, sizeof(double) * c->ncell, hipMemcpyDeviceToHost, s));
+  IMC_HIP(hipStreamSynchronize(s));
   return kOk;
 }
 
@@ -2008,6 +2099,7 @@ int mgimc_sphere_transport(mgimc_ctx* c, double dt, const int32_t* reflect, doub
                                                             c->precision == IMC_FP32_TRACK, c->vol.p, dt, c->dep.p,
                                                             c->si.p, c->bl.p);
   IMC_HIP(hipGetLastError());
+  c->dep_valid = true;
   if (dep_out) IMC_HIP(hipMemcpyAsync(dep_out, c->dep.p, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
   if (si_out) IMC_HIP(hipMemcpyAsync(si_out, c->si.p, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
   if (bl_out) IMC_HIP(hipMemcpyAsync(bl_out, c->bl.p, sizeof(double) * G, hipMemcpyDeviceToHost, s));
@@ -2029,6 +2121,7 @@ int mgimc_sphere_transport(mgimc_ctx* c, double dt, const int32_t* reflect, doub
 // the energy of bins left without copies); n_alive, combed.
 int mgimc_sphere_comb(mgimc_ctx* c, int64_t Nmax, int radiation, double* disc_out, int64_t* n_alive_out,
                       int* combed_out) {
+  if (c) c->post_valid = false;  // the sphere comb keeps no post-comb histogram
   int rc;
   if ((rc = require_sphere(c, "mgimc_sphere_comb"))) return rc;
   IMC_HIP(hipSetDevice(c->device));
