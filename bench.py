@@ -12,14 +12,17 @@ FP32-track step (whole step, host orchestration included).  FOM = 1/(R^2 t)
 with R^2 the median per-cell relative variance of the final-step absorbed
 energy over ``--fom-seeds`` independent Philox seeds (32 by default) and t the
 timed wall time (``fom``) or the summed transport-kernel time of the timed
-steps (``fom_transport``).  The CPU baseline is the FP64 C oracle (OpenMP) on a bounded sample of the same
-workload.
+steps (``fom_transport``).  The CPU baseline is the FP64 C oracle (OpenMP) on
+a bounded sample of the same workload.  ``roofline.traffic`` is the HBM bytes
+per transport launch that rocprofv3 measured for the same workload and timed
+region (profiles/), null for other sizes.
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -35,6 +38,25 @@ HBM_PEAK_GBS = 8000.0
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def profiled_traffic(profile, precision, steps, warmup, defaults_match):
+    """HBM bytes per transport launch measured by rocprofv3 for this workload
+    (FETCH_SIZE + WRITE_SIZE per dispatch, each scaled by the same counter's
+    reading for a copy kernel of known size, averaged over the timed
+    launches), from the committed summary in profiles/.  None unless this run
+    has the profiled sizes, timed steps and warm-up steps."""
+    if not defaults_match:
+        return None
+    try:
+        with open(os.path.join(ROOT, "profiles", profile)) as fh:
+            text = fh.read()
+    except OSError:
+        return None
+    m = re.search(r"traffic_bytes_per_launch %s (\S+) steps=(\d+) warmup=(\d+)" % precision, text)
+    if not m or int(m.group(2)) != steps or int(m.group(3)) != warmup:
+        return None
+    return float(m.group(1))
 
 
 def run_gpu(prob, precision, steps, warmup, fom_seeds):
@@ -385,7 +407,9 @@ def main_mg(args):
         "histories_per_s": head["histories"] / head["wall"],
         "transport_segments_per_s": head["segments"] / (head["kernel_ms"].sum() / 1e3),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": profiled_traffic("k_mg_transport_crooked_pipe_240x240x10.md", precs[0], steps, warmup,
+                                                 npart == 20_000_000),
                      "kernel": f"k_mg_transport<{'float' if precs[0] == 'fp32' else 'double'},xy>",
                      "bytes_per_segment": BYTES_PER_SEGMENT_MG[precs[0]]},
     }
@@ -583,7 +607,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": profiled_traffic("k_transport_marshak2d_200x200_1e7.md", precs[0], args.steps, args.warmup,
+                                        args.cells == 200 and int(args.particles) == 10_000_000),
             "kernel": f"k_transport<{'float' if precs[0] == 'fp32' else 'double'},xy>",
             "bytes_per_segment": BYTES_PER_SEGMENT[precs[0]],
         },
