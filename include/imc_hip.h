@@ -38,6 +38,7 @@ extern "C" {
 /* volume-source kinds */
 #define IMC_SRC_EMISSION 0
 #define IMC_SRC_FIXED 1
+#define IMC_SRC_CF 2 /* Carter-Forest emission: QV per cell, truncated-exponential times */
 
 typedef struct imc2d_ctx imc2d_ctx;
 typedef struct imc1d_ctx imc1d_ctx;
@@ -120,6 +121,26 @@ int imc2d_transport(imc2d_ctx* ctx, double dt, const int32_t* reflect, int64_t m
                     double* dep_out, double* si_out, double* bl_out, int64_t* stats_out);
 /* Device time of the last transport kernel in milliseconds (HIP events). */
 float imc2d_last_transport_ms(imc2d_ctx* ctx);
+
+/* ---- Carter-Forest variant (IMC/IMC2D_CarterForest.py), FP64 contexts ---- */
+/* Material for the Carter-Forest step (IMC/IMC2D_CarterForest.py:948-949):
+ * T (rz linear emission tilt), the true sigma_a and beta = 4 a max(T,1e-12)^3
+ * / c_v per cell.  Emission then goes through imc2d_sample_volume with
+ * kind IMC_SRC_CF and QV = the per-cell emission energies of
+ * _sample_internal_emission_cf (876-906): the fixed-source sampler with each
+ * time drawn from the truncated exponential of rate c sigma_a beta (51-56). */
+int imc2d_set_material_cf(imc2d_ctx* ctx, const double* T, const double* sigma_a, const double* beta);
+/* _transport_particles_2d_cf (IMC/IMC2D_CarterForest.py:346-508): analog
+ * absorption with delayed re-emission or capture.  weight_floor < 0 means
+ * 1e-10 sum(w)/N; fastpath_threshold, no_progress_rel_tol and
+ * no_progress_max_streak as the reference's step arguments.  Outputs
+ * (nullable): dep = captured energy / V and si = sum w s / (dt V) per cell,
+ * boundary loss, stats[8] = events, boundary crossings, absorb-continue
+ * events, census events, captures, floor kills (deposited in their cell),
+ * reflections, event-cap hits. */
+int imc2d_transport_cf(imc2d_ctx* ctx, double dt, const int32_t* reflect, int64_t max_events, double weight_floor,
+                       double fastpath_threshold, double no_progress_rel_tol, int64_t no_progress_max_streak,
+                       double* dep_out, double* si_out, double* bl_out, int64_t* stats_out);
 
 /* Per-cell stochastic comb to about Nmax particles: _comb (IMC/IMC2D.py:1448-1511).
  * Optional outputs: ecen_out, pre-comb census energy per cell; disc_out,

@@ -813,3 +813,326 @@ int64_t orc_comb_counts(int64_t n, const double* w, const int64_t* cell, int64_t
   }
   return tot;
 }
+
+/* ------------------------------------------------------------------------- */
+/* Carter-Forest variant, IMC/IMC2D_CarterForest.py.                          */
+/* Emission times: the truncated exponential of 51-56, from slot 0 of the     */
+/* particle's emission stream (the slot the fixed-source sampler uses for a   */
+/* uniform time), so the whole emission is one sampler call plus this map.   */
+/* Transport (103-508): analog absorption at the true sigma_a; an absorbed    */
+/* particle is re-emitted isotropically after a delay -ln(xi)/(c sigma_a      */
+/* beta) if that ends before census, else it is captured and its weight       */
+/* deposited; every segment adds w s to the scalar-intensity tally; a face    */
+/* crossing nudges the position 1e-12 cell widths past the face.              */
+/* Draw slots for event e: 6e absorption distance, 6e+1 capped redraw, 6e+2   */
+/* delay, 6e+3 fast-path capture test, 6e+4 and 6e+5 the new direction.       */
+/* ------------------------------------------------------------------------- */
+double orc_cf_time(double rate, double dt, double xi) {
+  if (rate <= 1e-12) return xi * dt;
+  return -log(1.0 - xi * (1.0 - exp(-rate * dt))) / rate;
+}
+
+void orc_cf_emission_times(int64_t n, const int32_t* ci, const int32_t* cj, int ny, const double* rate, double dt,
+                           uint64_t seed, uint32_t step, int32_t tag, double* t) {
+  for (int64_t k = 0; k < n; ++k) {
+    orc_stream s;
+    orc_stream_init(&s, seed, tag, (uint64_t)k, step);
+    t[k] = orc_cf_time(rate[(int64_t)ci[k] * ny + cj[k]], dt, orc_u01(&s, 0));
+  }
+}
+
+enum { CF_CENSUS = 0, CF_BOUNDARY = 1, CF_ABSORB = 2, CF_CAPTURE = 4 };
+
+typedef struct {
+  double w, d1, d2, p1, p2, t, dep_w, dep_i, dcen;
+  int crossed, evt;
+} orc_cf_out;
+
+static double orc_cf_exp_dist(orc_stream* st, uint32_t slot, double rate) {
+  double u = 1.0 - orc_u01(st, slot);
+  return -log(u > 1e-16 ? u : 1e-16) / rate;
+}
+
+/* the absorption branch shared by both geometries (IMC/IMC2D_CarterForest.py:
+ * 183-213, 317-343): o already holds the moved particle; returns with evt,
+ * w, dep_w, dcen, t and (on re-emission) the new direction set */
+static void orc_cf_absorb(orc_stream* st, uint32_t ev, int geom, double w, double sa, double beta, double se,
+                          double tl, double eps_s, double fastpath, orc_cf_out* o) {
+  if (beta > 1e-15 && sa > 1e-15) {
+    double rate = ORC_C * sa * beta;
+    double trem = o->dcen / ORC_C;
+    int iso = 0;
+    if (fastpath > 0.0 && trem > 1e-14 && rate * trem > fastpath) {
+      double pcap = exp(-rate * trem);
+      if (orc_u01(st, 6 * ev + 3) < pcap) {
+        o->w = 0.0; o->dep_w = w; o->dcen = 0.0; o->t = tl; o->evt = CF_CAPTURE;
+        return;
+      }
+      iso = 1;
+      o->t = tl;
+      o->evt = CF_ABSORB;
+    } else {
+      double xi = orc_u01(st, 6 * ev + 2);
+      if (xi < 1e-15) xi = 1e-15;
+      double delay = -log(xi) / rate;
+      if (delay < trem - 1e-15) {
+        iso = 1;
+        o->t = tl + se / ORC_C + delay;
+        double dc = (trem - delay) * ORC_C;
+        o->dcen = dc > 0.0 ? dc : 0.0;
+        if (o->dcen <= eps_s) {
+          o->dcen = 0.0;
+          o->evt = CF_CENSUS;
+        } else {
+          o->evt = CF_ABSORB;
+        }
+      }
+    }
+    if (iso) {
+      o->w = w; o->dep_w = 0.0;
+      if (geom == 0) {
+        double uz = 2.0 * orc_u01(st, 6 * ev + 4) - 1.0;
+        double phi = 2.0 * ORC_PI * orc_u01(st, 6 * ev + 5);
+        double rxy = sqrt(fmax(0.0, 1.0 - uz * uz));
+        o->d1 = rxy * cos(phi);
+        o->d2 = rxy * sin(phi);
+      } else {
+        o->d2 = 2.0 * orc_u01(st, 6 * ev + 4) - 1.0;
+        o->d1 = cos(2.0 * ORC_PI * orc_u01(st, 6 * ev + 5));
+      }
+      return;
+    }
+  }
+  o->w = 0.0; o->dep_w = w; o->dcen = 0.0; o->t = tl; o->evt = CF_CAPTURE;
+}
+
+static void orc_cf_move_xy(orc_stream* st, uint32_t ev, double w, double ux, double uy, double x, double y, int i,
+                           int j, const double* e1, const double* e2, double sa, double beta, double dcen, double tl,
+                           double fastpath, orc_cf_out* o) {
+  double xl = e1[i], xr = e1[i + 1], yl = e2[j], yr = e2[j + 1];
+  double sx = 1e30, sy = 1e30;
+  if (ux > 0.0)
+    sx = (xr - x) / ux;
+  else if (ux < 0.0)
+    sx = (xl - x) / ux;
+  if (uy > 0.0)
+    sy = (yr - y) / uy;
+  else if (uy < 0.0)
+    sy = (yl - y) / uy;
+  double sb = sx < sy ? sx : sy;
+  double s_abs = sa > 1e-12 ? orc_cf_exp_dist(st, 6 * ev, sa) : 1e30;
+  double cl = (xr - xl) < (yr - yl) ? (xr - xl) : (yr - yl);
+  double eps_s = fmax(1e-14, 1e-12 * cl), eps_x = fmax(1e-14, 1e-12 * (xr - xl));
+  double eps_y = fmax(1e-14, 1e-12 * (yr - yl));
+  if (sa * cl > 10000.0) s_abs = orc_cf_exp_dist(st, 6 * ev + 1, 10000.0 / cl);
+  double se = sb;
+  if (s_abs < se) se = s_abs;
+  if (dcen < se) se = dcen;
+  if (fabs(se - sb) < 1e-11 && se < eps_s && dcen > eps_s) se = eps_s;
+  o->w = w; o->d1 = ux; o->d2 = uy; o->p1 = x; o->p2 = y; o->t = tl; o->dep_w = 0.0; o->dep_i = 0.0;
+  o->crossed = X_NONE;
+  if (se <= 0.0) {
+    o->dcen = 0.0; o->evt = CF_CENSUS;
+    return;
+  }
+  o->p1 = x + ux * se;
+  o->p2 = y + uy * se;
+  o->dep_i = w * se;
+  o->dcen = dcen - se;
+  if (fabs(se - sb) < 1e-11) {
+    if (sx <= sy) {
+      if (ux > 0.0) { o->p1 = xr + eps_x; o->crossed = X_IP; } else { o->p1 = xl - eps_x; o->crossed = X_IM; }
+    } else {
+      if (uy > 0.0) { o->p2 = yr + eps_y; o->crossed = X_JP; } else { o->p2 = yl - eps_y; o->crossed = X_JM; }
+    }
+    o->evt = CF_BOUNDARY;
+    return;
+  }
+  if (fabs(se - s_abs) < 1e-11) {
+    orc_cf_absorb(st, ev, 0, w, sa, beta, se, tl, eps_s, fastpath, o);
+    return;
+  }
+  o->dcen = 0.0; o->evt = CF_CENSUS;
+}
+
+static void orc_cf_move_rz(orc_stream* st, uint32_t ev, double w, double mu, double eta, double r, double z, int i,
+                           int j, const double* e1, const double* e2, double sa, double beta, double dcen, double tl,
+                           double fastpath, orc_cf_out* o) {
+  double rin = e1[i], rout = e1[i + 1], zlo = e2[j], zhi = e2[j + 1];
+  double s_rin = rin > 0.0 ? orc_dist_radial(r, mu, eta, rin) : 1e30;
+  double s_rout = orc_dist_radial(r, mu, eta, rout);
+  double s_zlo, s_zhi;
+  if (eta > 1e-15) {
+    s_zlo = 1e30; s_zhi = (zhi - z) / eta;
+  } else if (eta < -1e-15) {
+    s_zlo = (zlo - z) / eta; s_zhi = 1e30;
+  } else {
+    s_zlo = 1e30; s_zhi = 1e30;
+  }
+  double sb = s_rin;
+  if (s_rout < sb) sb = s_rout;
+  if (s_zlo < sb) sb = s_zlo;
+  if (s_zhi < sb) sb = s_zhi;
+  double s_abs = sa > 1e-12 ? orc_cf_exp_dist(st, 6 * ev, sa) : 1e30;
+  double cl = (rout - rin) < (zhi - zlo) ? (rout - rin) : (zhi - zlo);
+  double eps_s = fmax(1e-14, 1e-12 * cl), eps_r = fmax(1e-14, 1e-12 * (rout - rin));
+  double eps_z = fmax(1e-14, 1e-12 * (zhi - zlo));
+  if (sa * cl > 10000.0) s_abs = orc_cf_exp_dist(st, 6 * ev + 1, 10000.0 / cl);
+  double se = sb;
+  if (s_abs < se) se = s_abs;
+  if (dcen < se) se = dcen;
+  if (fabs(se - sb) < 1e-11 && se < eps_s && dcen > eps_s) se = eps_s;
+  o->w = w; o->d1 = mu; o->d2 = eta; o->p1 = r; o->p2 = z; o->t = tl; o->dep_w = 0.0; o->dep_i = 0.0;
+  o->crossed = X_NONE;
+  if (se <= 0.0) {
+    o->dcen = 0.0; o->evt = CF_CENSUS;
+    return;
+  }
+  double ome2 = 1.0 - eta * eta;
+  if (ome2 < 0.0) ome2 = 0.0;
+  double l = se * sqrt(ome2);
+  double rn2 = r * r + 2.0 * r * mu * l + l * l;
+  double rn = sqrt(rn2 > 0.0 ? rn2 : 0.0);
+  double zn = z + se * eta;
+  double mun = mu;
+  if (rn > 1e-15 && l > 0.0) {
+    mun = (r * mu + l) / rn;
+    if (mun > 1.0) mun = 1.0; else if (mun < -1.0) mun = -1.0;
+  }
+  o->d1 = mun; o->p1 = rn; o->p2 = zn;
+  o->dep_i = w * se;
+  o->dcen = dcen - se;
+  if (fabs(se - sb) < 1e-11) {
+    if (sb == s_rin) {
+      double rr = rin - eps_r;
+      o->p1 = rin > 0.0 ? (rr > 1e-12 ? rr : 1e-12) : 1e-12;
+      o->crossed = X_IM;
+    } else if (sb == s_rout) {
+      o->p1 = rout + eps_r; o->crossed = X_IP;
+    } else if (sb == s_zlo) {
+      o->p2 = zlo - eps_z; o->crossed = X_JM;
+    } else {
+      o->p2 = zhi + eps_z; o->crossed = X_JP;
+    }
+    o->evt = CF_BOUNDARY;
+    return;
+  }
+  if (fabs(se - s_abs) < 1e-11) {
+    orc_cf_absorb(st, ev, 1, w, sa, beta, se, tl, eps_s, fastpath, o);
+    return;
+  }
+  o->dcen = 0.0; o->evt = CF_CENSUS;
+}
+
+/* _transport_particles_2d_cf (IMC/IMC2D_CarterForest.py:346-508).  dep: raw
+ * captured energy per cell; si: raw sum of w s per cell (the caller divides
+ * by dt V); both (nx,ny) C-order, zeroed by the caller.  stats[8]: events,
+ * boundary, absorb-continue, census, capture, floor kills, reflections, cap
+ * hits.  A weight that falls to the floor is deposited in its cell (the
+ * reference drops it unbooked).  times are updated in place. */
+void orc_transport_2d_cf(int64_t n, double* w, double* d1, double* d2, double* times, double* p1, double* p2,
+                         int32_t* ci, int32_t* cj, const double* e1, const double* e2, int nx, int ny,
+                         const double* sigma_a, const double* beta, double dt, const int32_t* reflect,
+                         int64_t max_events, int32_t geometry, double weight_floor, double fastpath,
+                         double no_progress_tol, int64_t no_progress_max, uint64_t seed, uint32_t step, double* dep,
+                         double* si, double* bl_out, int64_t* stats_out, int32_t nthreads) {
+  int nt = 1;
+#ifdef _OPENMP
+  nt = nthreads > 0 ? nthreads : omp_get_max_threads();
+#endif
+  int64_t ncell = (int64_t)nx * ny;
+  double* depT = (double*)calloc((size_t)nt * (size_t)ncell, sizeof(double));
+  double* siT = (double*)calloc((size_t)nt * (size_t)ncell, sizeof(double));
+  double* blT = (double*)calloc((size_t)nt, sizeof(double));
+  int64_t* stT = (int64_t*)calloc((size_t)nt * 8, sizeof(int64_t));
+
+#pragma omp parallel for schedule(dynamic, 256) num_threads(nt)
+  for (int64_t k = 0; k < n; ++k) {
+    int tid = 0;
+#ifdef _OPENMP
+    tid = omp_get_thread_num();
+#endif
+    double* dp = depT + (size_t)tid * (size_t)ncell;
+    double* sp = siT + (size_t)tid * (size_t)ncell;
+    int64_t* stt = stT + (size_t)tid * 8;
+    orc_stream st;
+    orc_stream_init(&st, seed, ORC_TAG_TRANSPORT, (uint64_t)k, step);
+    double dcen = (dt - times[k]) * ORC_C;
+    int64_t events = 0, stuck = 0;
+    while (dcen > 0.0 && w[k] > weight_floor) {
+      double dcen_before = dcen;
+      int i = ci[k], j = cj[k];
+      if (i < 0 || i >= nx || j < 0 || j >= ny) break;
+      events += 1;
+      stt[0] += 1;
+      if (max_events > 0 && events > max_events) {
+        stt[7] += 1;
+        break;
+      }
+      int64_t c = (int64_t)i * ny + j;
+      orc_cf_out o;
+      uint32_t ev = (uint32_t)(events - 1);
+      if (geometry == 0)
+        orc_cf_move_xy(&st, ev, w[k], d1[k], d2[k], p1[k], p2[k], i, j, e1, e2, sigma_a[c], beta[c], dcen, times[k],
+                       fastpath, &o);
+      else
+        orc_cf_move_rz(&st, ev, w[k], d1[k], d2[k], p1[k], p2[k], i, j, e1, e2, sigma_a[c], beta[c], dcen, times[k],
+                       fastpath, &o);
+      dp[c] += o.dep_w;
+      sp[c] += o.dep_i;
+      w[k] = o.w; d1[k] = o.d1; d2[k] = o.d2; p1[k] = o.p1; p2[k] = o.p2; times[k] = o.t;
+      dcen = o.dcen;
+      if (o.evt == CF_BOUNDARY) stt[1] += 1;
+      else if (o.evt == CF_ABSORB) stt[2] += 1;
+      else if (o.evt == CF_CAPTURE) stt[4] += 1;
+      else stt[3] += 1;
+      if (dcen >= dcen_before * (1.0 - no_progress_tol)) {
+        stuck += 1;
+        if (stuck >= no_progress_max) break;
+      } else {
+        stuck = 0;
+      }
+      if (o.crossed != X_NONE) {
+        if (o.crossed == X_IP) ci[k] += 1;
+        else if (o.crossed == X_IM) ci[k] -= 1;
+        else if (o.crossed == X_JP) cj[k] += 1;
+        else cj[k] -= 1;
+        int i2 = ci[k], j2 = cj[k];
+        if (i2 < 0) {
+          if (reflect[0]) { d1[k] = -d1[k]; ci[k] = 0; stt[6] += 1; }
+          else { blT[tid] += w[k]; w[k] = 0.0; dcen = 0.0; }
+        } else if (i2 >= nx) {
+          if (reflect[1]) { d1[k] = -d1[k]; ci[k] = nx - 1; stt[6] += 1; }
+          else { blT[tid] += w[k]; w[k] = 0.0; dcen = 0.0; }
+        } else if (j2 < 0) {
+          if (reflect[2]) { d2[k] = -d2[k]; cj[k] = 0; stt[6] += 1; }
+          else { blT[tid] += w[k]; w[k] = 0.0; dcen = 0.0; }
+        } else if (j2 >= ny) {
+          if (reflect[3]) { d2[k] = -d2[k]; cj[k] = ny - 1; stt[6] += 1; }
+          else { blT[tid] += w[k]; w[k] = 0.0; dcen = 0.0; }
+        }
+      }
+      if (w[k] <= weight_floor && w[k] > 0.0) {
+        stt[5] += 1;
+        dp[(int64_t)ci[k] * ny + cj[k]] += w[k];
+        w[k] = 0.0;
+        dcen = 0.0;
+      }
+    }
+  }
+  double bl = 0.0;
+  for (int t = 0; t < 8; ++t) stats_out[t] = 0;
+  for (int th = 0; th < nt; ++th) {
+    for (int64_t c = 0; c < ncell; ++c) {
+      dep[c] += depT[(size_t)th * ncell + c];
+      si[c] += siT[(size_t)th * ncell + c];
+    }
+    bl += blT[th];
+    for (int t = 0; t < 8; ++t) stats_out[t] += stT[th * 8 + t];
+  }
+  *bl_out = bl;
+  free(depT);
+  free(siT);
+  free(blT);
+  free(stT);
+}

@@ -21,6 +21,7 @@ FP64 = 0
 FP32_TRACK = 1
 SRC_EMISSION = 0
 SRC_FIXED = 1
+SRC_CF = 2
 
 A_RAD = 0.01372
 C_LIGHT = 29.98
@@ -49,6 +50,9 @@ SIGNATURES = {
     "imc2d_sample_boundary": (C.c_int, [_vp, C.c_int, C.c_int64, C.c_double, C.c_double, _dp, C.c_int]),
     "imc2d_sample_volume": (C.c_int, [_vp, C.c_int, C.c_int64, _dp, C.c_double, C.c_int, _dp, _dp, _i64p]),
     "imc2d_transport": (C.c_int, [_vp, C.c_double, _i32p, C.c_int64, C.c_double, _dp, _dp, _dp, _i64p]),
+    "imc2d_set_material_cf": (C.c_int, [_vp, _dp, _dp, _dp]),
+    "imc2d_transport_cf": (C.c_int, [_vp, C.c_double, _i32p, C.c_int64, C.c_double, C.c_double, C.c_double,
+                                     C.c_int64, _dp, _dp, _dp, _i64p]),
     "imc2d_last_transport_ms": (C.c_float, [_vp]),
     "imc2d_comb": (C.c_int, [_vp, C.c_int64, _dp, _dp, _dp, _dp]),
     "imc2d_enable_adjoint": (C.c_int, [_vp, C.c_int]),
@@ -259,6 +263,25 @@ class Engine2D:
         st = np.zeros(8, dtype=np.int64)
         check(self._L.imc2d_transport(self._h, float(dt), _p(refl, _i32p), int(max_events), float(weight_floor),
                                       _p(dep), _p(si), C.byref(bl), _p(st, _i64p)))
+        return dep, si, bl.value, st
+
+    def set_material_cf(self, T, sigma_a, beta):
+        """Carter-Forest material: T, the true sigma_a and beta per cell."""
+        check(self._L.imc2d_set_material_cf(self._h, _p(_cell(T, self.shape)), _p(_cell(sigma_a, self.shape)),
+                                            _p(_cell(beta, self.shape))))
+
+    def transport_cf(self, dt, reflect, max_events, weight_floor=-1.0, fastpath_threshold=0.0,
+                     no_progress_rel_tol=1e-12, no_progress_max_streak=8, want_tallies=True):
+        """Carter-Forest transport; returns (dep = captured energy / V,
+        si = sum w s / (dt V), boundary loss, stats[8])."""
+        refl = np.array([1 if r else 0 for r in reflect], dtype=np.int32)
+        dep = np.empty(self.shape) if want_tallies else None
+        si = np.empty(self.shape) if want_tallies else None
+        bl = C.c_double()
+        st = np.zeros(8, dtype=np.int64)
+        check(self._L.imc2d_transport_cf(self._h, float(dt), _p(refl, _i32p), int(max_events), float(weight_floor),
+                                         float(fastpath_threshold), float(no_progress_rel_tol),
+                                         int(no_progress_max_streak), _p(dep), _p(si), C.byref(bl), _p(st, _i64p)))
         return dep, si, bl.value, st
 
     def last_transport_ms(self):

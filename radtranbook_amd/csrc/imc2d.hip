@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "../../include/imc_hip.h"
+#include "imc2d_cf_kernels.hpp"
 #include "imc2d_kernels.hpp"
 #include "imc_device.hpp"
 #include "imc_util.hpp"
@@ -250,7 +251,8 @@ __global__ void k_sample_volume(PoolView<R> P, int64_t base, int64_t total, cons
                                 int64_t ncell, int ny, const double* __restrict__ e1, const double* __restrict__ e2,
                                 const double* __restrict__ QV, int kind, int geometry, const double* __restrict__ Ar,
                                 const double* __restrict__ Mr, const double* __restrict__ Pn, double dt, Key key,
-                                uint32_t step, uint32_t tag) {
+                                uint32_t step, uint32_t tag, const double* __restrict__ cf_sa,
+                                const double* __restrict__ cf_beta) {
 #pragma clang fp contract(off)
   int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= total) return;
@@ -260,7 +262,9 @@ __global__ void k_sample_volume(PoolView<R> P, int64_t base, int64_t total, cons
   double x0 = e1[i], x1 = e1[i + 1], y0 = e2[j], y1 = e2[j + 1];
   Ctr ctr = make_ctr(tag, (uint64_t)k, step);
   Pair b0 = block_d(ctr, key, 0), b1 = block_d(ctr, key, 1), b2 = block_d(ctr, key, 2);
-  double t = (kind == 1) ? 0.0 : dt * b0.u0;
+  // kind 0 uniform time, 1 time 0 (census), 2 Carter-Forest emission time
+  // from the same uniform (IMC/IMC2D_CarterForest.py:900-903)
+  double t = (kind == 1) ? 0.0 : (kind == 2 ? cf_time(kC * cf_sa[c] * cf_beta[c], dt, b0.u0) : dt * b0.u0);
   double p1, p2, d1, d2;
   if (geometry == 0) {
     p1 = x0 + (x1 - x0) * b0.u1;
@@ -1026,6 +1030,242 @@ __global__ void k_f64_to_f32(int64_t n, const double* __restrict__ a, float* __r
 // ---------------------------------------------------------------------------
 // context
 // ---------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
+// Carter-Forest tracker (IMC/IMC2D_CarterForest.py:346-508), FP64, with the
+// persistent-wave design of k_transport.  Per history: the reference's loop
+// (range check, event count and cap, move, tallies, no-progress streak, face
+// crossing with reflection or escape, weight floor).  dep_raw: captured
+// energy per cell; si_raw: sum of w s per cell (both per private copy).  A
+// weight at or below the floor is deposited in its cell (the reference drops
+// it unbooked).  stats: events, boundary, absorb-continue, census, capture,
+// floor, reflections, cap hits.
+// ---------------------------------------------------------------------------
+struct CfArgs {
+  PoolView<double> P;
+  int64_t n;
+  const double2* e1p;
+  const double2* e2p;
+  int nx, ny;
+  const double* sa;
+  const double* beta;
+  double dt;
+  int reflect;
+  int64_t max_events;
+  const double* floor_ptr;
+  double fastpath, tol;
+  int64_t max_streak;
+  Key key;
+  uint32_t step;
+  unsigned long long* next;
+  int batch;
+  double* dep_raw;
+  double* si_raw;
+  int64_t ncell;
+  int copy_mask;
+  double* bl;
+  unsigned long long* stats;
+};
+
+template <int GEOM>
+__global__ __launch_bounds__(256) void k_transport_cf(CfArgs A) {
+#pragma clang fp contract(off)
+  const double wfloor = *A.floor_ptr;
+  const int64_t copy_off = (int64_t)(blockIdx.x & A.copy_mask) * A.ncell;
+  double* const dep_t = A.dep_raw + copy_off;
+  double* const si_t = A.si_raw + copy_off;
+  __shared__ int lkey[kLdsSlots];
+  __shared__ double lval[kLdsSlots];
+  lds_tally_init(lkey, lval);
+  __syncthreads();
+  const int ny = A.ny, nx = A.nx;
+  const int ncell = (int)A.ncell;
+  // keys [0, ncell) are captured energy, [ncell, 2 ncell) path tallies
+  auto add = [&](int key, double v) {
+    if (!lds_tally_add(lkey, lval, key, v)) atomic_add_f64(key < ncell ? dep_t + key : si_t + (key - ncell), v);
+  };
+  int64_t qbase = 0, qend = 0;
+  bool exhausted = false, active = false;
+  int64_t k = 0;
+  double w = 0.0, d1 = 0.0, d2 = 0.0, p1 = 0.0, p2 = 0.0, tl = 0.0, dcen = 0.0;
+  int i = 0, j = 0;
+  uint32_t ev = 0;
+  int64_t stuck = 0;
+  double acc_si = 0.0;
+  int acc_cell = -1;
+  double bl = 0.0;
+  uint32_t st_ev = 0, st_bnd = 0, st_abs = 0, st_cen = 0, st_cap = 0, st_floor = 0, st_refl = 0, st_hit = 0;
+  while (true) {
+    uint64_t need = __ballot(!active);
+    if (need) {
+      if (qbase >= qend && !exhausted) {
+        unsigned long long b = 0;
+        if (__lane_id() == 0) b = atomicAdd(A.next, (unsigned long long)A.batch);
+        b = __shfl(b, 0, 64);
+        if ((int64_t)b >= A.n) {
+          exhausted = true;
+        } else {
+          qbase = (int64_t)b;
+          qend = ((int64_t)b + A.batch < A.n) ? (int64_t)b + A.batch : A.n;
+        }
+      }
+      int64_t avail = qend - qbase;
+      if (avail > 0) {
+        int64_t rank = __popcll(need & lanemask_lt());
+        if (!active && rank < avail) {
+          k = qbase + rank;
+          w = A.P.w[k];
+          d1 = A.P.d1[k];
+          d2 = A.P.d2[k];
+          p1 = A.P.p1[k];
+          p2 = A.P.p2[k];
+          tl = A.P.t[k];
+          i = A.P.ci[k];
+          j = A.P.cj[k];
+          dcen = (A.dt - tl) * kC;
+          ev = 0;
+          stuck = 0;
+          active = dcen > 0.0 && w > wfloor;
+        }
+        int64_t take = __popcll(need);
+        qbase += take < avail ? take : avail;
+      }
+    }
+    if (__ballot(active) == 0) {
+      if (exhausted) break;
+      continue;
+    }
+    if (!active) continue;
+    bool finish = false;
+    if (i < 0 || i >= nx || j < 0 || j >= ny) {
+      finish = true;
+    } else {
+      ++ev;
+      ++st_ev;
+      if (A.max_events > 0 && (int64_t)ev > A.max_events) {
+        ++st_hit;
+        finish = true;
+      } else {
+        const int c = i * ny + j;
+        const double2 ex = A.e1p[i], ey = A.e2p[j];
+        Ctr ctr = make_ctr(kTagTransport, (uint64_t)k, A.step);
+        CfOut o;
+        const double dcen_before = dcen;
+        if (GEOM == 0)
+          cf_move_xy(ctr, A.key, ev - 1u, w, d1, d2, p1, p2, ex.x, ex.y, ey.x, ey.y, A.sa[c], A.beta[c], dcen, tl,
+                     A.fastpath, o);
+        else
+          cf_move_rz(ctr, A.key, ev - 1u, w, d1, d2, p1, p2, ex.x, ex.y, ey.x, ey.y, A.sa[c], A.beta[c], dcen, tl,
+                     A.fastpath, o);
+        if (o.dep_w != 0.0) add(c, o.dep_w);
+        if (c != acc_cell) {
+          if (acc_cell >= 0 && acc_si != 0.0) add(ncell + acc_cell, acc_si);
+          acc_cell = c;
+          acc_si = 0.0;
+        }
+        acc_si += o.dep_i;
+        w = o.w; d1 = o.d1; d2 = o.d2; p1 = o.p1; p2 = o.p2; tl = o.t;
+        dcen = o.dcen;
+        st_bnd += o.evt == kCfBoundary;
+        st_abs += o.evt == kCfAbsorb;
+        st_cap += o.evt == kCfCapture;
+        st_cen += o.evt == kCfCensus;
+        if (dcen >= dcen_before * (1.0 - A.tol)) {
+          if (++stuck >= A.max_streak) finish = true;
+        } else {
+          stuck = 0;
+        }
+        if (!finish) {
+          if (o.crossed != kCrossNone) {
+            if (o.crossed == kCrossIPlus) ++i;
+            else if (o.crossed == kCrossIMinus) --i;
+            else if (o.crossed == kCrossJPlus) ++j;
+            else --j;
+            int side = -1;
+            if (i < 0) side = 0;
+            else if (i >= nx) side = 1;
+            else if (j < 0) side = 2;
+            else if (j >= ny) side = 3;
+            if (side >= 0) {
+              if ((A.reflect >> side) & 1) {
+                if (side < 2) { d1 = -d1; i = side == 0 ? 0 : nx - 1; }
+                else { d2 = -d2; j = side == 2 ? 0 : ny - 1; }
+                ++st_refl;
+              } else {
+                bl += w;
+                w = 0.0;
+                dcen = 0.0;
+              }
+            }
+          }
+          if (w <= wfloor && w > 0.0) {
+            ++st_floor;
+            add(i * ny + j, w);
+            w = 0.0;
+            dcen = 0.0;
+          }
+          if (!(dcen > 0.0 && w > wfloor)) finish = true;
+        }
+      }
+    }
+    if (finish) {
+      A.P.w[k] = w;
+      A.P.d1[k] = d1;
+      A.P.d2[k] = d2;
+      A.P.p1[k] = p1;
+      A.P.p2[k] = p2;
+      A.P.t[k] = tl;
+      A.P.ci[k] = i;
+      A.P.cj[k] = j;
+      active = false;
+    }
+  }
+  if (acc_cell >= 0 && acc_si != 0.0) add(ncell + acc_cell, acc_si);
+  __syncthreads();
+  for (int s = threadIdx.x; s < kLdsSlots; s += blockDim.x) {
+    int kk = lkey[s];
+    if (kk >= 0) {
+      double v = lval[s];
+      if (v != 0.0) atomic_add_f64(kk < ncell ? dep_t + kk : si_t + (kk - ncell), v);
+    }
+  }
+  unsigned long long s0 = wave_sum<unsigned long long>(st_ev);
+  unsigned long long s1 = wave_sum<unsigned long long>(st_bnd);
+  unsigned long long s2 = wave_sum<unsigned long long>(st_abs);
+  unsigned long long s3 = wave_sum<unsigned long long>(st_cen);
+  unsigned long long s4 = wave_sum<unsigned long long>(st_cap);
+  unsigned long long s5 = wave_sum<unsigned long long>(st_floor);
+  unsigned long long s6 = wave_sum<unsigned long long>(st_refl);
+  unsigned long long s7 = wave_sum<unsigned long long>(st_hit);
+  double blw = wave_sum<double>(bl);
+  if (__lane_id() == 0) {
+    atomicAdd(A.stats + 0, s0);
+    atomicAdd(A.stats + 1, s1);
+    atomicAdd(A.stats + 2, s2);
+    atomicAdd(A.stats + 3, s3);
+    atomicAdd(A.stats + 4, s4);
+    atomicAdd(A.stats + 5, s5);
+    atomicAdd(A.stats + 6, s6);
+    atomicAdd(A.stats + 7, s7);
+    if (blw != 0.0) atomic_add_f64(A.bl, blw);
+  }
+}
+
+// dep = captured energy / V, si = sum w s / (dt V), summed over the copies.
+__global__ void k_cf_finalize(int64_t n, int copies, const double* __restrict__ dep_raw,
+                              const double* __restrict__ si_raw, const double* __restrict__ vol, double dt,
+                              double* __restrict__ dep, double* __restrict__ si) {
+#pragma clang fp contract(off)
+  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  double a = 0.0, b = 0.0;
+  for (int k = 0; k < copies; ++k) {
+    a += dep_raw[(int64_t)k * n + c];
+    b += si_raw[(int64_t)k * n + c];
+  }
+  dep[c] = a / vol[c];
+  si[c] = b / (dt * vol[c]);
+}
+
 struct Imc2dCtx {
   int geometry = 0;  // 0 xy, 1 rz
   int nx = 0, ny = 0;
@@ -1040,7 +1280,8 @@ struct Imc2dCtx {
   double *e1 = nullptr, *e2 = nullptr, *vol = nullptr;
   float *e1f = nullptr, *e2f = nullptr;
   DevBuf<double> T, sig, cv, sa, ss, E, Ar, Mr, Pn, dep_raw, si_raw, dep, si, ecen, ew, disc, prof, sum_scr, wbuf,
-      volsrc;
+      volsrc, beta;
+  bool cf_ready = false;  // Carter-Forest material (sigma_a, beta) set
   DevBuf<float> saf, ssf;
   DevBuf<float2> sas32, e1p32, e2p32;
   DevBuf<double2> sas64, e1p64, e2p64;
@@ -1169,7 +1410,8 @@ int imc2d_create(imc2d_ctx** out, int geometry, int nx, int ny, const double* ed
   int64_t nc = c->ncell;
   int rc = 0;
   DevBuf<double>* cellbufs[] = {&c->T, &c->sig, &c->cv, &c->sa, &c->ss, &c->E, &c->Ar, &c->Mr, &c->Pn,
-                                &c->dep_raw, &c->si_raw, &c->dep, &c->si, &c->ecen, &c->ew, &c->disc, &c->volsrc};
+                                &c->dep_raw, &c->si_raw, &c->dep, &c->si, &c->ecen, &c->ew, &c->disc, &c->volsrc,
+                                &c->beta};
   for (auto* b : cellbufs)
     if ((rc = b->reserve(nc))) return rc;
   if ((rc = c->dep_raw.reserve(nc * kMaxTallyCopies))) return rc;
@@ -1227,7 +1469,7 @@ void imc2d_destroy(imc2d_ctx* c) {
   (void)hipStreamSynchronize(c->stream);
   DevBuf<double>* cellbufs[] = {&c->T, &c->sig, &c->cv, &c->sa, &c->ss, &c->E, &c->Ar, &c->Mr, &c->Pn,
                                 &c->dep_raw, &c->si_raw, &c->dep, &c->si, &c->ecen, &c->ew, &c->disc, &c->prof,
-                                &c->sum_scr, &c->wbuf, &c->volsrc};
+                                &c->sum_scr, &c->wbuf, &c->volsrc, &c->beta};
   for (auto* b : cellbufs) b->release();
   c->stage.release();
   c->saf.release();
@@ -1387,11 +1629,11 @@ static int launch_volume(imc2d_ctx* c, int64_t base, int64_t total, const double
   if (c->precision == IMC_FP64)
     k_sample_volume<double><<<g, 256, 0, c->stream>>>(c->A.view<double>(), base, total, c->offs.p, c->ncell, c->ny,
                                                       c->e1, c->e2, QV, kind, geom, c->Ar.p, c->Mr.p, c->Pn.p, dt,
-                                                      key, step, tag);
+                                                      key, step, tag, c->sa.p, c->beta.p);
   else
     k_sample_volume<float><<<g, 256, 0, c->stream>>>(c->A.view<float>(), base, total, c->offs.p, c->ncell, c->ny,
                                                      c->e1, c->e2, QV, kind, geom, c->Ar.p, c->Mr.p, c->Pn.p, dt,
-                                                     key, step, tag);
+                                                     key, step, tag, c->sa.p, c->beta.p);
   IMC_HIP(hipGetLastError());
   return kOk;
 }
@@ -1473,7 +1715,10 @@ int imc2d_sample_boundary(imc2d_ctx* c, int side, int64_t n, double E_total, dou
 int imc2d_sample_volume(imc2d_ctx* c, int kind, int64_t N, const double* QV_host, double dt, int rz_linear,
                         double* E_out, double* total_out, int64_t* count_out) {
   IMC_REQUIRE(c && N >= 0, kErrArg, "imc2d_sample_volume: bad arguments");
-  IMC_REQUIRE(kind == IMC_SRC_EMISSION || (kind == IMC_SRC_FIXED && QV_host), kErrArg, "imc2d_sample_volume: bad kind");
+  IMC_REQUIRE(kind == IMC_SRC_EMISSION || ((kind == IMC_SRC_FIXED || kind == IMC_SRC_CF) && QV_host), kErrArg,
+              "imc2d_sample_volume: bad kind");
+  IMC_REQUIRE(kind != IMC_SRC_CF || c->cf_ready, kErrState,
+              "imc2d_sample_volume: IMC_SRC_CF needs imc2d_set_material_cf first");
   IMC_HIP(hipSetDevice(c->device));
   const double* QV;
   uint32_t tag;
@@ -1488,7 +1733,7 @@ int imc2d_sample_volume(imc2d_ctx* c, int kind, int64_t N, const double* QV_host
     int rc0 = copy_jobs(&in, 1, c->stream);
     if (rc0) return rc0;
     QV = c->volsrc.p;
-    tag = kTagSource;
+    tag = kind == IMC_SRC_CF ? kTagEmission : kTagSource;
   }
   int64_t total = 0;
   bool fixed = c->fixed_counts && kind == IMC_SRC_EMISSION;
@@ -1520,7 +1765,7 @@ int imc2d_sample_volume(imc2d_ctx* c, int kind, int64_t N, const double* QV_host
     IMC_HIP(hipGetLastError());
   }
   if ((rc = pool_ensure(c, c->n + total))) return rc;
-  if ((rc = launch_volume(c, c->n, total, QV, 0, geom, dt, tag, c->step))) return rc;
+  if ((rc = launch_volume(c, c->n, total, QV, kind == IMC_SRC_CF ? 2 : 0, geom, dt, tag, c->step))) return rc;
   c->n += total;
   return kOk;
 }
@@ -1628,6 +1873,97 @@ int imc2d_transport(imc2d_ctx* c, double dt, const int32_t* reflect, int64_t max
 }
 
 float imc2d_last_transport_ms(imc2d_ctx* c) { return c ? c->last_transport_ms : -1.0f; }
+
+// Carter-Forest material (IMC/IMC2D_CarterForest.py:948-949): T (for the rz
+// linear emission tilt), the true sigma_a and beta = 4 a max(T,1e-12)^3 / c_v,
+// per cell.  The tracker runs in FP64 only.
+int imc2d_set_material_cf(imc2d_ctx* c, const double* T, const double* sigma_a, const double* beta) {
+  IMC_REQUIRE(c && T && sigma_a && beta, kErrArg, "imc2d_set_material_cf: null argument");
+  IMC_REQUIRE(c->precision == IMC_FP64, kErrArg,
+              "imc2d_set_material_cf: the Carter-Forest tracker runs in FP64 (create the context with IMC_FP64)");
+  IMC_HIP(hipSetDevice(c->device));
+  CopyJob in[3] = {stage_in(c, 0, c->T.p, T), stage_in(c, 1, c->sa.p, sigma_a), stage_in(c, 2, c->beta.p, beta)};
+  int rc = copy_jobs(in, 3, c->stream);
+  if (rc) return rc;
+  IMC_HIP(hipStreamSynchronize(c->stream));
+  c->cf_ready = true;
+  return kOk;
+}
+
+// Carter-Forest transport of the whole pool (IMC/IMC2D_CarterForest.py:346-508).
+// weight_floor < 0: 1e-10 sum(w)/N on the device (1036).  Outputs nullable:
+// dep (captured energy / V) and si (sum w s / (dt V)) per cell, bl, stats[8].
+int imc2d_transport_cf(imc2d_ctx* c, double dt, const int32_t* reflect, int64_t max_events, double weight_floor,
+                       double fastpath_threshold, double no_progress_rel_tol, int64_t no_progress_max_streak,
+                       double* dep_out, double* si_out, double* bl_out, int64_t* stats_out) {
+  IMC_REQUIRE(c && reflect && max_events >= 0 && no_progress_max_streak >= 1, kErrArg,
+              "imc2d_transport_cf: bad arguments");
+  IMC_REQUIRE(c->cf_ready, kErrState, "imc2d_transport_cf: call imc2d_set_material_cf first");
+  IMC_HIP(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  int64_t n = c->n;
+  IMC_HIP(hipMemsetAsync(c->dep_raw.p, 0, sizeof(double) * c->ncell * c->tally_copies, s));
+  IMC_HIP(hipMemsetAsync(c->si_raw.p, 0, sizeof(double) * c->ncell * c->tally_copies, s));
+  IMC_HIP(hipMemsetAsync(c->stats, 0, sizeof(unsigned long long) * 8, s));
+  IMC_HIP(hipMemsetAsync(c->scal + 3, 0, sizeof(double), s));
+  IMC_HIP(hipMemsetAsync(c->next, 0, sizeof(unsigned long long), s));
+  int rc;
+  if (weight_floor < 0.0) {
+    if ((rc = tree_sum(c->A.w, n, c->scal + 1, c->tree_scr.p, s))) return rc;
+    k_weight_floor<<<1, 1, 0, s>>>(c->scal + 1, n, c->scal + 2);
+    IMC_HIP(hipGetLastError());
+  } else {
+    IMC_HIP(hipMemcpyAsync(c->scal + 2, &weight_floor, sizeof(double), hipMemcpyHostToDevice, s));
+  }
+  int refl = 0;
+  for (int a = 0; a < 4; ++a) refl |= (reflect[a] ? 1 : 0) << a;
+  const int block = 256;
+  const void* fn = c->geometry == IMC_GEOM_XY ? (const void*)k_transport_cf<0> : (const void*)k_transport_cf<1>;
+  int grid;
+  if (c->grid_override > 0) {
+    grid = c->grid_override;
+  } else {
+    int per_cu = 0;
+    IMC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, block, 0));
+    grid = std::max(per_cu, 1) * c->n_cu;
+  }
+  int64_t need_blocks = (n + block - 1) / block;
+  if (need_blocks < grid) grid = (int)std::max<int64_t>(need_blocks, 1);
+  IMC_HIP(hipEventRecord(c->ev0, s));
+  if (n > 0) {
+    CfArgs A{c->A.view<double>(), n, c->e1p64.p, c->e2p64.p, c->nx, c->ny, c->sa.p, c->beta.p, dt, refl, max_events,
+             c->scal + 2, fastpath_threshold, no_progress_rel_tol, no_progress_max_streak, make_key(c->seed),
+             c->step, c->next, c->batch, c->dep_raw.p, c->si_raw.p, c->ncell, c->tally_copies - 1, c->scal + 3,
+             c->stats};
+    if (c->geometry == IMC_GEOM_XY)
+      k_transport_cf<0><<<grid, block, 0, s>>>(A);
+    else
+      k_transport_cf<1><<<grid, block, 0, s>>>(A);
+    IMC_HIP(hipGetLastError());
+  }
+  IMC_HIP(hipEventRecord(c->ev1, s));
+  k_cf_finalize<<<launch_blocks(c->ncell, 256), 256, 0, s>>>(c->ncell, c->tally_copies, c->dep_raw.p, c->si_raw.p,
+                                                             c->vol, dt, c->dep.p, c->si.p);
+  IMC_HIP(hipGetLastError());
+  double* hs = stage_scalars(c);
+  unsigned long long* hst = (unsigned long long*)(hs + 1);
+  {
+    double* hsd = stage_scalars_dev(c);
+    CopyJob out[4] = {dep_out ? stage_out(c, 3, c->dep.p) : CopyJob{}, si_out ? stage_out(c, 4, c->si.p) : CopyJob{},
+                      CopyJob{c->scal + 3, hsd, 8}, CopyJob{c->stats, hsd + 1, 64}};
+    int rc1 = copy_jobs(out, 4, s);
+    if (rc1) return rc1;
+  }
+  IMC_HIP(hipStreamSynchronize(s));
+  IMC_HIP(hipEventElapsedTime(&c->last_transport_ms, c->ev0, c->ev1));
+  size_t bytes = sizeof(double) * c->ncell;
+  if (dep_out) std::memcpy(dep_out, c->stage.slot(c->ncell, 3), bytes);
+  if (si_out) std::memcpy(si_out, c->stage.slot(c->ncell, 4), bytes);
+  if (bl_out) *bl_out = hs[0];
+  if (stats_out)
+    for (int a = 0; a < 8; ++a) stats_out[a] = (int64_t)hst[a];
+  return kOk;
+}
 
 // Census energy per cell of the current pool into c->ecen (device).
 static int census_histogram(imc2d_ctx* c) {
