@@ -513,14 +513,115 @@ def main_diff(args):
     print(json.dumps(out), flush=True)
 
 
+# ---------------------------------------------------------------------------
+# Carter-Forest variant on the C2 problem (FP64 tracker)
+# ---------------------------------------------------------------------------
+# event model: state (pos 2x8, dir 2x8, weight 8, d_census 8, time 8, cell 4,
+# history id 4, event counter 4 = 68 B) loaded and stored once per segment,
+# (sigma_a, beta) gathered, two FP64 tallies
+BYTES_PER_SEGMENT_CF = 2 * 68 + 16 + 16
+
+
+def main_cf(args):
+    from radtranbook_amd import IMC2D_CarterForest as CFM, _lib
+    from radtranbook_amd.problems import marshak_2d_heterogeneous
+
+    _lib.load()
+    _lib.require_gpu()
+    CFM.VERBOSE = False
+    steps = args.steps if args.steps is not None else 20
+    warmup = args.warmup if args.warmup is not None else 2
+    npart = int(args.particles) if args.particles is not None else 10_000_000
+    prob = marshak_2d_heterogeneous(n=args.cells, n_particles=npart, n_steps=steps)
+    e1, e2 = prob.edges1, prob.edges2
+    T0 = np.full(prob.shape, prob.T0)
+    src = np.zeros(prob.shape)
+    eng = [None]
+
+    def one_run(seed, nsteps, timed):
+        st = CFM.init_simulation(prob.Ntarget, T0, T0, e1, e2, prob.eos.eos, prob.eos.inv_eos, geometry=prob.geometry,
+                                 precision="fp64", seed=seed, engine=eng[0])
+        eng[0] = st._engine
+        infos = []
+        t0 = time.perf_counter()
+        for s in range(nsteps):
+            st, info = CFM.step(st, prob.Ntarget, prob.Nboundary, 0, prob.Nmax, prob.T_boundary, prob.dt, e1, e2,
+                                prob.opacity, prob.eos.inv_eos, prob.eos.cv_func, src, reflect=prob.reflect,
+                                geometry=prob.geometry, max_events_per_particle=prob.max_events)
+            infos.append(info)
+            if timed:
+                te = info["profiling"]["transport_events"]
+                log(f"  [cf] step {s + 1}/{nsteps}: {info['profiling']['phase_times_s']['total'] * 1e3:.1f} ms, "
+                    f"transport kernel {info['profiling']['transport_kernel_ms']:.2f} ms, segments {te['total']:.3e}, "
+                    f"captures {te['absorption_capture_events']:.3e}, re-emissions "
+                    f"{te['absorption_continue_events']:.3e}")
+        return infos, time.perf_counter() - t0
+
+    if warmup > 0:
+        one_run(1000, warmup, False)
+    infos, wall = one_run(1, steps, True)
+    eng[0].close()
+    segs = sum(i["profiling"]["transport_events"]["total"] for i in infos)
+    hist = sum(i["profiling"]["transport_events"]["n_particles_transported"] for i in infos)
+    kern = np.array([i["profiling"]["transport_kernel_ms"] for i in infos])
+    seg_launch = np.array([i["profiling"]["transport_events"]["total"] for i in infos], dtype=np.float64)
+    resid = max(abs(i["energy_residual"]) / i["total_energy"] for i in infos)
+    value = segs / wall
+    achieved = seg_launch.mean() * BYTES_PER_SEGMENT_CF / (kern.mean() / 1e3) / 1e9
+    out = {
+        "metric": "IMC segments/s, Carter-Forest 2D 200x200 heterogeneous Marshak, 1e7 particles/step",
+        "value": value, "unit": "segments/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
+        "ms_per_step": wall / steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic (seeded sigma0 field, Philox particle streams)",
+        "config": {"workload": f"Carter-Forest gray IMC 2D xy {args.cells}x{args.cells} heterogeneous Marshak, "
+                               f"{npart:.0e} particles/step, dt=0.005 ns x {steps} steps",
+                   "cells": args.cells * args.cells, "particles_per_step": npart},
+        "histories_per_s": hist / wall,
+        "transport_segments_per_s": segs / (kern.sum() / 1e3),
+        "transport_kernel_ms_per_step": float(kern.mean()),
+        "segments_per_step": float(seg_launch.mean()),
+        "max_energy_residual_rel": resid,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_transport_cf<xy>",
+                     "bytes_per_segment": BYTES_PER_SEGMENT_CF},
+    }
+    if not args.no_cpu:
+        from oracle import oracle as O, oracle_cf as OCF
+
+        O.build()
+        ncpu = int(args.cpu_particles) if args.cpu_particles is not None else 4_000_000
+        nst = args.cpu_steps if args.cpu_steps is not None else 8
+        cores = len(os.sched_getaffinity(0))
+        orc = OCF.OracleCF2D(e1, e2, prob.geometry, seed=1, nthreads=cores)
+        st = orc.init_simulation(ncpu // 2, T0, T0, prob.eos.eos, prob.eos.inv_eos)
+        csegs, ct = 0, 0.0
+        for s in range(nst):
+            t0 = time.perf_counter()
+            st, info = orc.step(st, ncpu // 2, ncpu // 2, 0, ncpu, prob.T_boundary, prob.dt, prob.opacity,
+                                prob.eos.inv_eos, prob.eos.cv_func, src, reflect=prob.reflect)
+            dtc = time.perf_counter() - t0
+            if s >= 1:
+                csegs += int(info["stats"][0])
+                ct += dtc
+            log(f"  [cpu cf] step {s + 1}/{nst}: {dtc:.2f} s, segments {int(info['stats'][0]):.3e}")
+        out["cpu_baseline"] = {"value": csegs / ct if ct > 0 else None, "unit": "segments/s", "cores": cores,
+                               "kind": "port",
+                               "sample": f"FP64 Carter-Forest C oracle (OpenMP, history-based), same problem at "
+                                         f"{ncpu:.0e} particles/step, steps 2..{nst} timed ({csegs:.3e} segments in "
+                                         f"{ct:.1f} s)"}
+        if ct > 0:
+            out["gpu_over_cpu_segments_per_s"] = value / (csegs / ct)
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=["marshak2d", "slab1d", "batch1d", "mg2d", "diff_rz"],
+    ap.add_argument("--workload", choices=["marshak2d", "slab1d", "batch1d", "mg2d", "diff_rz", "cf2d"],
                     default="marshak2d",
                     help="marshak2d: 2D 200x200 heterogeneous (default, headline); slab1d: the reference's 1D slab "
                          "case; batch1d: 256 independent slabs; mg2d: multigroup crooked pipe; diff_rz: "
-                         "differentiable rz step")
+                         "differentiable rz step; cf2d: Carter-Forest variant of marshak2d")
     ap.add_argument("--problems", type=int, default=256)
     ap.add_argument("--tune", action=argparse.BooleanOptionalAction, default=True,
                     help="autotune the 1D transport launch shape on the first warm-up step")
@@ -547,6 +648,9 @@ def main():
         return
     if args.workload == "diff_rz":
         main_diff(args)
+        return
+    if args.workload == "cf2d":
+        main_cf(args)
         return
     if args.workload != "marshak2d":
         main_1d(args, args.workload)
