@@ -582,7 +582,10 @@ def main_cf(args):
         "segments_per_step": float(seg_launch.mean()),
         "max_energy_residual_rel": resid,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_transport_cf<xy>",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": profiled_traffic("k_transport_cf_marshak2d_200x200_1e7.md", "fp64", steps, warmup,
+                                                 args.cells == 200 and npart == 10_000_000),
+                     "kernel": "k_transport_cf<xy>",
                      "bytes_per_segment": BYTES_PER_SEGMENT_CF},
     }
     if not args.no_cpu:
