@@ -405,6 +405,7 @@ struct TransportArgs {
   int copy_mask;  // number of tally copies - 1 (power of two)
   double* bl;
   unsigned long long* stats;  // [8]
+  QueueMap Q;
 };
 
 // Off: offset type of the particle-array accesses (uint32_t while every
@@ -422,8 +423,7 @@ __global__ __launch_bounds__(256) void k_transport(TransportArgs<R> A) {
   lds_tally_init(lkey, lval);
   __syncthreads();
   const int ny = A.ny, nx = A.nx;
-  int64_t qbase = 0, qend = 0;
-  bool exhausted = false;
+  WaveQueue wq;
   bool active = false;
   int64_t k = 0;
   double w = 0.0;
@@ -438,40 +438,24 @@ __global__ __launch_bounds__(256) void k_transport(TransportArgs<R> A) {
   while (true) {
     uint64_t need = __ballot(!active);
     if (need) {
-      if (qbase >= qend && !exhausted) {
-        unsigned long long b = 0;
-        if (__lane_id() == 0) b = atomicAdd(A.next, (unsigned long long)A.batch);
-        b = __shfl(b, 0, 64);
-        if ((int64_t)b >= A.n) {
-          exhausted = true;
-        } else {
-          qbase = (int64_t)b;
-          qend = ((int64_t)b + A.batch < A.n) ? (int64_t)b + A.batch : A.n;
-        }
-      }
-      int64_t avail = qend - qbase;
-      if (avail > 0) {
-        int64_t rank = __popcll(need & lanemask_lt());
-        if (!active && rank < avail) {
-          k = qbase + rank;
-          const Off ko = (Off)k;
-          w = elem(A.P.w, ko);
-          d1 = elem(A.P.d1, ko);
-          d2 = elem(A.P.d2, ko);
-          p1 = elem(A.P.p1, ko);
-          p2 = elem(A.P.p2, ko);
-          i = elem(A.P.ci, ko);
-          j = elem(A.P.cj, ko);
-          dcen = (R)((A.dt - (double)elem(A.P.t, ko)) * kC);
-          ev = 0;
-          active = dcen > (R)0;
-        }
-        int64_t take = __popcll(need);
-        qbase += take < avail ? take : avail;
+      const int64_t kk = wave_take(wq, !active, need, A.next, A.batch, A.Q, A.n);
+      if (kk >= 0) {
+        k = kk;
+        const Off ko = (Off)k;
+        w = elem(A.P.w, ko);
+        d1 = elem(A.P.d1, ko);
+        d2 = elem(A.P.d2, ko);
+        p1 = elem(A.P.p1, ko);
+        p2 = elem(A.P.p2, ko);
+        i = elem(A.P.ci, ko);
+        j = elem(A.P.cj, ko);
+        dcen = (R)((A.dt - (double)elem(A.P.t, ko)) * kC);
+        ev = 0;
+        active = dcen > (R)0;
       }
     }
     if (__ballot(active) == 0) {
-      if (exhausted) break;
+      if (wq.exhausted) break;
       continue;
     }
     if (!active) continue;
@@ -585,6 +569,45 @@ __global__ __launch_bounds__(256) void k_transport(TransportArgs<R> A) {
   }
 }
 
+// Expected segment count of the histories of chunk c (particles c << shift
+// onward), from its first particle: path L = min(distance to census, 23 /
+// sigma_a) (23 e-folds take a weight to the 1e-10 floor), times the event
+// rate sigma_s (capped at 1e4 / h, as the tracker does) + 2 / h for face
+// crossings, h the smaller cell width.  Key 0 = longest, in quarter octaves.
+// Also leaves the per-workgroup key histograms for chunk_order.
+template <typename R>
+__global__ __launch_bounds__(kChunkBlock) void k_chunk_key(PoolView<R> P, int64_t n, int shift, int nx, int ny,
+                                                           const double* __restrict__ e1,
+                                                           const double* __restrict__ e2,
+                                                           const double* __restrict__ sa,
+                                                           const double* __restrict__ ss, double dt,
+                                                           uint8_t* __restrict__ key, int32_t* __restrict__ hist) {
+  __shared__ int h[kChunkKeys];
+  chunk_hist_zero(h);
+  const int64_t nchunk = (n + (1ll << shift) - 1) >> shift;
+  for (int64_t c = blockIdx.x * (int64_t)kChunkBlock + threadIdx.x; c < nchunk;
+       c += (int64_t)gridDim.x * kChunkBlock) {
+    const int64_t k = c << shift;
+    const int i = P.ci[k], j = P.cj[k];
+    int kk = kChunkKeys - 1;
+    if (i >= 0 && i < nx && j >= 0 && j < ny) {
+      const double dc = (dt - (double)P.t[k]) * kC;
+      if (dc > 0.0) {
+        const int cell = i * ny + j;
+        const double h = fmin(e1[i + 1] - e1[i], e2[j + 1] - e2[j]);
+        const double len = fmin(dc, 23.0 / fmax(sa[cell], 1e-30));
+        const double segs = len * (fmin(ss[cell], 1e4 / h) + 2.0 / h);
+        const double q = 4.0 * log2(1.0 + segs);  // NaN and inf handled before the conversion
+        const int b = q >= (double)(kChunkKeys - 1) ? kChunkKeys - 1 : (q > 0.0 ? (int)q : 0);
+        kk = kChunkKeys - 1 - b;
+      }
+    }
+    key[c] = (uint8_t)kk;
+    atomicAdd(&h[kk], 1);
+  }
+  chunk_hist_publish(h, hist);
+}
+
 __global__ void k_weight_floor(const double* __restrict__ sumw, int64_t n, double* __restrict__ out) {
 #pragma clang fp contract(off)
   *out = 1e-10 * (*sumw) / (double)(n > 1 ? n : 1);
@@ -646,6 +669,7 @@ struct AdjointArgs {
   double* g_raw;  // copies x 2 ncell: gA then gS
   int64_t ncell;
   int copy_mask;
+  QueueMap Q;
 };
 
 template <typename R, int GEOM>
@@ -659,8 +683,8 @@ __global__ __launch_bounds__(256) void k_transport_adjoint(AdjointArgs<R> A) {
   __syncthreads();
   const int ny = A.ny, nx = A.nx;
   const int ncell = (int)A.ncell;
-  int64_t qbase = 0, qend = 0;
-  bool exhausted = false, active = false;
+  WaveQueue wq;
+  bool active = false;
   int64_t k = 0;
   int phase = 0;
   double w = 0.0, Lam = 0.0, P = 0.0;
@@ -693,32 +717,16 @@ __global__ __launch_bounds__(256) void k_transport_adjoint(AdjointArgs<R> A) {
   while (true) {
     uint64_t need = __ballot(!active);
     if (need) {
-      if (qbase >= qend && !exhausted) {
-        unsigned long long b = 0;
-        if (__lane_id() == 0) b = atomicAdd(A.next, (unsigned long long)A.batch);
-        b = __shfl(b, 0, 64);
-        if ((int64_t)b >= A.n) {
-          exhausted = true;
-        } else {
-          qbase = (int64_t)b;
-          qend = ((int64_t)b + A.batch < A.n) ? (int64_t)b + A.batch : A.n;
-        }
-      }
-      int64_t avail = qend - qbase;
-      if (avail > 0) {
-        int64_t rank = __popcll(need & lanemask_lt());
-        if (!active && rank < avail) {
-          k = qbase + rank;
-          load(k);
-          phase = 0;
-          active = dcen > (R)0;
-        }
-        int64_t take = __popcll(need);
-        qbase += take < avail ? take : avail;
+      const int64_t kk = wave_take(wq, !active, need, A.next, A.batch, A.Q, A.n);
+      if (kk >= 0) {
+        k = kk;
+        load(k);
+        phase = 0;
+        active = dcen > (R)0;
       }
     }
     if (__ballot(active) == 0) {
-      if (exhausted) break;
+      if (wq.exhausted) break;
       continue;
     }
     if (!active) continue;
@@ -1064,6 +1072,7 @@ struct CfArgs {
   int copy_mask;
   double* bl;
   unsigned long long* stats;
+  QueueMap Q;
 };
 
 template <int GEOM>
@@ -1083,8 +1092,8 @@ __global__ __launch_bounds__(256) void k_transport_cf(CfArgs A) {
   auto add = [&](int key, double v) {
     if (!lds_tally_add(lkey, lval, key, v)) atomic_add_f64(key < ncell ? dep_t + key : si_t + (key - ncell), v);
   };
-  int64_t qbase = 0, qend = 0;
-  bool exhausted = false, active = false;
+  WaveQueue wq;
+  bool active = false;
   int64_t k = 0;
   double w = 0.0, d1 = 0.0, d2 = 0.0, p1 = 0.0, p2 = 0.0, tl = 0.0, dcen = 0.0;
   int i = 0, j = 0;
@@ -1097,41 +1106,25 @@ __global__ __launch_bounds__(256) void k_transport_cf(CfArgs A) {
   while (true) {
     uint64_t need = __ballot(!active);
     if (need) {
-      if (qbase >= qend && !exhausted) {
-        unsigned long long b = 0;
-        if (__lane_id() == 0) b = atomicAdd(A.next, (unsigned long long)A.batch);
-        b = __shfl(b, 0, 64);
-        if ((int64_t)b >= A.n) {
-          exhausted = true;
-        } else {
-          qbase = (int64_t)b;
-          qend = ((int64_t)b + A.batch < A.n) ? (int64_t)b + A.batch : A.n;
-        }
-      }
-      int64_t avail = qend - qbase;
-      if (avail > 0) {
-        int64_t rank = __popcll(need & lanemask_lt());
-        if (!active && rank < avail) {
-          k = qbase + rank;
-          w = A.P.w[k];
-          d1 = A.P.d1[k];
-          d2 = A.P.d2[k];
-          p1 = A.P.p1[k];
-          p2 = A.P.p2[k];
-          tl = A.P.t[k];
-          i = A.P.ci[k];
-          j = A.P.cj[k];
-          dcen = (A.dt - tl) * kC;
-          ev = 0;
-          stuck = 0;
-          active = dcen > 0.0 && w > wfloor;
-        }
-        int64_t take = __popcll(need);
-        qbase += take < avail ? take : avail;
+      const int64_t kk = wave_take(wq, !active, need, A.next, A.batch, A.Q, A.n);
+      if (kk >= 0) {
+        k = kk;
+        w = A.P.w[k];
+        d1 = A.P.d1[k];
+        d2 = A.P.d2[k];
+        p1 = A.P.p1[k];
+        p2 = A.P.p2[k];
+        tl = A.P.t[k];
+        i = A.P.ci[k];
+        j = A.P.cj[k];
+        dcen = (A.dt - tl) * kC;
+        ev = 0;
+        stuck = 0;
+        active = dcen > 0.0 && w > wfloor;
       }
     }
     if (__ballot(active) == 0) {
-      if (exhausted) break;
+      if (wq.exhausted) break;
       continue;
     }
     if (!active) continue;
@@ -1307,6 +1300,8 @@ struct Imc2dCtx {
   DevBuf<double> g_raw, g;
   int grid_override = 0;
   int batch = 256;
+  DevBuf<uint8_t> chunk_key;
+  DevBuf<int32_t> chunk_ord, chunk_hist;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   float last_transport_ms = 0.0f;
   double last_energy_emit = 0.0;
@@ -1333,6 +1328,34 @@ static double* stage_scalars_dev(Imc2dCtx* c) { return c->stage.dslot(c->ncell, 
 static int ensure_sum_scratch(Imc2dCtx* c, int64_t n) { return c->sum_scr.reserve(canonical_sum_scratch(n) + 8); }
 
 static int pool_ensure(Imc2dCtx* c, int64_t n_total) { return c->A.grow(n_total, c->n, c->stream); }
+
+// Pickup order of the gray tracker over the current pool: chunks of `batch`
+// particles, longest expected histories first (pool order when the batch is
+// not a power of two).
+static int longest_first_queue(Imc2dCtx* c, int64_t n, double dt, QueueMap* Q) {
+  *Q = pool_order_queue(n);
+  const int batch = c->batch;
+  if (n <= 0 || (batch & (batch - 1)) != 0) return kOk;
+  int shift = 0;
+  while ((1 << shift) < batch) ++shift;
+  const int64_t nchunk = (n + batch - 1) >> shift;
+  int rc;
+  if ((rc = c->chunk_key.reserve(nchunk)) || (rc = c->chunk_ord.reserve(nchunk)) ||
+      (rc = c->chunk_hist.reserve((int64_t)kChunkGrid * kChunkKeys)))
+    return rc;
+  hipStream_t s = c->stream;
+  const int g = chunk_grid(nchunk);
+  if (c->precision == IMC_FP64)
+    k_chunk_key<double><<<g, kChunkBlock, 0, s>>>(c->A.view<double>(), n, shift, c->nx, c->ny, c->e1, c->e2,
+                                                  c->sa.p, c->ss.p, dt, c->chunk_key.p, c->chunk_hist.p);
+  else
+    k_chunk_key<float><<<g, kChunkBlock, 0, s>>>(c->A.view<float>(), n, shift, c->nx, c->ny, c->e1, c->e2,
+                                                 c->sa.p, c->ss.p, dt, c->chunk_key.p, c->chunk_hist.p);
+  IMC_HIP(hipGetLastError());
+  if ((rc = chunk_order(c->chunk_key.p, nchunk, g, c->chunk_hist.p, c->chunk_ord.p, s))) return rc;
+  *Q = QueueMap{c->chunk_ord.p, nchunk << shift, shift};
+  return kOk;
+}
 
 }  // namespace imc
 
@@ -1476,6 +1499,9 @@ void imc2d_destroy(imc2d_ctx* c) {
   c->ssf.release();
   c->sas32.release();
   c->sas64.release();
+  c->chunk_key.release();
+  c->chunk_ord.release();
+  c->chunk_hist.release();
   c->e1p32.release();
   c->e2p32.release();
   c->e1p64.release();
@@ -1822,13 +1848,15 @@ int imc2d_transport(imc2d_ctx* c, double dt, const int32_t* reflect, int64_t max
       if (n > 0) IMC_HIP(hipMemcpyAsync(dst[a], src[a], esz[a] * (size_t)n, hipMemcpyDeviceToDevice, s));
     c->n_saved = n;
   }
+  QueueMap Q;
+  if ((rc = longest_first_queue(c, n, dt, &Q))) return rc;
   IMC_HIP(hipEventRecord(c->ev0, s));
   if (n > 0) {
     const bool narrow = n <= kNarrowMax;
     if (c->precision == IMC_FP64) {
       TransportArgs<double> A{c->A.view<double>(), n, c->e1p64.p, c->e2p64.p, c->nx, c->ny, c->sas64.p, dt, refl,
                               max_events, c->scal + 2, key, c->step, c->next, c->batch, c->dep_raw.p, c->si_raw.p,
-                              c->ncell, c->tally_copies - 1, c->scal + 3, c->stats};
+                              c->ncell, c->tally_copies - 1, c->scal + 3, c->stats, Q};
       if (narrow)
         launch_transport<double, uint32_t>(c->geometry, grid, block, s, A);
       else
@@ -1836,7 +1864,7 @@ int imc2d_transport(imc2d_ctx* c, double dt, const int32_t* reflect, int64_t max
     } else {
       TransportArgs<float> A{c->A.view<float>(), n, c->e1p32.p, c->e2p32.p, c->nx, c->ny, c->sas32.p, dt, refl,
                              max_events, c->scal + 2, key, c->step, c->next, c->batch, c->dep_raw.p, c->si_raw.p,
-                             c->ncell, c->tally_copies - 1, c->scal + 3, c->stats};
+                             c->ncell, c->tally_copies - 1, c->scal + 3, c->stats, Q};
       if (narrow)
         launch_transport<float, uint32_t>(c->geometry, grid, block, s, A);
       else
@@ -1934,7 +1962,7 @@ int imc2d_transport_cf(imc2d_ctx* c, double dt, const int32_t* reflect, int64_t 
     CfArgs A{c->A.view<double>(), n, c->e1p64.p, c->e2p64.p, c->nx, c->ny, c->sa.p, c->beta.p, dt, refl, max_events,
              c->scal + 2, fastpath_threshold, no_progress_rel_tol, no_progress_max_streak, make_key(c->seed),
              c->step, c->next, c->batch, c->dep_raw.p, c->si_raw.p, c->ncell, c->tally_copies - 1, c->scal + 3,
-             c->stats};
+             c->stats, pool_order_queue(n)};
     if (c->geometry == IMC_GEOM_XY)
       k_transport_cf<0><<<grid, block, 0, s>>>(A);
     else
@@ -2180,7 +2208,7 @@ int imc2d_adjoint(imc2d_ctx* c, double dt, const int32_t* reflect, int64_t max_e
     if (c->precision == IMC_FP64) {
       AdjointArgs<double> A{c->S.view<double>(), n, c->e1p64.p, c->e2p64.p, c->nx, c->ny, c->sas64.p, c->sa.p, dt,
                             refl, max_events, c->scal + 2, key, c->step, c->next, c->batch, c->volsrc.p, c->emit_lo,
-                            c->emit_hi, c->g_raw.p, nc, c->tally_copies - 1};
+                            c->emit_hi, c->g_raw.p, nc, c->tally_copies - 1, pool_order_queue(n)};
       if (c->geometry == IMC_GEOM_XY)
         k_transport_adjoint<double, 0><<<grid, block, 0, s>>>(A);
       else
@@ -2188,7 +2216,7 @@ int imc2d_adjoint(imc2d_ctx* c, double dt, const int32_t* reflect, int64_t max_e
     } else {
       AdjointArgs<float> A{c->S.view<float>(), n, c->e1p32.p, c->e2p32.p, c->nx, c->ny, c->sas32.p, c->sa.p, dt,
                            refl, max_events, c->scal + 2, key, c->step, c->next, c->batch, c->volsrc.p, c->emit_lo,
-                           c->emit_hi, c->g_raw.p, nc, c->tally_copies - 1};
+                           c->emit_hi, c->g_raw.p, nc, c->tally_copies - 1, pool_order_queue(n)};
       if (c->geometry == IMC_GEOM_XY)
         k_transport_adjoint<float, 0><<<grid, block, 0, s>>>(A);
       else

@@ -141,6 +141,57 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
   return (__lane_id() == 0) ? 0ull : ((~0ull) >> (64 - __lane_id()));
 }
 
+// Pickup order of the persistent trackers.  The queue hands out batches of
+// `batch` consecutive particles (consecutive particles share cells, which is
+// what keeps the LDS tally caches hitting).  Pools are laid out cell by
+// cell, so a batch from an optically thick region is a batch of long
+// histories; handed out late, it keeps its wave busy long after the rest of
+// the device has drained.  With `order` set, queue batch b is particle chunk
+// order[b] (chunk size 2^shift == batch), and the host sorts the chunks by
+// expected history length, longest first.  Only the order of pickup changes:
+// a particle keeps its index and so its random stream.
+struct QueueMap {
+  const int32_t* order;  // chunk permutation, or null for pool order
+  int64_t len;           // queue positions: chunks << shift, or n
+  int shift;             // log2 of the chunk size
+};
+inline QueueMap pool_order_queue(int64_t n) { return QueueMap{nullptr, n, 0}; }
+
+// Wave-local batch of particles; one global atomic per batch.
+struct WaveQueue {
+  int64_t qbase = 0, qend = 0;
+  bool exhausted = false;
+};
+// Called by every lane of the wave when `need` (the ballot of lanes without
+// a history) is non-zero.  Hands the next particles of the wave's batch to
+// the needy lanes in lane order, fetching a new batch of `batch` queue
+// positions when the current one is used up.  Returns the particle index a
+// needy lane takes, or -1.
+__device__ __forceinline__ int64_t wave_take(WaveQueue& W, bool needy, uint64_t need, unsigned long long* next,
+                                             int batch, const QueueMap& Q, int64_t n) {
+  if (W.qbase >= W.qend && !W.exhausted) {
+    unsigned long long b = 0;
+    if (__lane_id() == 0) b = atomicAdd(next, (unsigned long long)batch);
+    b = __shfl(b, 0, 64);
+    if ((int64_t)b >= Q.len) {
+      W.exhausted = true;
+    } else {
+      const int64_t base = Q.order ? ((int64_t)Q.order[b >> Q.shift] << Q.shift) : (int64_t)b;
+      W.qbase = base;
+      W.qend = (base + batch < n) ? base + batch : n;
+    }
+  }
+  int64_t k = -1;
+  const int64_t avail = W.qend - W.qbase;
+  if (avail > 0) {
+    const int64_t rank = __popcll(need & lanemask_lt());
+    if (needy && rank < avail) k = W.qbase + rank;
+    const int64_t take = __popcll(need);
+    W.qbase += take < avail ? take : avail;
+  }
+  return k;
+}
+
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {
 #pragma unroll

@@ -19,6 +19,62 @@ void set_error(const char* fmt, ...) {
 
 const char* get_error() { return g_err; }
 
+int chunk_grid(int64_t nchunk) {
+  return (int)std::max<int64_t>(1, std::min<int64_t>(kChunkGrid, (nchunk + kChunkBlock - 1) / kChunkBlock));
+}
+
+// Offsets: off[b][key] = (chunks with a smaller key) + (chunks with this key
+// in workgroups before b).  One workgroup; histograms staged in LDS.
+__global__ __launch_bounds__(1024) void k_chunk_scan(int32_t* __restrict__ hist, int grid) {
+  __shared__ int h[kChunkGrid * kChunkKeys];
+  __shared__ int tot[kChunkKeys];
+  const int n = grid * kChunkKeys;
+  for (int a = threadIdx.x; a < n; a += blockDim.x) h[a] = hist[a];
+  __syncthreads();
+  if (threadIdx.x < kChunkKeys) {
+    int run = 0;
+    for (int b = 0; b < grid; ++b) {
+      const int v = h[b * kChunkKeys + threadIdx.x];
+      h[b * kChunkKeys + threadIdx.x] = run;
+      run += v;
+    }
+    tot[threadIdx.x] = run;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int run = 0;
+    for (int k = 0; k < kChunkKeys; ++k) {
+      const int v = tot[k];
+      tot[k] = run;
+      run += v;
+    }
+  }
+  __syncthreads();
+  for (int a = threadIdx.x; a < n; a += blockDim.x) hist[a] = h[a] + tot[a % kChunkKeys];
+}
+
+__global__ __launch_bounds__(kChunkBlock) void k_chunk_scatter(const uint8_t* __restrict__ key, int64_t nchunk,
+                                                               const int32_t* __restrict__ off,
+                                                               int32_t* __restrict__ order) {
+  __shared__ int cur[kChunkKeys];
+  if (threadIdx.x < kChunkKeys) cur[threadIdx.x] = off[blockIdx.x * kChunkKeys + threadIdx.x];
+  __syncthreads();
+  for (int64_t c = blockIdx.x * (int64_t)kChunkBlock + threadIdx.x; c < nchunk;
+       c += (int64_t)gridDim.x * kChunkBlock)
+    order[atomicAdd(&cur[key[c]], 1)] = (int32_t)c;
+}
+
+int chunk_order(const uint8_t* key, int64_t nchunk, int grid, int32_t* hist, int32_t* order, hipStream_t s) {
+  IMC_REQUIRE(nchunk >= 0 && nchunk <= INT32_MAX, kErrArg, "chunk_order: %lld chunks", (long long)nchunk);
+  IMC_REQUIRE(grid >= 1 && grid <= kChunkGrid, kErrArg, "chunk_order: grid %d", grid);
+  if (nchunk == 0) return kOk;
+  k_chunk_scan<<<1, 1024, 0, s>>>(hist, grid);
+  IMC_HIP(hipGetLastError());
+  k_chunk_scatter<<<grid, kChunkBlock, 0, s>>>(key, nchunk, hist, order);
+  IMC_HIP(hipGetLastError());
+  return kOk;
+}
+
 int launch_blocks(int64_t n, int block) {
   int64_t b = (n + block - 1) / block;
   if (b < 1) b = 1;

@@ -138,4 +138,26 @@ int exclusive_scan_i32_to_i64(const int32_t* in, int64_t n, int64_t* out, int64_
 
 int launch_blocks(int64_t n, int block);
 
+// Counting sort of per-chunk keys in [0, kChunkKeys) into a permutation of
+// chunk ids, key 0 first; order within a key is arbitrary.  The keys are
+// written by a kernel that also leaves per-workgroup key histograms in
+// `hist` (kChunkGrid x kChunkKeys ints); chunk c belongs to workgroup
+// (c / kChunkBlock) % grid of a grid-strided loop over chunks with
+// kChunkBlock threads per workgroup.  `grid` <= kChunkGrid.
+constexpr int kChunkKeys = 64;
+constexpr int kChunkGrid = 128;
+constexpr int kChunkBlock = 256;
+int chunk_grid(int64_t nchunk);
+int chunk_order(const uint8_t* key, int64_t nchunk, int grid, int32_t* hist, int32_t* order, hipStream_t s);
+
+// Block-local key histogram for the key kernels: zero it, count, publish.
+__device__ __forceinline__ void chunk_hist_zero(int* h) {
+  if (threadIdx.x < kChunkKeys) h[threadIdx.x] = 0;
+  __syncthreads();
+}
+__device__ __forceinline__ void chunk_hist_publish(const int* h, int32_t* hist) {
+  __syncthreads();
+  if (threadIdx.x < kChunkKeys) hist[blockIdx.x * kChunkKeys + threadIdx.x] = h[threadIdx.x];
+}
+
 }  // namespace imc
