@@ -220,8 +220,12 @@ int imc1d_sample_boundary(imc1d_ctx* ctx, int side, const int64_t* n, const doub
  * 629-707) for IMC_SRC_EMISSION (a c T^4 sigma_a dt V from the last
  * imc1d_set_material, zero below T_emit_floor), or sample_source /
  * _spherical (IMC/IMC1D.py:294-316, 710-741) for IMC_SRC_FIXED with QV per
- * cell.  Counts ceil(N E_c / E_p).  E_out per cell and Etot_out per problem
- * nullable. */
+ * cell.  Counts ceil(N E_c / E_p).  IMC_SRC_CF: emitted_particles /
+ * _spherical of IMC/IMC1D_CarterForest.py (443-549, 890-965) with QV = the
+ * Carter-Forest emission energies per cell: max(1, round(N E_c / E_p)) per
+ * cell, none when E_p < 1e-30, the emission tilt of the T given to
+ * imc1d_set_material_cf and times from the truncated exponential of rate
+ * c sigma_a beta.  E_out per cell and Etot_out per problem nullable. */
 int imc1d_sample_volume(imc1d_ctx* ctx, int kind, const int64_t* N, const double* QV_host,
                         const double* T_emit_floor, double* E_out, double* Etot_out);
 
@@ -245,6 +249,25 @@ int imc1d_get_pool(imc1d_ctx* ctx, int64_t n, double* w, double* mu, double* t, 
 int imc1d_transport(imc1d_ctx* ctx, const int32_t* reflect, const double* weight_floor, double* dep_out,
                     double* si_out, double* leak_out, double* floor_leak_out, int64_t* stats_out);
 float imc1d_last_transport_ms(imc1d_ctx* ctx);
+
+/* ---- Carter-Forest variant (IMC/IMC1D_CarterForest.py), FP64 contexts ---- */
+/* Material of a Carter-Forest step (IMC/IMC1D_CarterForest.py:1177-1180): T
+ * per cell (emission tilt), the true sigma_a and beta = 4 a T^3 / c_v per
+ * cell, dt per problem.  Starts a new step (no source blocks). */
+int imc1d_set_material_cf(imc1d_ctx* ctx, const double* T, const double* sigma_a, const double* beta,
+                          const double* dt);
+/* move_particles_cf / move_particles_spherical_cf
+ * (IMC/IMC1D_CarterForest.py:210-332, 723-819) over the census and this
+ * step's source blocks.  reflect: 2 per problem; weight_floor per problem
+ * (< 0: 1e-10 sum(w)/N); max_events: event_cap_per_particle (<= 0: none);
+ * fastpath_threshold as the reference's.  Outputs (nullable): dep = captured
+ * energy / V and si = sum w s / (V dt) per cell, leak per problem, stats[8] =
+ * events, boundary crossings, absorb-continue events, census events,
+ * captures, floor kills (kept in census), reflections, event-cap hits;
+ * transported_out: particles in the transport pool. */
+int imc1d_transport_cf(imc1d_ctx* ctx, const int32_t* reflect, const double* weight_floor, int64_t max_events,
+                       double fastpath_threshold, double* dep_out, double* si_out, double* leak_out,
+                       int64_t* stats_out, int64_t* transported_out);
 
 /* Per-problem comb to Nmax[p]: comb (IMC/IMC1D.py:770-828); times reset to 0
  * (IMC/IMC1D.py:1068).  Outputs (nullable): ecen, disc per cell; ecen_total,

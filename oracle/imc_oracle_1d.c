@@ -375,3 +375,221 @@ int64_t o1_counts_ic(int I, const double* E, int64_t N, int64_t* counts) {
   }
   return tot;
 }
+
+/* ------------------------------------------------------------------------- */
+/* Carter-Forest variant, IMC/IMC1D_CarterForest.py                           */
+/* move_particle_cf 64-207, move_particles_cf 210-332 (slab);                 */
+/* move_particle_spherical_cf 615-720, move_particles_spherical_cf 723-819.   */
+/* Draw slots for event e: 6e absorption distance, 6e+1 capped redraw, 6e+2   */
+/* delay, 6e+3 fast-path test, 6e+4 new direction.  A history whose weight is */
+/* at or below the floor is not moved and stays in census (counted as a floor */
+/* kill).  Outputs per cell: dep_raw = captured weight, path_raw = sum w s.   */
+/* stats[8]: events, boundary, absorb-continue, census, capture, floor,       */
+/* reflections, event-cap hits.                                               */
+/* ------------------------------------------------------------------------- */
+
+/* Truncated-exponential emission time (IMC/IMC1D_CarterForest.py:531-539). */
+double o1_cf_time(double rate, double dt, double xi) {
+  if (rate > 1e-10) return -log(1.0 - xi * (1.0 - exp(-rate * dt))) / rate;
+  return xi * dt;
+}
+
+/* Times of an emission batch written by o1_sample_volume (same seed, step,
+ * tag, stream base): slot 0 of each particle's stream through o1_cf_time. */
+void o1_cf_emission_times(int64_t n, const int32_t* cell, const double* rate, double dt, uint64_t seed,
+                          uint32_t step, int32_t tag, uint64_t stream_base, double* t) {
+  for (int64_t k = 0; k < n; ++k) {
+    orc_stream s;
+    orc_stream_init(&s, seed, tag, stream_base + (uint64_t)k, step);
+    t[k] = o1_cf_time(rate[cell[k]], dt, orc_u01(&s, 0));
+  }
+}
+
+/* One Carter-Forest segment; returns the event code (1 boundary, 2 absorb and
+ * continue, 3 census, 4 capture) and updates the particle in place. */
+static int o1_cf_segment(int sph, orc_stream* rs, uint32_t e, double* w, double* mu, double* x, double* t,
+                         double r0, double r1, double sa, double beta, double* dcen, double fastpath, int* newloc,
+                         double* dep_w, double* path) {
+  *newloc = 0;
+  *dep_w = 0.0;
+  *path = 0.0;
+  if (*dcen < 1e-14) {
+    *dcen = 0.0;
+    return 3;
+  }
+  const double m = *mu, r = *x;
+  const double width = r1 - r0;
+  double db;
+  int outer = 1;
+  if (!sph) {
+    if (m > 0.0)
+      db = fmax(0.0, (r1 - r) / m);
+    else if (m < 0.0)
+      db = fmax(0.0, (r0 - r) / m);
+    else
+      db = 1e30;
+  } else {
+    double b2 = r * r * (1.0 - m * m);
+    double dout = -r * m + sqrt(fmax(r1 * r1 - b2, 0.0));
+    double din = 1e30;
+    double dd = r0 * r0 - b2;
+    if (m < 0.0 && dd > 0.0) din = -r * m - sqrt(dd);
+    db = din < dout ? din : dout;
+    outer = dout <= din;
+  }
+  double da = 1e30;
+  if (sa > 1e-10) da = -log(1.0 - orc_u01(rs, 6 * e + 0)) / sa;
+  if (sa * width > 10000) da = -log(1.0 - orc_u01(rs, 6 * e + 1)) / (10000.0 / width);
+  double s = db;
+  if (da < s) s = da;
+  if (*dcen < s) s = *dcen;
+  if (s <= 0.0) {
+    *dcen = 0.0;
+    return 3;
+  }
+  double xn, mn = m;
+  if (!sph) {
+    xn = r + m * s;
+  } else {
+    double q = r * r + 2.0 * r * s * m + s * s;
+    xn = q > 0.0 ? sqrt(q) : 0.0;
+    mn = xn > 1e-15 ? (r * m + s) / xn : 1.0;
+    mn = fmax(-1.0, fmin(1.0, mn));
+  }
+  *path = *w * s;
+  int code = 3;
+  if (fabs(db - s) < 1e-10) {
+    code = 1;
+    if (!sph) {
+      if (m > 0.0) { xn = r1; *newloc = 1; } else { xn = r0; *newloc = -1; }
+    } else if (outer) {
+      xn = r1;
+      mn = fmax(-1.0, fmin(1.0, r1 > 1e-15 ? (r * m + s) / r1 : 1.0));
+      *newloc = 1;
+    } else {
+      xn = r0;
+      mn = fmax(-1.0, fmin(1.0, r0 > 1e-15 ? (r * m + s) / r0 : -1.0));
+      *newloc = -1;
+    }
+    *dcen -= s;
+  } else if (fabs(da - s) < 1e-10) {
+    code = 2;
+    if (beta > 1e-15 && sa > 1e-15) {
+      double rate = O1_C * sa * beta;
+      double tr = (*dcen - s) / O1_C;
+      int cap;
+      if (fastpath > 0.0 && tr > 1e-14 && rate * tr > fastpath) {
+        cap = orc_u01(rs, 6 * e + 3) < exp(-rate * tr);
+        if (!cap) {
+          mn = -1.0 + 2.0 * orc_u01(rs, 6 * e + 4);
+          *dcen -= s;
+        }
+      } else {
+        double xi = orc_u01(rs, 6 * e + 2);
+        if (xi < 1e-15) xi = 1e-15;
+        double delay = -log(xi) / rate;
+        cap = !(tr > 1e-14 && delay < tr);
+        if (!cap) {
+          mn = -1.0 + 2.0 * orc_u01(rs, 6 * e + 4);
+          *t = (*t + s / O1_C) + delay;
+          *dcen = (tr - delay) * O1_C;
+        }
+      }
+      if (cap) {
+        *dep_w = *w;
+        *w = 0.0;
+        code = 4;
+      }
+    } else {
+      mn = -1.0 + 2.0 * orc_u01(rs, 6 * e + 4);
+      *dcen -= s;
+    }
+  } else {
+    *dcen = 0.0;
+  }
+  *x = xn;
+  *mu = mn;
+  return code;
+}
+
+void o1_transport_cf(int sph, int64_t n, double* w, double* mu, double* times, double* x, int32_t* cell, int I,
+                     const double* edges, const double* sigma_a, const double* beta, double dt, const int32_t* refl,
+                     double weight_floor, int64_t max_events, double fastpath, uint64_t seed, uint32_t step,
+                     uint64_t stream_base, double* dep_raw, double* path_raw, double* leak_out, int64_t* stats_out,
+                     int32_t nthreads) {
+  int nt = 1;
+#ifdef _OPENMP
+  nt = nthreads > 0 ? nthreads : omp_get_max_threads();
+#endif
+  double* depT = (double*)calloc((size_t)nt * (size_t)I, sizeof(double));
+  double* pathT = (double*)calloc((size_t)nt * (size_t)I, sizeof(double));
+  double* lkT = (double*)calloc((size_t)nt, sizeof(double));
+  int64_t* stT = (int64_t*)calloc((size_t)nt * 8, sizeof(int64_t));
+#pragma omp parallel for schedule(dynamic, 256) num_threads(nt)
+  for (int64_t k = 0; k < n; ++k) {
+    int tid = 0;
+#ifdef _OPENMP
+    tid = omp_get_thread_num();
+#endif
+    int64_t* st = stT + (size_t)tid * 8;
+    orc_stream rs;
+    orc_stream_init(&rs, seed, ORC_TAG_TRANSPORT, stream_base + (uint64_t)k, step);
+    double dcen = (dt - times[k]) * O1_C;
+    if (!(w[k] > weight_floor)) {
+      if (w[k] > 0.0) st[5] += 1;
+      continue;
+    }
+    int64_t events = 0;
+    while (dcen > 0.0) {
+      st[0] += 1;
+      events += 1;
+      if (max_events > 0 && events >= max_events) {
+        st[7] += 1;
+        dcen = 0.0;
+        break;
+      }
+      int loc = cell[k];
+      if (loc < 0) {
+        if (refl[0]) { cell[k] = loc = 0; mu[k] = -mu[k]; } else break;
+      } else if (loc >= I) {
+        if (refl[1]) { cell[k] = loc = I - 1; mu[k] = -mu[k]; } else break;
+      }
+      int newloc;
+      double dw, pth;
+      int code = o1_cf_segment(sph, &rs, (uint32_t)(events - 1), &w[k], &mu[k], &x[k], &times[k], edges[loc],
+                               edges[loc + 1], sigma_a[loc], beta[loc], &dcen, fastpath, &newloc, &dw, &pth);
+      cell[k] += newloc;
+      depT[(size_t)tid * I + loc] += dw;
+      pathT[(size_t)tid * I + loc] += pth;
+      if (code == 1) st[1] += 1;
+      else if (code == 2) st[2] += 1;
+      else if (code == 4) st[4] += 1;
+      else st[3] += 1;
+      if (code == 4) break;
+      if (cell[k] == I) {
+        if (refl[1]) { mu[k] = -mu[k]; cell[k] -= 1; st[6] += 1; } else break;
+      } else if (cell[k] == -1) {
+        if (refl[0]) { mu[k] = -mu[k]; cell[k] += 1; st[6] += 1; } else break;
+      }
+    }
+    if (cell[k] < 0 || cell[k] >= I) {
+      lkT[tid] += w[k];
+      w[k] = 0.0;
+    }
+  }
+  for (int a = 0; a < 8; ++a) stats_out[a] = 0;
+  double lk = 0.0;
+  for (int th = 0; th < nt; ++th) {
+    for (int i = 0; i < I; ++i) {
+      dep_raw[i] += depT[(size_t)th * I + i];
+      path_raw[i] += pathT[(size_t)th * I + i];
+    }
+    lk += lkT[th];
+    for (int a = 0; a < 8; ++a) stats_out[a] += stT[th * 8 + a];
+  }
+  *leak_out = lk;
+  free(depT);
+  free(pathT);
+  free(lkT);
+  free(stT);
+}

@@ -76,6 +76,8 @@ SIGNATURES = {
     "imc1d_get_pool": (C.c_int, [_vp, C.c_int64, _dp, _dp, _dp, _dp, _i64p, _i32p]),
     "imc1d_transport": (C.c_int, [_vp, _i32p, _dp, _dp, _dp, _dp, _dp, _i64p]),
     "imc1d_last_transport_ms": (C.c_float, [_vp]),
+    "imc1d_set_material_cf": (C.c_int, [_vp, _dp, _dp, _dp, _dp]),
+    "imc1d_transport_cf": (C.c_int, [_vp, _i32p, _dp, C.c_int64, C.c_double, _dp, _dp, _dp, _i64p, _i64p]),
     "imc1d_comb": (C.c_int, [_vp, _i64p, _dp, _dp, _dp, _dp]),
     "mgimc_create": (C.c_int, [C.POINTER(_vp), C.c_int, C.c_int, C.c_int, _dp, _dp, C.c_int, _dp, C.c_int,
                                C.c_uint64, C.c_int]),
@@ -467,6 +469,27 @@ class Engine1D:
 
     def last_transport_ms(self):
         return float(self._L.imc1d_last_transport_ms(self._h))
+
+    def set_material_cf(self, T, sigma_a, beta, dt):
+        """Carter-Forest material (FP64 contexts): T, true sigma_a and beta =
+        4 a T^3 / c_v per cell, dt per problem; starts a new step."""
+        check(self._L.imc1d_set_material_cf(self._h, _p(_per(T, self.ncell)), _p(_per(sigma_a, self.ncell)),
+                                            _p(_per(beta, self.ncell)), _p(_per(dt, self.P))))
+
+    def transport_cf(self, reflect, weight_floor=-1.0, max_events=0, fastpath_threshold=0.0, want_tallies=True):
+        """Carter-Forest transport.  Returns (dep, si, leak[P], stats[8],
+        particles transported)."""
+        r = np.asarray(reflect, dtype=bool)
+        r = np.ascontiguousarray(np.broadcast_to(r.reshape(-1, 2), (self.P, 2)), dtype=np.int32)
+        wf = _per(weight_floor, self.P)
+        dep = np.empty(self.ncell) if want_tallies else None
+        si = np.empty(self.ncell) if want_tallies else None
+        lk = np.zeros(self.P)
+        st = np.zeros(8, dtype=np.int64)
+        ntr = np.zeros(1, dtype=np.int64)
+        check(self._L.imc1d_transport_cf(self._h, _p(r, _i32p), _p(wf), int(max_events), float(fastpath_threshold),
+                                         _p(dep), _p(si), _p(lk), _p(st, _i64p), _p(ntr, _i64p)))
+        return dep, si, lk, st, int(ntr[0])
 
     def comb(self, Nmax, want_ecen=False, want_disc=False):
         """Returns (ecen per cell or None, disc per cell or None, pre-comb

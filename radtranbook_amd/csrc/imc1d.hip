@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "../../include/imc_hip.h"
+#include "imc1d_cf_kernels.hpp"
 #include "imc1d_kernels.hpp"
 #include "imc_device.hpp"
 #include "imc_util.hpp"
@@ -143,7 +144,11 @@ __global__ __launch_bounds__(256) void k1_problem_sums(int P, const int64_t* __r
 }
 
 // counts: mode 0 ceil(N E_c / E_p), zero when E_p <= 0 (IMC/IMC1D.py:277-281);
-// mode 1 ceil(E_c / E_p * N), zero when N <= 0 or E_p <= 0 (IMC/IMC1D.py:229-232)
+// mode 1 ceil(E_c / E_p * N), zero when N <= 0 or E_p <= 0 (IMC/IMC1D.py:229-232);
+// Carter-Forest emission, none when E_p < 1e-30, else at least one per cell:
+// mode 2 round(N (E_c / E_p)) (slab, IMC/IMC1D_CarterForest.py:504-505),
+// mode 3 round(N E_c / E_p) (sphere, IMC/IMC1D_CarterForest.py:927); round
+// half to even, as numpy.round
 __global__ void k1_counts(int64_t n, const int32_t* __restrict__ cprob, const double* __restrict__ E,
                           const double* __restrict__ Ep, const int64_t* __restrict__ N, int mode,
                           int64_t* __restrict__ counts) {
@@ -153,7 +158,13 @@ __global__ void k1_counts(int64_t n, const int32_t* __restrict__ cprob, const do
   int p = cprob[c];
   double et = Ep[p];
   int64_t k = 0;
-  if (et > 0.0 && N[p] > 0) {
+  if (mode >= 2) {
+    if (et >= 1e-30) {
+      const double q = mode == 2 ? (double)N[p] * (E[c] / et) : (double)N[p] * E[c] / et;
+      k = (int64_t)rint(q);
+      if (k < 1) k = 1;
+    }
+  } else if (et > 0.0 && N[p] > 0) {
     if (mode == 0)
       k = (int64_t)ceil((double)N[p] * E[c] / et);
     else
@@ -267,14 +278,18 @@ __device__ __forceinline__ int64_t bin_of(const int64_t* __restrict__ offs, int6
 
 // Volume sources (IMC/IMC1D.py:220-316, 535-741).  kind 0 emission (tilted,
 // t ~ U(0,dt)), 1 equilibrium (uniform, t = 0), 2 fixed source (uniform; slab
-// times U(0,1) as IMC/IMC1D.py:313).  Slots: 0 time, 1 position, 2 mu.
+// times U(0,1) as IMC/IMC1D.py:313), 3 Carter-Forest emission (tilted, the
+// time from the truncated exponential of rate c sigma_a beta,
+// IMC/IMC1D_CarterForest.py:443-549, 890-965).  Slots: 0 time, 1 position,
+// 2 mu.
 template <typename R>
 __global__ void k1_sample_volume(Pool1View<R> D, int64_t base, int64_t total, const int64_t* __restrict__ offs,
                                  int64_t ncell, const int32_t* __restrict__ cprob, const int64_t* __restrict__ coff,
                                  const double2* __restrict__ ep, const double* __restrict__ QV,
                                  const double* __restrict__ T, const double* __restrict__ slope,
                                  const double* __restrict__ acoef, const double* __restrict__ dt, int kind, int sph,
-                                 Key key, uint32_t step, uint32_t tag) {
+                                 const double* __restrict__ cf_sa, const double* __restrict__ cf_beta, Key key,
+                                 uint32_t step, uint32_t tag) {
 #pragma clang fp contract(off)
   int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= total) return;
@@ -287,10 +302,12 @@ __global__ void k1_sample_volume(Pool1View<R> D, int64_t base, int64_t total, co
   double ut = b0.u0, ux = b0.u1, um = b1.u0;
   double r0 = ep[c].x, r1 = ep[c].y, dx = r1 - r0;
   double t = (kind == 1) ? 0.0 : ((kind == 2 && !sph) ? ut : dt[p] * ut);
+  if (kind == 3) t = cf1_time(kC * cf_sa[c] * cf_beta[c], dt[p], ut);
+  const bool tilted = kind == 0 || kind == 3;
   double x;
   if (!sph) {
-    x = (kind == 0) ? linear_density(dx, slope[c], T[c], ux) + r0 : r0 + dx * ux;
-  } else if (kind == 0 && !(fabs(slope[c]) < 1e-30 || fabs(acoef[c]) < 1e-30)) {
+    x = tilted ? linear_density(dx, slope[c], T[c], ux) + r0 : r0 + dx * ux;
+  } else if (tilted && !(fabs(slope[c]) < 1e-30 || fabs(acoef[c]) < 1e-30)) {
     x = solve_sph_cdf(r0, r1, acoef[c], slope[c], ux);
   } else {
     x = pow(r0 * r0 * r0 + (r1 * r1 * r1 - r0 * r0 * r0) * ux, 1.0 / 3.0);
@@ -598,6 +615,213 @@ __global__ void k1_finalize(int64_t ncell, int P, int copies, const int32_t* __r
 }
 
 // ---------------------------------------------------------------------------
+// Carter-Forest transport (IMC/IMC1D_CarterForest.py:210-332 slab, 723-819
+// sphere), FP64.  Per history: while the distance to census is positive, count
+// the event (a positive event cap ends the history when reached), reflect or
+// drop an out-of-range index, move, tally, stop on capture, then reflect or
+// escape at a domain face.  A history whose weight is at or below the floor
+// is not moved and stays in census (counted as a floor kill; the reference
+// drops its weight unbooked).  Escaped weight is the problem's boundary loss.
+// dep_raw: captured energy per cell then (leak, unused) per problem; path_raw:
+// sum of w s per cell.  stats: events, boundary, absorb-continue, census,
+// capture, floor, reflections, cap hits.
+// ---------------------------------------------------------------------------
+struct Cf1Args {
+  Pool1View<double> P;
+  int64_t n;
+  const int64_t* poff;
+  const int64_t* coff;
+  const double2* ep;
+  const double* sa;
+  const double* beta;
+  const double* dt;
+  const double* wfloor;
+  const int32_t* refl;
+  int64_t max_events;  // <= 0: no cap
+  double fastpath;
+  Key key;
+  uint32_t step;
+  unsigned long long* next;
+  int batch;
+  double* dep_raw;
+  double* path_raw;
+  int64_t ncell;
+  int nprob;
+  int copy_mask;
+  unsigned long long* stats;
+  QueueMap Q;
+};
+
+template <int SPH>
+__global__ __launch_bounds__(256) void k1_transport_cf(Cf1Args A) {
+#pragma clang fp contract(off)
+  const int64_t tally_n = A.ncell + 2 * (int64_t)A.nprob;
+  const int copy = blockIdx.x & A.copy_mask;
+  double* const dep_t = A.dep_raw + (int64_t)copy * tally_n;
+  double* const path_t = A.path_raw + (int64_t)copy * A.ncell;
+  __shared__ int lkey[kLdsSlots];
+  __shared__ double lval[kLdsSlots];
+  lds_tally_init(lkey, lval);
+  __syncthreads();
+  // keys [0, tally_n): captured energy and leaks; [tally_n, tally_n + ncell): path
+  auto add = [&](int64_t key, double v) {
+    if (!lds_tally_add(lkey, lval, (int)key, v))
+      atomic_add_f64(key < tally_n ? dep_t + key : path_t + (key - tally_n), v);
+  };
+  WaveQueue wq;
+  bool active = false;
+  int64_t k = 0, cb = 0;
+  double w = 0.0, mu = 0.0, x = 0.0, t = 0.0, dcen = 0.0;
+  int cell = 0, p = 0, I = 0;
+  uint32_t ev = 0;
+  double acc_path = 0.0;
+  int64_t acc_cell = -1;
+  uint32_t st_ev = 0, st_bnd = 0, st_abs = 0, st_cen = 0, st_cap = 0, st_floor = 0, st_refl = 0, st_hit = 0;
+  while (true) {
+    uint64_t need = __ballot(!active);
+    if (need) {
+      const int64_t kk = wave_take(wq, !active, need, A.next, A.batch, A.Q, A.n);
+      if (kk >= 0) {
+        k = kk;
+        w = A.P.w[k];
+        mu = A.P.mu[k];
+        x = A.P.x[k];
+        t = A.P.t[k];
+        cell = A.P.cell[k];
+        p = A.P.prob[k];
+        cb = A.coff[p];
+        I = (int)(A.coff[p + 1] - cb);
+        dcen = (A.dt[p] - t) * kC;
+        ev = 0;
+        if (w > A.wfloor[p]) {
+          active = dcen > 0.0;
+        } else {
+          st_floor += w > 0.0;
+          active = false;
+        }
+      }
+    }
+    if (__ballot(active) == 0) {
+      if (wq.exhausted) break;
+      continue;
+    }
+    if (!active) continue;
+    bool finish = false;
+    ++ev;
+    ++st_ev;
+    if (A.max_events > 0 && (int64_t)ev >= A.max_events) {
+      ++st_hit;
+      dcen = 0.0;
+      finish = true;
+    } else {
+      if (cell < 0) {
+        if (A.refl[2 * p]) { cell = 0; mu = -mu; } else finish = true;
+      } else if (cell >= I) {
+        if (A.refl[2 * p + 1]) { cell = I - 1; mu = -mu; } else finish = true;
+      }
+      if (!finish) {
+        const int64_t c = cb + cell;
+        const double2 e = A.ep[c];
+        Ctr ctr = make_ctr(kTagTransport, ((uint64_t)p << 32) | (uint64_t)(k - A.poff[p]), A.step);
+        Cf1Out o;
+        if (SPH)
+          cf1_move_sph(ctr, A.key, ev - 1u, w, mu, x, t, e.x, e.y, A.sa[c], A.beta[c], dcen, A.fastpath, o);
+        else
+          cf1_move_slab(ctr, A.key, ev - 1u, w, mu, x, t, e.x, e.y, A.sa[c], A.beta[c], dcen, A.fastpath, o);
+        if (o.dep_w != 0.0) add(c, o.dep_w);
+        if (c != acc_cell) {
+          if (acc_cell >= 0 && acc_path != 0.0) add(tally_n + acc_cell, acc_path);
+          acc_cell = c;
+          acc_path = 0.0;
+        }
+        acc_path += o.path;
+        w = o.w;
+        mu = o.mu;
+        x = o.x;
+        t = o.t;
+        dcen = o.dcen;
+        cell += o.newloc;
+        st_bnd += o.code == kCf1Boundary;
+        st_abs += o.code == kCf1Absorb;
+        st_cen += o.code == kCf1Census;
+        st_cap += o.code == kCf1Capture;
+        if (o.code == kCf1Capture) {
+          finish = true;
+        } else if (cell == I) {
+          if (A.refl[2 * p + 1]) { mu = -mu; cell -= 1; ++st_refl; } else finish = true;
+        } else if (cell == -1) {
+          if (A.refl[2 * p]) { mu = -mu; cell += 1; ++st_refl; } else finish = true;
+        }
+        if (!(dcen > 0.0)) finish = true;
+      }
+    }
+    if (finish) {
+      if (cell < 0 || cell >= I) {  // escaped: boundary loss of the problem
+        if (w != 0.0) add(A.ncell + 2 * p, w);
+        w = 0.0;
+      }
+      A.P.w[k] = w;
+      A.P.mu[k] = mu;
+      A.P.x[k] = x;
+      A.P.t[k] = t;
+      A.P.cell[k] = cell;
+      active = false;
+    }
+  }
+  if (acc_cell >= 0 && acc_path != 0.0) add(tally_n + acc_cell, acc_path);
+  __syncthreads();
+  for (int s = threadIdx.x; s < kLdsSlots; s += blockDim.x) {
+    const int kk = lkey[s];
+    if (kk >= 0) {
+      const double v = lval[s];
+      if (v != 0.0) atomic_add_f64(kk < tally_n ? dep_t + kk : path_t + (kk - tally_n), v);
+    }
+  }
+  unsigned long long s0 = wave_sum<unsigned long long>(st_ev);
+  unsigned long long s1 = wave_sum<unsigned long long>(st_bnd);
+  unsigned long long s2 = wave_sum<unsigned long long>(st_abs);
+  unsigned long long s3 = wave_sum<unsigned long long>(st_cen);
+  unsigned long long s4 = wave_sum<unsigned long long>(st_cap);
+  unsigned long long s5 = wave_sum<unsigned long long>(st_floor);
+  unsigned long long s6 = wave_sum<unsigned long long>(st_refl);
+  unsigned long long s7 = wave_sum<unsigned long long>(st_hit);
+  if (__lane_id() == 0) {
+    atomicAdd(A.stats + 0, s0);
+    atomicAdd(A.stats + 1, s1);
+    atomicAdd(A.stats + 2, s2);
+    atomicAdd(A.stats + 3, s3);
+    atomicAdd(A.stats + 4, s4);
+    atomicAdd(A.stats + 5, s5);
+    atomicAdd(A.stats + 6, s6);
+    atomicAdd(A.stats + 7, s7);
+  }
+}
+
+// dep = captured / V, si = sum(w s) / (V dt) per cell (the reference's
+// per-segment w s / V / dt, IMC/IMC1D_CarterForest.py:125, 282-283); leaks
+// per problem.
+__global__ void k1_cf_finalize(int64_t ncell, int P, int copies, const int32_t* __restrict__ cprob,
+                               const double* __restrict__ dep_raw, const double* __restrict__ path_raw,
+                               const double* __restrict__ vol, const double* __restrict__ dt,
+                               double* __restrict__ dep, double* __restrict__ si, double* __restrict__ leaks) {
+  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t tally_n = ncell + 2 * (int64_t)P;
+  if (c < ncell) {
+    double a = 0.0, b = 0.0;
+    for (int k = 0; k < copies; ++k) {
+      a += dep_raw[(int64_t)k * tally_n + c];
+      b += path_raw[(int64_t)k * ncell + c];
+    }
+    dep[c] = a / vol[c];
+    si[c] = b / vol[c] / dt[cprob[c]];
+  } else if (c < tally_n) {
+    double a = 0.0;
+    for (int k = 0; k < copies; ++k) a += dep_raw[(int64_t)k * tally_n + c];
+    leaks[c - ncell] = a;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // comb (IMC/IMC1D.py:770-828), per problem
 // ---------------------------------------------------------------------------
 template <typename R>
@@ -778,6 +1002,8 @@ struct Imc1dCtx {
       disc, Ep, theta, dt, tfloor, floor_v, psum, tree_scr, sum_scr, Ebnd;
   DevBuf<double2> ep64, sas64;
   DevBuf<float2> ep32, sas32;
+  DevBuf<double> cfsa, cfbeta;  // Carter-Forest: true sigma_a and beta per cell
+  bool cf_ready = false;
   DevBuf<unsigned int> count_part;
   DevBuf<unsigned long long> cellcount;
   unsigned long long* stats = nullptr;
@@ -889,7 +1115,7 @@ void imc1d_destroy(imc1d_ctx* c) {
   DevBuf<double>* d[] = {&c->vol, &c->T, &c->sig, &c->cv, &c->sa, &c->ss, &c->E, &c->slope, &c->acoef,
                          &c->dep_raw, &c->path_raw, &c->dep, &c->si, &c->leaks, &c->ecen, &c->ecen_part, &c->ew,
                          &c->disc, &c->Ep, &c->theta, &c->dt, &c->tfloor, &c->floor_v, &c->psum, &c->tree_scr,
-                         &c->sum_scr, &c->Ebnd};
+                         &c->sum_scr, &c->Ebnd, &c->cfsa, &c->cfbeta};
   for (auto* b : d) b->release();
   c->ep64.release();
   c->sas64.release();
@@ -1176,11 +1402,13 @@ static int launch_volume1(imc1d_ctx* c, Pool1Buf& dst, int64_t base, int64_t tot
   if (c->precision == IMC_FP64)
     k1_sample_volume<double><<<g, 256, 0, c->stream>>>(dst.view<double>(), base, total, c->offs.p, c->ncell,
                                                        c->cprob.p, c->coff.p, c->ep64.p, QV, c->T.p, c->slope.p,
-                                                       c->acoef.p, c->dt.p, kind, c->sph, key, step, tag);
+                                                       c->acoef.p, c->dt.p, kind, c->sph, c->cfsa.p, c->cfbeta.p,
+                                                       key, step, tag);
   else
     k1_sample_volume<float><<<g, 256, 0, c->stream>>>(dst.view<float>(), base, total, c->offs.p, c->ncell,
                                                       c->cprob.p, c->coff.p, c->ep64.p, QV, c->T.p, c->slope.p,
-                                                      c->acoef.p, c->dt.p, kind, c->sph, key, step, tag);
+                                                      c->acoef.p, c->dt.p, kind, c->sph, c->cfsa.p, c->cfbeta.p,
+                                                      key, step, tag);
   IMC_HIP(hipGetLastError());
   return kOk;
 }
@@ -1213,12 +1441,16 @@ int imc1d_equilibrium_sample(imc1d_ctx* c, const int64_t* N, const double* Tr, c
 }
 
 // Volume source block: IMC_SRC_EMISSION (a c T^4 sigma_a dt V from the last
-// imc1d_set_material, linear tilt) or IMC_SRC_FIXED (QV per cell, uniform).
+// imc1d_set_material, linear tilt), IMC_SRC_FIXED (QV per cell, uniform) or
+// IMC_SRC_CF (QV = Carter-Forest emission energies, tilt from the T of the
+// last imc1d_set_material_cf, truncated-exponential times).
 int imc1d_sample_volume(imc1d_ctx* c, int kind, const int64_t* N, const double* QV_host, const double* T_emit_floor,
                         double* E_out, double* Etot_out) {
   IMC_REQUIRE(c && N && T_emit_floor, kErrArg, "imc1d_sample_volume: bad arguments");
-  IMC_REQUIRE(kind == IMC_SRC_EMISSION || (kind == IMC_SRC_FIXED && QV_host), kErrArg,
+  IMC_REQUIRE(kind == IMC_SRC_EMISSION || ((kind == IMC_SRC_FIXED || kind == IMC_SRC_CF) && QV_host), kErrArg,
               "imc1d_sample_volume: bad kind");
+  IMC_REQUIRE(kind != IMC_SRC_CF || c->cf_ready, kErrArg,
+              "imc1d_sample_volume: Carter-Forest emission needs imc1d_set_material_cf");
   IMC_HIP(hipSetDevice(c->device));
   hipStream_t s = c->stream;
   IMC_HIP(hipMemcpyAsync(c->tfloor.p, T_emit_floor, sizeof(double) * c->P, hipMemcpyHostToDevice, s));
@@ -1234,9 +1466,15 @@ int imc1d_sample_volume(imc1d_ctx* c, int kind, const int64_t* N, const double* 
   } else {
     IMC_HIP(hipMemcpyAsync(c->ecen.p, QV_host, sizeof(double) * c->ncell, hipMemcpyHostToDevice, s));
     QV = c->ecen.p;
+    if (kind == IMC_SRC_CF) {
+      k1_slopes<<<launch_blocks(c->ncell, 256), 256, 0, s>>>(c->P, c->coff.p, c->T.p, c->ep64.p, c->sph,
+                                                             c->slope.p, c->acoef.p);
+      IMC_HIP(hipGetLastError());
+    }
   }
+  const int count_mode = kind == IMC_SRC_CF ? (c->sph ? 3 : 2) : 0;
   std::vector<int64_t> off;
-  int rc = volume_counts(c, QV, N, 0, off, Etot_out);
+  int rc = volume_counts(c, QV, N, count_mode, off, Etot_out);
   if (rc) return rc;
   if (E_out) {
     IMC_HIP(hipMemcpyAsync(E_out, QV, sizeof(double) * c->ncell, hipMemcpyDeviceToHost, s));
@@ -1246,9 +1484,9 @@ int imc1d_sample_volume(imc1d_ctx* c, int kind, const int64_t* N, const double* 
   if (total == 0) return kOk;
   int64_t base = c->nN;
   if ((rc = add_block(c, off))) return rc;
-  if ((rc = launch_volume1(c, c->N, base - off[0], total, QV,
-                           kind == IMC_SRC_EMISSION ? 0 : 2, kind == IMC_SRC_EMISSION ? kTagEmission : kTagSource,
-                           c->step)))
+  const int skind = kind == IMC_SRC_EMISSION ? 0 : (kind == IMC_SRC_CF ? 3 : 2);
+  if ((rc = launch_volume1(c, c->N, base - off[0], total, QV, skind,
+                           kind == IMC_SRC_FIXED ? kTagSource : kTagEmission, c->step)))
     return rc;
   IMC_HIP(hipStreamSynchronize(s));
   return kOk;
@@ -1454,6 +1692,113 @@ int imc1d_transport(imc1d_ctx* c, const int32_t* reflect, const double* weight_f
 }
 
 float imc1d_last_transport_ms(imc1d_ctx* c) { return c ? c->last_ms : -1.0f; }
+
+// ---- Carter-Forest variant (IMC/IMC1D_CarterForest.py), FP64 contexts ----
+int imc1d_set_material_cf(imc1d_ctx* c, const double* T, const double* sigma_a, const double* beta,
+                          const double* dt) {
+  IMC_REQUIRE(c && T && sigma_a && beta && dt, kErrArg, "imc1d_set_material_cf: null argument");
+  IMC_REQUIRE(c->precision == IMC_FP64, kErrArg, "imc1d_set_material_cf: the Carter-Forest tracker runs in FP64");
+  IMC_HIP(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  int rc;
+  if ((rc = c->cfsa.reserve(c->ncell)) || (rc = c->cfbeta.reserve(c->ncell))) return rc;
+  size_t bytes = sizeof(double) * c->ncell;
+  IMC_HIP(hipMemcpyAsync(c->T.p, T, bytes, hipMemcpyHostToDevice, s));
+  IMC_HIP(hipMemcpyAsync(c->cfsa.p, sigma_a, bytes, hipMemcpyHostToDevice, s));
+  IMC_HIP(hipMemcpyAsync(c->cfbeta.p, beta, bytes, hipMemcpyHostToDevice, s));
+  IMC_HIP(hipMemcpyAsync(c->dt.p, dt, sizeof(double) * c->P, hipMemcpyHostToDevice, s));
+  IMC_HIP(hipStreamSynchronize(s));
+  c->cf_ready = true;
+  // a new step starts: no source blocks yet
+  c->staged = false;
+  c->nN = 0;
+  c->nblocks = 0;
+  c->h_blk_off.clear();
+  c->h_blk_base.clear();
+  return kOk;
+}
+
+int imc1d_transport_cf(imc1d_ctx* c, const int32_t* reflect, const double* weight_floor, int64_t max_events,
+                       double fastpath_threshold, double* dep_out, double* si_out, double* leak_out,
+                       int64_t* stats_out, int64_t* transported_out) {
+  IMC_REQUIRE(c && reflect && weight_floor, kErrArg, "imc1d_transport_cf: bad arguments");
+  IMC_REQUIRE(c->cf_ready, kErrArg, "imc1d_transport_cf: call imc1d_set_material_cf first");
+  IMC_REQUIRE(c->ncell + 2 * (int64_t)c->P + c->ncell < INT32_MAX, kErrArg, "imc1d_transport_cf: too many cells");
+  IMC_HIP(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  int P = c->P;
+  int rc = build_transport_pool(c);
+  if (rc) return rc;
+  int64_t n = c->nB;
+  int64_t tally_n = c->ncell + 2 * (int64_t)P;
+  IMC_HIP(hipMemsetAsync(c->dep_raw.p, 0, sizeof(double) * tally_n * c->tally_copies, s));
+  IMC_HIP(hipMemsetAsync(c->path_raw.p, 0, sizeof(double) * c->ncell * c->tally_copies, s));
+  IMC_HIP(hipMemsetAsync(c->stats, 0, sizeof(unsigned long long) * 8, s));
+  IMC_HIP(hipMemsetAsync(c->next, 0, sizeof(unsigned long long), s));
+  IMC_HIP(hipMemcpyAsync(c->refl.p, reflect, sizeof(int32_t) * 2 * P, hipMemcpyHostToDevice, s));
+  // weight floors: given, or 1e-10 * sum(w) / N per problem (IMC/IMC1D_CarterForest.py:1243)
+  std::vector<double> fl(weight_floor, weight_floor + P);
+  bool any_auto = false;
+  for (int p = 0; p < P; ++p) any_auto |= fl[p] < 0.0;
+  if (any_auto) {
+    if (n > 0) {
+      k1_segment_sums<<<P, 256, 0, s>>>(c->B.w, c->poff.p, c->psum.p);
+      IMC_HIP(hipGetLastError());
+    } else {
+      IMC_HIP(hipMemsetAsync(c->psum.p, 0, sizeof(double) * P, s));
+    }
+    k1_floor<<<launch_blocks(P, 64), 64, 0, s>>>(P, c->psum.p, c->poff.p, c->floor_v.p);
+    IMC_HIP(hipGetLastError());
+    std::vector<double> autof(P);
+    IMC_HIP(hipMemcpyAsync(autof.data(), c->floor_v.p, sizeof(double) * P, hipMemcpyDeviceToHost, s));
+    IMC_HIP(hipStreamSynchronize(s));
+    for (int p = 0; p < P; ++p)
+      if (fl[p] < 0.0) fl[p] = autof[p];
+  }
+  IMC_HIP(hipMemcpyAsync(c->floor_v.p, fl.data(), sizeof(double) * P, hipMemcpyHostToDevice, s));
+  int block = 256, grid;
+  if (c->grid_override > 0) {
+    grid = c->grid_override;
+  } else {
+    int per_cu = 0;
+    const void* fn = c->sph ? (const void*)k1_transport_cf<1> : (const void*)k1_transport_cf<0>;
+    IMC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, block, 0));
+    grid = std::max(per_cu, 1) * c->n_cu;
+  }
+  grid = (int)std::max<int64_t>(std::min<int64_t>(grid, (n + block - 1) / block), 1);
+  Key key = make_key(c->seed);
+  IMC_HIP(hipEventRecord(c->ev0, s));
+  if (n > 0) {
+    Cf1Args A{c->B.view<double>(), n, c->poff.p, c->coff.p, c->ep64.p, c->cfsa.p, c->cfbeta.p, c->dt.p,
+              c->floor_v.p, c->refl.p, max_events, fastpath_threshold, key, c->step, c->next, c->batch,
+              c->dep_raw.p, c->path_raw.p, c->ncell, P, c->tally_copies - 1, c->stats, pool_order_queue(n)};
+    if (c->sph)
+      k1_transport_cf<1><<<grid, block, 0, s>>>(A);
+    else
+      k1_transport_cf<0><<<grid, block, 0, s>>>(A);
+    IMC_HIP(hipGetLastError());
+  }
+  IMC_HIP(hipEventRecord(c->ev1, s));
+  k1_cf_finalize<<<launch_blocks(tally_n, 256), 256, 0, s>>>(c->ncell, P, c->tally_copies, c->cprob.p,
+                                                             c->dep_raw.p, c->path_raw.p, c->vol.p, c->dt.p,
+                                                             c->dep.p, c->si.p, c->leaks.p);
+  IMC_HIP(hipGetLastError());
+  size_t bytes = sizeof(double) * c->ncell;
+  if (dep_out) IMC_HIP(hipMemcpyAsync(dep_out, c->dep.p, bytes, hipMemcpyDeviceToHost, s));
+  if (si_out) IMC_HIP(hipMemcpyAsync(si_out, c->si.p, bytes, hipMemcpyDeviceToHost, s));
+  std::vector<double> lk(2 * P);
+  IMC_HIP(hipMemcpyAsync(lk.data(), c->leaks.p, sizeof(double) * 2 * P, hipMemcpyDeviceToHost, s));
+  unsigned long long st[8];
+  IMC_HIP(hipMemcpyAsync(st, c->stats, sizeof(st), hipMemcpyDeviceToHost, s));
+  IMC_HIP(hipStreamSynchronize(s));
+  IMC_HIP(hipEventElapsedTime(&c->last_ms, c->ev0, c->ev1));
+  if (leak_out)
+    for (int p = 0; p < P; ++p) leak_out[p] = lk[2 * p];
+  if (stats_out)
+    for (int a = 0; a < 8; ++a) stats_out[a] = (int64_t)st[a];
+  if (transported_out) *transported_out = n;
+  return kOk;
+}
 
 // Comb each problem to Nmax[p] (IMC/IMC1D.py:770-828); the transported pool
 // becomes the new census, grouped by problem.  Optional outputs: ecen, disc

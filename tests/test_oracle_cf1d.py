@@ -1,0 +1,147 @@
+"""Pins of the 1-D Carter-Forest oracle (oracle/oracle_cf1d.py,
+oracle/imc_oracle_1d.c) by closed forms, on the CPU.
+
+- Emission time: the inverse of the truncated-exponential CDF.
+- Emission energies a c sigma T^4 dt V (1 - e^{-x})/x with x = c sigma beta
+  dt, and the count rule max(1, round(N E_c / E)), round half to even.
+- Transport in one reflecting slab cell, and in one sphere with a
+  reflecting surface, with constant sigma_a and beta: a history is a
+  two-state Markov chain (radiation -> material at l1 = c sigma_a, back at
+  l2 = c sigma_a beta), so the census fraction is l2/s + l1/s e^{-s dt}
+  (s = l1 + l2) and the path length per particle
+  c [l2/s dt + l1/s^2 (1 - e^{-s dt})], whatever the geometry.
+- Full step: energy balance to 1e-12 (slab, sphere), and one step of a
+  closed cell against the exact solution of the linear two-state system."""
+import numpy as np
+import pytest
+
+C_LIGHT = 29.98
+A_RAD = 0.01372
+
+
+@pytest.fixture(scope="module")
+def CF(oracle):
+    from oracle import oracle_cf1d
+
+    return oracle_cf1d
+
+
+def test_emission_time_inverts_the_truncated_exponential(CF):
+    dt = 0.01
+    for rate in (0.0, 1e-11, 3.0, 80.0, 2000.0):
+        for xi in np.linspace(0.0, 0.999999, 23):
+            t = CF.cf1_time(rate, dt, xi)
+            assert 0.0 <= t <= dt * (1 + 1e-12)
+            if rate <= 1e-10:
+                assert t == xi * dt
+            else:
+                F = (1.0 - np.exp(-rate * t)) / (1.0 - np.exp(-rate * dt))
+                assert F == pytest.approx(xi, abs=1e-12)
+
+
+def test_emission_energies_and_counts(CF):
+    T = np.array([0.3, 1.0, 2.0, 0.5])
+    sig = np.array([10.0, 1e-16, 0.5, 4.0])
+    beta = CF.cf1_beta(T, np.full(4, 0.2))
+    np.testing.assert_allclose(beta, 4 * A_RAD * T ** 3 / 0.2, rtol=1e-15)
+    vol = np.array([1.0, 2.0, 0.5, 1.0])
+    dt = 0.003
+    rate, E = CF.cf1_emission(T, sig, beta, vol, dt)
+    x = rate * dt
+    lin = A_RAD * C_LIGHT * sig * T ** 4 * dt * vol
+    ratio = np.where(rate > 1e-10, (1 - np.exp(-x)) / np.where(x > 0, x, 1), 1.0)
+    np.testing.assert_allclose(E, lin * ratio, rtol=1e-12, atol=1e-300)
+    # max(1, round(N E_c / E)), half to even; a zero-energy cell still gets one
+    Ec = np.array([0.0, 1.0, 2.5, 6.5])  # E = 10
+    assert list(CF.cf1_counts(Ec, 10, False)) == [1, 1, 2, 6]
+    assert list(CF.cf1_counts(Ec, 10, True)) == [1, 1, 2, 6]
+    assert list(CF.cf1_counts(np.zeros(3), 10, False)) == [0, 0, 0]
+
+
+def _one_cell_pool(O, n, x0, sph):
+    p = O.Pool1D.empty(n)
+    rng = np.random.default_rng(3)
+    p.w[:] = 1.0
+    p.mu[:] = rng.uniform(-1, 1, n)
+    p.x[:] = x0
+    return p
+
+
+@pytest.mark.parametrize("sph", [False, True])
+@pytest.mark.parametrize("sigma,beta", [(5.0, 0.5), (2.0, 3.0)])
+def test_one_cell_markov_closed_form(oracle, CF, sph, sigma, beta):
+    from oracle import oracle as O
+
+    n, dt = 200_000, 0.01
+    edges = np.array([0.0, 1.0])
+    pool = _one_cell_pool(O, n, 0.5, sph)
+    dep, path, leak, st = CF.transport_1d_cf(sph, pool, edges, np.array([sigma]), np.array([beta]), dt,
+                                             (True, True), 0.0, 0, 0.0, seed=7, step=0, base=0)
+    l1, l2 = C_LIGHT * sigma, C_LIGHT * sigma * beta
+    s = l1 + l2
+    p_rad = l2 / s + l1 / s * np.exp(-s * dt)
+    se = np.sqrt(p_rad * (1 - p_rad) / n)
+    census = pool.w.sum() / n
+    assert abs(census - p_rad) < 4 * se, (census, p_rad, se)
+    assert pool.w.sum() + dep.sum() == pytest.approx(n, rel=1e-12)
+    assert leak == 0.0 and st[4] == np.count_nonzero(pool.w == 0.0)
+    expect = C_LIGHT * (l2 / s * dt + l1 / s ** 2 * (1 - np.exp(-s * dt)))
+    assert path.sum() / n == pytest.approx(expect, rel=5e-3)
+    assert np.all((pool.x >= 0.0) & (pool.x <= 1.0))
+
+
+def _run(CF, nsteps, N, seed, geometry, T0=0.3, Tb=(0.8, 0.0), reflect=(False, False), I=12, cvc=0.3):
+    mesh = np.stack([np.linspace(0, 0.5, I + 1)[:-1], np.linspace(0, 0.5, I + 1)[1:]], axis=1)
+    orc = CF.OracleCF1D(mesh, geometry, seed=seed, nthreads=2)
+    Ti = np.full(I, T0)
+    st = orc.init_simulation(N // 2, Ti, Ti, lambda T: cvc * T, lambda u: u / cvc)
+    out = []
+    for _ in range(nsteps):
+        st, info = orc.step(st, N // 2, N // 2, 0, N, Tb, 0.01, lambda T: 20.0 * np.maximum(T, 1e-3) ** -1,
+                            lambda u: u / cvc, lambda T: cvc + 0 * T, np.zeros(I), reflect=reflect)
+        out.append(info)
+    return out
+
+
+@pytest.mark.parametrize("geometry", ["slab", "spherical"])
+def test_full_step_balance(oracle, CF, geometry):
+    # the sphere is driven through its outer surface (the inner one has radius 0)
+    Tb = (0.8, 0.0) if geometry == "slab" else (0.0, 0.8)
+    for info in _run(CF, 3, 20000, 5, geometry, Tb=Tb):
+        resid = info["energy_loss"] + info["comb_discrepancy"]
+        assert abs(resid) <= 1e-12 * info["total_energy"], resid
+        assert info["stats"][2] > 0 and info["stats"][4] > 0  # re-emissions and captures happen
+        assert info["boundary_emission"] > 0
+
+
+def _one_step_radiation(CF, T0, Tr0, sigma, cvc, dt, N, seed):
+    """Pre-comb radiation energy of one step of a closed one-cell slab and the
+    closed form of the frozen-coefficient two-state system: with E0 = a Tr0^4,
+    U0 = a T0^4, beta = 4 a T0^3 / c_v, s = c sigma (1 + beta),
+    E(dt) = [beta E0 + U0 + (E0 - U0) exp(-s dt)] / (1 + beta)."""
+    mesh = np.array([[0.0, 1.0]])
+    orc = CF.OracleCF1D(mesh, "slab", seed=seed, nthreads=2)
+    st = orc.init_simulation(N, np.array([T0]), np.array([Tr0]), lambda T: cvc * T, lambda u: u / cvc)
+    # small comb target: the estimate is read before the comb, and long
+    # sequential sums of equal copy weights in one cell would lose ~n eps
+    st, info = orc.step(st, N, 0, 0, N // 4, (0.0, 0.0), dt, lambda T: sigma + 0 * T, lambda u: u / cvc,
+                        lambda T: cvc + 0 * T, np.zeros(1), reflect=(True, True))
+    E1 = info["total_radiation_energy"] + info["comb_discrepancy"]
+    beta = 4 * A_RAD * T0 ** 3 / cvc
+    E0, U0 = A_RAD * Tr0 ** 4, A_RAD * T0 ** 4
+    s = C_LIGHT * sigma * (1 + beta)
+    return E1, (beta * E0 + U0 + (E0 - U0) * np.exp(-s * dt)) / (1 + beta), info
+
+
+@pytest.mark.parametrize("T0,Tr0", [(1.0, 0.5), (0.4, 0.9)])
+def test_one_step_linear_closed_form(oracle, CF, T0, Tr0):
+    """Emission (energy and truncated-exponential times) and transport
+    together against the exact solution (4 standard errors over 8 seeds)."""
+    errs = []
+    for seed in range(1, 9):
+        E1, exact, info = _one_step_radiation(CF, T0, Tr0, 3.0, 0.01, 0.001, 40000, seed)
+        errs.append(E1 / exact - 1.0)
+        assert abs(info["energy_loss"] + info["comb_discrepancy"]) <= 1e-12 * info["total_energy"]
+    errs = np.array(errs)
+    se = errs.std(ddof=1) / np.sqrt(errs.size)
+    assert abs(errs.mean()) < 4 * se + 1e-4, (errs.mean(), se)
