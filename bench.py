@@ -617,14 +617,108 @@ def main_cf(args):
     print(json.dumps(out), flush=True)
 
 
+# ---------------------------------------------------------------------------
+# Carter-Forest 1-D: the reference's own FC vs CF Marshak-wave comparison case
+# ---------------------------------------------------------------------------
+# event model: state (w, mu, x, t, d_census 5x8, cell 4, history id 4, event
+# counter 4 = 52 B) loaded and stored once per segment, (sigma_a, beta)
+# gathered, two FP64 tallies
+BYTES_PER_SEGMENT_CF1D = 2 * 52 + 16 + 16
+# the reference's published run of this case on its CPU path
+# (IMC/marshak_wave_profile_compare_1ns.json: wall_s, phase_totals_s.transport,
+# event_totals.total)
+REFERENCE_CF1D = {"cf": {"wall_s": 7.253435118997004, "transport_s": 4.730065080017084, "segments": 67963139},
+                  "fc": {"wall_s": 16.950405291994684, "transport_s": 15.079742463007278, "segments": 427251321}}
+
+
+def main_cf1d(args):
+    from radtranbook_amd import IMC1D, IMC1D_CarterForest as CFM, _lib
+    from radtranbook_amd.problems import marshak_wave_compare
+
+    _lib.load()
+    _lib.require_gpu()
+    IMC1D.VERBOSE = False
+    CFM.VERBOSE = False
+    case = marshak_wave_compare()
+    warmup = args.warmup if args.warmup is not None else 2
+    T0 = np.full(case.I, case.T0)
+
+    def run(mod, seed, nsteps):
+        st = mod.init_simulation(case.n_target, T0, T0, case.mesh, case.eos, case.inv_eos, geometry="slab", seed=seed)
+        segs, kern, k = 0, 0.0, 0
+        t0 = time.perf_counter()
+        while st.time < case.t_final - 1e-12 and k < nsteps:
+            st, info = mod.step(st, case.n_target, case.n_boundary, 0, case.n_max, case.T_boundary,
+                                min(case.dt, case.t_final - st.time), case.mesh, case.sigma_a, case.inv_eos,
+                                case.cv_func, np.zeros(case.I), reflect=case.reflect, geometry="slab")
+            segs += info["profiling"]["transport_events"]["total"]
+            kern += info["profiling"]["transport_kernel_ms"]
+            k += 1
+        return dict(wall=time.perf_counter() - t0, segs=segs, kern_ms=kern, steps=k,
+                    energy=st.previous_total_energy)
+
+    full = int(round(case.t_final / case.dt))
+    res = {}
+    for name, mod in (("cf", CFM), ("fc", IMC1D)):
+        if warmup > 0:
+            run(mod, 1000, warmup)
+        res[name] = run(mod, 1, full)
+        log(f"  [{name}] {res[name]['steps']} steps: {res[name]['wall'] * 1e3:.1f} ms, {res[name]['segs']:.4e} "
+            f"segments, kernels {res[name]['kern_ms']:.1f} ms, final energy {res[name]['energy']:.6e}")
+    cf = res["cf"]
+    value = cf["segs"] / cf["wall"]
+    achieved = cf["segs"] * BYTES_PER_SEGMENT_CF1D / (cf["kern_ms"] / 1e3) / 1e9
+    out = {
+        "metric": "IMC segments/s, Carter-Forest 1-D slab Marshak wave (the reference's FC vs CF comparison case)",
+        "value": value, "unit": "segments/s", "n_gpus": 1, "steps": cf["steps"], "warmup": warmup,
+        "ms_per_step": cf["wall"] / cf["steps"] * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "the reference's deterministic case, Philox particle streams",
+        "config": {"workload": "Carter-Forest gray IMC 1-D slab, 50 cells on [0, 0.2] cm, sigma_a = 300 T^-3, "
+                               "c_v = 0.3, T0 = 1e-4 keV, 1 keV left face, reflecting right face, dt = 0.025 ns "
+                               "to 1 ns, 1e4 + 1e4 new particles per step, census combed to 4e4",
+                   "cells": case.I},
+        "transport_segments_per_s": cf["segs"] / (cf["kern_ms"] / 1e3),
+        "final_total_energy": cf["energy"],
+        "fc": {"segments_per_s": res["fc"]["segs"] / res["fc"]["wall"], "wall_s": res["fc"]["wall"],
+               "segments": res["fc"]["segs"], "final_total_energy": res["fc"]["energy"]},
+        "reference_published_cpu": REFERENCE_CF1D,
+        "wall_speedup_vs_reference_published": {m: REFERENCE_CF1D[m]["wall_s"] / res[m]["wall"] for m in res},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k1_transport_cf<slab>",
+                     "bytes_per_segment": BYTES_PER_SEGMENT_CF1D,
+                     "note": "5e4-8e4 histories per launch: latency-bound, far below the device's size"},
+    }
+    if not args.no_cpu:
+        from oracle import oracle as O, oracle_cf1d as OCF
+
+        O.build()
+        cores = len(os.sched_getaffinity(0))
+        orc = OCF.OracleCF1D(case.mesh, "slab", seed=1, nthreads=cores)
+        st = orc.init_simulation(case.n_target, T0, T0, case.eos, case.inv_eos)
+        csegs = 0
+        t0 = time.perf_counter()
+        while st.time < case.t_final - 1e-12:
+            st, info = orc.step(st, case.n_target, case.n_boundary, 0, case.n_max, case.T_boundary,
+                                min(case.dt, case.t_final - st.time), case.sigma_a, case.inv_eos, case.cv_func,
+                                np.zeros(case.I), reflect=case.reflect)
+            csegs += int(info["stats"][0])
+        ct = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": csegs / ct, "unit": "segments/s", "cores": cores, "kind": "port",
+                               "sample": f"FP64 C Carter-Forest oracle (OpenMP), the whole case: {csegs:.3e} "
+                                         f"segments in {ct:.2f} s"}
+        out["gpu_over_cpu_segments_per_s"] = value / (csegs / ct)
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=["marshak2d", "slab1d", "batch1d", "mg2d", "diff_rz", "cf2d"],
+    ap.add_argument("--workload", choices=["marshak2d", "slab1d", "batch1d", "mg2d", "diff_rz", "cf2d", "cf1d"],
                     default="marshak2d",
                     help="marshak2d: 2D 200x200 heterogeneous (default, headline); slab1d: the reference's 1D slab "
                          "case; batch1d: 256 independent slabs; mg2d: multigroup crooked pipe; diff_rz: "
-                         "differentiable rz step; cf2d: Carter-Forest variant of marshak2d")
+                         "differentiable rz step; cf2d: Carter-Forest variant of marshak2d; cf1d: the reference's "
+                         "FC vs Carter-Forest 1D Marshak-wave comparison case")
     ap.add_argument("--problems", type=int, default=256)
     ap.add_argument("--tune", action=argparse.BooleanOptionalAction, default=True,
                     help="autotune the 1D transport launch shape on the first warm-up step")
@@ -654,6 +748,9 @@ def main():
         return
     if args.workload == "cf2d":
         main_cf(args)
+        return
+    if args.workload == "cf1d":
+        main_cf1d(args)
         return
     if args.workload != "marshak2d":
         main_1d(args, args.workload)

@@ -14,6 +14,12 @@
   (half emission, half from the hot face, census combed to that count),
   sigma0 ~ U(10, 1000) with sigma_a = sigma0 max(T,1e-3)^-3, c_v ~ U(0.05, 0.5),
   T_b ~ U(0.1, 1) keV on the left face, T0 = 1e-3 keV, dt = 0.01 ns x 10.
+* ``marshak_wave_compare`` -- the reference's Fleck-Cummings vs Carter-Forest
+  comparison case (IMC/marshak_wave_profile_compare.py, defaults): gray slab,
+  50 cells on [0, 0.2] cm, sigma_a = 300 T^-3 (no floor), c_v = 0.3,
+  T0 = Tr0 = 1e-4 keV, left face 1 keV, right face reflecting,
+  dt = 0.025 ns to 1 ns, 1e4 emission and 1e4 face particles per step,
+  census combed to 4e4.
 * ``crooked_pipe_mg`` -- multigroup 2-D xy, 240x240 cells on [0,2]^2, 10 groups
   with edges logspace(-4, 1, 11) keV and nu_g = sqrt(lo hi).  The thin
   channel is every cell within 0.05 cm of a polyline from (0, y0) to (2, y5)
@@ -160,6 +166,42 @@ def marshak_slab(I=100, n_particles=100_000, n_steps=10):
         name=f"marshak_slab_{I}", mesh=_slab_mesh(I, 0.2),
         opacity=PowerLawOpacity(np.full(I, 300.0), 3.0, 1e-3), eos=LinearEOS(0.3), T0=1e-3,
         T_boundary=(1.0, 0.0), reflect=(False, False), dt=0.01, n_steps=n_steps, n_particles=int(n_particles))
+
+
+@dataclass
+class WaveCompareCase:
+    mesh: np.ndarray
+    T0: float = 1e-4
+    T_boundary: tuple = (1.0, 0.0)
+    reflect: tuple = (False, True)
+    dt: float = 0.025
+    t_final: float = 1.0
+    n_target: int = 10_000
+    n_boundary: int = 10_000
+    n_max: int = 40_000
+    cv: float = 0.3
+
+    @property
+    def I(self):
+        return self.mesh.shape[0]
+
+    @staticmethod
+    def sigma_a(T):
+        return 300.0 * T ** -3
+
+    def eos(self, T):
+        return self.cv * T
+
+    def inv_eos(self, u):
+        return u / self.cv
+
+    def cv_func(self, T):
+        return self.cv + 0.0 * T
+
+
+def marshak_wave_compare(cells=50, length=0.2):
+    """IMC/marshak_wave_profile_compare.py:65-98 with its default arguments."""
+    return WaveCompareCase(mesh=_slab_mesh(cells, length))
 
 
 def batch_slabs(P=256, seed=1, n_steps=10, scale=1.0):
