@@ -435,6 +435,7 @@ struct MgArgs {
   int64_t nb;
   int copy_mask;
   unsigned long long* stats;
+  QueueMap Q;
 };
 
 constexpr int kMgWaveSlots = 512;  // tally cache slots per wave of the 2-D tracker
@@ -458,8 +459,8 @@ __global__ __launch_bounds__(256) void k_mg_transport(MgArgs<R> A) {
   Tally tally(tally_lds + (threadIdx.x >> 6) * Tally::kBytes);
   const int nx = A.nx, ny = A.ny;
   const int64_t ncell = (int64_t)nx * ny;
-  int64_t qbase = 0, qend = 0;
-  bool exhausted = false, active = false;
+  WaveQueue wq;
+  bool active = false;
   int64_t k = 0;
   double w = 0.0, t = 0.0;
   R d1 = 0, d2 = 0, p1 = 0, p2 = 0, dcen = 0;
@@ -471,44 +472,28 @@ __global__ __launch_bounds__(256) void k_mg_transport(MgArgs<R> A) {
   while (true) {
     uint64_t need = __ballot(!active);
     if (need) {
-      if (qbase >= qend && !exhausted) {
-        unsigned long long b = 0;
-        if (__lane_id() == 0) b = atomicAdd(A.next, (unsigned long long)A.batch);
-        b = __shfl(b, 0, 64);
-        if ((int64_t)b >= A.n) {
-          exhausted = true;
-        } else {
-          qbase = (int64_t)b;
-          qend = ((int64_t)b + A.batch < A.n) ? (int64_t)b + A.batch : A.n;
-        }
-      }
-      int64_t avail = qend - qbase;
-      if (avail > 0) {
-        int64_t rank = __popcll(need & lanemask_lt());
-        if (!active && rank < avail) {
-          k = qbase + rank;
-          w = A.P.w[k];
-          d1 = A.P.d1[k];
-          d2 = A.P.d2[k];
-          p1 = A.P.p1[k];
-          p2 = A.P.p2[k];
-          t = (double)A.P.t[k];
-          i = A.P.ci[k];
-          j = A.P.cj[k];
-          g = A.P.g[k];
-          // FP32 times are stored rounded to nearest: a time sampled just below
-          // dt can read back above it, and a negative distance to census would
-          // grow the weight by e^{sigma_a |dcen|}.  Such a particle is at census.
-          dcen = (R)fmax((A.dt - t) * kC, 0.0);
-          ev = 0;
-          active = w > wfloor;  // below the floor: skipped, unchanged (MG_IMC2D.py:842-843)
-        }
-        int64_t take = __popcll(need);
-        qbase += take < avail ? take : avail;
+      const int64_t kk = wave_take(wq, !active, need, A.next, A.batch, A.Q, A.n);
+      if (kk >= 0) {
+        k = kk;
+        w = A.P.w[k];
+        d1 = A.P.d1[k];
+        d2 = A.P.d2[k];
+        p1 = A.P.p1[k];
+        p2 = A.P.p2[k];
+        t = (double)A.P.t[k];
+        i = A.P.ci[k];
+        j = A.P.cj[k];
+        g = A.P.g[k];
+        // FP32 times are stored rounded to nearest: a time sampled just below
+        // dt can read back above it, and a negative distance to census would
+        // grow the weight by e^{sigma_a |dcen|}.  Such a particle is at census.
+        dcen = (R)fmax((A.dt - t) * kC, 0.0);
+        ev = 0;
+        active = w > wfloor;  // below the floor: skipped, unchanged (MG_IMC2D.py:842-843)
       }
     }
     if (__ballot(active) == 0) {
-      if (exhausted) break;
+      if (wq.exhausted) break;
       continue;
     }
     if (!active) continue;
@@ -965,8 +950,8 @@ __global__ __launch_bounds__(256) void k_m1_transport(MgArgs<R> A) {
   lds_tally_init(lkey, lval);
   __syncthreads();
   const int I = A.nx;
-  int64_t qbase = 0, qend = 0;
-  bool exhausted = false, active = false;
+  WaveQueue wq;
+  bool active = false;
   int64_t k = 0;
   double w = 0.0;
   R mu = 0, x = 0, dcen = 0;
@@ -978,37 +963,21 @@ __global__ __launch_bounds__(256) void k_m1_transport(MgArgs<R> A) {
   while (true) {
     uint64_t need = __ballot(!active);
     if (need) {
-      if (qbase >= qend && !exhausted) {
-        unsigned long long b = 0;
-        if (__lane_id() == 0) b = atomicAdd(A.next, (unsigned long long)A.batch);
-        b = __shfl(b, 0, 64);
-        if ((int64_t)b >= A.n) {
-          exhausted = true;
-        } else {
-          qbase = (int64_t)b;
-          qend = ((int64_t)b + A.batch < A.n) ? (int64_t)b + A.batch : A.n;
-        }
-      }
-      int64_t avail = qend - qbase;
-      if (avail > 0) {
-        int64_t rank = __popcll(need & lanemask_lt());
-        if (!active && rank < avail) {
-          k = qbase + rank;
-          w = A.P.w[k];
-          mu = A.P.d1[k];
-          x = A.P.p1[k];
-          cell = A.P.ci[k];
-          g = A.P.g[k];
-          dcen = (R)((A.dt - (double)A.P.t[k]) * kC);
-          ev = 0;
-          active = dcen > (R)1e-15 && cell >= 0 && cell < I;
-        }
-        int64_t take = __popcll(need);
-        qbase += take < avail ? take : avail;
+      const int64_t kk = wave_take(wq, !active, need, A.next, A.batch, A.Q, A.n);
+      if (kk >= 0) {
+        k = kk;
+        w = A.P.w[k];
+        mu = A.P.d1[k];
+        x = A.P.p1[k];
+        cell = A.P.ci[k];
+        g = A.P.g[k];
+        dcen = (R)((A.dt - (double)A.P.t[k]) * kC);
+        ev = 0;
+        active = dcen > (R)1e-15 && cell >= 0 && cell < I;
       }
     }
     if (__ballot(active) == 0) {
-      if (exhausted) break;
+      if (wq.exhausted) break;
       continue;
     }
     if (!active) continue;
@@ -1756,12 +1725,15 @@ int mgimc_transport(mgimc_ctx* c, double dt, const int32_t* reflect, int64_t max
   }
   grid = (int)std::max<int64_t>(std::min<int64_t>(grid, (n + block - 1) / block), 1);
   Key key = make_key(c->seed);
+  // pool order: ordering the chunks by expected history length (as the gray
+  // tracker does) measured no gain here, where the tally atomics bound the time
+  const QueueMap Q = pool_order_queue(n);
   IMC_HIP(hipEventRecord(c->ev0, s));
   if (n > 0) {
     if (c->precision == IMC_FP64) {
       MgArgs<double> A{c->A.view<double>(), n, c->e1p64.p, c->e2p64.p, c->nx, c->ny, G, c->sas64.p, dt, refl,
                        max_events, c->scal.p + 2, key, c->step, c->next, c->batch, c->dep_raw.p, c->path_raw.p, nb,
-                       c->tally_copies - 1, c->stats};
+                       c->tally_copies - 1, c->stats, Q};
       if (c->geometry == IMC_GEOM_XY)
         k_mg_transport<double, 0><<<grid, block, 0, s>>>(A);
       else
@@ -1769,7 +1741,7 @@ int mgimc_transport(mgimc_ctx* c, double dt, const int32_t* reflect, int64_t max
     } else {
       MgArgs<float> A{c->A.view<float>(), n, c->e1p32.p, c->e2p32.p, c->nx, c->ny, G, c->sas32.p, dt, refl,
                       max_events, c->scal.p + 2, key, c->step, c->next, c->batch, c->dep_raw.p, c->path_raw.p, nb,
-                      c->tally_copies - 1, c->stats};
+                      c->tally_copies - 1, c->stats, Q};
       if (c->geometry == IMC_GEOM_XY)
         k_mg_transport<float, 0><<<grid, block, 0, s>>>(A);
       else
@@ -2089,12 +2061,12 @@ int mgimc_sphere_transport(mgimc_ctx* c, double dt, const int32_t* reflect, doub
     if (c->precision == IMC_FP64) {
       MgArgs<double> A{c->A.view<double>(), n, c->e1p64.p, c->e2p64.p, c->nx, 1, G, c->sas64.p, dt, refl, 0,
                        c->scal.p + 2, key, c->step, c->next, c->batch, c->dep_raw.p, c->path_raw.p, nb,
-                       c->tally_copies - 1, c->stats};
+                       c->tally_copies - 1, c->stats, pool_order_queue(n)};
       k_m1_transport<double><<<grid, block, 0, s>>>(A);
     } else {
       MgArgs<float> A{c->A.view<float>(), n, c->e1p32.p, c->e2p32.p, c->nx, 1, G, c->sas32.p, dt, refl, 0,
                       c->scal.p + 2, key, c->step, c->next, c->batch, c->dep_raw.p, c->path_raw.p, nb,
-                      c->tally_copies - 1, c->stats};
+                      c->tally_copies - 1, c->stats, pool_order_queue(n)};
       k_m1_transport<float><<<grid, block, 0, s>>>(A);
     }
     IMC_HIP(hipGetLastError());
