@@ -1,4 +1,6 @@
-"""End-to-end acceptance against a run the reference published: its
+"""End-to-end acceptance runs through the drop-in modules.
+
+1. Against a run the reference published: its
 Fleck-Cummings vs Carter-Forest Marshak-wave comparison
 (IMC/marshak_wave_profile_compare.py with default arguments, result file
 IMC/marshak_wave_profile_compare_1ns.json, kept here as
@@ -8,7 +10,12 @@ Both drop-in modules (IMC1D, IMC1D_CarterForest) run the same case to
 1 ns over four seeds.  The final total energy must agree with the published
 single run within 4 combined standard deviations (one-run spreads measured
 on this case: 0.45% FC, 0.26% CF; tolerances 2% and 1.2%), and the total
-event count within 5%."""
+event count within 5%.
+
+2. The checks of the reference's 2-D Marshak-wave driver
+(IMC/MarshakWave2D.py): x- vs y-driven waves on a square (FC and CF), an rz
+shell far from the axis against the 1-D slab, and the self-similar Marshak
+solution."""
 import json
 import os
 
@@ -57,3 +64,88 @@ def test_marshak_wave_compare_matches_published_run(published, method, rtol):
           f"events {events:.4e} (published {ref['event_totals']['total']:.4e})", flush=True)
     assert energy == pytest.approx(ref["final_total_energy"], rel=rtol)
     assert events == pytest.approx(ref["event_totals"]["total"], rel=0.05)
+
+
+# ---------------------------------------------------------------------------
+# IMC/MarshakWave2D.py: the Cartesian symmetry and rz planar-limit checks
+# (run_xy_symmetry_check, run_rz_planar_limit_check with the driver's
+# defaults: 60 x 60 cells on [0, 0.2]^2, sigma_a = 300 T^-3, c_v = 0.3,
+# T0 = 1e-4 keV, a 1 keV face, dt = 0.01 ns to 1 ns).  The reference reports
+# the relative L2 differences without thresholds; the bounds here are about
+# twice what this engine measures (x- vs y-driven 0.4-0.7% FC, 0.7-1.3% CF;
+# rz vs slab 2.5-3.0%, against 0.4-1.0% between two slab seeds: the rz and
+# slab emission tilts differ; IMC vs the self-similar diffusion solution
+# 7-8%).
+# ---------------------------------------------------------------------------
+A = 0.01372
+C = 29.98
+
+
+def _sig(T):
+    return 300.0 * T ** -3
+
+
+def _self_similar(x, t):
+    """Marshak-wave self-similar temperature for sigma = 300 T^-3, c_v = 0.3,
+    a 1 keV face (IMC/MarshakWave2D.py:187-211)."""
+    K = 8.0 * A * C / (7.0 * 3.0 * 300.0 * 0.3)
+    xi = x / np.sqrt(K * t)
+    core = np.maximum((1.0 - xi / 1.11305) * (1.0 + 0.05989 * xi / 1.11305), 1e-30)
+    return np.where(xi < 1.11305, core ** (1.0 / 6.0), 0.0)
+
+
+def _rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def _run2d(mod, geometry, e1, e2, Tb, reflect, Nt, Nb, Nm, seed):
+    from radtranbook_amd import IMC2D
+
+    nx, ny = e1.size - 1, e2.size - 1
+    T0 = np.full((nx, ny), 1e-4)
+    st = IMC2D.init_simulation(Nt, T0, T0, e1, e2, lambda T: 0.3 * T, lambda u: u / 0.3, geometry=geometry,
+                               seed=seed)
+    while st.time < 1.0 - 1e-12:
+        st, info = mod.step(st, Nt, Nb, 0, Nm, Tb, min(0.01, 1.0 - st.time), e1, e2, _sig, lambda u: u / 0.3,
+                            lambda T: 0.3 + 0 * T, np.zeros((nx, ny)), reflect=reflect, geometry=geometry,
+                            max_events_per_particle=10**6)
+        assert abs(info["energy_residual"]) <= 1e-12 * info["total_energy"]
+    return st.temperature
+
+
+@pytest.mark.parametrize("method,bound", [("fc", 0.03), ("cf", 0.04)])
+def test_marshak2d_xy_directional_symmetry(method, bound):
+    from radtranbook_amd import IMC2D, IMC2D_CarterForest
+
+    IMC2D.VERBOSE = False
+    IMC2D_CarterForest.VERBOSE = False
+    mod = IMC2D if method == "fc" else IMC2D_CarterForest
+    e = np.linspace(0.0, 0.2, 61)
+    Tx = _run2d(mod, "xy", e, e, (1.0, 0.0, 0.0, 0.0), (False, True, True, True), 20000, 12000, 120000, 1).mean(1)
+    Ty = _run2d(mod, "xy", e, e, (0.0, 0.0, 1.0, 0.0), (True, True, False, True), 20000, 12000, 120000, 2).mean(0)
+    err = _rel_l2(Tx, Ty)
+    ss = _rel_l2(Tx, _self_similar(0.5 * (e[1:] + e[:-1]), 1.0))
+    print(f"{method}: x- vs y-driven {err:.4f}, vs self-similar {ss:.4f}", flush=True)
+    assert err < bound
+    assert ss < 0.15
+
+
+def test_marshak2d_rz_planar_limit():
+    from radtranbook_amd import IMC1D, IMC2D
+
+    IMC1D.VERBOSE = False
+    IMC2D.VERBOSE = False
+    z = np.linspace(0.0, 0.2, 61)
+    r = np.linspace(10.0, 10.2, 11)
+    Trz = _run2d(IMC2D, "rz", r, z, (0.0, 0.0, 1.0, 0.0), (True, True, False, True), 24000, 12000, 120000, 1)
+    vol = np.pi * (r[1:] ** 2 - r[:-1] ** 2)[:, None] * np.diff(z)[None, :]
+    Tz = ((vol / vol.sum(0, keepdims=True)) * Trz).sum(0)
+    mesh = np.stack([z[:-1], z[1:]], axis=1)
+    T0 = np.full(60, 1e-4)
+    st = IMC1D.init_simulation(24000, T0, T0, mesh, lambda T: 0.3 * T, lambda u: u / 0.3, geometry="slab", seed=1)
+    while st.time < 1.0 - 1e-12:
+        st, _ = IMC1D.step(st, 24000, 12000, 0, 120000, (1.0, 0.0), min(0.01, 1.0 - st.time), mesh, _sig,
+                           lambda u: u / 0.3, lambda T: 0.3 + 0 * T, np.zeros(60), reflect=(False, True))
+    err = _rel_l2(Tz, st.temperature)
+    print(f"rz planar limit vs slab {err:.4f}", flush=True)
+    assert err < 0.06
