@@ -1725,8 +1725,10 @@ int mgimc_transport(mgimc_ctx* c, double dt, const int32_t* reflect, int64_t max
   }
   grid = (int)std::max<int64_t>(std::min<int64_t>(grid, (n + block - 1) / block), 1);
   Key key = make_key(c->seed);
-  // pool order: ordering the chunks by expected history length (as the gray
-  // tracker does) measured no gain here, where the tally atomics bound the time
+  // pool order: ordering the chunks by expected history length, as the gray
+  // tracker does, measured no gain here, where the tally atomics bound the
+  // time (crooked pipe, FP32 tracker: all chunks sorted 7.25 ms against 7.28
+  // in pool order; only long chunks first 7.50 against 7.26)
   const QueueMap Q = pool_order_queue(n);
   IMC_HIP(hipEventRecord(c->ev0, s));
   if (n > 0) {
