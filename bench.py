@@ -281,7 +281,9 @@ def main_1d(args, workload):
         "transport_segments_per_s": head["segments"] / (head["kernel_ms"].sum() / 1e3),
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": profiled_traffic("k1_transport_batch1d_256slabs.md", precs[0], steps, warmup,
+                                        workload == "batch1d" and args.problems == 256 and not args.tune),
             "kernel": f"k1_transport<{'float' if precs[0] == 'fp32' else 'double'},0>",
             "bytes_per_segment": BYTES_PER_SEGMENT_1D[precs[0]],
         },
