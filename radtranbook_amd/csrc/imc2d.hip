@@ -1346,10 +1346,11 @@ static int ensure_sum_scratch(Imc2dCtx* c, int64_t n) { return c->sum_scr.reserv
 
 static int pool_ensure(Imc2dCtx* c, int64_t n_total) { return c->A.grow(n_total, c->n, c->stream); }
 
-// Pickup order of the gray tracker (beta null) or of the Carter-Forest one
-// over the current pool: chunks of `batch` particles, longest expected
-// histories first (pool order when the batch is not a power of two).
-static int longest_first_queue(Imc2dCtx* c, int64_t n, double dt, QueueMap* Q, const double* beta = nullptr) {
+// Pickup order of the gray tracker or its adjoint replay (beta null) or of
+// the Carter-Forest tracker over pool P: chunks of `batch` particles, longest
+// expected histories first (pool order when the batch is not a power of two).
+static int longest_first_queue(Imc2dCtx* c, PoolBuf& P, int64_t n, double dt, QueueMap* Q,
+                               const double* beta = nullptr) {
   *Q = pool_order_queue(n);
   const int batch = c->batch;
   if (n <= 0 || (batch & (batch - 1)) != 0) return kOk;
@@ -1363,10 +1364,10 @@ static int longest_first_queue(Imc2dCtx* c, int64_t n, double dt, QueueMap* Q, c
   hipStream_t s = c->stream;
   const int g = chunk_grid(nchunk);
   if (c->precision == IMC_FP64)
-    k_chunk_key<double><<<g, kChunkBlock, 0, s>>>(c->A.view<double>(), n, shift, c->nx, c->ny, c->e1, c->e2,
+    k_chunk_key<double><<<g, kChunkBlock, 0, s>>>(P.view<double>(), n, shift, c->nx, c->ny, c->e1, c->e2,
                                                   c->sa.p, c->ss.p, beta, dt, c->chunk_key.p, c->chunk_hist.p);
   else
-    k_chunk_key<float><<<g, kChunkBlock, 0, s>>>(c->A.view<float>(), n, shift, c->nx, c->ny, c->e1, c->e2,
+    k_chunk_key<float><<<g, kChunkBlock, 0, s>>>(P.view<float>(), n, shift, c->nx, c->ny, c->e1, c->e2,
                                                  c->sa.p, c->ss.p, beta, dt, c->chunk_key.p, c->chunk_hist.p);
   IMC_HIP(hipGetLastError());
   if ((rc = chunk_order(c->chunk_key.p, nchunk, g, c->chunk_hist.p, c->chunk_ord.p, s))) return rc;
@@ -1866,7 +1867,7 @@ int imc2d_transport(imc2d_ctx* c, double dt, const int32_t* reflect, int64_t max
     c->n_saved = n;
   }
   QueueMap Q;
-  if ((rc = longest_first_queue(c, n, dt, &Q))) return rc;
+  if ((rc = longest_first_queue(c, c->A, n, dt, &Q))) return rc;
   IMC_HIP(hipEventRecord(c->ev0, s));
   if (n > 0) {
     const bool narrow = n <= kNarrowMax;
@@ -1977,7 +1978,7 @@ int imc2d_transport_cf(imc2d_ctx* c, double dt, const int32_t* reflect, int64_t 
   // long expected histories first (200x200 heterogeneous Marshak, 1e7
   // particles/step: tracker 2.01 -> 1.85 ms against pool order)
   QueueMap Q;
-  if ((rc = longest_first_queue(c, n, dt, &Q, c->beta.p))) return rc;
+  if ((rc = longest_first_queue(c, c->A, n, dt, &Q, c->beta.p))) return rc;
   IMC_HIP(hipEventRecord(c->ev0, s));
   if (n > 0) {
     CfArgs A{c->A.view<double>(), n, c->e1p64.p, c->e2p64.p, c->nx, c->ny, c->sa.p, c->beta.p, dt, refl, max_events,
@@ -2225,11 +2226,13 @@ int imc2d_adjoint(imc2d_ctx* c, double dt, const int32_t* reflect, int64_t max_e
   int grid = c->grid_override > 0 ? c->grid_override : c->n_cu * 4;
   grid = (int)std::max<int64_t>(std::min<int64_t>(grid, (n + block - 1) / block), 1);
   Key key = make_key(c->seed);
+  QueueMap Q;  // the forward launch's order, from the saved pool
+  if ((rc = longest_first_queue(c, c->S, n, dt, &Q))) return rc;
   if (n > 0) {
     if (c->precision == IMC_FP64) {
       AdjointArgs<double> A{c->S.view<double>(), n, c->e1p64.p, c->e2p64.p, c->nx, c->ny, c->sas64.p, c->sa.p, dt,
                             refl, max_events, c->scal + 2, key, c->step, c->next, c->batch, c->volsrc.p, c->emit_lo,
-                            c->emit_hi, c->g_raw.p, nc, c->tally_copies - 1, pool_order_queue(n)};
+                            c->emit_hi, c->g_raw.p, nc, c->tally_copies - 1, Q};
       if (c->geometry == IMC_GEOM_XY)
         k_transport_adjoint<double, 0><<<grid, block, 0, s>>>(A);
       else
@@ -2237,7 +2240,7 @@ int imc2d_adjoint(imc2d_ctx* c, double dt, const int32_t* reflect, int64_t max_e
     } else {
       AdjointArgs<float> A{c->S.view<float>(), n, c->e1p32.p, c->e2p32.p, c->nx, c->ny, c->sas32.p, c->sa.p, dt,
                            refl, max_events, c->scal + 2, key, c->step, c->next, c->batch, c->volsrc.p, c->emit_lo,
-                           c->emit_hi, c->g_raw.p, nc, c->tally_copies - 1, pool_order_queue(n)};
+                           c->emit_hi, c->g_raw.p, nc, c->tally_copies - 1, Q};
       if (c->geometry == IMC_GEOM_XY)
         k_transport_adjoint<float, 0><<<grid, block, 0, s>>>(A);
       else
