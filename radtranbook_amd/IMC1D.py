@@ -20,6 +20,8 @@ import time as _time
 
 import numpy as np
 
+from ._census import DeviceCensus
+
 from ._lib import quarter_root, A_RAD, C_LIGHT, SRC_EMISSION, SRC_FIXED, Engine1D, ImcError
 
 __c = C_LIGHT
@@ -50,7 +52,7 @@ def _boundary_energy(geometry, Tb, dt, R):
     return e
 
 
-class SimulationState:
+class SimulationState(DeviceCensus):
     """Mutable 1-D IMC state (IMC/IMC1D.py:836-848).
 
     The particle arrays live on the GPU; reading ``weights``, ``mus``, ...
@@ -61,8 +63,7 @@ class SimulationState:
 
     def __init__(self, engine, internal_energy, temperature, radiation_temperature, time, previous_total_energy,
                  count=0):
-        self._engine = engine
-        self._cache = None
+        DeviceCensus.__init__(self, engine)
         self.internal_energy = internal_energy
         self.temperature = temperature
         self.radiation_temperature = radiation_temperature
@@ -70,32 +71,13 @@ class SimulationState:
         self.previous_total_energy = previous_total_energy
         self.count = count
 
-    def _census(self):
-        if self._cache is None:
-            self._cache = self._engine.get_census()[:5]
-        return self._cache
+    def _download(self):
+        return tuple(self._engine.get_census()[:5])
 
-    def _invalidate(self):
-        self._cache = None
+    def _upload(self, arrs):
+        self._engine.put_census([len(arrs[0])], *arrs)
 
-    def __getattr__(self, name):
-        if name in SimulationState._PARTICLE_FIELDS:
-            return self._census()[SimulationState._PARTICLE_FIELDS.index(name)]
-        raise AttributeError(name)
-
-    def __setattr__(self, name, value):
-        if name in SimulationState._PARTICLE_FIELDS:
-            arrs = list(self._census())
-            arrs[SimulationState._PARTICLE_FIELDS.index(name)] = np.asarray(value)
-            n = min(len(a) for a in arrs)
-            arrs = [a[:n] for a in arrs]
-            self._engine.put_census([n], *arrs)
-            self._cache = None
-            return
-        object.__setattr__(self, name, value)
-
-    @property
-    def n_particles(self):
+    def _device_count(self):
         return self._engine.census_counts()[1]
 
 

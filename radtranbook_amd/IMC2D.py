@@ -17,6 +17,8 @@ import time as _time
 
 import numpy as np
 
+from ._census import DeviceCensus
+
 from ._lib import quarter_root, A_RAD, C_LIGHT, SRC_EMISSION, SRC_FIXED, Engine2D
 
 __c = C_LIGHT
@@ -27,7 +29,7 @@ VERBOSE = True
 _SIDES = {"xy": ("left", "right", "bottom", "top"), "rz": ("rmin", "rmax", "zmin", "zmax")}
 
 
-class SimulationState2D:
+class SimulationState2D(DeviceCensus):
     """Mutable 2-D IMC state (IMC/IMC2D.py:65-84).
 
     The particle arrays live on the GPU; reading ``weights``, ``dir1``, ...
@@ -38,42 +40,13 @@ class SimulationState2D:
 
     def __init__(self, engine, internal_energy, temperature, radiation_temperature, time, previous_total_energy,
                  count=0):
-        self._engine = engine
-        self._cache = None
+        DeviceCensus.__init__(self, engine)
         self.internal_energy = internal_energy
         self.temperature = temperature
         self.radiation_temperature = radiation_temperature
         self.time = time
         self.previous_total_energy = previous_total_energy
         self.count = count
-
-    def _census(self):
-        if self._cache is None:
-            self._cache = self._engine.get_census()
-        return self._cache
-
-    def _invalidate(self):
-        self._cache = None
-
-    def __getattr__(self, name):
-        if name in SimulationState2D._PARTICLE_FIELDS:
-            return self._census()[SimulationState2D._PARTICLE_FIELDS.index(name)]
-        raise AttributeError(name)
-
-    def __setattr__(self, name, value):
-        if name in SimulationState2D._PARTICLE_FIELDS:
-            arrs = list(self._census())
-            arrs[SimulationState2D._PARTICLE_FIELDS.index(name)] = np.asarray(value)
-            n = min(len(a) for a in arrs)
-            arrs = [a[:n] for a in arrs]
-            self._engine.put_census(*arrs)
-            self._cache = None
-            return
-        object.__setattr__(self, name, value)
-
-    @property
-    def n_particles(self):
-        return self._engine.n
 
 
 def _shape_from_edges(edges1, edges2):

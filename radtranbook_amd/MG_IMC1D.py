@@ -27,6 +27,8 @@ import time as _time
 
 import numpy as np
 
+from ._census import DeviceCensus
+
 from ._lib import quarter_root, A_RAD, C_LIGHT, SRC_EMISSION, SRC_FIXED, EngineMG1D
 
 __c = C_LIGHT
@@ -35,7 +37,7 @@ __a = A_RAD
 VERBOSE = True
 
 
-class SimulationState1DMG:
+class SimulationState1DMG(DeviceCensus):
     """Mutable multigroup spherical state (MG_IMC/MG_IMC1D.py:116-138); the
     particle arrays are downloaded from the device on access."""
 
@@ -43,8 +45,7 @@ class SimulationState1DMG:
 
     def __init__(self, engine, internal_energy, temperature, radiation_temperature, radiation_energy_by_group,
                  radiation_flux_by_group, time, previous_total_energy, count=0):
-        self._engine = engine
-        self._cache = None
+        DeviceCensus.__init__(self, engine)
         self.internal_energy = internal_energy
         self.temperature = temperature
         self.radiation_temperature = radiation_temperature
@@ -54,29 +55,6 @@ class SimulationState1DMG:
         self.time = time
         self.previous_total_energy = previous_total_energy
         self.count = count
-
-    def _census(self):
-        if self._cache is None:
-            self._cache = self._engine.get_census()
-        return self._cache
-
-    def _invalidate(self):
-        self._cache = None
-
-    def __getattr__(self, name):
-        if name in SimulationState1DMG._PARTICLE_FIELDS:
-            return self._census()[SimulationState1DMG._PARTICLE_FIELDS.index(name)]
-        raise AttributeError(name)
-
-    def __setattr__(self, name, value):
-        if name in SimulationState1DMG._PARTICLE_FIELDS:
-            arrs = list(self._census())
-            arrs[SimulationState1DMG._PARTICLE_FIELDS.index(name)] = np.asarray(value)
-            n = min(len(a) for a in arrs)
-            self._engine.put_census(*[a[:n] for a in arrs])
-            self._cache = None
-            return
-        object.__setattr__(self, name, value)
 
 
 def _edges(mesh):

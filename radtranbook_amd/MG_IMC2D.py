@@ -27,6 +27,8 @@ import time as _time
 
 import numpy as np
 
+from ._census import DeviceCensus
+
 from ._lib import A_RAD, C_LIGHT, EngineMG, quarter_root
 
 __c = C_LIGHT
@@ -37,7 +39,7 @@ VERBOSE = True
 _SIDES = {"xy": ("left", "right", "bottom", "top"), "rz": ("rmin", "rmax", "zmin", "zmax")}
 
 
-class SimulationState2DMG:
+class SimulationState2DMG(DeviceCensus):
     """Mutable multigroup 2-D state (MG_IMC/MG_IMC2D.py:116-136); the
     particle arrays are downloaded from the device on access."""
 
@@ -45,8 +47,7 @@ class SimulationState2DMG:
 
     def __init__(self, engine, internal_energy, temperature, radiation_temperature, radiation_energy_by_group, time,
                  previous_total_energy, count=0):
-        self._engine = engine
-        self._cache = None
+        DeviceCensus.__init__(self, engine)
         self.internal_energy = internal_energy
         self.temperature = temperature
         self.radiation_temperature = radiation_temperature
@@ -55,29 +56,6 @@ class SimulationState2DMG:
         self.time = time
         self.previous_total_energy = previous_total_energy
         self.count = count
-
-    def _census(self):
-        if self._cache is None:
-            self._cache = self._engine.get_census()
-        return self._cache
-
-    def _invalidate(self):
-        self._cache = None
-
-    def __getattr__(self, name):
-        if name in SimulationState2DMG._PARTICLE_FIELDS:
-            return self._census()[SimulationState2DMG._PARTICLE_FIELDS.index(name)]
-        raise AttributeError(name)
-
-    def __setattr__(self, name, value):
-        if name in SimulationState2DMG._PARTICLE_FIELDS:
-            arrs = list(self._census())
-            arrs[SimulationState2DMG._PARTICLE_FIELDS.index(name)] = np.asarray(value)
-            n = min(len(a) for a in arrs)
-            self._engine.put_census(*[a[:n] for a in arrs])
-            self._cache = None
-            return
-        object.__setattr__(self, name, value)
 
 
 def _cell_volumes(edges1, edges2, geometry):

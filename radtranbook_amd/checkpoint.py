@@ -74,6 +74,7 @@ def save_checkpoint(state, path, extra=None):
     if kind_name not in _KINDS:
         raise TypeError(f"save_checkpoint(): not a drop-in simulation state: {kind_name}")
     kind, fields, scalars = _KINDS[kind_name]
+    state._flush()  # pending census field assignments reach the device first
     eng = state._engine
     data = {"format": np.array(FORMAT), "kind": np.array(kind)}
     for k, v in _engine_record(kind, eng).items():
