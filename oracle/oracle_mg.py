@@ -121,8 +121,27 @@ def volumes(e1, e2, geometry):
     return (np.pi * dr2[:, None] * np.diff(e2)[None, :]).ravel()
 
 
-def fleck_mg(T, sigma, cv, theta, dt, edges):
+def fleck_mg_gray_bg(T, sigma, cv, theta, dt):
+    """Fleck block of MG_IMC/MG_IMC2D.py:1702-1719 and the b* of 1879-1885
+    under the reference's gray B_g fallback (MG_IMC/MG_IMC2D.py:60-94,
+    MG_IMC/MG_IMC1D.py:63-94, used when planck_integrals is absent): every
+    group gets the same B = (c a / 4) T^4 / (4 pi), so sigma_P is the group
+    mean and b*_g = 1/G (0 at T = 0).  Test infrastructure for the fixtures the
+    reference writes in that mode (tests/golden/make_reference_fixtures.py)."""
+    T = _f64(T).ravel(); sig = _f64(sigma).reshape(-1, T.size); cv = _f64(cv).ravel()
+    G = sig.shape[0]
+    B = np.broadcast_to((C_LIGHT * A_RAD / 4.0) * T ** 4 / (4.0 * np.pi), sig.shape)
+    bs_sum = np.sum(B, axis=0) + 1e-300
+    sP = np.sum(sig * B, axis=0) / bs_sum
+    beta = 4.0 * A_RAD * T ** 3 / cv
+    f = np.clip(1.0 / (1.0 + theta * beta * sP * C_LIGHT * dt), 0.0, 1.0)
+    return sig * f[None, :], sig * (1.0 - f)[None, :], B / bs_sum[None, :], f
+
+
+def fleck_mg(T, sigma, cv, theta, dt, edges, gray_bg=False):
     """sigma (G, ncell) -> (sa, ss, bstar, f)."""
+    if gray_bg:
+        return fleck_mg_gray_bg(T, sigma, cv, theta, dt)
     T = _f64(T).ravel(); sig = _f64(sigma).reshape(-1, T.size); cv = _f64(cv).ravel()
     G, n = sig.shape
     sa = np.zeros_like(sig); ss = np.zeros_like(sig); bs = np.zeros_like(sig); f = np.zeros(n)
@@ -299,7 +318,8 @@ class StateMG:
 class OracleMG2D:
     """FP64 CPU restatement of the multigroup 2-D IMC step."""
 
-    def __init__(self, edges1, edges2, energy_edges, geometry="xy", seed=1):
+    def __init__(self, edges1, edges2, energy_edges, geometry="xy", seed=1, gray_bg=False):
+        self.gray_bg = bool(gray_bg)  # the reference's B_g fallback (fleck_mg_gray_bg)
         self.e1 = _f64(edges1); self.e2 = _f64(edges2); self.edges = _f64(energy_edges)
         self.nx, self.ny = self.e1.size - 1, self.e2.size - 1
         self.ncell = self.nx * self.ny
@@ -335,7 +355,7 @@ class OracleMG2D:
         T = state.temperature
         sig = np.stack([np.asarray(sigma_a_funcs[g](T), dtype=np.float64) * np.ones((nx, ny)) for g in range(G)])
         cvv = np.asarray(cv(T), dtype=np.float64) * np.ones((nx, ny))
-        sa, ss, bstar_planck, _ = fleck_mg(T, sig.reshape(G, nc), cvv, theta, dt, self.edges)
+        sa, ss, bstar_planck, _ = fleck_mg(T, sig.reshape(G, nc), cvv, theta, dt, self.edges, self.gray_bg)
         Tb = [Tb_(state.time) if callable(Tb_) else Tb_ for Tb_ in T_boundary]
         if Ntotal > 0:
             E_bc = boundary_estimate(self.geometry, self.e1, self.e2, Tb, dt, boundary_source_func)

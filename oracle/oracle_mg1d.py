@@ -199,7 +199,8 @@ class StateMG1D:
 class OracleMG1D:
     """FP64 CPU restatement of ``MG_IMC/MG_IMC1D.py`` (spherical)."""
 
-    def __init__(self, mesh, energy_edges, seed=1, nthreads=0):
+    def __init__(self, mesh, energy_edges, seed=1, nthreads=0, gray_bg=False):
+        self.gray_bg = bool(gray_bg)  # the reference's B_g fallback (oracle_mg.fleck_mg_gray_bg)
         mesh = np.asarray(mesh, dtype=np.float64)
         self.edges = np.concatenate((mesh[:, 0], mesh[-1:, 1]))
         self.I = mesh.shape[0]
@@ -241,7 +242,7 @@ class OracleMG1D:
         T = state.temperature
         sig = np.array([np.asarray(f(T), dtype=np.float64) * np.ones(I) for f in sigma_a_funcs])
         cvv = np.asarray(cv(T), dtype=np.float64) * np.ones(I)
-        sa, ss, bstar, f = OM.fleck_mg(T, sig, cvv, theta, dt, self.energy_edges)
+        sa, ss, bstar, f = OM.fleck_mg(T, sig, cvv, theta, dt, self.energy_edges, self.gray_bg)
         Ebin, Ecell = OM.emission_mg(T, sa, bstar, vol, dt, T_emit_floor)
         Tin = T_boundary[0](state.time) if callable(T_boundary[0]) else T_boundary[0]
         Tout = T_boundary[1](state.time) if callable(T_boundary[1]) else T_boundary[1]

@@ -180,6 +180,7 @@ class WaveCompareCase:
     n_boundary: int = 10_000
     n_max: int = 40_000
     cv: float = 0.3
+    geometry: str = "slab"
 
     @property
     def I(self):
@@ -199,9 +200,16 @@ class WaveCompareCase:
         return self.cv + 0.0 * T
 
 
-def marshak_wave_compare(cells=50, length=0.2):
-    """IMC/marshak_wave_profile_compare.py:65-98 with its default arguments."""
-    return WaveCompareCase(mesh=_slab_mesh(cells, length))
+def marshak_wave_compare(cells=50, length=0.2, geometry="slab"):
+    """IMC/marshak_wave_profile_compare.py:65-98 with its default arguments.
+    ``geometry="spherical"`` is the driver's ``--geometry spherical`` case
+    (lines 85-91): reflecting centre, a 1 keV bath at the outer radius."""
+    if geometry == "slab":
+        return WaveCompareCase(mesh=_slab_mesh(cells, length))
+    if geometry == "spherical":
+        return WaveCompareCase(mesh=_slab_mesh(cells, length), T_boundary=(0.0, 1.0), reflect=(True, False),
+                               geometry="spherical")
+    raise ValueError(f"geometry must be 'slab' or 'spherical', got {geometry!r}")
 
 
 def batch_slabs(P=256, seed=1, n_steps=10, scale=1.0):
@@ -304,3 +312,85 @@ def crooked_pipe_mg(n=240, n_particles=20_000_000, n_steps=20, seed=1):
                      energy_edges=np.logspace(-4, 1, 11), geometry="xy", thin=thin, cv=cv, T0=0.05, dt=0.002,
                      n_steps=n_steps, n_particles=int(n_particles), inlet=(vy[0] - 0.05, vy[0] + 0.05),
                      T_inlet=0.5)
+
+
+@dataclass
+class MGPowerLawMarshak:
+    """The reference's 10-group power-law Marshak wave
+    (MG_IMC/problems/test_marshak_wave_multigroup_powerlaw.py:149-320, 726-836,
+    ``--mode publication``): 140 uniform cells on [0, 7] cm x one cell of
+    height 1 (xy, top and bottom reflecting), rho = 0.01 g/cm^3,
+    sigma_a(T, E) = min(10 rho max(T, 1e-2)^-1/2 E^-3, 1e14) with each group
+    opacity the geometric mean of its edge values, c_v = 0.05 rho, T0 = Tr0 =
+    0.005 keV, left face ramped from 0.05 to 0.25 keV over 5 ns, right face
+    at T0, dt = 0.01 ns; 5e5 emission and 5e5 face particles per step, census
+    combed to Nmax = 2e6 growing by 1e5 per step to 2e7; T_emit_floor 0.02 keV
+    (the driver's command-line default); max_events_per_particle 1e5."""
+
+    nx: int = 140
+    n_groups: int = 10
+    rho: float = 0.01
+    dt: float = 0.01
+    T0: float = 0.005
+    Ntarget: int = 500_000
+    Nboundary: int = 500_000
+    Nmax: int = 2_000_000
+    Nmax_growth: int = 100_000
+    Nmax_final: int = 20_000_000
+    T_emit_floor: float = 0.02
+    max_events: int = 100_000
+
+    @property
+    def edges1(self):
+        return np.linspace(0.0, 7.0, self.nx + 1)
+
+    @property
+    def edges2(self):
+        return np.array([0.0, 1.0])
+
+    @property
+    def energy_edges(self):
+        return np.logspace(np.log10(1e-4), np.log10(10.0), self.n_groups + 1)
+
+    @property
+    def shape(self):
+        return (self.nx, 1)
+
+    @property
+    def cv(self):
+        return 0.05 * self.rho
+
+    def _sigma_at(self, T, E):
+        return np.minimum(10.0 * self.rho * np.maximum(T, 1e-2) ** -0.5 * E ** -3.0, 1e14)
+
+    @property
+    def sigma_a_funcs(self):
+        e = self.energy_edges
+
+        def make(lo, hi):
+            return lambda T: np.sqrt(self._sigma_at(T, lo) * self._sigma_at(T, hi))
+        return [make(e[g], e[g + 1]) for g in range(self.n_groups)]
+
+    def T_left(self, t):
+        return 0.05 + (0.25 - 0.05) * (t / 5.0) if t < 5.0 else 0.25
+
+    @property
+    def T_boundary(self):
+        return (self.T_left, self.T0, 0.0, 0.0)
+
+    @property
+    def reflect(self):
+        return (False, False, True, True)
+
+    def eos(self, T):
+        return self.cv * T
+
+    def inv_eos(self, u):
+        return u / self.cv
+
+    def cv_func(self, T):
+        return self.cv * np.ones_like(T)
+
+
+def mg_powerlaw_marshak(**kw):
+    return MGPowerLawMarshak(**kw)

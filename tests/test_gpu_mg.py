@@ -370,3 +370,27 @@ def test_mg_fp32_vs_fp64_ensemble(L, M):
     assert mask.sum() > 40
     assert np.mean(z > 3) < 0.05
     assert z.max() < 5.0
+
+
+def test_mg_device_planck_fractions_match_reference_printed_chi(L):
+    """Device b*_g against the Planck emission fractions the reference prints
+    at T = 0.05 keV from its planck_integrals module
+    (results/marshak_wave_powerlaw/imc_10g_publication/mgimc_10.out:15-25):
+    with equal group opacities the per-group emission energies are
+    proportional to b*_g."""
+    import json
+    import os
+
+    _announce("test_mg_device_planck_fractions_match_reference_printed_chi")
+    with open(os.path.join(os.path.dirname(__file__), "golden", "mgimc_10_powerlaw.json")) as fh:
+        ref = json.load(fh)
+    e1, e2 = _mesh(4, 3)
+    T = np.full((4, 3), ref["T_chi"])
+    sig = np.ones((10, 4, 3))
+    eng = L.EngineMG(e1, e2, EDGES, "xy", "fp64", seed=3)
+    mat, Et = eng.set_material(T, sig, 0.3, 1.0, 0.01, None, 0.0, want=("E",))
+    E = mat["E"].reshape(10, -1)
+    chi = E / E.sum(0)
+    for c in range(chi.shape[1]):
+        np.testing.assert_allclose(chi[:, c], ref["chi"], rtol=0, atol=5.000001e-7)
+    eng.close()

@@ -196,3 +196,46 @@ def test_mg_step_balance_and_gray_limit(M, oracle):
     se = np.sqrt(mg.var(0, ddof=1) / len(mg) + gr.var(0, ddof=1) / len(gr))
     z = np.abs(mg.mean(0) - gr.mean(0)) / np.maximum(se, 1e-300)
     assert np.max(z) < 4.5, z
+
+
+def _mgimc_10():
+    import json
+    import os
+
+    with open(os.path.join(os.path.dirname(__file__), "golden", "mgimc_10_powerlaw.json")) as fh:
+        return json.load(fh)
+
+
+def test_planck_fractions_match_reference_printed_chi(M):
+    """The reference's 10-group power-law Marshak run prints its Planck
+    emission fractions chi_g at T = 0.05 keV, computed by its planck_integrals
+    module (MG_IMC/problems/test_marshak_wave_multigroup_powerlaw.py:262-273;
+    results/marshak_wave_powerlaw/imc_10g_publication/mgimc_10.out:15-25).
+    The oracle's b*_g reproduces all ten to the printed 6 decimals."""
+    ref = _mgimc_10()
+    T = ref["T_chi"]
+    frac = np.array([M.planck_fraction(a / T, b / T) for a, b in zip(EDGES[:-1], EDGES[1:])])
+    chi = frac / (frac.sum() + 1e-300)
+    np.testing.assert_allclose(chi, ref["chi"], rtol=0, atol=5.000001e-7)
+    # the printed group edges (4 decimals) are the logspace(-4, 1, 11) of the run
+    np.testing.assert_allclose(EDGES[:-1], ref["group_lo_printed"], rtol=0, atol=5.000001e-5)
+    # b*_g through the Fleck/emission path: equal group opacities make the
+    # per-group emission energies proportional to b*_g
+    sa, ss, bs, f = M.fleck_mg(np.array([T]), np.ones((10, 1)), np.array([0.3]), 1.0, 0.01, EDGES)
+    np.testing.assert_allclose(bs[:, 0], ref["chi"], rtol=0, atol=5.000001e-7)
+    # a plausible slip -- x = E T instead of E / T -- misses the table
+    bad = np.array([M.planck_fraction(a * T, b * T) for a, b in zip(EDGES[:-1], EDGES[1:])])
+    assert np.max(np.abs(bad / bad.sum() - ref["chi"])) > 0.1
+
+
+def test_powerlaw_group_opacities_match_reference_printed(M):
+    """Geometric-mean group opacities of the power-law material at 0.05 keV
+    (mgimc_10.out:16-25): the problem generator's opacity reproduces the
+    printed values to their 4 digits."""
+    from radtranbook_amd.problems import mg_powerlaw_marshak
+
+    ref = _mgimc_10()
+    prob = mg_powerlaw_marshak()
+    T = np.full(prob.shape, ref["T_chi"])
+    sig = np.array([float(np.mean(f(T))) for f in prob.sigma_a_funcs])
+    np.testing.assert_allclose(sig, ref["sigma_a"], rtol=6e-4)

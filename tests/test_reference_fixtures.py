@@ -120,3 +120,146 @@ def test_oracle_slab_matches_reference_ensemble(oracle):
     print("slab", {k: float(v.max()) for k, v in zs.items()}, "frac z>3", np.mean(allz > 3.0))
     assert allz.max() < 5.0, {k: float(v.max()) for k, v in zs.items()}
     assert np.mean(allz > 3.0) < 0.02
+
+
+def _stats(rows, nsteps):
+    res = {}
+    for k, name in enumerate(["T", "Tr", "E_total", "E_rad", "boundary_loss", "N"]):
+        arr = np.array([[r[s][k] for s in range(nsteps)] for r in rows], dtype=np.float64)
+        res[name + "_mean"] = arr.mean(0)
+        res[name + "_sem"] = arr.std(0, ddof=1) / np.sqrt(arr.shape[0])
+    return res
+
+
+def _row(info):
+    return (np.array(info["temperature"], dtype=np.float64), np.array(info["radiation_temperature"], dtype=np.float64),
+            info["total_energy"], info["total_radiation_energy"], info["boundary_loss"], info["N_particles"])
+
+
+def _check(ref, got, label, names=("T", "Tr", "E_total", "E_rad", "boundary_loss"), count_slack=2):
+    zs = {name: _compare(ref, got, name) for name in names}
+    allz = np.concatenate([z.ravel() for z in zs.values()])
+    print(label, {k: float(v.max()) for k, v in zs.items()}, "frac z>3", np.mean(allz > 3.0))
+    assert allz.max() < 5.0, {k: float(v.max()) for k, v in zs.items()}
+    assert np.mean(allz > 3.0) < 0.02
+    assert np.all(np.abs(got["N_mean"] - ref["N_mean"])
+                  < 5 * np.sqrt(ref["N_sem"] ** 2 + got["N_sem"] ** 2) + count_slack)
+
+
+def _sphere_rows(oracle, p, seed0):
+    I = p["I"]
+    edges = np.linspace(0, p["R"], I + 1)
+    mesh = np.column_stack([edges[:-1], edges[1:]])
+    cv, s0 = p["cv"], p["s0"]
+    sf = lambda T: s0 * np.maximum(T, 1e-3) ** -3
+    T0 = np.full(I, p["T0"])
+    rows = []
+    for seed in range(SEEDS):
+        orc = oracle.Oracle1D(mesh, "spherical", seed=seed0 + seed, nthreads=1)
+        st = orc.init_simulation(p["N"], T0, T0, lambda T: cv * T, lambda u: u / cv)
+        out = []
+        for _ in range(p["steps"]):
+            st, info = orc.step(st, p["N"], p["Nb"], 0, p["Nmax"], (0.0, p["Tb"]), p["dt"], sf, lambda u: u / cv,
+                                lambda T: cv + 0 * T, np.zeros(I), reflect=(False, False))
+            out.append(_row(info))
+        rows.append(out)
+    return _stats(rows, p["steps"])
+
+
+def test_oracle_sphere_matches_reference_ensemble(oracle):
+    """Gray 1-D spherical step (IMC/IMC1D.py:908-1124 with the spherical
+    sourcing and tracker, 336-763) against the reference's own ensemble."""
+    ref = dict(np.load(os.path.join(HERE, "golden", "ref_imc1d_sphere.npz")))
+    _check(ref, _sphere_rows(oracle, F.SPH, 700), "sphere")
+
+
+def _mg2d_rows(p, seed0, gray_bg=True):
+    from oracle import oracle_mg as OM
+
+    e1 = np.linspace(0, p["Lx"], p["nx"] + 1)
+    e2 = np.linspace(0, p["Ly"], p["ny"] + 1)
+    edges = np.array(F.MG_EDGES)
+    sfs = F.mg_sigma_funcs(F.sigma0_mg2d())
+    cv = p["cv"]
+    T0 = np.full((p["nx"], p["ny"]), p["T0"])
+    rows = []
+    for seed in range(SEEDS):
+        orc = OM.OracleMG2D(e1, e2, edges, "xy", seed=seed0 + seed, gray_bg=gray_bg)
+        st = orc.init_simulation(p["N"], T0, T0, lambda T: cv * T, lambda u: u / cv)
+        out = []
+        for _ in range(p["steps"]):
+            st, info = orc.step(st, p["N"], p["Nb"], 0, p["Nmax"], (p["Tb"], 0.0, 0.0, 0.0), p["dt"], sfs,
+                                lambda u: u / cv, lambda T: cv + 0 * T, np.zeros_like(T0),
+                                reflect=(False, False, True, True))
+            out.append(_row(info))
+        rows.append(out)
+    return _stats(rows, p["steps"])
+
+
+def test_oracle_mg2d_matches_reference_ensemble(oracle):
+    """Multigroup 2-D xy step (MG_IMC/MG_IMC2D.py:1635-2122) with three groups
+    of different opacity, against the reference's ensemble in its gray-B_g
+    import mode (the oracle takes the same B_g, oracle_mg.fleck_mg_gray_bg)."""
+    ref = dict(np.load(os.path.join(HERE, "golden", "ref_mgimc2d_xy.npz")))
+    _check(ref, _mg2d_rows(F.MG2D, 900), "mg2d")
+
+
+def _mg1d_rows(p, seed0):
+    from oracle import oracle_mg1d as OM1
+
+    I = p["I"]
+    edges = np.linspace(0, p["R"], I + 1)
+    mesh = np.column_stack([edges[:-1], edges[1:]])
+    sfs = F.mg_sigma_funcs(p["s0"] * np.linspace(1.0, 2.0, I))
+    cv = p["cv"]
+    T0 = np.full(I, p["T0"])
+    rows = []
+    for seed in range(SEEDS):
+        orc = OM1.OracleMG1D(mesh, np.array(F.MG_EDGES), seed=seed0 + seed, nthreads=1, gray_bg=True)
+        st = orc.init_simulation(p["N"], T0, T0, lambda T: cv * T, lambda u: u / cv)
+        out = []
+        for _ in range(p["steps"]):
+            st, info = orc.step(st, p["N"], p["Nb"], 0, p["Nmax"], (0.0, p["Tb"]), p["dt"], sfs, lambda u: u / cv,
+                                lambda T: cv + 0 * T, np.zeros(I), reflect=(False, False),
+                                conserve_comb_energy=p["conserve"], particle_budget_fmin=p["fmin"])
+            out.append(_row(info))
+        rows.append(out)
+    return _stats(rows, p["steps"])
+
+
+@pytest.mark.parametrize("name,seed0", [("budget", 1100), ("radcomb", 1400)])
+def test_oracle_mg1d_matches_reference_ensemble(oracle, name, seed0):
+    """Multigroup spherical step (MG_IMC/MG_IMC1D.py:917-1309) against the
+    reference's ensembles: energy-proportional particle budget with the comb
+    discrepancy into the material ("budget"), and fixed counts with the
+    threshold comb rescaling survivors per bin ("radcomb")."""
+    p = F.MG1D_BUDGET if name == "budget" else F.MG1D_RADCOMB
+    ref = dict(np.load(os.path.join(HERE, "golden", f"ref_mgimc1d_{name}.npz")))
+    _check(ref, _mg1d_rows(p, seed0), "mg1d_" + name)
+
+
+def _max_z(ref, got, names=("T", "Tr", "E_total", "E_rad", "boundary_loss")):
+    return max(float(_compare(ref, got, name).max()) for name in names)
+
+
+@pytest.mark.parametrize("case", ["sphere_Tb", "mg2d_Tb", "mg2d_planck_bg", "mg1d_budget_Tb", "mg1d_radcomb_dt"])
+def test_fixtures_catch_planted_mistakes(oracle, case):
+    """Each composed-step fixture is sharp enough to reject a plausible slip
+    planted in the oracle's inputs: a 4% face temperature (17% more face
+    energy), the Planck B_g in place of the reference's gray fallback, a 5%
+    longer time step.  The unmodified oracle passes the same comparison
+    (the tests above), so max |z| >= 5 here means the fixture sees the slip."""
+    if case == "sphere_Tb":
+        p = dict(F.SPH, Tb=F.SPH["Tb"] * 1.04)
+        got, ref = _sphere_rows(oracle, p, 700), "ref_imc1d_sphere"
+    elif case == "mg2d_Tb":
+        got, ref = _mg2d_rows(dict(F.MG2D, Tb=F.MG2D["Tb"] * 1.04), 900), "ref_mgimc2d_xy"
+    elif case == "mg2d_planck_bg":
+        got, ref = _mg2d_rows(F.MG2D, 900, gray_bg=False), "ref_mgimc2d_xy"
+    elif case == "mg1d_budget_Tb":
+        got, ref = _mg1d_rows(dict(F.MG1D_BUDGET, Tb=F.MG1D_BUDGET["Tb"] * 1.04), 1100), "ref_mgimc1d_budget"
+    else:
+        got, ref = _mg1d_rows(dict(F.MG1D_RADCOMB, dt=F.MG1D_RADCOMB["dt"] * 1.05), 1400), "ref_mgimc1d_radcomb"
+    z = _max_z(dict(np.load(os.path.join(HERE, "golden", ref + ".npz"))), got)
+    print(case, "max z", z)
+    assert z >= 5.0, z
