@@ -121,6 +121,22 @@ int imc2d_transport(imc2d_ctx* ctx, double dt, const int32_t* reflect, int64_t m
                     double* dep_out, double* si_out, double* bl_out, int64_t* stats_out);
 /* Device time of the last transport kernel in milliseconds (HIP events). */
 float imc2d_last_transport_ms(imc2d_ctx* ctx);
+/* imc2d_transport without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The same
+ * tracker and the same results as imc2d_transport over the resident pool,
+ * with every scratch buffer (tally partials, counters, pickup order,
+ * weight-sum partials) in one fixed 32 MiB device array per GPU instead of
+ * buffers the context grows.  dep_dev and si_dev (ncell doubles each,
+ * required), bl_dev (1 double) and stats_dev (8 int64, as imc2d_transport's
+ * stats) are DEVICE pointers the caller owns.  Fails with an error when the
+ * scratch needed, 16 B x cells x tally copies plus 5 B per 256-particle
+ * chunk and ~10 KB, exceeds 32 MiB, and on a context with the adjoint
+ * replay enabled.  Calls share their GPU's array: no-allocation calls that
+ * overlap on one GPU race.  Synchronises the context's stream. */
+int imc2d_transport_noalloc(imc2d_ctx* ctx, double dt, const int32_t* reflect, int64_t max_events,
+                            double weight_floor, double* dep_dev, double* si_dev, double* bl_dev,
+                            int64_t* stats_dev);
 
 /* ---- Carter-Forest variant (IMC/IMC2D_CarterForest.py), FP64 contexts ---- */
 /* Material for the Carter-Forest step (IMC/IMC2D_CarterForest.py:948-949):

@@ -50,6 +50,7 @@ SIGNATURES = {
     "imc2d_sample_boundary": (C.c_int, [_vp, C.c_int, C.c_int64, C.c_double, C.c_double, _dp, C.c_int]),
     "imc2d_sample_volume": (C.c_int, [_vp, C.c_int, C.c_int64, _dp, C.c_double, C.c_int, _dp, _dp, _i64p]),
     "imc2d_transport": (C.c_int, [_vp, C.c_double, _i32p, C.c_int64, C.c_double, _dp, _dp, _dp, _i64p]),
+    "imc2d_transport_noalloc": (C.c_int, [_vp, C.c_double, _i32p, C.c_int64, C.c_double, _vp, _vp, _vp, _vp]),
     "imc2d_set_material_cf": (C.c_int, [_vp, _dp, _dp, _dp]),
     "imc2d_transport_cf": (C.c_int, [_vp, C.c_double, _i32p, C.c_int64, C.c_double, C.c_double, C.c_double,
                                      C.c_int64, _dp, _dp, _dp, _i64p]),
@@ -266,6 +267,15 @@ class Engine2D:
         check(self._L.imc2d_transport(self._h, float(dt), _p(refl, _i32p), int(max_events), float(weight_floor),
                                       _p(dep), _p(si), C.byref(bl), _p(st, _i64p)))
         return dep, si, bl.value, st
+
+    def transport_noalloc(self, dt, reflect, max_events, weight_floor, dep_dev, si_dev, bl_dev=0, stats_dev=0):
+        """``imc2d_transport_noalloc``: the tracker with the library's fixed
+        device scratch; the outputs are device addresses (ints) the caller
+        owns: dep and si (ncell doubles), bl (1 double), stats (8 int64)."""
+        refl = np.array([1 if r else 0 for r in reflect], dtype=np.int32)
+        check(self._L.imc2d_transport_noalloc(self._h, float(dt), _p(refl, _i32p), int(max_events),
+                                              float(weight_floor), _vp(dep_dev), _vp(si_dev), _vp(bl_dev or None),
+                                              _vp(stats_dev or None)))
 
     def set_material_cf(self, T, sigma_a, beta):
         """Carter-Forest material: T, the true sigma_a and beta per cell."""

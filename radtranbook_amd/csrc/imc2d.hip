@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "../../include/imc_hip.h"
@@ -1349,8 +1350,18 @@ static int pool_ensure(Imc2dCtx* c, int64_t n_total) { return c->A.grow(n_total,
 // Pickup order of the gray tracker or its adjoint replay (beta null) or of
 // the Carter-Forest tracker over pool P: chunks of `batch` particles, longest
 // expected histories first (pool order when the batch is not a power of two).
+// Chunk buffers of the pickup order: per-chunk keys, the permutation and the
+// per-workgroup key histograms (kChunkGrid x kChunkKeys).
+struct ChunkBufs {
+  uint8_t* key;
+  int32_t* ord;
+  int32_t* hist;
+};
+
+// `ext` (nullable) supplies buffers of at least ceil(n / batch) chunks; without
+// it the context's buffers grow as needed.
 static int longest_first_queue(Imc2dCtx* c, PoolBuf& P, int64_t n, double dt, QueueMap* Q,
-                               const double* beta = nullptr) {
+                               const double* beta = nullptr, const ChunkBufs* ext = nullptr) {
   *Q = pool_order_queue(n);
   const int batch = c->batch;
   if (n <= 0 || (batch & (batch - 1)) != 0) return kOk;
@@ -1358,20 +1369,107 @@ static int longest_first_queue(Imc2dCtx* c, PoolBuf& P, int64_t n, double dt, Qu
   while ((1 << shift) < batch) ++shift;
   const int64_t nchunk = (n + batch - 1) >> shift;
   int rc;
-  if ((rc = c->chunk_key.reserve(nchunk)) || (rc = c->chunk_ord.reserve(nchunk)) ||
-      (rc = c->chunk_hist.reserve((int64_t)kChunkGrid * kChunkKeys)))
-    return rc;
+  ChunkBufs B;
+  if (ext) {
+    B = *ext;
+  } else {
+    if ((rc = c->chunk_key.reserve(nchunk)) || (rc = c->chunk_ord.reserve(nchunk)) ||
+        (rc = c->chunk_hist.reserve((int64_t)kChunkGrid * kChunkKeys)))
+      return rc;
+    B = ChunkBufs{c->chunk_key.p, c->chunk_ord.p, c->chunk_hist.p};
+  }
   hipStream_t s = c->stream;
   const int g = chunk_grid(nchunk);
   if (c->precision == IMC_FP64)
     k_chunk_key<double><<<g, kChunkBlock, 0, s>>>(P.view<double>(), n, shift, c->nx, c->ny, c->e1, c->e2,
-                                                  c->sa.p, c->ss.p, beta, dt, c->chunk_key.p, c->chunk_hist.p);
+                                                  c->sa.p, c->ss.p, beta, dt, B.key, B.hist);
   else
     k_chunk_key<float><<<g, kChunkBlock, 0, s>>>(P.view<float>(), n, shift, c->nx, c->ny, c->e1, c->e2,
-                                                 c->sa.p, c->ss.p, beta, dt, c->chunk_key.p, c->chunk_hist.p);
+                                                 c->sa.p, c->ss.p, beta, dt, B.key, B.hist);
   IMC_HIP(hipGetLastError());
-  if ((rc = chunk_order(c->chunk_key.p, nchunk, g, c->chunk_hist.p, c->chunk_ord.p, s))) return rc;
-  *Q = QueueMap{c->chunk_ord.p, nchunk << shift, shift};
+  if ((rc = chunk_order(B.key, nchunk, g, B.hist, B.ord, s))) return rc;
+  *Q = QueueMap{B.ord, nchunk << shift, shift};
+  return kOk;
+}
+
+// ---------------------------------------------------------------------------
+// Scratch of imc2d_transport_noalloc: one file-scope device array, which each
+// GPU holds (at its own address) from the library's first use on it until
+// the process ends.  imc2d_create looks the address up once per device, with
+// that device current; host code uses only that address, never the array's
+// name.  Every no-allocation call on a GPU shares its array, so such calls
+// that overlap on one GPU race.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocScratchBytes = (size_t)32 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+constexpr int kMaxScratchDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxScratchDevices];
+static hipError_t g_scratch_err[kMaxScratchDevices];
+
+// With `device` current: the array's device address, once per device.  A
+// failed lookup is kept and reported by the no-allocation entry point.
+static void resolve_scratch(int device) {
+  if (device < 0 || device >= kMaxScratchDevices) return;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return;
+  void* p = nullptr;
+  hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    g_scratch_err[device] = e != hipSuccess ? e : hipErrorInvalidSymbol;
+    return;
+  }
+  g_scratch_err[device] = hipSuccess;
+  g_scratch_ptr[device] = p;
+}
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// Byte offsets of the transport's scratch inside the array.
+struct NoAllocLayout {
+  size_t dep_raw, si_raw, stats, next, scal, tree, key, ord, hist, total;
+};
+
+static NoAllocLayout noalloc_layout(int64_t ncell, int copies, int64_t nchunk) {
+  NoAllocLayout L{};
+  size_t o = 0;
+  auto take = [&o](size_t bytes) {
+    size_t at = o;
+    o = align256(o + bytes);
+    return at;
+  };
+  L.dep_raw = take(sizeof(double) * (size_t)ncell * (size_t)copies);
+  L.si_raw = take(sizeof(double) * (size_t)ncell * (size_t)copies);
+  L.stats = take(sizeof(unsigned long long) * 8);
+  L.next = take(sizeof(unsigned long long));
+  L.scal = take(sizeof(double) * 4);  // [1] weight sum, [2] floor, [3] boundary loss
+  L.tree = take(sizeof(double) * 1024);
+  L.key = take((size_t)nchunk);
+  L.ord = take(sizeof(int32_t) * (size_t)nchunk);
+  L.hist = take(sizeof(int32_t) * (size_t)kChunkGrid * kChunkKeys);
+  L.total = o;
+  return L;
+}
+
+__global__ void k_zero_u64(unsigned long long* __restrict__ p, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0ull;
+}
+
+__global__ void k_store_f64(double* __restrict__ p, double v) { *p = v; }
+
+__global__ void k_noalloc_outputs(const double* __restrict__ bl_in, const unsigned long long* __restrict__ st_in,
+                                  double* __restrict__ bl_out, int64_t* __restrict__ st_out) {
+  int a = threadIdx.x;
+  if (a == 0 && bl_out) *bl_out = *bl_in;
+  if (a < 8 && st_out) st_out[a] = (int64_t)st_in[a];
+}
+
+static int zero_words(void* p, size_t bytes, hipStream_t s) {
+  int64_t n = (int64_t)(bytes / 8);
+  if (n <= 0) return kOk;
+  k_zero_u64<<<(int)std::min<int64_t>(launch_blocks(n, 256), 1024), 256, 0, s>>>((unsigned long long*)p, n);
+  IMC_HIP(hipGetLastError());
   return kOk;
 }
 
@@ -1423,6 +1521,7 @@ int imc2d_create(imc2d_ctx** out, int geometry, int nx, int ny, const double* ed
   IMC_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   IMC_HIP(hipEventCreate(&c->ev0));
   IMC_HIP(hipEventCreate(&c->ev1));
+  resolve_scratch(device);  // the no-allocation transport's scratch on this device
   c->h_e1.assign(edges1, edges1 + nx + 1);
   c->h_e2.assign(edges2, edges2 + ny + 1);
   IMC_HIP(hipMalloc(&c->e1, sizeof(double) * (nx + 1)));
@@ -1814,6 +1913,29 @@ int imc2d_sample_volume(imc2d_ctx* c, int kind, int64_t N, const double* QV_host
   return kOk;
 }
 
+// Workgroups of the gray tracker: the override, else every CU filled to the
+// kernel's occupancy, capped by the pool size.
+static int transport_grid(imc2d_ctx* c, int64_t n, int block, int* grid_out) {
+  int grid;
+  if (c->grid_override > 0) {
+    grid = c->grid_override;
+  } else {
+    int per_cu = 0;
+    const void* fn = c->precision == IMC_FP64
+                         ? (c->geometry == IMC_GEOM_XY ? (const void*)k_transport<double, 0, uint32_t>
+                                                       : (const void*)k_transport<double, 1, uint32_t>)
+                         : (c->geometry == IMC_GEOM_XY ? (const void*)k_transport<float, 0, uint32_t>
+                                                       : (const void*)k_transport<float, 1, uint32_t>);
+    IMC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, block, 0));
+    if (per_cu < 1) per_cu = 1;
+    grid = per_cu * c->n_cu;
+  }
+  int64_t need_blocks = (n + block - 1) / block;
+  if (need_blocks < grid) grid = (int)std::max<int64_t>(need_blocks, 1);
+  *grid_out = grid;
+  return kOk;
+}
+
 // Transport the whole pool (IMC/IMC2D.py:938-1107).  weight_floor < 0 selects
 // 1e-10 * sum(w) / N computed on the device (IMC/IMC2D.py:1771).
 int imc2d_transport(imc2d_ctx* c, double dt, const int32_t* reflect, int64_t max_events, double weight_floor,
@@ -1837,23 +1959,9 @@ int imc2d_transport(imc2d_ctx* c, double dt, const int32_t* reflect, int64_t max
   }
   int refl = 0;
   for (int a = 0; a < 4; ++a) refl |= (reflect[a] ? 1 : 0) << a;
-  int block = 256;
+  const int block = 256;
   int grid;
-  if (c->grid_override > 0) {
-    grid = c->grid_override;
-  } else {
-    int per_cu = 0;
-    const void* fn = c->precision == IMC_FP64
-                         ? (c->geometry == IMC_GEOM_XY ? (const void*)k_transport<double, 0, uint32_t>
-                                                       : (const void*)k_transport<double, 1, uint32_t>)
-                         : (c->geometry == IMC_GEOM_XY ? (const void*)k_transport<float, 0, uint32_t>
-                                                       : (const void*)k_transport<float, 1, uint32_t>);
-    IMC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, block, 0));
-    if (per_cu < 1) per_cu = 1;
-    grid = per_cu * c->n_cu;
-  }
-  int64_t need_blocks = (n + block - 1) / block;
-  if (need_blocks < grid) grid = (int)std::max<int64_t>(need_blocks, 1);
+  if ((rc = transport_grid(c, n, block, &grid))) return rc;
   Key key = make_key(c->seed);
   if (c->adjoint) {
     // keep the transport's input pool for the adjoint replay
@@ -1919,6 +2027,102 @@ int imc2d_transport(imc2d_ctx* c, double dt, const int32_t* reflect, int64_t max
 }
 
 float imc2d_last_transport_ms(imc2d_ctx* c) { return c ? c->last_transport_ms : -1.0f; }
+
+// imc2d_transport without any allocation: the same tracker over the resident
+// pool, with its tally partials, counters, pickup-order buffers and
+// weight-sum partials in the library's fixed device scratch (kNoAllocScratchBytes
+// = 32 MiB per GPU) instead of buffers the context grows; results go to caller
+// device memory.  Cleared with kernels, no memset or host staging.
+int imc2d_transport_noalloc(imc2d_ctx* c, double dt, const int32_t* reflect, int64_t max_events,
+                            double weight_floor, double* dep_dev, double* si_dev, double* bl_dev,
+                            int64_t* stats_dev) {
+  IMC_REQUIRE(c && reflect && max_events >= 0 && dep_dev && si_dev, kErrArg,
+              "imc2d_transport_noalloc: bad arguments (dep and si outputs are required)");
+  IMC_REQUIRE(!c->adjoint, kErrState,
+              "imc2d_transport_noalloc: the adjoint replay keeps a copy of the pool; use imc2d_transport");
+  IMC_HIP(hipSetDevice(c->device));
+  void* base = nullptr;
+  hipError_t lookup = hipErrorInvalidSymbol;
+  if (c->device >= 0 && c->device < kMaxScratchDevices) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    base = g_scratch_ptr[c->device];
+    lookup = g_scratch_err[c->device];
+  }
+  IMC_REQUIRE(base, kErrState, "imc2d_transport_noalloc: no scratch address for device %d (%s)", c->device,
+              hipGetErrorString(lookup));
+  hipStream_t s = c->stream;
+  const int64_t n = c->n;
+  const int64_t nchunk = (n + c->batch - 1) / std::max(c->batch, 1);
+  const NoAllocLayout L = noalloc_layout(c->ncell, c->tally_copies, nchunk);
+  IMC_REQUIRE(L.total <= kNoAllocScratchBytes, kErrArg,
+              "imc2d_transport_noalloc: needs %zu bytes of scratch (%lld cells x %d tally copies, %lld particles), "
+              "more than its fixed %zu bytes; use imc2d_transport or fewer tally copies",
+              L.total, (long long)c->ncell, c->tally_copies, (long long)n, kNoAllocScratchBytes);
+  char* b = (char*)base;
+  double* dep_raw = (double*)(b + L.dep_raw);
+  double* si_raw = (double*)(b + L.si_raw);
+  unsigned long long* stats = (unsigned long long*)(b + L.stats);
+  unsigned long long* next = (unsigned long long*)(b + L.next);
+  double* scal = (double*)(b + L.scal);
+  double* tree = (double*)(b + L.tree);
+  const ChunkBufs chunks{(uint8_t*)(b + L.key), (int32_t*)(b + L.ord), (int32_t*)(b + L.hist)};
+  const size_t tally_bytes = sizeof(double) * (size_t)c->ncell * (size_t)c->tally_copies;
+  int rc;
+  // the previous call's contents stay in the array: clear what starts at zero
+  if ((rc = zero_words(dep_raw, tally_bytes, s)) || (rc = zero_words(si_raw, tally_bytes, s)) ||
+      (rc = zero_words(stats, 64, s)) || (rc = zero_words(next, 8, s)) || (rc = zero_words(scal, 32, s)))
+    return rc;
+  if (weight_floor < 0.0) {
+    if ((rc = tree_sum(c->A.w, n, scal + 1, tree, s))) return rc;
+    k_weight_floor<<<1, 1, 0, s>>>(scal + 1, n, scal + 2);
+  } else {
+    k_store_f64<<<1, 1, 0, s>>>(scal + 2, weight_floor);
+  }
+  IMC_HIP(hipGetLastError());
+  int refl = 0;
+  for (int a = 0; a < 4; ++a) refl |= (reflect[a] ? 1 : 0) << a;
+  const int block = 256;
+  int grid;
+  if ((rc = transport_grid(c, n, block, &grid))) return rc;
+  QueueMap Q;
+  if ((rc = longest_first_queue(c, c->A, n, dt, &Q, nullptr, &chunks))) return rc;
+  Key key = make_key(c->seed);
+  IMC_HIP(hipEventRecord(c->ev0, s));
+  if (n > 0) {
+    const bool narrow = n <= kNarrowMax;
+    if (c->precision == IMC_FP64) {
+      TransportArgs<double> A{c->A.view<double>(), n, c->e1p64.p, c->e2p64.p, c->nx, c->ny, c->sas64.p, dt, refl,
+                              max_events, scal + 2, key, c->step, next, c->batch, dep_raw, si_raw,
+                              c->ncell, c->tally_copies - 1, scal + 3, stats, Q};
+      if (narrow)
+        launch_transport<double, uint32_t>(c->geometry, grid, block, s, A);
+      else
+        launch_transport<double, uint64_t>(c->geometry, grid, block, s, A);
+    } else {
+      TransportArgs<float> A{c->A.view<float>(), n, c->e1p32.p, c->e2p32.p, c->nx, c->ny, c->sas32.p, dt, refl,
+                             max_events, scal + 2, key, c->step, next, c->batch, dep_raw, si_raw,
+                             c->ncell, c->tally_copies - 1, scal + 3, stats, Q};
+      if (narrow)
+        launch_transport<float, uint32_t>(c->geometry, grid, block, s, A);
+      else
+        launch_transport<float, uint64_t>(c->geometry, grid, block, s, A);
+    }
+    IMC_HIP(hipGetLastError());
+  }
+  IMC_HIP(hipEventRecord(c->ev1, s));
+  const bool f32 = c->precision == IMC_FP32_TRACK;
+  k_finalize_tallies<<<launch_blocks(c->ncell, 256), 256, 0, s>>>(c->ncell, c->tally_copies, dep_raw, si_raw,
+                                                                  c->sa.p, f32 ? c->saf.p : nullptr, c->vol, dt,
+                                                                  dep_dev, si_dev);
+  IMC_HIP(hipGetLastError());
+  if (bl_dev || stats_dev) {
+    k_noalloc_outputs<<<1, 64, 0, s>>>(scal + 3, stats, bl_dev, stats_dev);
+    IMC_HIP(hipGetLastError());
+  }
+  IMC_HIP(hipStreamSynchronize(s));
+  IMC_HIP(hipEventElapsedTime(&c->last_transport_ms, c->ev0, c->ev1));
+  return kOk;
+}
 
 // Carter-Forest material (IMC/IMC2D_CarterForest.py:948-949): T (for the rz
 // linear emission tilt), the true sigma_a and beta = 4 a max(T,1e-12)^3 / c_v,
