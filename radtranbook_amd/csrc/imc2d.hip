@@ -409,6 +409,42 @@ struct TransportArgs {
   QueueMap Q;
 };
 
+// Finished histories of a wave are staged in LDS and written back to their
+// pool slots in bulk.  A wave's vector-memory counter covers stores as well as
+// loads, so a store issued at every finish made the next segment's material
+// gathers wait for that store to complete (200x200 heterogeneous Marshak,
+// FP32 tracker: 2.60 ms with a store at each finish against 2.14 ms with the
+// stores left out).  Each of the four waves of a workgroup owns kStageSlots
+// entries; the wave flushes at the top of its loop once kStageFlushAt wait,
+// so at most kStageFlushAt - 1 + 64 are ever held.
+constexpr int kStageSlots = 96;
+constexpr int kStageFlushAt = 32;
+static_assert(kStageFlushAt - 1 + 64 <= kStageSlots, "a full wave of finishes must fit after a flush check");
+
+template <typename R, typename Off>
+struct CensusStage {
+  double w[4][kStageSlots];
+  R d1[4][kStageSlots], d2[4][kStageSlots], p1[4][kStageSlots], p2[4][kStageSlots];
+  int32_t ci[4][kStageSlots], cj[4][kStageSlots];
+  Off k[4][kStageSlots];
+  int n[4];
+};
+
+// Every lane of wave `wv` calls it with the same nb.
+template <typename R, typename Off>
+__device__ __forceinline__ void stage_flush(CensusStage<R, Off>& S, int wv, int nb, const PoolView<R>& P) {
+  for (int e = __lane_id(); e < nb; e += 64) {
+    const Off ko = S.k[wv][e];
+    elem(P.w, ko) = S.w[wv][e];
+    elem(P.d1, ko) = S.d1[wv][e];
+    elem(P.d2, ko) = S.d2[wv][e];
+    elem(P.p1, ko) = S.p1[wv][e];
+    elem(P.p2, ko) = S.p2[wv][e];
+    elem(P.ci, ko) = S.ci[wv][e];
+    elem(P.cj, ko) = S.cj[wv][e];
+  }
+}
+
 // Off: offset type of the particle-array accesses (uint32_t while every
 // byte offset fits 32 bits, else uint64_t).
 template <typename R, int GEOM, typename Off>
@@ -421,7 +457,10 @@ __global__ __launch_bounds__(256) void k_transport(TransportArgs<R> A) {
   double* const si_t = A.si_raw + copy_off;
   __shared__ int lkey[kLdsSlots];
   __shared__ double lval[kLdsSlots];
+  __shared__ CensusStage<R, Off> stg;
+  const int wv = threadIdx.x >> 6;
   lds_tally_init(lkey, lval);
+  if (__lane_id() == 0) stg.n[wv] = 0;
   __syncthreads();
   const int ny = A.ny, nx = A.nx;
   WaveQueue wq;
@@ -437,6 +476,13 @@ __global__ __launch_bounds__(256) void k_transport(TransportArgs<R> A) {
   uint32_t st_ev = 0, st_bnd = 0, st_sc = 0, st_cen = 0, st_floor = 0, st_refl = 0, st_cap = 0;
 
   while (true) {
+    {
+      const int nb = stg.n[wv];
+      if (nb >= kStageFlushAt) {
+        stage_flush(stg, wv, nb, A.P);
+        stg.n[wv] = 0;
+      }
+    }
     uint64_t need = __ballot(!active);
     if (need) {
       const int64_t kk = wave_take(wq, !active, need, A.next, A.batch, A.Q, A.n);
@@ -532,17 +578,24 @@ __global__ __launch_bounds__(256) void k_transport(TransportArgs<R> A) {
       }
     }
     if (finish) {
-      const Off ko = (Off)k;
-      elem(A.P.w, ko) = w;
-      elem(A.P.d1, ko) = d1;
-      elem(A.P.d2, ko) = d2;
-      elem(A.P.p1, ko) = p1;
-      elem(A.P.p2, ko) = p2;
-      elem(A.P.ci, ko) = i;
-      elem(A.P.cj, ko) = j;
+      // stage the history for the bulk write-back (slots after the ones
+      // already waiting, in lane order)
+      const uint64_t fm = __ballot(true);
+      const int nb = stg.n[wv];
+      const int slot = nb + (int)__popcll(fm & lanemask_lt());
+      stg.w[wv][slot] = w;
+      stg.d1[wv][slot] = d1;
+      stg.d2[wv][slot] = d2;
+      stg.p1[wv][slot] = p1;
+      stg.p2[wv][slot] = p2;
+      stg.ci[wv][slot] = i;
+      stg.cj[wv][slot] = j;
+      stg.k[wv][slot] = (Off)k;
+      if (__lane_id() == (int)__builtin_ctzll(fm)) stg.n[wv] = nb + (int)__popcll(fm);
       active = false;
     }
   }
+  stage_flush(stg, wv, stg.n[wv], A.P);
   if (acc_cell >= 0) {
     if (!lds_tally_add(lkey, lval, acc_cell, acc_dep)) atomic_add_f64(&elem(dep_t, (uint32_t)acc_cell), acc_dep);
     if (acc_si != 0.0) atomic_add_f64(&elem(si_t, (uint32_t)acc_cell), acc_si);

@@ -239,7 +239,7 @@ __device__ __forceinline__ void move_xy(Ctr ctr, Key key, uint32_t ev, double w,
   float sx = (ux != 0.0f) ? fmaxf(fdiv_fast((ux > 0.0f ? xr : xl) - x, ux), 0.0f) : 1e30f;
   float sy = (uy != 0.0f) ? fmaxf(fdiv_fast((uy > 0.0f ? yr : yl) - y, uy), 0.0f) : 1e30f;
   float cl = CAP ? fminf(xr - xl, yr - yl) : 0.0f;
-  QuadF u = (ss > 1e-12f) ? block4_f(ctr, key, ev) : QuadF{0.0f, 0.0f, 0.0f, 0.0f};
+  QuadF u = block4_f(ctr, key, ev);  // independent of the material gathers: overlaps their latency
   float s_scat = scatter_distance_f32(u, ss, cl);
   float sb = fminf(sx, sy);
   o.crossed = kCrossNone;
@@ -313,7 +313,7 @@ __device__ __forceinline__ void move_rz(Ctr ctr, Key key, uint32_t ev, double w,
   else if (eta < 0.0f)
     s_zlo = fmaxf(fdiv_fast(zlo - z, eta), 0.0f);
   float cl = CAP ? fminf(rout - rin, zhi - zlo) : 0.0f;
-  QuadF u = (ss > 1e-12f) ? block4_f(ctr, key, ev) : QuadF{0.0f, 0.0f, 0.0f, 0.0f};
+  QuadF u = block4_f(ctr, key, ev);  // independent of the material gathers: overlaps their latency
   float s_scat = scatter_distance_f32(u, ss, cl);
   float sr = fminf(s_rin, s_rout), sz = fminf(s_zlo, s_zhi);
   float sb = fminf(sr, sz);

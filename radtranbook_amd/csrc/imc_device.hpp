@@ -53,6 +53,12 @@ struct U4 {
 __device__ __forceinline__ U4 philox10(Ctr c, uint32_t block, Key k) {
   uint32_t x0 = c.c0, x1 = c.c1, x2 = c.c2, x3 = block;
   uint32_t k0 = k.k0, k1 = k.k1;
+  // The key is uniform (a kernel argument).  Left visible, the compiler
+  // hoists the 20 round keys out of a tracker's loop and, short of scalar
+  // registers, parks them in vector-register lanes: two v_readlane and an
+  // extra xor per round.  Made opaque here, the round keys are rebuilt with
+  // scalar adds at every call and each xor takes its key as a scalar operand.
+  asm volatile("" : "+s"(k0), "+s"(k1));
 #pragma unroll
   for (int r = 0; r < 10; ++r) {
     if (r > 0) {
@@ -63,8 +69,9 @@ __device__ __forceinline__ U4 philox10(Ctr c, uint32_t block, Key k) {
     // mul_lo/mul_hi pair
     uint64_t p0 = (uint64_t)0xD2511F53u * x0;
     uint64_t p1 = (uint64_t)0xCD9E8D57u * x2;
-    uint32_t n0 = (uint32_t)(p1 >> 32) ^ x1 ^ k0;
-    uint32_t n2 = (uint32_t)(p0 >> 32) ^ x3 ^ k1;
+    // three-input xor in one v_bitop3_b32 (truth table 0x96 = a ^ b ^ c)
+    uint32_t n0 = __builtin_amdgcn_bitop3_b32((uint32_t)(p1 >> 32), x1, k0, 0x96);
+    uint32_t n2 = __builtin_amdgcn_bitop3_b32((uint32_t)(p0 >> 32), x3, k1, 0x96);
     x0 = n0;
     x1 = (uint32_t)p1;
     x2 = n2;
