@@ -90,6 +90,23 @@ int imc2d_census_by_cell(imc2d_ctx* ctx, double* out);
 int imc2d_set_material(imc2d_ctx* ctx, const double* T, const double* sigma_true, const double* cv, double theta,
                        double dt, int clip, double* sigma_a_out, double* sigma_s_out);
 
+/* Device-held material of the configs, replacing the host callables of
+ * IMC2D.step when they have this form (IMC/IMC2D.py:1646-1651 sigma_a_func,
+ * cv; 1800-1810 eos/inv_eos): sigma_a = sigma0 * max(T, T_floor)^-power and
+ * u = c_v * T per cell.  Copies sigma0, c_v, the temperature T and the
+ * internal energy density u (per cell). */
+int imc2d_set_material_powerlaw(imc2d_ctx* ctx, const double* sigma0, double power, double T_floor, const double* cv,
+                                const double* T, const double* u);
+/* Fleck block of IMC/IMC2D.py:1646-1651 on the device-held T: sigma from the
+ * power law, then f and the effective opacities as imc2d_set_material. */
+int imc2d_fleck_powerlaw(imc2d_ctx* ctx, double theta, double dt, int clip);
+/* Material update of IMC/IMC2D.py:1800-1810 after the step's transport (with
+ * use_scalar_intensity_Tr): u += dep - E_emit / V (the emission energies of
+ * the step's imc2d_sample_volume(IMC_SRC_EMISSION)), T = u / c_v,
+ * T_r = (si / (a c))^(1/4).  Outputs (nullable): T, T_r, u per cell and the
+ * total internal energy sum(u V). */
+int imc2d_update_material(imc2d_ctx* ctx, double* T_out, double* Tr_out, double* u_out, double* internal_total_out);
+
 /* Equilibrium census at t=0, replacing the pool: _equilibrium_sample_xy/_rz
  * (IMC/IMC2D.py:1110-1188). */
 int imc2d_equilibrium_sample(imc2d_ctx* ctx, int64_t N, const double* Tr);

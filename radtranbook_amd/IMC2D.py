@@ -20,6 +20,7 @@ import numpy as np
 from ._census import DeviceCensus
 
 from ._lib import quarter_root, A_RAD, C_LIGHT, SRC_EMISSION, SRC_FIXED, Engine2D
+from .materials import device_material
 
 __c = C_LIGHT
 __a = A_RAD
@@ -149,10 +150,21 @@ def step(state, Ntarget, Nboundary, Nsource, Nmax, T_boundary, dt, edges1, edges
     state._invalidate()
     eng.set_step(state.count)
 
-    temperature = state.temperature
-    sig = sigma_a_func(temperature)
-    cvv = cv(temperature)
-    eng.set_material(temperature, sig, cvv, theta, dt, clip=True)
+    # a power-law opacity with a linear EOS runs on the device-held T and u
+    dm = device_material(sigma_a_func, cv, inv_eos, eng.shape) if (use_scalar_intensity_Tr and
+                                                                   not conserve_comb_energy) else None
+    if dm is not None:
+        refs = getattr(eng, "_device_material", None)
+        cur = (sigma_a_func, cv.__self__, state.temperature, state.internal_energy)
+        if refs is None or any(a is not b for a, b in zip(refs, cur)):
+            eng.set_material_powerlaw(dm[0], dm[1], dm[2], dm[3], state.temperature, state.internal_energy)
+        eng.fleck_powerlaw(theta, dt, clip=True)
+    else:
+        eng._device_material = None
+        temperature = state.temperature
+        sig = sigma_a_func(temperature)
+        cvv = cv(temperature)
+        eng.set_material(temperature, sig, cvv, theta, dt, clip=True)
 
     boundary_emission = 0.0
     if Nboundary > 0:
@@ -188,20 +200,26 @@ def step(state, Ntarget, Nboundary, Nsource, Nmax, T_boundary, dt, edges1, edges
     dep_cell, si_cell, boundary_loss, stats = eng.transport(dt, reflect, max_events_per_particle, -1.0)
     t_post_start = _time.perf_counter()
 
-    internal_energy = state.internal_energy + dep_cell - E_emit_cells / volumes
-    temperature = inv_eos(internal_energy)
-
-    ecen, comb_disc, ecen_total, total_rad = eng.comb(Nmax, want_ecen=not use_scalar_intensity_Tr,
-                                                      want_disc=conserve_comb_energy)
-    if use_scalar_intensity_Tr:
-        radiation_temperature = quarter_root(si_cell * (1.0 / (__a * __c)))
+    if dm is not None:
+        # u += dep - E_emit / V, T = u / c_v, T_r from si, sum(u V) on the device
+        temperature, radiation_temperature, internal_energy, total_internal = eng.update_material()
+        eng._device_material = (sigma_a_func, cv.__self__, temperature, internal_energy)
+        ecen, comb_disc, ecen_total, total_rad = eng.comb(Nmax, want_ecen=False, want_disc=False)
     else:
-        radiation_temperature = quarter_root(ecen / volumes / __a)
-    if conserve_comb_energy:
-        internal_energy = internal_energy + comb_disc / volumes
+        internal_energy = state.internal_energy + dep_cell - E_emit_cells / volumes
         temperature = inv_eos(internal_energy)
 
-    total_internal = float(np.sum(internal_energy * volumes))
+        ecen, comb_disc, ecen_total, total_rad = eng.comb(Nmax, want_ecen=not use_scalar_intensity_Tr,
+                                                          want_disc=conserve_comb_energy)
+        if use_scalar_intensity_Tr:
+            radiation_temperature = quarter_root(si_cell * (1.0 / (__a * __c)))
+        else:
+            radiation_temperature = quarter_root(ecen / volumes / __a)
+        if conserve_comb_energy:
+            internal_energy = internal_energy + comb_disc / volumes
+            temperature = inv_eos(internal_energy)
+
+        total_internal = float(np.sum(internal_energy * volumes))
     total_energy = total_internal + total_rad
     energy_loss = total_energy - state.previous_total_energy - boundary_emission + boundary_loss - source_emission
     # comb discrepancy booked as a sink unless it went back into the material

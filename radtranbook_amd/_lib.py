@@ -46,6 +46,9 @@ SIGNATURES = {
     "imc2d_census_energy": (C.c_int, [_vp, _dp]),
     "imc2d_census_by_cell": (C.c_int, [_vp, _dp]),
     "imc2d_set_material": (C.c_int, [_vp, _dp, _dp, _dp, C.c_double, C.c_double, C.c_int, _dp, _dp]),
+    "imc2d_set_material_powerlaw": (C.c_int, [_vp, _dp, C.c_double, C.c_double, _dp, _dp, _dp]),
+    "imc2d_fleck_powerlaw": (C.c_int, [_vp, C.c_double, C.c_double, C.c_int]),
+    "imc2d_update_material": (C.c_int, [_vp, _dp, _dp, _dp, _dp]),
     "imc2d_equilibrium_sample": (C.c_int, [_vp, C.c_int64, _dp]),
     "imc2d_sample_boundary": (C.c_int, [_vp, C.c_int, C.c_int64, C.c_double, C.c_double, _dp, C.c_int]),
     "imc2d_sample_volume": (C.c_int, [_vp, C.c_int, C.c_int64, _dp, C.c_double, C.c_int, _dp, _dp, _i64p]),
@@ -239,6 +242,23 @@ class Engine2D:
         check(self._L.imc2d_set_material(self._h, _p(T), _p(s), _p(c), float(theta), float(dt), 1 if clip else 0,
                                          _p(sa), _p(ss)))
         return sa, ss
+
+    def set_material_powerlaw(self, sigma0, power, T_floor, cv, T, u):
+        """Device-held power-law material: sigma0, c_v, T and u per cell."""
+        a = [_cell(x, self.shape) for x in (sigma0, cv, T, u)]
+        check(self._L.imc2d_set_material_powerlaw(self._h, _p(a[0]), float(power), float(T_floor), _p(a[1]),
+                                                  _p(a[2]), _p(a[3])))
+
+    def fleck_powerlaw(self, theta, dt, clip=True):
+        check(self._L.imc2d_fleck_powerlaw(self._h, float(theta), float(dt), 1 if clip else 0))
+
+    def update_material(self, want=("T", "Tr", "u")):
+        """u += dep - E_emit/V, T = u/c_v, T_r from si on the device; returns
+        (T, T_r, u) (None where not wanted) and sum(u V)."""
+        out = {k: (np.empty(self.shape) if k in want else None) for k in ("T", "Tr", "u")}
+        tot = C.c_double()
+        check(self._L.imc2d_update_material(self._h, _p(out["T"]), _p(out["Tr"]), _p(out["u"]), C.byref(tot)))
+        return out["T"], out["Tr"], out["u"], tot.value
 
     def equilibrium_sample(self, N, Tr):
         Tr = _cell(Tr, self.shape)

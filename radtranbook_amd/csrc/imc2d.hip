@@ -122,6 +122,35 @@ __global__ void k_emission_energy(int64_t n, const double* __restrict__ T, const
   E[c] = kA * kC * t4 * sa[c] * dt * vol[c];
 }
 
+// Power-law opacity on the device: sigma = sigma0 * max(T, T_floor)^-power,
+// with power 3 evaluated as sigma0 / (m m m), the form of the host
+// PowerLawOpacity (radtranbook_amd/materials.py), so both give the same bits.
+__global__ void k_powerlaw_sigma(int64_t n, const double* __restrict__ T, const double* __restrict__ sigma0,
+                                 double power, double T_floor, double* __restrict__ sig) {
+#pragma clang fp contract(off)
+  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const double m = T[c] > T_floor ? T[c] : T_floor;
+  sig[c] = power == 3.0 ? sigma0[c] / (m * m * m) : sigma0[c] * pow(m, -power);
+}
+
+// Material update of IMC/IMC2D.py:1800-1810 for u = c_v T: u += dep -
+// E_emit / V, T = u / c_v, T_r = (si / (a c))^(1/4) as two square roots (the
+// scalar-intensity estimator), in the host's operation order.
+__global__ void k_update_material(int64_t n, double* __restrict__ u, double* __restrict__ T,
+                                  double* __restrict__ Tr, const double* __restrict__ dep,
+                                  const double* __restrict__ E, const double* __restrict__ vol,
+                                  const double* __restrict__ cv, const double* __restrict__ si, double* __restrict__ uV) {
+#pragma clang fp contract(off)
+  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const double un = (u[c] + dep[c]) - E[c] / vol[c];
+  u[c] = un;
+  T[c] = un / cv[c];
+  Tr[c] = sqrt(sqrt(si[c] * (1.0 / (kA * kC))));
+  uV[c] = un * vol[c];
+}
+
 // a (T_r) source for the equilibrium census: a T^4 V
 __global__ void k_equilibrium_energy(int64_t n, const double* __restrict__ Tr, const double* __restrict__ vol,
                                      double* __restrict__ E) {
@@ -1345,6 +1374,10 @@ struct Imc2dCtx {
   float *e1f = nullptr, *e2f = nullptr;
   DevBuf<double> T, sig, cv, sa, ss, E, Ar, Mr, Pn, dep_raw, si_raw, dep, si, ecen, ew, disc, prof, sum_scr, wbuf,
       volsrc, beta;
+  // device-held power-law material (imc2d_set_material_powerlaw)
+  DevBuf<double> sigma0, u, Tr, uV;
+  double pl_power = 3.0, pl_floor = 0.0;
+  bool pl_ready = false;
   bool cf_ready = false;  // Carter-Forest material (sigma_a, beta) set
   DevBuf<float> saf, ssf;
   DevBuf<float2> sas32, e1p32, e2p32;
@@ -1355,7 +1388,7 @@ struct Imc2dCtx {
   DevBuf<unsigned long long> cellcount;
   DevBuf<double> ecen_part, tree_scr;
   DevBuf<unsigned int> count_part;
-  double* scal = nullptr;  // [0] Etot  [1] weight sum  [2] floor  [3] bl  [4] comb E  [5] rad total
+  double* scal = nullptr;  // [0] Etot [1] weight sum [2] floor [3] bl [4] comb E [5] rad total [6] sum u V
   unsigned long long* stats = nullptr;
   unsigned long long* next = nullptr;
   int* bad = nullptr;
@@ -1662,7 +1695,7 @@ void imc2d_destroy(imc2d_ctx* c) {
   (void)hipStreamSynchronize(c->stream);
   DevBuf<double>* cellbufs[] = {&c->T, &c->sig, &c->cv, &c->sa, &c->ss, &c->E, &c->Ar, &c->Mr, &c->Pn,
                                 &c->dep_raw, &c->si_raw, &c->dep, &c->si, &c->ecen, &c->ew, &c->disc, &c->prof,
-                                &c->sum_scr, &c->wbuf, &c->volsrc, &c->beta};
+                                &c->sum_scr, &c->wbuf, &c->volsrc, &c->beta, &c->sigma0, &c->u, &c->Tr, &c->uV};
   for (auto* b : cellbufs) b->release();
   c->stage.release();
   c->saf.release();
@@ -1814,6 +1847,92 @@ int imc2d_set_material(imc2d_ctx* c, const double* T, const double* sigma_true, 
   if (sigma_s_out) std::memcpy(sigma_s_out, c->stage.slot(c->ncell, 4), bytes);
   IMC_REQUIRE(!*hbad, kErrArg, "Fleck factor out of bounds");
   c->emit_lo = c->emit_hi = 0;
+  return kOk;
+}
+
+// Device-held material of the configs: sigma_a = sigma0 max(T, T_floor)^-power
+// and u = c_v T per cell (the host callables of IMC/IMC2D.py:1646-1651,1800-1810
+// when they have this form).  Copies sigma0, c_v, T and u.
+int imc2d_set_material_powerlaw(imc2d_ctx* c, const double* sigma0, double power, double T_floor, const double* cv,
+                                const double* T, const double* u) {
+  IMC_REQUIRE(c && sigma0 && cv && T && u, kErrArg, "imc2d_set_material_powerlaw: null argument");
+  IMC_REQUIRE(std::isfinite(power) && std::isfinite(T_floor) && T_floor >= 0.0, kErrArg,
+              "imc2d_set_material_powerlaw: bad power %g or T_floor %g", power, T_floor);
+  IMC_HIP(hipSetDevice(c->device));
+  int rc;
+  const int64_t nc = c->ncell;
+  if ((rc = c->sigma0.reserve(nc)) || (rc = c->u.reserve(nc)) || (rc = c->Tr.reserve(nc)) ||
+      (rc = c->uV.reserve(nc)))
+    return rc;
+  {
+    CopyJob in[3] = {stage_in(c, 0, c->sigma0.p, sigma0), stage_in(c, 1, c->cv.p, cv), stage_in(c, 2, c->T.p, T)};
+    if ((rc = copy_jobs(in, 3, c->stream))) return rc;
+    CopyJob in2 = stage_in(c, 5, c->u.p, u);
+    if ((rc = copy_jobs(&in2, 1, c->stream))) return rc;
+  }
+  IMC_HIP(hipStreamSynchronize(c->stream));
+  c->pl_power = power;
+  c->pl_floor = T_floor;
+  c->pl_ready = true;
+  return kOk;
+}
+
+// Fleck block on the device-held T (IMC/IMC2D.py:1646-1651): sigma from the
+// power law, then the effective opacities as imc2d_set_material.
+int imc2d_fleck_powerlaw(imc2d_ctx* c, double theta, double dt, int clip) {
+  IMC_REQUIRE(c, kErrArg, "imc2d_fleck_powerlaw: null ctx");
+  IMC_REQUIRE(c->pl_ready, kErrState, "imc2d_fleck_powerlaw: call imc2d_set_material_powerlaw first");
+  IMC_HIP(hipSetDevice(c->device));
+  const int64_t nc = c->ncell;
+  const int g = launch_blocks(nc, 256);
+  k_powerlaw_sigma<<<g, 256, 0, c->stream>>>(nc, c->T.p, c->sigma0.p, c->pl_power, c->pl_floor, c->sig.p);
+  IMC_HIP(hipGetLastError());
+  bool f32 = c->precision == IMC_FP32_TRACK;
+  if (!clip) IMC_HIP(hipMemsetAsync(c->bad, 0, sizeof(int), c->stream));
+  k_fleck<<<g, 256, 0, c->stream>>>(nc, c->T.p, c->sig.p, c->cv.p, theta, dt, clip, c->sa.p, c->ss.p,
+                                    f32 ? c->saf.p : nullptr, f32 ? c->ssf.p : nullptr, c->sas64.p, c->sas32.p,
+                                    c->bad);
+  IMC_HIP(hipGetLastError());
+  if (!clip) {
+    int* hbad = (int*)stage_scalars(c);
+    CopyJob out{c->bad, stage_scalars_dev(c), 4};
+    int rc = copy_jobs(&out, 1, c->stream);
+    if (rc) return rc;
+    IMC_HIP(hipStreamSynchronize(c->stream));
+    IMC_REQUIRE(!*hbad, kErrArg, "Fleck factor out of bounds");
+  }
+  c->emit_lo = c->emit_hi = 0;
+  return kOk;
+}
+
+// Material update after the transport of the step (IMC/IMC2D.py:1800-1810,
+// use_scalar_intensity_Tr): u += dep - E_emit / V with the emission energies
+// of the step's imc2d_sample_volume(IMC_SRC_EMISSION), T = u / c_v, T_r from
+// the path-length tally.  Outputs (host, nullable): T, T_r and u per cell and
+// sum(u V) in canonical order.
+int imc2d_update_material(imc2d_ctx* c, double* T_out, double* Tr_out, double* u_out, double* internal_total_out) {
+  IMC_REQUIRE(c, kErrArg, "imc2d_update_material: null ctx");
+  IMC_REQUIRE(c->pl_ready, kErrState, "imc2d_update_material: call imc2d_set_material_powerlaw first");
+  IMC_HIP(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  const int64_t nc = c->ncell;
+  k_update_material<<<launch_blocks(nc, 256), 256, 0, s>>>(nc, c->u.p, c->T.p, c->Tr.p, c->dep.p, c->E.p, c->vol,
+                                                           c->cv.p, c->si.p, c->uV.p);
+  IMC_HIP(hipGetLastError());
+  int rc;
+  if ((rc = canonical_sum(c->uV.p, nc, c->scal + 6, c->sum_scr.p, s))) return rc;
+  double* hs = stage_scalars(c);
+  {
+    CopyJob out[4] = {T_out ? stage_out(c, 3, c->T.p) : CopyJob{}, Tr_out ? stage_out(c, 4, c->Tr.p) : CopyJob{},
+                      u_out ? stage_out(c, 5, c->u.p) : CopyJob{}, CopyJob{c->scal + 6, stage_scalars_dev(c) + 3, 8}};
+    if ((rc = copy_jobs(out, 4, s))) return rc;
+  }
+  IMC_HIP(hipStreamSynchronize(s));
+  const size_t bytes = sizeof(double) * nc;
+  if (T_out) std::memcpy(T_out, c->stage.slot(nc, 3), bytes);
+  if (Tr_out) std::memcpy(Tr_out, c->stage.slot(nc, 4), bytes);
+  if (u_out) std::memcpy(u_out, c->stage.slot(nc, 5), bytes);
+  if (internal_total_out) *internal_total_out = hs[3];
   return kOk;
 }
 

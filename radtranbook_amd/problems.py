@@ -35,6 +35,8 @@ from dataclasses import dataclass
 
 import numpy as np
 
+from .materials import LinearEOS, PowerLawOpacity  # noqa: F401  (the problems' material models)
+
 
 def gaussian_field(n1, n2, corr_cells, seed):
     """Zero-mean, unit-variance periodic Gaussian random field."""
@@ -47,35 +49,6 @@ def gaussian_field(n1, n2, corr_cells, seed):
     f -= f.mean()
     f /= f.std()
     return f
-
-
-@dataclass
-class PowerLawOpacity:
-    """sigma(T) = sigma0 * max(T, T_floor)^-power, per cell."""
-
-    sigma0: np.ndarray
-    power: float = 3.0
-    T_floor: float = 1e-3
-
-    def __call__(self, T):
-        m = np.maximum(T, self.T_floor)
-        if self.power == 3.0:
-            return self.sigma0 / (m * m * m)  # the generic power is several times slower on the host
-        return self.sigma0 * m ** (-self.power)
-
-
-@dataclass
-class LinearEOS:
-    cv: float
-
-    def eos(self, T):
-        return self.cv * T
-
-    def inv_eos(self, u):
-        return u / self.cv
-
-    def cv_func(self, T):
-        return self.cv + 0.0 * T
 
 
 @dataclass

@@ -77,3 +77,4 @@ def test_rejects_other_mesh_and_files(tmp_path):
         checkpoint.load_checkpoint(bad)
     with pytest.raises(TypeError):
         checkpoint.save_checkpoint(object(), tmp_path / "x.npz")
+
