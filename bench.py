@@ -40,23 +40,67 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def profiled_traffic(profile, precision, steps, warmup, defaults_match):
-    """HBM bytes per transport launch measured by rocprofv3 for this workload
-    (FETCH_SIZE + WRITE_SIZE per dispatch, each scaled by the same counter's
-    reading for a copy kernel of known size, averaged over the timed
-    launches), from the committed summary in profiles/.  None unless this run
-    has the profiled sizes, timed steps and warm-up steps."""
-    if not defaults_match:
+def profiled(profile, precision, sizes_match):
+    """Figures rocprofv3 measured for this workload (the committed summary in
+    profiles/): HBM bytes per segment of the transport kernel (FETCH_SIZE +
+    WRITE_SIZE per dispatch, each scaled by the same counter's reading for a
+    copy kernel of known size, over the timed launches) and, where recorded,
+    its VALU busy fraction and active-lane fraction.  None unless this run
+    has the profiled sizes."""
+    if not sizes_match:
         return None
     try:
         with open(os.path.join(ROOT, "profiles", profile)) as fh:
             text = fh.read()
     except OSError:
         return None
-    m = re.search(r"traffic_bytes_per_launch %s (\S+) steps=(\d+) warmup=(\d+)" % precision, text)
-    if not m or int(m.group(2)) != steps or int(m.group(3)) != warmup:
-        return None
-    return float(m.group(1))
+    out = {}
+    m = re.search(r"^bytes_per_segment %s (\S+)" % precision, text, re.M)
+    if m:
+        out["bytes_per_segment"] = float(m.group(1))
+    m = re.search(r"^valu_busy %s (\S+) lanes_active (\S+)" % precision, text, re.M)
+    if m:
+        out["valu_busy"] = float(m.group(1))
+        out["lanes_active"] = float(m.group(2))
+    return out or None
+
+
+# Algorithmic HBM bytes per history of each tracker: the history's state read
+# once when a lane picks it up and written once at census (a persistent
+# tracker keeps the history in registers in between).  The per-cell material
+# and tallies are a few MB per launch and are left out.
+BYTES_PER_HISTORY = {
+    "k_transport<float,xy>": 36 + 32, "k_transport<double,xy>": 56 + 48,
+    "k_mg_transport<float,xy>": 40 + 32, "k_mg_transport<double,xy>": 60 + 48,
+    "k1_transport<float,0>": 28 + 20, "k1_transport<double,0>": 40 + 28,
+    "k_transport_cf<xy>": 56 + 56, "k1_transport_cf<0>": 40 + 36,
+}
+
+
+def roofline(kernel, precision, seg_per_launch, hist_per_launch, avg_launch_s, model_bps, profile, sizes_match):
+    """The roofline object of the bench line.  ``achieved``/``frac``: the
+    algorithmic bytes per launch (BYTES_PER_HISTORY x histories) over the
+    launch's average duration.  ``traffic``: HBM bytes per launch that
+    rocprofv3 measured (bytes per segment x this run's segments per launch),
+    ``traffic_frac`` the bandwidth that moves.  ``issue_frac``: the VALU busy
+    fraction rocprofv3 measured (the kernels are issue/latency bound).  The
+    per-segment event model (every segment loads and stores the state) is kept
+    as ``model_*``."""
+    alg = hist_per_launch * BYTES_PER_HISTORY[kernel]
+    achieved = alg / avg_launch_s / 1e9
+    meas = profiled(profile, precision, sizes_match)
+    traffic = meas["bytes_per_segment"] * seg_per_launch if meas and "bytes_per_segment" in meas else None
+    model = seg_per_launch * model_bps / avg_launch_s / 1e9
+    return {
+        "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+        "traffic": traffic,
+        "traffic_frac": traffic / avg_launch_s / 1e9 / HBM_PEAK_GBS if traffic else None,
+        "issue_frac": meas.get("valu_busy") if meas else None,
+        "lanes_active_frac": meas.get("lanes_active") if meas else None,
+        "kernel": kernel, "algorithmic_bytes_per_history": BYTES_PER_HISTORY[kernel],
+        "histories_per_launch": hist_per_launch, "segments_per_launch": seg_per_launch,
+        "model_bytes_per_segment": model_bps, "model_achieved": model, "model_frac": model / HBM_PEAK_GBS,
+    }
 
 
 def run_gpu(prob, precision, steps, warmup, fom_seeds):
@@ -261,7 +305,6 @@ def main_1d(args, workload):
     head = res[precs[0]]
     value = head["segments"] / head["wall"]
     avg_launch_s = head["kernel_ms"].mean() / 1e3
-    achieved = head["seg_per_launch"].mean() * BYTES_PER_SEGMENT_1D[precs[0]] / avg_launch_s / 1e9
     out = {
         "metric": metric,
         "value": value,
@@ -279,14 +322,10 @@ def main_1d(args, workload):
                    "new_particles_per_step": int(sum(p.n_particles for p in probs))},
         "histories_per_s": head["histories"] / head["wall"],
         "transport_segments_per_s": head["segments"] / (head["kernel_ms"].sum() / 1e3),
-        "roofline": {
-            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": profiled_traffic("k1_transport_batch1d_256slabs.md", precs[0], steps, warmup,
-                                        workload == "batch1d" and args.problems == 256 and not args.tune),
-            "kernel": f"k1_transport<{'float' if precs[0] == 'fp32' else 'double'},0>",
-            "bytes_per_segment": BYTES_PER_SEGMENT_1D[precs[0]],
-        },
+        "roofline": roofline(f"k1_transport<{'float' if precs[0] == 'fp32' else 'double'},0>", precs[0],
+                             float(head["seg_per_launch"].mean()), head["histories"] / steps, avg_launch_s,
+                             BYTES_PER_SEGMENT_1D[precs[0]], "k1_transport_batch1d_256slabs.md",
+                             workload == "batch1d" and args.problems == 256 and not args.tune),
         "autotuned_launch": head["tuned"],
     }
     for p in precs:
@@ -395,7 +434,6 @@ def main_mg(args):
     res = {p: run_gpu_mg(prob, p, steps, warmup, npart) for p in precs}
     head = res[precs[0]]
     value = head["segments"] / head["wall"]
-    achieved = head["seg_per_launch"].mean() * BYTES_PER_SEGMENT_MG[precs[0]] / (head["kernel_ms"].mean() / 1e3) / 1e9
     out = {
         "metric": "IMC segments/s, multigroup 2D crooked pipe 240x240 x 10 groups, 2e7 particles/step",
         "value": value, "unit": "segments/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
@@ -408,12 +446,10 @@ def main_mg(args):
                    "particles_per_step": npart},
         "histories_per_s": head["histories"] / head["wall"],
         "transport_segments_per_s": head["segments"] / (head["kernel_ms"].sum() / 1e3),
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": profiled_traffic("k_mg_transport_crooked_pipe_240x240x10.md", precs[0], steps, warmup,
-                                                 npart == 20_000_000),
-                     "kernel": f"k_mg_transport<{'float' if precs[0] == 'fp32' else 'double'},xy>",
-                     "bytes_per_segment": BYTES_PER_SEGMENT_MG[precs[0]]},
+        "roofline": roofline(f"k_mg_transport<{'float' if precs[0] == 'fp32' else 'double'},xy>", precs[0],
+                             float(head["seg_per_launch"].mean()), head["histories"] / steps,
+                             head["kernel_ms"].mean() / 1e3, BYTES_PER_SEGMENT_MG[precs[0]],
+                             "k_mg_transport_crooked_pipe_240x240x10.md", npart == 20_000_000),
     }
     for p in precs:
         r = res[p]
@@ -569,7 +605,6 @@ def main_cf(args):
     seg_launch = np.array([i["profiling"]["transport_events"]["total"] for i in infos], dtype=np.float64)
     resid = max(abs(i["energy_residual"]) / i["total_energy"] for i in infos)
     value = segs / wall
-    achieved = seg_launch.mean() * BYTES_PER_SEGMENT_CF / (kern.mean() / 1e3) / 1e9
     out = {
         "metric": "IMC segments/s, Carter-Forest 2D 200x200 heterogeneous Marshak, 1e7 particles/step",
         "value": value, "unit": "segments/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
@@ -583,12 +618,9 @@ def main_cf(args):
         "transport_kernel_ms_per_step": float(kern.mean()),
         "segments_per_step": float(seg_launch.mean()),
         "max_energy_residual_rel": resid,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": profiled_traffic("k_transport_cf_marshak2d_200x200_1e7.md", "fp64", steps, warmup,
-                                                 args.cells == 200 and npart == 10_000_000),
-                     "kernel": "k_transport_cf<xy>",
-                     "bytes_per_segment": BYTES_PER_SEGMENT_CF},
+        "roofline": roofline("k_transport_cf<xy>", "fp64", float(seg_launch.mean()), hist / steps, kern.mean() / 1e3,
+                             BYTES_PER_SEGMENT_CF, "k_transport_cf_marshak2d_200x200_1e7.md",
+                             args.cells == 200 and npart == 10_000_000),
     }
     if not args.no_cpu:
         from oracle import oracle as O, oracle_cf as OCF
@@ -647,16 +679,17 @@ def main_cf1d(args):
 
     def run(mod, seed, nsteps):
         st = mod.init_simulation(case.n_target, T0, T0, case.mesh, case.eos, case.inv_eos, geometry="slab", seed=seed)
-        segs, kern, k = 0, 0.0, 0
+        segs, hist, kern, k = 0, 0, 0.0, 0
         t0 = time.perf_counter()
         while st.time < case.t_final - 1e-12 and k < nsteps:
             st, info = mod.step(st, case.n_target, case.n_boundary, 0, case.n_max, case.T_boundary,
                                 min(case.dt, case.t_final - st.time), case.mesh, case.sigma_a, case.inv_eos,
                                 case.cv_func, np.zeros(case.I), reflect=case.reflect, geometry="slab")
             segs += info["profiling"]["transport_events"]["total"]
+            hist += info["profiling"]["transport_events"]["n_particles_transported"]
             kern += info["profiling"]["transport_kernel_ms"]
             k += 1
-        return dict(wall=time.perf_counter() - t0, segs=segs, kern_ms=kern, steps=k,
+        return dict(wall=time.perf_counter() - t0, segs=segs, hist=hist, kern_ms=kern, steps=k,
                     energy=st.previous_total_energy)
 
     full = int(round(case.t_final / case.dt))
@@ -669,7 +702,6 @@ def main_cf1d(args):
             f"segments, kernels {res[name]['kern_ms']:.1f} ms, final energy {res[name]['energy']:.6e}")
     cf = res["cf"]
     value = cf["segs"] / cf["wall"]
-    achieved = cf["segs"] * BYTES_PER_SEGMENT_CF1D / (cf["kern_ms"] / 1e3) / 1e9
     out = {
         "metric": "IMC segments/s, Carter-Forest 1-D slab Marshak wave (the reference's FC vs CF comparison case)",
         "value": value, "unit": "segments/s", "n_gpus": 1, "steps": cf["steps"], "warmup": warmup,
@@ -685,10 +717,9 @@ def main_cf1d(args):
                "segments": res["fc"]["segs"], "final_total_energy": res["fc"]["energy"]},
         "reference_published_cpu": REFERENCE_CF1D,
         "wall_speedup_vs_reference_published": {m: REFERENCE_CF1D[m]["wall_s"] / res[m]["wall"] for m in res},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k1_transport_cf<slab>",
-                     "bytes_per_segment": BYTES_PER_SEGMENT_CF1D,
-                     "note": "5e4-8e4 histories per launch: latency-bound, far below the device's size"},
+        "roofline": dict(roofline("k1_transport_cf<0>", "fp64", cf["segs"] / cf["steps"], cf["hist"] / cf["steps"],
+                                  cf["kern_ms"] / cf["steps"] / 1e3, BYTES_PER_SEGMENT_CF1D, "", False),
+                         note="5e4-8e4 histories per launch: latency-bound, far below the device's size"),
     }
     if not args.no_cpu:
         from oracle import oracle as O, oracle_cf1d as OCF
@@ -783,9 +814,8 @@ def main():
     head = res[precs[0]]
     value = head["segments"] / head["wall"]
     kern_s = head["kernel_ms"].sum() / 1e3
-    seg_per_launch = head["seg_per_launch"].mean()
+    seg_per_launch = float(head["seg_per_launch"].mean())
     avg_launch_s = head["kernel_ms"].mean() / 1e3
-    achieved = seg_per_launch * BYTES_PER_SEGMENT[precs[0]] / avg_launch_s / 1e9
     out = {
         "metric": METRIC,
         "value": value,
@@ -809,23 +839,29 @@ def main():
         "histories_per_s": head["histories"] / head["wall"],
         "transport_segments_per_s": head["segments"] / kern_s,
         "fom_seeds": args.fom_seeds,
-        "roofline": {
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": profiled_traffic("k_transport_marshak2d_200x200_1e7.md", precs[0], args.steps, args.warmup,
-                                        args.cells == 200 and int(args.particles) == 10_000_000),
-            "kernel": f"k_transport<{'float' if precs[0] == 'fp32' else 'double'},xy>",
-            "bytes_per_segment": BYTES_PER_SEGMENT[precs[0]],
-        },
+        "roofline": roofline(f"k_transport<{'float' if precs[0] == 'fp32' else 'double'},xy>", precs[0],
+                             seg_per_launch, head["histories"] / args.steps, avg_launch_s,
+                             BYTES_PER_SEGMENT[precs[0]], "k_transport_marshak2d_200x200_1e7.md",
+                             args.cells == 200 and int(args.particles) == 10_000_000),
     }
     mask = fom_cells(res, prob.T0)
     out["fom_cells"] = int(mask.sum())
+    # the FOM-cell set of the FOM definition itself: every cell with E_abs
+    # above 1e-3 of the maximum (cold cells that re-absorb their own emission
+    # included)
+    mask_all = None
+    for r in res.values():
+        mean = r["deps"].mean(0)
+        m = mean > 1e-3 * mean.max()
+        mask_all = m if mask_all is None else (mask_all & m)
+    out["fom_cells_all"] = int(mask_all.sum())
+    out["fom_cells_all_note"] = ("R2 median over every cell with E_abs > 1e-3 max: most of those cells are cold and "
+                                 "re-absorb their own emission, so their FP64 tally has no Monte Carlo noise and "
+                                 "the median is degenerate; fom uses the fom_cells the front has heated")
     for p in precs:
         r = res[p]
         r["R2"], r["fom"], r["fom_transport"] = fom_of(r, mask)
+        r["R2_all"], r["fom_all"], _ = fom_of(r, mask_all)
         r["final_dep"] = r["deps"].mean(0)
         out[p] = {
             "segments_per_s": r["segments"] / r["wall"],
@@ -835,6 +871,8 @@ def main():
             "R2_median": r["R2"],
             "fom": r["fom"],
             "fom_transport": r["fom_transport"],
+            "R2_median_all_cells": r["R2_all"],
+            "fom_all_cells": r["fom_all"],
             "max_energy_residual_rel": r["residual"],
         }
     if "fp32" in res and "fp64" in res:
@@ -845,6 +883,14 @@ def main():
         a, b = res["fp32"]["final_dep"], res["fp64"]["final_dep"]
         m = b > 1e-3 * b.max()
         out["fp32_vs_fp64_final_dep_max_rel"] = float(np.max(np.abs(a[m] - b[m]) / b[m]))
+        # FP32-track tallies against FP64 in combined standard errors of the
+        # seed means, over the FOM cells
+        d32, d64 = res["fp32"]["deps"], res["fp64"]["deps"]
+        if len(d32) > 1 and len(d64) > 1:
+            se = np.sqrt(d32.var(0, ddof=1) / len(d32) + d64.var(0, ddof=1) / len(d64))
+            z = np.abs(d32.mean(0) - d64.mean(0))[mask] / np.maximum(se[mask], 1e-300)
+            out["fp32_vs_fp64_final_dep_z"] = {"max": float(z.max()), "frac_above_3": float(np.mean(z > 3.0)),
+                                               "cells": int(mask.sum()), "seeds": int(len(d32))}
     if not args.no_cpu:
         cb = run_cpu_baseline(prob, args.cpu_particles, args.cpu_steps)
         out["cpu_baseline"] = {
@@ -857,6 +903,15 @@ def main():
         }
         if cb["value"]:
             out["gpu_over_cpu_segments_per_s"] = value / cb["value"]
+            # the FOM the CPU oracle would reach on the same timed steps: the
+            # same FP64 histories (R^2 of the FP64 GPU runs) at its measured
+            # segments/s
+            r64 = res.get("fp64")
+            if r64 is not None and r64["R2"] and r64["R2"] > 0:
+                t_cpu = r64["segments"] / cb["value"]
+                out["cpu_baseline"]["fom_estimate"] = 1.0 / (r64["R2"] * t_cpu)
+                if head.get("fom"):
+                    out["gpu_over_cpu_fom"] = head["fom"] / out["cpu_baseline"]["fom_estimate"]
     print(json.dumps(out), flush=True)
 
 

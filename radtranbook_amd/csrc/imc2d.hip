@@ -1909,7 +1909,7 @@ int imc2d_fleck_powerlaw(imc2d_ctx* c, double theta, double dt, int clip) {
 // use_scalar_intensity_Tr): u += dep - E_emit / V with the emission energies
 // of the step's imc2d_sample_volume(IMC_SRC_EMISSION), T = u / c_v, T_r from
 // the path-length tally.  Outputs (host, nullable): T, T_r and u per cell and
-// sum(u V) in canonical order.
+// sum(u V).
 int imc2d_update_material(imc2d_ctx* c, double* T_out, double* Tr_out, double* u_out, double* internal_total_out) {
   IMC_REQUIRE(c, kErrArg, "imc2d_update_material: null ctx");
   IMC_REQUIRE(c->pl_ready, kErrState, "imc2d_update_material: call imc2d_set_material_powerlaw first");
@@ -1920,7 +1920,9 @@ int imc2d_update_material(imc2d_ctx* c, double* T_out, double* Tr_out, double* u
                                                            c->cv.p, c->si.p, c->uV.p);
   IMC_HIP(hipGetLastError());
   int rc;
-  if ((rc = canonical_sum(c->uV.p, nc, c->scal + 6, c->sum_scr.p, s))) return rc;
+  // a tree (pairwise-like) sum: the balance residual stays at rounding level
+  // (a left-to-right sum over 1024-cell chunks left it near 4e-14)
+  if ((rc = tree_sum(c->uV.p, nc, c->scal + 6, c->tree_scr.p, s))) return rc;
   double* hs = stage_scalars(c);
   {
     CopyJob out[4] = {T_out ? stage_out(c, 3, c->T.p) : CopyJob{}, Tr_out ? stage_out(c, 4, c->Tr.p) : CopyJob{},
