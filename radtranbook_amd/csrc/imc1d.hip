@@ -412,7 +412,7 @@ struct Transport1Args {
   int64_t n;
   const int64_t* poff;   // pool offset of each problem (P+1)
   const int64_t* coff;   // cell offset of each problem (P+1)
-  const R2* ep;          // (r0, r1) per cell
+  const double2* ep;     // (r0, r1) per cell, FP64 in both precisions
   const R2* sas;         // (sigma_a, sigma_s) per cell
   const double* dt;      // per problem
   const double* wfloor;  // per problem
@@ -450,7 +450,8 @@ __global__ __launch_bounds__(256) void k1_transport(Transport1Args<R> A) {
   bool active = false;
   int64_t k = 0;
   double w = 0.0;
-  R mu = 0, x = 0, dcen = 0;
+  R mu = 0, dcen = 0;
+  double x = 0.0;  // FP64 in both precisions (the FP32 mover works on cell offsets)
   int cell = 0, p = 0, I = 0;
   int64_t cb = 0;
   double wfl = 0.0;
@@ -486,7 +487,7 @@ __global__ __launch_bounds__(256) void k1_transport(Transport1Args<R> A) {
     ++st_ev;
     const int loc = cell;
     const int64_t c = cb + loc;
-    R2 edg = A.ep[c];
+    const double2 edg = A.ep[c];
     R2 mat = A.sas[c];
     Move1D<R> o;
     Ctr ctr = make_ctr(kTagTransport, ((uint64_t)p << 32) | (uint64_t)(k - A.poff[p]), A.step);
@@ -543,7 +544,7 @@ __global__ __launch_bounds__(256) void k1_transport(Transport1Args<R> A) {
       }
       A.P.w[k] = w;
       A.P.mu[k] = mu;
-      A.P.x[k] = x;
+      A.P.x[k] = (R)x;
       A.P.cell[k] = cell;
       active = false;
     }
@@ -1710,7 +1711,7 @@ int imc1d_transport(imc1d_ctx* c, const int32_t* reflect, const double* weight_f
       else
         k1_transport<double, 0><<<grid, block, 0, s>>>(A);
     } else {
-      Transport1Args<float> A{c->B.view<float>(), n, c->poff.p, c->coff.p, c->ep32.p, c->sas32.p, c->dt.p,
+      Transport1Args<float> A{c->B.view<float>(), n, c->poff.p, c->coff.p, c->ep64.p, c->sas32.p, c->dt.p,
                               c->floor_v.p, c->refl.p, key, c->step, c->next, c->batch, c->dep_raw.p,
                               c->path_raw.p, c->ncell, P, c->tally_copies - 1, c->stats, Q};
       if (c->sph)

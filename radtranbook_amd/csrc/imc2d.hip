@@ -417,8 +417,8 @@ struct TransportArgs {
   using R2 = typename Vec2<R>::type;
   PoolView<R> P;
   int64_t n;
-  const R2* e1p;  // (e1[i], e1[i+1])
-  const R2* e2p;  // (e2[j], e2[j+1])
+  const double2* e1p;  // (e1[i], e1[i+1]), FP64 in both precisions
+  const double2* e2p;  // (e2[j], e2[j+1])
   int nx, ny;
   const R2* sas;  // (sigma_a, sigma_s) per cell
   double dt;
@@ -496,7 +496,8 @@ __global__ __launch_bounds__(256) void k_transport(TransportArgs<R> A) {
   bool active = false;
   int64_t k = 0;
   double w = 0.0;
-  R d1 = 0, d2 = 0, p1 = 0, p2 = 0, dcen = 0;
+  R d1 = 0, d2 = 0, dcen = 0;
+  double p1 = 0.0, p2 = 0.0;  // FP64 in both precisions (cell offsets are FP32 in the FP32 movers)
   int i = 0, j = 0;
   uint32_t ev = 0;
   double acc_dep = 0.0, acc_si = 0.0;
@@ -548,8 +549,8 @@ __global__ __launch_bounds__(256) void k_transport(TransportArgs<R> A) {
     } else {
       int c = i * ny + j;
       R2 mat = elem(A.sas, (uint32_t)c);
-      R2 ex = elem(A.e1p, (uint32_t)i);
-      R2 ey = elem(A.e2p, (uint32_t)j);
+      const double2 ex = elem(A.e1p, (uint32_t)i);
+      const double2 ey = elem(A.e2p, (uint32_t)j);
       MoveOut<R> o;
       Ctr ctr = make_ctr(kTagTransport, (uint64_t)k, A.step);
       if (GEOM == 0)
@@ -751,8 +752,8 @@ struct AdjointArgs {
   using R2 = typename Vec2<R>::type;
   PoolView<R> S;  // transport input pool (read only)
   int64_t n;
-  const R2* e1p;
-  const R2* e2p;
+  const double2* e1p;  // FP64 in both precisions
+  const double2* e2p;
   int nx, ny;
   const R2* sas;
   const double* sa64;  // emission weights use the FP64 sigma_a
@@ -788,7 +789,8 @@ __global__ __launch_bounds__(256) void k_transport_adjoint(AdjointArgs<R> A) {
   int64_t k = 0;
   int phase = 0;
   double w = 0.0, Lam = 0.0, P = 0.0;
-  R d1 = 0, d2 = 0, p1 = 0, p2 = 0, dcen = 0;
+  R d1 = 0, d2 = 0, dcen = 0;
+  double p1 = 0.0, p2 = 0.0;  // FP64 in both precisions (cell offsets are FP32 in the FP32 movers)
   int i = 0, j = 0;
   uint32_t ev = 0;
   double acc_a = 0.0, acc_s = 0.0;
@@ -837,8 +839,8 @@ __global__ __launch_bounds__(256) void k_transport_adjoint(AdjointArgs<R> A) {
     } else {
       int c = i * ny + j;
       R2 mat = A.sas[c];
-      R2 ex = A.e1p[i];
-      R2 ey = A.e2p[j];
+      const double2 ex = A.e1p[i];
+      const double2 ey = A.e2p[j];
       MoveOut<R> o;
       Ctr ctr = make_ctr(kTagTransport, (uint64_t)k, A.step);
       if (GEOM == 0)
@@ -2162,7 +2164,7 @@ int imc2d_transport(imc2d_ctx* c, double dt, const int32_t* reflect, int64_t max
       else
         launch_transport<double, uint64_t>(c->geometry, grid, block, s, A);
     } else {
-      TransportArgs<float> A{c->A.view<float>(), n, c->e1p32.p, c->e2p32.p, c->nx, c->ny, c->sas32.p, dt, refl,
+      TransportArgs<float> A{c->A.view<float>(), n, c->e1p64.p, c->e2p64.p, c->nx, c->ny, c->sas32.p, dt, refl,
                              max_events, c->scal + 2, key, c->step, c->next, c->batch, c->dep_raw.p, c->si_raw.p,
                              c->ncell, c->tally_copies - 1, c->scal + 3, c->stats, Q};
       if (narrow)
@@ -2273,7 +2275,7 @@ int imc2d_transport_noalloc(imc2d_ctx* c, double dt, const int32_t* reflect, int
       else
         launch_transport<double, uint64_t>(c->geometry, grid, block, s, A);
     } else {
-      TransportArgs<float> A{c->A.view<float>(), n, c->e1p32.p, c->e2p32.p, c->nx, c->ny, c->sas32.p, dt, refl,
+      TransportArgs<float> A{c->A.view<float>(), n, c->e1p64.p, c->e2p64.p, c->nx, c->ny, c->sas32.p, dt, refl,
                              max_events, scal + 2, key, c->step, next, c->batch, dep_raw, si_raw,
                              c->ncell, c->tally_copies - 1, scal + 3, stats, Q};
       if (narrow)
@@ -2616,7 +2618,7 @@ int imc2d_adjoint(imc2d_ctx* c, double dt, const int32_t* reflect, int64_t max_e
       else
         k_transport_adjoint<double, 1><<<grid, block, 0, s>>>(A);
     } else {
-      AdjointArgs<float> A{c->S.view<float>(), n, c->e1p32.p, c->e2p32.p, c->nx, c->ny, c->sas32.p, c->sa.p, dt,
+      AdjointArgs<float> A{c->S.view<float>(), n, c->e1p64.p, c->e2p64.p, c->nx, c->ny, c->sas32.p, c->sa.p, dt,
                            refl, max_events, c->scal + 2, key, c->step, c->next, c->batch, c->volsrc.p, c->emit_lo,
                            c->emit_hi, c->g_raw.p, nc, c->tally_copies - 1, Q};
       if (c->geometry == IMC_GEOM_XY)

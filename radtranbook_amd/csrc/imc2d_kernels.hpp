@@ -12,7 +12,12 @@
 //    argmin (boundary wins ties, then scatter, as in the reference), non-face
 //    moves are clamped into the cell, the cylinder roots use the
 //    cancellation-free forms, and the absorption split keeps
-//    dep + w_new == w in FP64 so energy bookkeeping stays exact.
+//    dep + w_new == w in FP64 so energy bookkeeping stays exact.  The
+//    history's position is carried in FP64 between segments and each segment
+//    works in FP32 on its offset from the cell's lower edges: with absolute
+//    FP32 coordinates a move shorter than a few ulps of the coordinate (the
+//    capped scatter rate 1e4 / h gives ~1e-4 h; 5e-7 cm against an ulp of
+//    6e-8 cm at x = 0.5 on the 200x200 mesh) is rounded away or biased.
 #pragma once
 #include "imc_device.hpp"
 
@@ -24,10 +29,13 @@ enum Event : int { kEvtCensus = 0, kEvtBoundary = 1, kEvtScatter = 2 };
 // dep_w: energy absorbed on the segment.  dep_i: w(1-e^{-sigma_a s})/sigma_a
 // (FP64) or, in FP32, the path weight w*s used only where sigma_a <= 1e-12.
 // s: the segment length (the adjoint replay needs it).
+// Positions are FP64 in both precisions: the FP32 movers compute in FP32 on
+// offsets from the cell's lower edges and return the new position in FP64.
 template <typename R>
 struct MoveOut {
   double w, dep_w, dep_i;
-  R d1, d2, p1, p2, dcen, s;
+  double p1, p2;
+  R d1, d2, dcen, s;
   int crossed, evt;
 };
 
@@ -232,13 +240,16 @@ __device__ __forceinline__ float scatter_distance_f32(const QuadF& u, float ss, 
 // CAP: apply the gray opacity cap (sigma_s * min cell width > 1e4); the
 // multigroup tracker has none.
 template <bool CAP = true>
-__device__ __forceinline__ void move_xy(Ctr ctr, Key key, uint32_t ev, double w, float ux, float uy, float x, float y,
-                                        float xl, float xr, float yl, float yr, float sa, float ss, float dcen,
-                                        MoveOut<float>& o) {
+__device__ __forceinline__ void move_xy(Ctr ctr, Key key, uint32_t ev, double w, float ux, float uy, double x,
+                                        double y, double xl, double xr, double yl, double yr, float sa, float ss,
+                                        float dcen, MoveOut<float>& o) {
+  // offsets from the cell's lower edges and the cell widths
+  const float lx = (float)(x - xl), ly = (float)(y - yl);
+  const float wx = (float)(xr - xl), wy = (float)(yr - yl);
   // face distances by reciprocal-multiply (about 2 ulp; clamped at 0)
-  float sx = (ux != 0.0f) ? fmaxf(fdiv_fast((ux > 0.0f ? xr : xl) - x, ux), 0.0f) : 1e30f;
-  float sy = (uy != 0.0f) ? fmaxf(fdiv_fast((uy > 0.0f ? yr : yl) - y, uy), 0.0f) : 1e30f;
-  float cl = CAP ? fminf(xr - xl, yr - yl) : 0.0f;
+  float sx = (ux != 0.0f) ? fmaxf(fdiv_fast((ux > 0.0f ? wx : 0.0f) - lx, ux), 0.0f) : 1e30f;
+  float sy = (uy != 0.0f) ? fmaxf(fdiv_fast((uy > 0.0f ? wy : 0.0f) - ly, uy), 0.0f) : 1e30f;
+  float cl = CAP ? fminf(wx, wy) : 0.0f;
   QuadF u = block4_f(ctr, key, ev);  // independent of the material gathers: overlaps their latency
   float s_scat = scatter_distance_f32(u, ss, cl);
   float sb = fminf(sx, sy);
@@ -248,19 +259,17 @@ __device__ __forceinline__ void move_xy(Ctr ctr, Key key, uint32_t ev, double w,
   if (sb <= s_scat && sb <= dcen) {
     se = sb;
     o.evt = kEvtBoundary;
-    float xn = x + ux * se, yn = y + uy * se;
     if (sx <= sy) {
-      if (ux > 0.0f) { xn = xr; o.crossed = kCrossIPlus; } else { xn = xl; o.crossed = kCrossIMinus; }
-      yn = fminf(fmaxf(yn, yl), yr);
+      if (ux > 0.0f) { o.p1 = xr; o.crossed = kCrossIPlus; } else { o.p1 = xl; o.crossed = kCrossIMinus; }
+      o.p2 = yl + (double)fminf(fmaxf(ly + uy * se, 0.0f), wy);
     } else {
-      if (uy > 0.0f) { yn = yr; o.crossed = kCrossJPlus; } else { yn = yl; o.crossed = kCrossJMinus; }
-      xn = fminf(fmaxf(xn, xl), xr);
+      if (uy > 0.0f) { o.p2 = yr; o.crossed = kCrossJPlus; } else { o.p2 = yl; o.crossed = kCrossJMinus; }
+      o.p1 = xl + (double)fminf(fmaxf(lx + ux * se, 0.0f), wx);
     }
-    o.p1 = xn; o.p2 = yn;
   } else {
     se = fminf(s_scat, dcen);
-    o.p1 = fminf(fmaxf(x + ux * se, xl), xr);
-    o.p2 = fminf(fmaxf(y + uy * se, yl), yr);
+    o.p1 = xl + (double)fminf(fmaxf(lx + ux * se, 0.0f), wx);
+    o.p2 = yl + (double)fminf(fmaxf(ly + uy * se, 0.0f), wy);
     if (s_scat <= dcen) {
       o.evt = kEvtScatter;
       float uz = 2.0f * u.u2 - 1.0f;
@@ -271,6 +280,8 @@ __device__ __forceinline__ void move_xy(Ctr ctr, Key key, uint32_t ev, double w,
       o.evt = kEvtCensus;
     }
   }
+  o.p1 = fmin(fmax(o.p1, xl), xr);
+  o.p2 = fmin(fmax(o.p2, yl), yr);
   o.dcen = dcen - se;
   o.s = se;
   absorb_f32(o, w, sa, se);
@@ -278,41 +289,45 @@ __device__ __forceinline__ void move_xy(Ctr ctr, Key key, uint32_t ev, double w,
 
 // Distance to the cylinder of radius R along the transverse chord, divided by
 // sqrt(1 - eta^2).  outer: the particle is inside (r <= R); inner: outside.
-__device__ __forceinline__ float dist_outer_f32(float r, float mu, float R, float den) {
+// gap = |R - r| computed in FP64 by the caller (the cancellation-free chord
+// (R - r)(R + r) / (root + r mu) keeps its precision for r next to R).
+__device__ __forceinline__ float dist_outer_f32(float r, float mu, float R, float gap, float den) {
   float ome = (1.0f - mu) * (1.0f + mu);
   float disc = fmaxf(R * R - r * r * ome, 0.0f);
   float root = __builtin_amdgcn_sqrtf(disc);
-  float l = (mu > 0.0f) ? fdiv_fast(fmaxf((R - r) * (R + r), 0.0f), root + r * mu) : root - r * mu;
+  float l = (mu > 0.0f) ? fdiv_fast(fmaxf(gap * (R + r), 0.0f), root + r * mu) : root - r * mu;
   if (!(l >= 0.0f)) l = 0.0f;  // r == R, mu == 1 degenerate: 0/0
   return fdiv_fast(l, den);
 }
-__device__ __forceinline__ float dist_inner_f32(float r, float mu, float R, float den) {
+__device__ __forceinline__ float dist_inner_f32(float r, float mu, float R, float gap, float den) {
   if (!(mu < 0.0f)) return 1e30f;
   float ome = (1.0f - mu) * (1.0f + mu);
   float disc = R * R - r * r * ome;
   if (disc <= 0.0f) return 1e30f;
   float root = __builtin_amdgcn_sqrtf(disc);
-  float l = fdiv_fast(fmaxf((r - R) * (r + R), 0.0f), root - r * mu);
+  float l = fdiv_fast(fmaxf(gap * (r + R), 0.0f), root - r * mu);
   return fdiv_fast(l, den);
 }
 
 template <bool CAP = true>
-__device__ __forceinline__ void move_rz(Ctr ctr, Key key, uint32_t ev, double w, float mu, float eta, float r,
-                                        float z, float rin, float rout, float zlo, float zhi, float sa, float ss,
-                                        float dcen, MoveOut<float>& o) {
+__device__ __forceinline__ void move_rz(Ctr ctr, Key key, uint32_t ev, double w, float mu, float eta, double r64,
+                                        double z64, double rin64, double rout64, double zlo64, double zhi64,
+                                        float sa, float ss, float dcen, MoveOut<float>& o) {
+  const float r = (float)r64, rin = (float)rin64, rout = (float)rout64;
+  const float lz = (float)(z64 - zlo64), hz = (float)(zhi64 - zlo64);
   float ome2 = fmaxf((1.0f - eta) * (1.0f + eta), 0.0f);
   float den = __builtin_amdgcn_sqrtf(ome2);
   float s_rin = 1e30f, s_rout = 1e30f;
   if (den > 0.0f) {
-    if (rin > 0.0f) s_rin = dist_inner_f32(r, mu, rin, den);
-    s_rout = dist_outer_f32(r, mu, rout, den);
+    if (rin64 > 0.0) s_rin = dist_inner_f32(r, mu, rin, (float)(r64 - rin64), den);
+    s_rout = dist_outer_f32(r, mu, rout, (float)(rout64 - r64), den);
   }
   float s_zlo = 1e30f, s_zhi = 1e30f;
   if (eta > 0.0f)
-    s_zhi = fmaxf(fdiv_fast(zhi - z, eta), 0.0f);
+    s_zhi = fmaxf(fdiv_fast(hz - lz, eta), 0.0f);
   else if (eta < 0.0f)
-    s_zlo = fmaxf(fdiv_fast(zlo - z, eta), 0.0f);
-  float cl = CAP ? fminf(rout - rin, zhi - zlo) : 0.0f;
+    s_zlo = fmaxf(fdiv_fast(-lz, eta), 0.0f);
+  float cl = CAP ? fminf((float)(rout64 - rin64), hz) : 0.0f;
   QuadF u = block4_f(ctr, key, ev);  // independent of the material gathers: overlaps their latency
   float s_scat = scatter_distance_f32(u, ss, cl);
   float sr = fminf(s_rin, s_rout), sz = fminf(s_zlo, s_zhi);
@@ -325,26 +340,26 @@ __device__ __forceinline__ void move_rz(Ctr ctr, Key key, uint32_t ev, double w,
   else
     se = fminf(s_scat, dcen);
   // stream along the chord: r'^2 = (r + mu l)^2 + l^2 (1 - mu^2), and the new
-  // radial cosine is mu' = (r mu + l) / r'
+  // radial cosine is mu' = (r mu + l) / r'; the radius moves by
+  // r' - r = (2 r mu l + l^2) / (r' + r), added in FP64
   float l = se * den;
   float a = r + mu * l;
   float rn = __builtin_amdgcn_sqrtf(a * a + l * l * fmaxf((1.0f - mu) * (1.0f + mu), 0.0f));
-  float zn = z + se * eta;
+  float sum = rn + r;
+  float dr = sum > 0.0f ? fdiv_fast(l * (2.0f * r * mu + l), sum) : 0.0f;
+  double rn64 = r64 + (double)dr;
+  double zn64 = zlo64 + (double)fminf(fmaxf(lz + se * eta, 0.0f), hz);
   float mun = mu;
   if (rn > 0.0f && l > 0.0f) mun = fminf(1.0f, fmaxf(-1.0f, fdiv_fast(r * mu + l, rn)));
   float etan = eta;
   if (bnd) {
     o.evt = kEvtBoundary;
     if (sr <= sz) {
-      if (s_rin <= s_rout) { rn = rin; o.crossed = kCrossIMinus; } else { rn = rout; o.crossed = kCrossIPlus; }
-      zn = fminf(fmaxf(zn, zlo), zhi);
+      if (s_rin <= s_rout) { rn64 = rin64; o.crossed = kCrossIMinus; } else { rn64 = rout64; o.crossed = kCrossIPlus; }
     } else {
-      if (s_zlo <= s_zhi) { zn = zlo; o.crossed = kCrossJMinus; } else { zn = zhi; o.crossed = kCrossJPlus; }
-      rn = fminf(fmaxf(rn, rin), rout);
+      if (s_zlo <= s_zhi) { zn64 = zlo64; o.crossed = kCrossJMinus; } else { zn64 = zhi64; o.crossed = kCrossJPlus; }
     }
   } else {
-    rn = fminf(fmaxf(rn, rin), rout);
-    zn = fminf(fmaxf(zn, zlo), zhi);
     if (s_scat <= dcen) {
       o.evt = kEvtScatter;
       etan = 2.0f * u.u2 - 1.0f;
@@ -353,7 +368,9 @@ __device__ __forceinline__ void move_rz(Ctr ctr, Key key, uint32_t ev, double w,
       o.evt = kEvtCensus;
     }
   }
-  o.d1 = mun; o.d2 = etan; o.p1 = rn; o.p2 = zn;
+  o.d1 = mun; o.d2 = etan;
+  o.p1 = fmin(fmax(rn64, rin64), rout64);
+  o.p2 = fmin(fmax(zn64, zlo64), zhi64);
   o.dcen = dcen - se;
   o.s = se;
   absorb_f32(o, w, sa, se);

@@ -418,8 +418,8 @@ struct MgArgs {
   using R2 = typename Vec2<R>::type;
   MgView<R> P;
   int64_t n;
-  const R2* e1p;
-  const R2* e2p;
+  const double2* e1p;  // FP64 in both precisions
+  const double2* e2p;
   int nx, ny, G;
   const R2* sas;  // (sigma_a, sigma_s) per bin
   double dt;
@@ -463,7 +463,8 @@ __global__ __launch_bounds__(256) void k_mg_transport(MgArgs<R> A) {
   bool active = false;
   int64_t k = 0;
   double w = 0.0, t = 0.0;
-  R d1 = 0, d2 = 0, p1 = 0, p2 = 0, dcen = 0;
+  R d1 = 0, d2 = 0, dcen = 0;
+  double p1 = 0.0, p2 = 0.0;  // FP64 in both precisions (cell offsets are FP32 in the FP32 movers)
   int i = 0, j = 0, g = 0;
   uint32_t ev = 0;
   double acc_dep = 0.0, acc_path = 0.0;
@@ -513,8 +514,8 @@ __global__ __launch_bounds__(256) void k_mg_transport(MgArgs<R> A) {
       const int64_t c = (int64_t)i * ny + j;
       const int bin = (int)((int64_t)g * ncell + c);
       R2 mat = A.sas[bin];
-      R2 ex = A.e1p[i];
-      R2 ey = A.e2p[j];
+      const double2 ex = A.e1p[i];
+      const double2 ey = A.e2p[j];
       Ctr ctr = make_ctr(kTagTransport, (uint64_t)k, A.step);
       double dep, path;
       int evt, crossed;
@@ -598,8 +599,8 @@ __global__ __launch_bounds__(256) void k_mg_transport(MgArgs<R> A) {
       A.P.w[k] = w;
       A.P.d1[k] = d1;
       A.P.d2[k] = d2;
-      A.P.p1[k] = p1;
-      A.P.p2[k] = p2;
+      A.P.p1[k] = (R)p1;
+      A.P.p2[k] = (R)p2;
       A.P.t[k] = (R)(sizeof(R) == 8 ? t : A.dt - (double)dcen / kC);
       A.P.ci[k] = i;
       A.P.cj[k] = j;
@@ -954,7 +955,8 @@ __global__ __launch_bounds__(256) void k_m1_transport(MgArgs<R> A) {
   bool active = false;
   int64_t k = 0;
   double w = 0.0;
-  R mu = 0, x = 0, dcen = 0;
+  R mu = 0, dcen = 0;
+  double x = 0.0;  // FP64 in both precisions (the FP32 mover works on shell gaps)
   int cell = 0, g = 0;
   uint32_t ev = 0;
   double acc_dep = 0.0, acc_path = 0.0;
@@ -984,7 +986,7 @@ __global__ __launch_bounds__(256) void k_m1_transport(MgArgs<R> A) {
     bool finish = false;
     ++st_ev;
     const int bin = g * I + cell;
-    R2 edg = A.e1p[cell];
+    const double2 edg = A.e1p[cell];
     R2 mat = A.sas[bin];
     Move1D<R> o;
     Ctr ctr = make_ctr(kTagTransport, (uint64_t)k, A.step);
@@ -1042,7 +1044,7 @@ __global__ __launch_bounds__(256) void k_m1_transport(MgArgs<R> A) {
     if (finish) {
       A.P.w[k] = w;
       A.P.d1[k] = mu;
-      A.P.p1[k] = x;
+      A.P.p1[k] = (R)x;
       A.P.ci[k] = cell;
       active = false;
     }
@@ -1741,7 +1743,7 @@ int mgimc_transport(mgimc_ctx* c, double dt, const int32_t* reflect, int64_t max
       else
         k_mg_transport<double, 1><<<grid, block, 0, s>>>(A);
     } else {
-      MgArgs<float> A{c->A.view<float>(), n, c->e1p32.p, c->e2p32.p, c->nx, c->ny, G, c->sas32.p, dt, refl,
+      MgArgs<float> A{c->A.view<float>(), n, c->e1p64.p, c->e2p64.p, c->nx, c->ny, G, c->sas32.p, dt, refl,
                       max_events, c->scal.p + 2, key, c->step, c->next, c->batch, c->dep_raw.p, c->path_raw.p, nb,
                       c->tally_copies - 1, c->stats, Q};
       if (c->geometry == IMC_GEOM_XY)
@@ -2066,7 +2068,7 @@ int mgimc_sphere_transport(mgimc_ctx* c, double dt, const int32_t* reflect, doub
                        c->tally_copies - 1, c->stats, pool_order_queue(n)};
       k_m1_transport<double><<<grid, block, 0, s>>>(A);
     } else {
-      MgArgs<float> A{c->A.view<float>(), n, c->e1p32.p, c->e2p32.p, c->nx, 1, G, c->sas32.p, dt, refl, 0,
+      MgArgs<float> A{c->A.view<float>(), n, c->e1p64.p, c->e2p64.p, c->nx, 1, G, c->sas32.p, dt, refl, 0,
                       c->scal.p + 2, key, c->step, c->next, c->batch, c->dep_raw.p, c->path_raw.p, nb,
                       c->tally_copies - 1, c->stats, pool_order_queue(n)};
       k_m1_transport<float><<<grid, block, 0, s>>>(A);

@@ -153,11 +153,14 @@ def test_autograd_matches_fd_without_scattering(L):
 
 def test_autograd_matches_fd_in_mean(L):
     """With scattering: the score-function gradient against 3% central
-    differences with common random numbers, compared over seeds."""
+    differences with common random numbers, compared over seeds, along a
+    random direction of the opacity scale and of the c_v scale (the chain rule
+    dsigma_a/dcv_scale = sigma f (1 - f) / cv_scale = -dsigma_s/dcv_scale)."""
     import torch
 
     _announce("test_autograd_matches_fd_in_mean")
     ad, fd = [], []
+    adc, fdc = [], []
     for seed in range(10):
         step, n, m = _rz_step(L, "fp64", 20 + seed, N=16000)
         rng = np.random.default_rng(99)
@@ -176,10 +179,15 @@ def test_autograd_matches_fd_in_mean(L):
         h = 0.03
         fd.append((_loss(step.forward_np(s0 + h * v, 1.0), w, Es) - _loss(step.forward_np(s0 - h * v, 1.0), w, Es))
                   / (2 * h))
-        assert np.all(np.isfinite(cvs.grad.numpy()))
-    ad = np.array(ad); fd = np.array(fd)
-    d = ad - fd
-    z = abs(d.mean()) / (d.std(ddof=1) / np.sqrt(len(d)) + 1e-300)
-    _announce(f"  autograd {ad.mean():.4e} +- {ad.std(ddof=1) / np.sqrt(len(ad)):.1e}, "
-              f"fd {fd.mean():.4e} +- {fd.std(ddof=1) / np.sqrt(len(fd)):.1e}, z {z:.2f}")
-    assert z < 4.0
+        c = np.random.default_rng(77).standard_normal((n, m))
+        adc.append(float(np.sum(cvs.grad.numpy() * c)))
+        fdc.append((_loss(step.forward_np(s0, 1.0 + h * c), w, Es) - _loss(step.forward_np(s0, 1.0 - h * c), w, Es))
+                   / (2 * h))
+    for name, a_, f_ in (("opacity scale", ad, fd), ("c_v scale", adc, fdc)):
+        a_ = np.array(a_); f_ = np.array(f_)
+        d = a_ - f_
+        z = abs(d.mean()) / (d.std(ddof=1) / np.sqrt(len(d)) + 1e-300)
+        _announce(f"  {name}: autograd {a_.mean():.4e} +- {a_.std(ddof=1) / np.sqrt(len(a_)):.1e}, "
+                  f"fd {f_.mean():.4e} +- {f_.std(ddof=1) / np.sqrt(len(f_)):.1e}, z {z:.2f}")
+        assert z < 4.0
+        assert np.sign(a_.mean()) == np.sign(f_.mean())
