@@ -412,7 +412,7 @@ struct Transport1Args {
   int64_t n;
   const int64_t* poff;   // pool offset of each problem (P+1)
   const int64_t* coff;   // cell offset of each problem (P+1)
-  const double2* ep;     // (r0, r1) per cell, FP64 in both precisions
+  const R2* ep;          // per cell: FP64 (r0, r1); FP32 (r0, r1 - r0)
   const R2* sas;         // (sigma_a, sigma_s) per cell
   const double* dt;      // per problem
   const double* wfloor;  // per problem
@@ -451,7 +451,11 @@ __global__ __launch_bounds__(256) void k1_transport(Transport1Args<R> A) {
   int64_t k = 0;
   double w = 0.0;
   R mu = 0, dcen = 0;
-  double x = 0.0;  // FP64 in both precisions (the FP32 mover works on cell offsets)
+  // FP64: the position.  FP32: the offset from the cell's inner edge
+  // (kAtUpperFace: on the outer face), or, while `fresh`, the position read
+  // from the pool; q holds the position to write back.
+  R x = 0, q = 0;
+  bool fresh = false;
   int cell = 0, p = 0, I = 0;
   int64_t cb = 0;
   double wfl = 0.0;
@@ -468,6 +472,8 @@ __global__ __launch_bounds__(256) void k1_transport(Transport1Args<R> A) {
         w = A.P.w[k];
         mu = A.P.mu[k];
         x = A.P.x[k];
+        q = x;
+        fresh = true;
         cell = A.P.cell[k];
         p = A.P.prob[k];
         cb = A.coff[p];
@@ -487,10 +493,14 @@ __global__ __launch_bounds__(256) void k1_transport(Transport1Args<R> A) {
     ++st_ev;
     const int loc = cell;
     const int64_t c = cb + loc;
-    const double2 edg = A.ep[c];
+    const R2 edg = A.ep[c];
     R2 mat = A.sas[c];
     Move1D<R> o;
     Ctr ctr = make_ctr(kTagTransport, ((uint64_t)p << 32) | (uint64_t)(k - A.poff[p]), A.step);
+    if constexpr (kF32<R>) {
+      x = fresh ? fminf(fmaxf(x - edg.x, 0.0f), edg.y) : resolve_offset(x, edg.y);
+      fresh = false;
+    }
     move_1d<SPH>(ctr, A.key, ev, w, mu, x, edg.x, edg.y, mat.x, mat.y, dcen, o);
     ++ev;
     if (o.bad) {
@@ -511,15 +521,28 @@ __global__ __launch_bounds__(256) void k1_transport(Transport1Args<R> A) {
       w = o.w;
       mu = o.mu;
       x = o.x;
+      if constexpr (kF32<R>) q = edg.x + x;
       cell += o.newloc;
+      if (o.newloc > 0) enter_lower(x);
+      else if (o.newloc < 0) enter_upper(x);
       dcen = o.dcen;
       st_bnd += o.code == 1;
       st_sc += o.code == 2;
       st_cen += o.code == 3;
       if (cell == I) {
-        if (A.refl[2 * p + 1]) { mu = -mu; cell -= 1; ++st_refl; } else dcen = 0;
+        if (A.refl[2 * p + 1]) {
+          mu = -mu; cell -= 1; ++st_refl;
+          enter_upper(x);
+        } else {
+          dcen = 0;
+        }
       } else if (cell == -1) {
-        if (A.refl[2 * p]) { mu = -mu; cell += 1; ++st_refl; } else dcen = 0;
+        if (A.refl[2 * p]) {
+          mu = -mu; cell += 1; ++st_refl;
+          enter_lower(x);
+        } else {
+          dcen = 0;
+        }
       }
       if (w < wfl) {
         ++st_floor;
@@ -544,7 +567,7 @@ __global__ __launch_bounds__(256) void k1_transport(Transport1Args<R> A) {
       }
       A.P.w[k] = w;
       A.P.mu[k] = mu;
-      A.P.x[k] = (R)x;
+      A.P.x[k] = kF32<R> ? q : x;
       A.P.cell[k] = cell;
       active = false;
     }
@@ -1092,7 +1115,7 @@ int imc1d_create(imc1d_ctx** out, int geometry, int nprob, const int32_t* ncells
       IMC_REQUIRE(r1 > r0 && (!c->sph || r0 >= 0.0), kErrArg, "imc1d_create: bad edges in problem %d", p);
       int64_t ci = c->h_coff[p] + i;
       e64[ci] = make_double2(r0, r1);
-      e32[ci] = make_float2((float)r0, (float)r1);
+      e32[ci] = make_float2((float)r0, (float)r1 - (float)r0);
       vol[ci] = c->sph ? (4.0 / 3.0) * M_PI * (r1 * r1 * r1 - r0 * r0 * r0) : (r1 - r0);
       cp[ci] = p;
     }
@@ -1711,7 +1734,7 @@ int imc1d_transport(imc1d_ctx* c, const int32_t* reflect, const double* weight_f
       else
         k1_transport<double, 0><<<grid, block, 0, s>>>(A);
     } else {
-      Transport1Args<float> A{c->B.view<float>(), n, c->poff.p, c->coff.p, c->ep64.p, c->sas32.p, c->dt.p,
+      Transport1Args<float> A{c->B.view<float>(), n, c->poff.p, c->coff.p, c->ep32.p, c->sas32.p, c->dt.p,
                               c->floor_v.p, c->refl.p, key, c->step, c->next, c->batch, c->dep_raw.p,
                               c->path_raw.p, c->ncell, P, c->tally_copies - 1, c->stats, Q};
       if (c->sph)

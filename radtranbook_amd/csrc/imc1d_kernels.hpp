@@ -5,9 +5,13 @@
 //    (move_particle_spherical) operation for operation: 1e-10 face
 //    tolerance, 1e-10 opacity thresholds, opacity cap sigma_s dx > 1e4,
 //    non-positive distances reported as failures (the reference asserts).
-//  * float: FP32 geometry with FP64 weights and positions; argmin event
-//    choice (face, then scatter, then census), moves clamped into the cell,
-//    cancellation-free sphere chords and radius updates.
+//  * float: FP32 geometry with FP64 weights; argmin event choice (face,
+//    then scatter, then census), moves clamped into the cell,
+//    cancellation-free sphere chords and radius updates.  The position is an
+//    FP32 offset from the cell's inner edge, carried by the tracker between
+//    segments (cell tables (r0, r1 - r0) in FP32), so that moves of ~1e-4 of
+//    a cell (the capped scatter rate) keep their precision, which against an
+//    absolute FP32 coordinate they would not.
 // Outputs are raw energies: dep = w (1 - e^{-sigma_a s}), path = w s; the
 // caller divides by the cell volume (dx for slab) once per cell.
 #pragma once
@@ -16,16 +20,12 @@
 
 namespace imc {
 
-// The position is FP64 in both precisions: the FP32 mover computes in FP32 on
-// the offset from the cell's inner edge (or, in the sphere, the radius and the
-// FP64 gaps to the cell's shells) and returns the new position in FP64, so
-// that moves of ~1e-4 of a cell (the capped scatter rate) are not rounded
-// away against the absolute coordinate.
+// x: the new position (FP64) or the new offset from the cell's inner edge
+// (FP32; a face for a crossing, in the coordinates of the cell moved in).
 template <typename R>
 struct Move1D {
   double w, dep, path;
-  double x;
-  R mu, dcen;
+  R x, mu, dcen;
   int newloc;  // -1, 0, +1
   int code;    // 1 boundary, 2 scatter, 3 census
   bool bad;    // non-positive distance (reference AssertionError)
@@ -127,25 +127,26 @@ __device__ __forceinline__ void move_1d(Ctr ctr, Key key, uint32_t ev, double w,
 // u2 new direction cosine.
 // ---------------------------------------------------------------------------
 template <int SPH>
-__device__ __forceinline__ void move_1d(Ctr ctr, Key key, uint32_t ev, double w, float mu, double x64, double r064,
-                                        double r164, float sa, float ss, float dcen, Move1D<float>& o) {
-  const float dxc = (float)(r164 - r064);
-  const float lo = (float)(x64 - r064), hi = (float)(r164 - x64);  // gaps to the cell's faces
-  const float x = (float)x64, r0 = (float)r064, r1 = (float)r164;
+__device__ __forceinline__ void move_1d(Ctr ctr, Key key, uint32_t ev, double w, float mu, float lx, float r0,
+                                        float dxc, float sa, float ss, float dcen, Move1D<float>& o) {
+  const float lo = lx, hi = dxc - lx;  // gaps to the cell's faces
+  const float x = r0 + lx, r1 = r0 + dxc;
   float db;
   int hit_outer = 1;
   if (!SPH) {
     db = (mu != 0.0f) ? fmaxf(fdiv_fast(mu > 0.0f ? hi : -lo, mu), 0.0f) : 1e30f;
   } else {
-    float ome = fmaxf((1.0f - mu) * (1.0f + mu), 0.0f);
-    float rootout = __builtin_amdgcn_sqrtf(fmaxf(r1 * r1 - x * x * ome, 0.0f));
-    float dout = (mu > 0.0f) ? fdiv_fast(fmaxf(hi * (r1 + x), 0.0f), rootout + x * mu) : rootout - x * mu;
+    // r1^2 - x^2 (1 - mu^2) = hi (r1 + x) + (x mu)^2 and
+    // r0^2 - x^2 (1 - mu^2) = (x mu)^2 - lo (x + r0): no cancellation next to a shell
+    const float xm = x * mu;
+    float rootout = __builtin_amdgcn_sqrtf(fmaxf(hi * (r1 + x) + xm * xm, 0.0f));
+    float dout = (mu > 0.0f) ? fdiv_fast(fmaxf(hi * (r1 + x), 0.0f), rootout + xm) : rootout - xm;
     if (!(dout >= 0.0f)) dout = 0.0f;
     float din = 1e30f;
-    float discin = r0 * r0 - x * x * ome;
-    if (mu < 0.0f && discin > 0.0f) {
+    float discin = xm * xm - lo * (x + r0);
+    if (mu < 0.0f && r0 > 0.0f && discin > 0.0f) {
       float rootin = __builtin_amdgcn_sqrtf(discin);
-      din = fdiv_fast(fmaxf(lo * (x + r0), 0.0f), rootin - x * mu);
+      din = fdiv_fast(fmaxf(lo * (x + r0), 0.0f), rootin - xm);
     }
     db = fminf(dout, din);
     hit_outer = (dout <= din);
@@ -156,8 +157,7 @@ __device__ __forceinline__ void move_1d(Ctr ctr, Key key, uint32_t ev, double w,
   if (ss * dxc > 10000.0f) dsc = neg_ln_fast(1.0f - u.u1) * (dxc * 1.0e-4f);
   o.bad = false;
   o.newloc = 0;
-  float s, mn = mu;
-  double rn;
+  float s, mn = mu, rn;
   if (db <= dsc && db <= dcen) {
     s = db;
     o.code = 1;
@@ -165,23 +165,23 @@ __device__ __forceinline__ void move_1d(Ctr ctr, Key key, uint32_t ev, double w,
       float a = x + mu * s;
       float rr = __builtin_amdgcn_sqrtf(a * a + s * s * fmaxf((1.0f - mu) * (1.0f + mu), 0.0f));
       mn = rr > 0.0f ? fminf(1.0f, fmaxf(-1.0f, fdiv_fast(x * mu + s, rr))) : mu;
-      if (hit_outer) { rn = r164; o.newloc = 1; } else { rn = r064; o.newloc = -1; }
+      if (hit_outer) { rn = dxc; o.newloc = 1; } else { rn = 0.0f; o.newloc = -1; }
     } else {
-      if (mu > 0.0f) { rn = r164; o.newloc = 1; } else { rn = r064; o.newloc = -1; }
+      if (mu > 0.0f) { rn = dxc; o.newloc = 1; } else { rn = 0.0f; o.newloc = -1; }
     }
   } else {
     s = fminf(dsc, dcen);
     if (!SPH) {
-      rn = r064 + (double)(lo + mu * s);
+      rn = lo + mu * s;
     } else {
-      // r' - r = s (2 r mu + s) / (r' + r), added in FP64
+      // r' - r = s (2 r mu + s) / (r' + r), added to the offset
       float a = x + mu * s;
       float rr = __builtin_amdgcn_sqrtf(a * a + s * s * fmaxf((1.0f - mu) * (1.0f + mu), 0.0f));
       mn = rr > 0.0f ? fminf(1.0f, fmaxf(-1.0f, fdiv_fast(x * mu + s, rr))) : 1.0f;
       float sum = rr + x;
-      rn = x64 + (double)(sum > 0.0f ? fdiv_fast(s * (2.0f * x * mu + s), sum) : 0.0f);
+      rn = lo + (sum > 0.0f ? fdiv_fast(s * (2.0f * x * mu + s), sum) : 0.0f);
     }
-    rn = fmin(fmax(rn, r064), r164);
+    rn = fminf(fmaxf(rn, 0.0f), dxc);
     if (dsc <= dcen) {
       o.code = 2;
       mn = 2.0f * u.u2 - 1.0f;

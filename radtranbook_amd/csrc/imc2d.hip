@@ -417,8 +417,8 @@ struct TransportArgs {
   using R2 = typename Vec2<R>::type;
   PoolView<R> P;
   int64_t n;
-  const double2* e1p;  // (e1[i], e1[i+1]), FP64 in both precisions
-  const double2* e2p;  // (e2[j], e2[j+1])
+  const R2* e1p;  // FP64: (e1[i], e1[i+1]); FP32: (e1[i], e1[i+1] - e1[i])
+  const R2* e2p;  // the same for e2
   int nx, ny;
   const R2* sas;  // (sigma_a, sigma_s) per cell
   double dt;
@@ -497,7 +497,11 @@ __global__ __launch_bounds__(256) void k_transport(TransportArgs<R> A) {
   int64_t k = 0;
   double w = 0.0;
   R d1 = 0, d2 = 0, dcen = 0;
-  double p1 = 0.0, p2 = 0.0;  // FP64 in both precisions (cell offsets are FP32 in the FP32 movers)
+  // FP64: the position.  FP32: offsets from the cell's lower edges
+  // (kAtUpperFace: on the upper face), or, while `fresh`, the position read
+  // from the pool; q1, q2 hold the position to write back.
+  R p1 = 0, p2 = 0, q1 = 0, q2 = 0;
+  bool fresh = false;
   int i = 0, j = 0;
   uint32_t ev = 0;
   double acc_dep = 0.0, acc_si = 0.0;
@@ -524,6 +528,9 @@ __global__ __launch_bounds__(256) void k_transport(TransportArgs<R> A) {
         d2 = elem(A.P.d2, ko);
         p1 = elem(A.P.p1, ko);
         p2 = elem(A.P.p2, ko);
+        q1 = p1;
+        q2 = p2;
+        fresh = true;
         i = elem(A.P.ci, ko);
         j = elem(A.P.cj, ko);
         dcen = (R)((A.dt - (double)elem(A.P.t, ko)) * kC);
@@ -549,14 +556,11 @@ __global__ __launch_bounds__(256) void k_transport(TransportArgs<R> A) {
     } else {
       int c = i * ny + j;
       R2 mat = elem(A.sas, (uint32_t)c);
-      const double2 ex = elem(A.e1p, (uint32_t)i);
-      const double2 ey = elem(A.e2p, (uint32_t)j);
+      const R2 ex = elem(A.e1p, (uint32_t)i);
+      const R2 ey = elem(A.e2p, (uint32_t)j);
       MoveOut<R> o;
       Ctr ctr = make_ctr(kTagTransport, (uint64_t)k, A.step);
-      if (GEOM == 0)
-        move_xy(ctr, A.key, ev - 1u, w, d1, d2, p1, p2, ex.x, ex.y, ey.x, ey.y, mat.x, mat.y, dcen, o);
-      else
-        move_rz(ctr, A.key, ev - 1u, w, d1, d2, p1, p2, ex.x, ex.y, ey.x, ey.y, mat.x, mat.y, dcen, o);
+      segment_move<GEOM>(ctr, A.key, ev - 1u, w, d1, d2, p1, p2, fresh, ex, ey, mat, dcen, o);
       dcen = o.dcen;
       st_bnd += o.evt == kEvtBoundary;
       st_sc += o.evt == kEvtScatter;
@@ -576,13 +580,15 @@ __global__ __launch_bounds__(256) void k_transport(TransportArgs<R> A) {
       if (o.w <= wfloor) {
         ++st_floor;
         finish = true;  // weight kept, position and direction not advanced
+        if constexpr (kF32<R>) { q1 = ex.x + p1; q2 = ey.x + p2; }
       } else {
         d1 = o.d1; d2 = o.d2; p1 = o.p1; p2 = o.p2;
+        if constexpr (kF32<R>) { q1 = ex.x + p1; q2 = ey.x + p2; }
         if (o.crossed != kCrossNone) {
-          if (o.crossed == kCrossIPlus) ++i;
-          else if (o.crossed == kCrossIMinus) --i;
-          else if (o.crossed == kCrossJPlus) ++j;
-          else --j;
+          if (o.crossed == kCrossIPlus) { ++i; enter_lower(p1); }
+          else if (o.crossed == kCrossIMinus) { --i; enter_upper(p1); }
+          else if (o.crossed == kCrossJPlus) { ++j; enter_lower(p2); }
+          else { --j; enter_upper(p2); }
           int side = -1;
           if (i < 0) side = 0;
           else if (i >= nx) side = 1;
@@ -590,8 +596,7 @@ __global__ __launch_bounds__(256) void k_transport(TransportArgs<R> A) {
           else if (j >= ny) side = 3;
           if (side >= 0) {
             if ((A.reflect >> side) & 1) {
-              if (side < 2) { d1 = -d1; i = side == 0 ? 0 : nx - 1; }
-              else { d2 = -d2; j = side == 2 ? 0 : ny - 1; }
+              reflect_at(side, nx, ny, i, j, d1, d2, p1, p2);
               ++st_refl;
             } else {
               bl += w;
@@ -616,8 +621,8 @@ __global__ __launch_bounds__(256) void k_transport(TransportArgs<R> A) {
       stg.w[wv][slot] = w;
       stg.d1[wv][slot] = d1;
       stg.d2[wv][slot] = d2;
-      stg.p1[wv][slot] = p1;
-      stg.p2[wv][slot] = p2;
+      stg.p1[wv][slot] = kF32<R> ? q1 : p1;
+      stg.p2[wv][slot] = kF32<R> ? q2 : p2;
       stg.ci[wv][slot] = i;
       stg.cj[wv][slot] = j;
       stg.k[wv][slot] = (Off)k;
@@ -752,8 +757,8 @@ struct AdjointArgs {
   using R2 = typename Vec2<R>::type;
   PoolView<R> S;  // transport input pool (read only)
   int64_t n;
-  const double2* e1p;  // FP64 in both precisions
-  const double2* e2p;
+  const R2* e1p;  // as in TransportArgs
+  const R2* e2p;
   int nx, ny;
   const R2* sas;
   const double* sa64;  // emission weights use the FP64 sigma_a
@@ -790,7 +795,8 @@ __global__ __launch_bounds__(256) void k_transport_adjoint(AdjointArgs<R> A) {
   int phase = 0;
   double w = 0.0, Lam = 0.0, P = 0.0;
   R d1 = 0, d2 = 0, dcen = 0;
-  double p1 = 0.0, p2 = 0.0;  // FP64 in both precisions (cell offsets are FP32 in the FP32 movers)
+  R p1 = 0, p2 = 0;  // as in k_transport
+  bool fresh = false;
   int i = 0, j = 0;
   uint32_t ev = 0;
   double acc_a = 0.0, acc_s = 0.0;
@@ -801,6 +807,7 @@ __global__ __launch_bounds__(256) void k_transport_adjoint(AdjointArgs<R> A) {
     d2 = A.S.d2[q];
     p1 = A.S.p1[q];
     p2 = A.S.p2[q];
+    fresh = true;
     i = A.S.ci[q];
     j = A.S.cj[q];
     dcen = (R)((A.dt - (double)A.S.t[q]) * kC);
@@ -839,14 +846,11 @@ __global__ __launch_bounds__(256) void k_transport_adjoint(AdjointArgs<R> A) {
     } else {
       int c = i * ny + j;
       R2 mat = A.sas[c];
-      const double2 ex = A.e1p[i];
-      const double2 ey = A.e2p[j];
+      const R2 ex = A.e1p[i];
+      const R2 ey = A.e2p[j];
       MoveOut<R> o;
       Ctr ctr = make_ctr(kTagTransport, (uint64_t)k, A.step);
-      if (GEOM == 0)
-        move_xy(ctr, A.key, ev - 1u, w, d1, d2, p1, p2, ex.x, ex.y, ey.x, ey.y, mat.x, mat.y, dcen, o);
-      else
-        move_rz(ctr, A.key, ev - 1u, w, d1, d2, p1, p2, ex.x, ex.y, ey.x, ey.y, mat.x, mat.y, dcen, o);
+      segment_move<GEOM>(ctr, A.key, ev - 1u, w, d1, d2, p1, p2, fresh, ex, ey, mat, dcen, o);
       dcen = o.dcen;
       const double lc = A.lam[c];
       const double Pprev = P;
@@ -858,7 +862,7 @@ __global__ __launch_bounds__(256) void k_transport_adjoint(AdjointArgs<R> A) {
         }
         const double sseg = (double)o.s;
         acc_a += -sseg * (Lam - P) + lc * o.w * sseg;
-        R cl = fmin(ex.y - ex.x, ey.y - ey.x);
+        R cl = fmin(span_width(ex), span_width(ey));
         if (mat.y > (R)1e-12 && !(mat.y * cl > (R)10000))
           acc_s += ((o.evt == kEvtScatter ? 1.0 / (double)mat.y : 0.0) - sseg) * (Lam - Pprev);
       }
@@ -868,10 +872,10 @@ __global__ __launch_bounds__(256) void k_transport_adjoint(AdjointArgs<R> A) {
       } else {
         d1 = o.d1; d2 = o.d2; p1 = o.p1; p2 = o.p2;
         if (o.crossed != kCrossNone) {
-          if (o.crossed == kCrossIPlus) ++i;
-          else if (o.crossed == kCrossIMinus) --i;
-          else if (o.crossed == kCrossJPlus) ++j;
-          else --j;
+          if (o.crossed == kCrossIPlus) { ++i; enter_lower(p1); }
+          else if (o.crossed == kCrossIMinus) { --i; enter_upper(p1); }
+          else if (o.crossed == kCrossJPlus) { ++j; enter_lower(p2); }
+          else { --j; enter_upper(p2); }
           int side = -1;
           if (i < 0) side = 0;
           else if (i >= nx) side = 1;
@@ -879,8 +883,7 @@ __global__ __launch_bounds__(256) void k_transport_adjoint(AdjointArgs<R> A) {
           else if (j >= ny) side = 3;
           if (side >= 0) {
             if ((A.reflect >> side) & 1) {
-              if (side < 2) { d1 = -d1; i = side == 0 ? 0 : nx - 1; }
-              else { d2 = -d2; j = side == 2 ? 0 : ny - 1; }
+              reflect_at(side, nx, ny, i, j, d1, d2, p1, p2);
             } else {
               w = 0.0;
               dcen = 0;
@@ -1657,11 +1660,11 @@ int imc2d_create(imc2d_ctx** out, int geometry, int nx, int ny, const double* ed
     std::vector<float2> a32(nx), b32(ny);
     std::vector<double2> a64(nx), b64(ny);
     for (int q = 0; q < nx; ++q) {
-      a32[q] = make_float2((float)edges1[q], (float)edges1[q + 1]);
+      a32[q] = make_float2((float)edges1[q], (float)edges1[q + 1] - (float)edges1[q]);
       a64[q] = make_double2(edges1[q], edges1[q + 1]);
     }
     for (int q = 0; q < ny; ++q) {
-      b32[q] = make_float2((float)edges2[q], (float)edges2[q + 1]);
+      b32[q] = make_float2((float)edges2[q], (float)edges2[q + 1] - (float)edges2[q]);
       b64[q] = make_double2(edges2[q], edges2[q + 1]);
     }
     IMC_HIP(hipMemcpy(c->e1p32.p, a32.data(), sizeof(float2) * nx, hipMemcpyHostToDevice));
@@ -2164,7 +2167,7 @@ int imc2d_transport(imc2d_ctx* c, double dt, const int32_t* reflect, int64_t max
       else
         launch_transport<double, uint64_t>(c->geometry, grid, block, s, A);
     } else {
-      TransportArgs<float> A{c->A.view<float>(), n, c->e1p64.p, c->e2p64.p, c->nx, c->ny, c->sas32.p, dt, refl,
+      TransportArgs<float> A{c->A.view<float>(), n, c->e1p32.p, c->e2p32.p, c->nx, c->ny, c->sas32.p, dt, refl,
                              max_events, c->scal + 2, key, c->step, c->next, c->batch, c->dep_raw.p, c->si_raw.p,
                              c->ncell, c->tally_copies - 1, c->scal + 3, c->stats, Q};
       if (narrow)
@@ -2275,7 +2278,7 @@ int imc2d_transport_noalloc(imc2d_ctx* c, double dt, const int32_t* reflect, int
       else
         launch_transport<double, uint64_t>(c->geometry, grid, block, s, A);
     } else {
-      TransportArgs<float> A{c->A.view<float>(), n, c->e1p64.p, c->e2p64.p, c->nx, c->ny, c->sas32.p, dt, refl,
+      TransportArgs<float> A{c->A.view<float>(), n, c->e1p32.p, c->e2p32.p, c->nx, c->ny, c->sas32.p, dt, refl,
                              max_events, scal + 2, key, c->step, next, c->batch, dep_raw, si_raw,
                              c->ncell, c->tally_copies - 1, scal + 3, stats, Q};
       if (narrow)
@@ -2618,7 +2621,7 @@ int imc2d_adjoint(imc2d_ctx* c, double dt, const int32_t* reflect, int64_t max_e
       else
         k_transport_adjoint<double, 1><<<grid, block, 0, s>>>(A);
     } else {
-      AdjointArgs<float> A{c->S.view<float>(), n, c->e1p64.p, c->e2p64.p, c->nx, c->ny, c->sas32.p, c->sa.p, dt,
+      AdjointArgs<float> A{c->S.view<float>(), n, c->e1p32.p, c->e2p32.p, c->nx, c->ny, c->sas32.p, c->sa.p, dt,
                            refl, max_events, c->scal + 2, key, c->step, c->next, c->batch, c->volsrc.p, c->emit_lo,
                            c->emit_hi, c->g_raw.p, nc, c->tally_copies - 1, Q};
       if (c->geometry == IMC_GEOM_XY)

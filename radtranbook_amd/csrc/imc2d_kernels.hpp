@@ -13,12 +13,14 @@
 //    moves are clamped into the cell, the cylinder roots use the
 //    cancellation-free forms, and the absorption split keeps
 //    dep + w_new == w in FP64 so energy bookkeeping stays exact.  The
-//    history's position is carried in FP64 between segments and each segment
-//    works in FP32 on its offset from the cell's lower edges: with absolute
-//    FP32 coordinates a move shorter than a few ulps of the coordinate (the
-//    capped scatter rate 1e4 / h gives ~1e-4 h; 5e-7 cm against an ulp of
-//    6e-8 cm at x = 0.5 on the 200x200 mesh) is rounded away or biased.
+//    tracker carries the history's position as FP32 offsets from its cell's
+//    lower edges: with absolute FP32 coordinates a move shorter than a few
+//    ulps of the coordinate (the capped scatter rate 1e4 / h gives ~1e-4 h;
+//    5e-7 cm against an ulp of 6e-8 cm at x = 0.5 on the 200x200 mesh) is
+//    rounded away or biased, while an offset keeps ~1e-7 of the cell width.
 #pragma once
+#include <type_traits>
+
 #include "imc_device.hpp"
 
 namespace imc {
@@ -29,13 +31,12 @@ enum Event : int { kEvtCensus = 0, kEvtBoundary = 1, kEvtScatter = 2 };
 // dep_w: energy absorbed on the segment.  dep_i: w(1-e^{-sigma_a s})/sigma_a
 // (FP64) or, in FP32, the path weight w*s used only where sigma_a <= 1e-12.
 // s: the segment length (the adjoint replay needs it).
-// Positions are FP64 in both precisions: the FP32 movers compute in FP32 on
-// offsets from the cell's lower edges and return the new position in FP64.
+// p1, p2: the new position (FP64) or the new offsets from the cell's lower
+// edges (FP32).
 template <typename R>
 struct MoveOut {
   double w, dep_w, dep_i;
-  double p1, p2;
-  R d1, d2, dcen, s;
+  R p1, p2, d1, d2, dcen, s;
   int crossed, evt;
 };
 
@@ -237,15 +238,23 @@ __device__ __forceinline__ float scatter_distance_f32(const QuadF& u, float ss, 
   return s;
 }
 
+// Cell tables of the FP32 trackers hold (lower edge, width) per index, both
+// FP32, the width the FP32 difference of the FP32-rounded edges (so that
+// lower edge + width gives the next lower edge).  A history's position is an
+// FP32 offset from its cell's lower edges, carried from segment to segment;
+// kAtUpperFace marks the upper face of the cell (a history that entered
+// through it; the width is known once the cell's entry is read).
+constexpr float kAtUpperFace = -1.0f;
+
+__device__ __forceinline__ float resolve_offset(float l, float width) { return l < 0.0f ? width : l; }
+
 // CAP: apply the gray opacity cap (sigma_s * min cell width > 1e4); the
-// multigroup tracker has none.
+// multigroup tracker has none.  lx, ly: offsets in [0, wx] x [0, wy]; the
+// new offsets (same cell; a face for a crossing) are returned in o.p1, o.p2.
 template <bool CAP = true>
-__device__ __forceinline__ void move_xy(Ctr ctr, Key key, uint32_t ev, double w, float ux, float uy, double x,
-                                        double y, double xl, double xr, double yl, double yr, float sa, float ss,
-                                        float dcen, MoveOut<float>& o) {
-  // offsets from the cell's lower edges and the cell widths
-  const float lx = (float)(x - xl), ly = (float)(y - yl);
-  const float wx = (float)(xr - xl), wy = (float)(yr - yl);
+__device__ __forceinline__ void move_xy(Ctr ctr, Key key, uint32_t ev, double w, float ux, float uy, float lx,
+                                        float ly, float wx, float wy, float sa, float ss, float dcen,
+                                        MoveOut<float>& o) {
   // face distances by reciprocal-multiply (about 2 ulp; clamped at 0)
   float sx = (ux != 0.0f) ? fmaxf(fdiv_fast((ux > 0.0f ? wx : 0.0f) - lx, ux), 0.0f) : 1e30f;
   float sy = (uy != 0.0f) ? fmaxf(fdiv_fast((uy > 0.0f ? wy : 0.0f) - ly, uy), 0.0f) : 1e30f;
@@ -260,16 +269,16 @@ __device__ __forceinline__ void move_xy(Ctr ctr, Key key, uint32_t ev, double w,
     se = sb;
     o.evt = kEvtBoundary;
     if (sx <= sy) {
-      if (ux > 0.0f) { o.p1 = xr; o.crossed = kCrossIPlus; } else { o.p1 = xl; o.crossed = kCrossIMinus; }
-      o.p2 = yl + (double)fminf(fmaxf(ly + uy * se, 0.0f), wy);
+      if (ux > 0.0f) { o.p1 = wx; o.crossed = kCrossIPlus; } else { o.p1 = 0.0f; o.crossed = kCrossIMinus; }
+      o.p2 = fminf(fmaxf(ly + uy * se, 0.0f), wy);
     } else {
-      if (uy > 0.0f) { o.p2 = yr; o.crossed = kCrossJPlus; } else { o.p2 = yl; o.crossed = kCrossJMinus; }
-      o.p1 = xl + (double)fminf(fmaxf(lx + ux * se, 0.0f), wx);
+      if (uy > 0.0f) { o.p2 = wy; o.crossed = kCrossJPlus; } else { o.p2 = 0.0f; o.crossed = kCrossJMinus; }
+      o.p1 = fminf(fmaxf(lx + ux * se, 0.0f), wx);
     }
   } else {
     se = fminf(s_scat, dcen);
-    o.p1 = xl + (double)fminf(fmaxf(lx + ux * se, 0.0f), wx);
-    o.p2 = yl + (double)fminf(fmaxf(ly + uy * se, 0.0f), wy);
+    o.p1 = fminf(fmaxf(lx + ux * se, 0.0f), wx);
+    o.p2 = fminf(fmaxf(ly + uy * se, 0.0f), wy);
     if (s_scat <= dcen) {
       o.evt = kEvtScatter;
       float uz = 2.0f * u.u2 - 1.0f;
@@ -280,8 +289,6 @@ __device__ __forceinline__ void move_xy(Ctr ctr, Key key, uint32_t ev, double w,
       o.evt = kEvtCensus;
     }
   }
-  o.p1 = fmin(fmax(o.p1, xl), xr);
-  o.p2 = fmin(fmax(o.p2, yl), yr);
   o.dcen = dcen - se;
   o.s = se;
   absorb_f32(o, w, sa, se);
@@ -289,45 +296,48 @@ __device__ __forceinline__ void move_xy(Ctr ctr, Key key, uint32_t ev, double w,
 
 // Distance to the cylinder of radius R along the transverse chord, divided by
 // sqrt(1 - eta^2).  outer: the particle is inside (r <= R); inner: outside.
-// gap = |R - r| computed in FP64 by the caller (the cancellation-free chord
-// (R - r)(R + r) / (root + r mu) keeps its precision for r next to R).
+// gap = |R - r| from the cell offset.  The outer discriminant is written
+// R^2 - r^2 (1 - mu^2) = gap (R + r) + (r mu)^2 and the chord
+// (R - r)(R + r) / (root + r mu): both free of cancellation for r next to R.
 __device__ __forceinline__ float dist_outer_f32(float r, float mu, float R, float gap, float den) {
-  float ome = (1.0f - mu) * (1.0f + mu);
-  float disc = fmaxf(R * R - r * r * ome, 0.0f);
+  const float rm = r * mu;
+  float disc = fmaxf(gap * (R + r) + rm * rm, 0.0f);
   float root = __builtin_amdgcn_sqrtf(disc);
-  float l = (mu > 0.0f) ? fdiv_fast(fmaxf(gap * (R + r), 0.0f), root + r * mu) : root - r * mu;
+  float l = (mu > 0.0f) ? fdiv_fast(fmaxf(gap * (R + r), 0.0f), root + rm) : root - rm;
   if (!(l >= 0.0f)) l = 0.0f;  // r == R, mu == 1 degenerate: 0/0
   return fdiv_fast(l, den);
 }
 __device__ __forceinline__ float dist_inner_f32(float r, float mu, float R, float gap, float den) {
   if (!(mu < 0.0f)) return 1e30f;
-  float ome = (1.0f - mu) * (1.0f + mu);
-  float disc = R * R - r * r * ome;
+  const float rm = r * mu;
+  float disc = rm * rm - gap * (r + R);
   if (disc <= 0.0f) return 1e30f;
   float root = __builtin_amdgcn_sqrtf(disc);
-  float l = fdiv_fast(fmaxf(gap * (r + R), 0.0f), root - r * mu);
+  float l = fdiv_fast(fmaxf(gap * (r + R), 0.0f), root - rm);
   return fdiv_fast(l, den);
 }
 
+// lr, lz: offsets from the cell's inner radius rin and lower z; wr, wz: its
+// widths.  The new offsets (same cell; a face for a crossing) are returned in
+// o.p1, o.p2.
 template <bool CAP = true>
-__device__ __forceinline__ void move_rz(Ctr ctr, Key key, uint32_t ev, double w, float mu, float eta, double r64,
-                                        double z64, double rin64, double rout64, double zlo64, double zhi64,
-                                        float sa, float ss, float dcen, MoveOut<float>& o) {
-  const float r = (float)r64, rin = (float)rin64, rout = (float)rout64;
-  const float lz = (float)(z64 - zlo64), hz = (float)(zhi64 - zlo64);
+__device__ __forceinline__ void move_rz(Ctr ctr, Key key, uint32_t ev, double w, float mu, float eta, float lr,
+                                        float lz, float rin, float wr, float wz, float sa, float ss, float dcen,
+                                        MoveOut<float>& o) {
+  const float r = rin + lr, rout = rin + wr;
   float ome2 = fmaxf((1.0f - eta) * (1.0f + eta), 0.0f);
   float den = __builtin_amdgcn_sqrtf(ome2);
   float s_rin = 1e30f, s_rout = 1e30f;
   if (den > 0.0f) {
-    if (rin64 > 0.0) s_rin = dist_inner_f32(r, mu, rin, (float)(r64 - rin64), den);
-    s_rout = dist_outer_f32(r, mu, rout, (float)(rout64 - r64), den);
+    if (rin > 0.0f) s_rin = dist_inner_f32(r, mu, rin, lr, den);
+    s_rout = dist_outer_f32(r, mu, rout, wr - lr, den);
   }
   float s_zlo = 1e30f, s_zhi = 1e30f;
   if (eta > 0.0f)
-    s_zhi = fmaxf(fdiv_fast(hz - lz, eta), 0.0f);
+    s_zhi = fmaxf(fdiv_fast(wz - lz, eta), 0.0f);
   else if (eta < 0.0f)
     s_zlo = fmaxf(fdiv_fast(-lz, eta), 0.0f);
-  float cl = CAP ? fminf((float)(rout64 - rin64), hz) : 0.0f;
+  float cl = CAP ? fminf(wr, wz) : 0.0f;
   QuadF u = block4_f(ctr, key, ev);  // independent of the material gathers: overlaps their latency
   float s_scat = scatter_distance_f32(u, ss, cl);
   float sr = fminf(s_rin, s_rout), sz = fminf(s_zlo, s_zhi);
@@ -341,23 +351,23 @@ __device__ __forceinline__ void move_rz(Ctr ctr, Key key, uint32_t ev, double w,
     se = fminf(s_scat, dcen);
   // stream along the chord: r'^2 = (r + mu l)^2 + l^2 (1 - mu^2), and the new
   // radial cosine is mu' = (r mu + l) / r'; the radius moves by
-  // r' - r = (2 r mu l + l^2) / (r' + r), added in FP64
+  // r' - r = (2 r mu l + l^2) / (r' + r), added to the offset
   float l = se * den;
   float a = r + mu * l;
   float rn = __builtin_amdgcn_sqrtf(a * a + l * l * fmaxf((1.0f - mu) * (1.0f + mu), 0.0f));
   float sum = rn + r;
   float dr = sum > 0.0f ? fdiv_fast(l * (2.0f * r * mu + l), sum) : 0.0f;
-  double rn64 = r64 + (double)dr;
-  double zn64 = zlo64 + (double)fminf(fmaxf(lz + se * eta, 0.0f), hz);
+  float lrn = fminf(fmaxf(lr + dr, 0.0f), wr);
+  float lzn = fminf(fmaxf(lz + se * eta, 0.0f), wz);
   float mun = mu;
   if (rn > 0.0f && l > 0.0f) mun = fminf(1.0f, fmaxf(-1.0f, fdiv_fast(r * mu + l, rn)));
   float etan = eta;
   if (bnd) {
     o.evt = kEvtBoundary;
     if (sr <= sz) {
-      if (s_rin <= s_rout) { rn64 = rin64; o.crossed = kCrossIMinus; } else { rn64 = rout64; o.crossed = kCrossIPlus; }
+      if (s_rin <= s_rout) { lrn = 0.0f; o.crossed = kCrossIMinus; } else { lrn = wr; o.crossed = kCrossIPlus; }
     } else {
-      if (s_zlo <= s_zhi) { zn64 = zlo64; o.crossed = kCrossJMinus; } else { zn64 = zhi64; o.crossed = kCrossJPlus; }
+      if (s_zlo <= s_zhi) { lzn = 0.0f; o.crossed = kCrossJMinus; } else { lzn = wz; o.crossed = kCrossJPlus; }
     }
   } else {
     if (s_scat <= dcen) {
@@ -369,11 +379,67 @@ __device__ __forceinline__ void move_rz(Ctr ctr, Key key, uint32_t ev, double w,
     }
   }
   o.d1 = mun; o.d2 = etan;
-  o.p1 = fmin(fmax(rn64, rin64), rout64);
-  o.p2 = fmin(fmax(zn64, zlo64), zhi64);
+  o.p1 = lrn;
+  o.p2 = lzn;
   o.dcen = dcen - se;
   o.s = se;
   absorb_f32(o, w, sa, se);
+}
+
+// ---------------------------------------------------------------------------
+// position state of the trackers
+// ---------------------------------------------------------------------------
+template <typename R>
+constexpr bool kF32 = std::is_same<R, float>::value;
+
+__device__ __forceinline__ double span_width(double2 e) { return e.y - e.x; }
+__device__ __forceinline__ float span_width(float2 e) { return e.y; }
+
+// One segment in cell tables ex, ey.  FP32: resolves the carried offsets
+// (the pool's position on a history's first segment, the upper-face marker)
+// against the cell first.
+template <int GEOM, bool CAP = true>
+__device__ __forceinline__ void segment_move(Ctr ctr, Key key, uint32_t ev, double w, double d1, double d2, double p1,
+                                             double p2, bool&, double2 ex, double2 ey, double2 mat, double dcen,
+                                             MoveOut<double>& o) {
+  if (GEOM == 0)
+    move_xy(ctr, key, ev, w, d1, d2, p1, p2, ex.x, ex.y, ey.x, ey.y, mat.x, mat.y, dcen, o);
+  else
+    move_rz(ctr, key, ev, w, d1, d2, p1, p2, ex.x, ex.y, ey.x, ey.y, mat.x, mat.y, dcen, o);
+}
+template <int GEOM, bool CAP = true>
+__device__ __forceinline__ void segment_move(Ctr ctr, Key key, uint32_t ev, double w, float d1, float d2, float& p1,
+                                             float& p2, bool& fresh, float2 ex, float2 ey, float2 mat, float dcen,
+                                             MoveOut<float>& o) {
+  p1 = fresh ? fminf(fmaxf(p1 - ex.x, 0.0f), ex.y) : resolve_offset(p1, ex.y);
+  p2 = fresh ? fminf(fmaxf(p2 - ey.x, 0.0f), ey.y) : resolve_offset(p2, ey.y);
+  fresh = false;
+  if (GEOM == 0)
+    move_xy<CAP>(ctr, key, ev, w, d1, d2, p1, p2, ex.y, ey.y, mat.x, mat.y, dcen, o);
+  else
+    move_rz<CAP>(ctr, key, ev, w, d1, d2, p1, p2, ex.x, ex.y, ey.y, mat.x, mat.y, dcen, o);
+}
+
+// FP32 offsets after a face crossing: the new cell's lower / upper face
+__device__ __forceinline__ void enter_lower(double&) {}
+__device__ __forceinline__ void enter_upper(double&) {}
+__device__ __forceinline__ void enter_lower(float& l) { l = 0.0f; }
+__device__ __forceinline__ void enter_upper(float& l) { l = kAtUpperFace; }
+
+// Specular reflection at domain side s (0: i < 0, 1: i >= nx, 2: j < 0,
+// 3: j >= ny) after the crossing moved the cell index out of the mesh: the
+// history stays on that face of its last cell.
+template <typename R>
+__device__ __forceinline__ void reflect_at(int side, int nx, int ny, int& i, int& j, R& d1, R& d2, R& p1, R& p2) {
+  if (side < 2) {
+    d1 = -d1;
+    i = side == 0 ? 0 : nx - 1;
+    if constexpr (kF32<R>) p1 = side == 0 ? 0.0f : kAtUpperFace;
+  } else {
+    d2 = -d2;
+    j = side == 2 ? 0 : ny - 1;
+    if constexpr (kF32<R>) p2 = side == 2 ? 0.0f : kAtUpperFace;
+  }
 }
 
 }  // namespace imc

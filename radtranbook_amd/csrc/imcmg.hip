@@ -418,8 +418,8 @@ struct MgArgs {
   using R2 = typename Vec2<R>::type;
   MgView<R> P;
   int64_t n;
-  const double2* e1p;  // FP64 in both precisions
-  const double2* e2p;
+  const R2* e1p;  // FP64: (e1[i], e1[i+1]); FP32: (e1[i], e1[i+1] - e1[i])
+  const R2* e2p;  // the same for e2
   int nx, ny, G;
   const R2* sas;  // (sigma_a, sigma_s) per bin
   double dt;
@@ -464,7 +464,11 @@ __global__ __launch_bounds__(256) void k_mg_transport(MgArgs<R> A) {
   int64_t k = 0;
   double w = 0.0, t = 0.0;
   R d1 = 0, d2 = 0, dcen = 0;
-  double p1 = 0.0, p2 = 0.0;  // FP64 in both precisions (cell offsets are FP32 in the FP32 movers)
+  // FP64: the position.  FP32: offsets from the cell's lower edges
+  // (kAtUpperFace: on the upper face), or, while `fresh`, the position read
+  // from the pool; q1, q2 hold the position to write back.
+  R p1 = 0, p2 = 0, q1 = 0, q2 = 0;
+  bool fresh = false;
   int i = 0, j = 0, g = 0;
   uint32_t ev = 0;
   double acc_dep = 0.0, acc_path = 0.0;
@@ -481,6 +485,9 @@ __global__ __launch_bounds__(256) void k_mg_transport(MgArgs<R> A) {
         d2 = A.P.d2[k];
         p1 = A.P.p1[k];
         p2 = A.P.p2[k];
+        q1 = p1;
+        q2 = p2;
+        fresh = true;
         t = (double)A.P.t[k];
         i = A.P.ci[k];
         j = A.P.cj[k];
@@ -514,8 +521,8 @@ __global__ __launch_bounds__(256) void k_mg_transport(MgArgs<R> A) {
       const int64_t c = (int64_t)i * ny + j;
       const int bin = (int)((int64_t)g * ncell + c);
       R2 mat = A.sas[bin];
-      const double2 ex = A.e1p[i];
-      const double2 ey = A.e2p[j];
+      const R2 ex = A.e1p[i];
+      const R2 ey = A.e2p[j];
       Ctr ctr = make_ctr(kTagTransport, (uint64_t)k, A.step);
       double dep, path;
       int evt, crossed;
@@ -535,16 +542,15 @@ __global__ __launch_bounds__(256) void k_mg_transport(MgArgs<R> A) {
         t = t + o.s / kC;
       } else {
         MoveOut<float> o;
-        if (GEOM == 0)
-          move_xy<false>(ctr, A.key, ev, w, d1, d2, p1, p2, ex.x, ex.y, ey.x, ey.y, mat.x, mat.y, dcen, o);
-        else
-          move_rz<false>(ctr, A.key, ev, w, d1, d2, p1, p2, ex.x, ex.y, ey.x, ey.y, mat.x, mat.y, dcen, o);
+        segment_move<GEOM, false>(ctr, A.key, ev, w, d1, d2, p1, p2, fresh, ex, ey, mat, dcen, o);
         dep = o.dep_w;
         path = o.dep_i;
         evt = o.evt;
         crossed = o.crossed;
         w = o.w;
         d1 = o.d1; d2 = o.d2; p1 = o.p1; p2 = o.p2;
+        q1 = ex.x + p1;
+        q2 = ey.x + p2;
         dcen = o.dcen;
       }
       ++ev;
@@ -567,10 +573,10 @@ __global__ __launch_bounds__(256) void k_mg_transport(MgArgs<R> A) {
         ++st_sc;
       } else {
         ++st_bnd;
-        if (crossed == kCrossIPlus) ++i;
-        else if (crossed == kCrossIMinus) --i;
-        else if (crossed == kCrossJPlus) ++j;
-        else if (crossed == kCrossJMinus) --j;
+        if (crossed == kCrossIPlus) { ++i; enter_lower(p1); }
+        else if (crossed == kCrossIMinus) { --i; enter_upper(p1); }
+        else if (crossed == kCrossJPlus) { ++j; enter_lower(p2); }
+        else if (crossed == kCrossJMinus) { --j; enter_upper(p2); }
         int side = -1;
         if (i < 0) side = 0;
         else if (i >= nx) side = 1;
@@ -578,8 +584,7 @@ __global__ __launch_bounds__(256) void k_mg_transport(MgArgs<R> A) {
         else if (j >= ny) side = 3;
         if (side >= 0) {
           if ((A.reflect >> side) & 1) {
-            if (side < 2) { d1 = -d1; i = side == 0 ? 0 : nx - 1; }
-            else { d2 = -d2; j = side == 2 ? 0 : ny - 1; }
+            reflect_at(side, nx, ny, i, j, d1, d2, p1, p2);
             ++st_refl;
           } else {
             int kg = (int)(A.nb + g), ks = (int)(A.nb + A.G + side);
@@ -599,8 +604,8 @@ __global__ __launch_bounds__(256) void k_mg_transport(MgArgs<R> A) {
       A.P.w[k] = w;
       A.P.d1[k] = d1;
       A.P.d2[k] = d2;
-      A.P.p1[k] = (R)p1;
-      A.P.p2[k] = (R)p2;
+      A.P.p1[k] = kF32<R> ? q1 : p1;
+      A.P.p2[k] = kF32<R> ? q2 : p2;
       A.P.t[k] = (R)(sizeof(R) == 8 ? t : A.dt - (double)dcen / kC);
       A.P.ci[k] = i;
       A.P.cj[k] = j;
@@ -956,7 +961,11 @@ __global__ __launch_bounds__(256) void k_m1_transport(MgArgs<R> A) {
   int64_t k = 0;
   double w = 0.0;
   R mu = 0, dcen = 0;
-  double x = 0.0;  // FP64 in both precisions (the FP32 mover works on shell gaps)
+  // FP64: the radius.  FP32: the offset from the cell's inner radius
+  // (kAtUpperFace: on the outer shell), or, while `fresh`, the radius read
+  // from the pool; q holds the radius to write back.
+  R x = 0, q = 0;
+  bool fresh = false;
   int cell = 0, g = 0;
   uint32_t ev = 0;
   double acc_dep = 0.0, acc_path = 0.0;
@@ -971,6 +980,8 @@ __global__ __launch_bounds__(256) void k_m1_transport(MgArgs<R> A) {
         w = A.P.w[k];
         mu = A.P.d1[k];
         x = A.P.p1[k];
+        q = x;
+        fresh = true;
         cell = A.P.ci[k];
         g = A.P.g[k];
         dcen = (R)((A.dt - (double)A.P.t[k]) * kC);
@@ -986,10 +997,14 @@ __global__ __launch_bounds__(256) void k_m1_transport(MgArgs<R> A) {
     bool finish = false;
     ++st_ev;
     const int bin = g * I + cell;
-    const double2 edg = A.e1p[cell];
+    const R2 edg = A.e1p[cell];
     R2 mat = A.sas[bin];
     Move1D<R> o;
     Ctr ctr = make_ctr(kTagTransport, (uint64_t)k, A.step);
+    if constexpr (kF32<R>) {
+      x = fresh ? fminf(fmaxf(x - edg.x, 0.0f), edg.y) : resolve_offset(x, edg.y);
+      fresh = false;
+    }
     move_1d<1>(ctr, A.key, ev, w, mu, x, edg.x, edg.y, mat.x, mat.y, dcen, o);
     ++ev;
     if (o.bad) {
@@ -1010,7 +1025,10 @@ __global__ __launch_bounds__(256) void k_m1_transport(MgArgs<R> A) {
       w = o.w;
       mu = o.mu;
       x = o.x;
+      if constexpr (kF32<R>) q = edg.x + x;
       cell += o.newloc;
+      if (o.newloc > 0) enter_lower(x);
+      else if (o.newloc < 0) enter_upper(x);
       dcen = o.dcen;
       st_bnd += o.code == 1;
       st_sc += o.code == 2;
@@ -1021,6 +1039,8 @@ __global__ __launch_bounds__(256) void k_m1_transport(MgArgs<R> A) {
         if ((A.reflect >> face) & 1) {
           mu = -mu;
           cell = face == 0 ? 0 : I - 1;
+          if (face == 0) enter_lower(x);
+          else enter_upper(x);
           ++st_refl;
         } else {
           int kg = (int)(A.nb + g);
@@ -1044,7 +1064,7 @@ __global__ __launch_bounds__(256) void k_m1_transport(MgArgs<R> A) {
     if (finish) {
       A.P.w[k] = w;
       A.P.d1[k] = mu;
-      A.P.p1[k] = (R)x;
+      A.P.p1[k] = kF32<R> ? q : x;
       A.P.ci[k] = cell;
       active = false;
     }
@@ -1314,11 +1334,11 @@ static int mg_create(mgimc_ctx** out, int geometry, int nx, int ny, const double
     std::vector<float2> a32(nx), b32(ny);
     std::vector<double2> a64(nx), b64(ny);
     for (int q = 0; q < nx; ++q) {
-      a32[q] = make_float2((float)edges1[q], (float)edges1[q + 1]);
+      a32[q] = make_float2((float)edges1[q], (float)edges1[q + 1] - (float)edges1[q]);
       a64[q] = make_double2(edges1[q], edges1[q + 1]);
     }
     for (int q = 0; q < ny; ++q) {
-      b32[q] = make_float2((float)edges2[q], (float)edges2[q + 1]);
+      b32[q] = make_float2((float)edges2[q], (float)edges2[q + 1] - (float)edges2[q]);
       b64[q] = make_double2(edges2[q], edges2[q + 1]);
     }
     IMC_HIP(hipMemcpy(c->e1p32.p, a32.data(), sizeof(float2) * nx, hipMemcpyHostToDevice));
@@ -1743,7 +1763,7 @@ int mgimc_transport(mgimc_ctx* c, double dt, const int32_t* reflect, int64_t max
       else
         k_mg_transport<double, 1><<<grid, block, 0, s>>>(A);
     } else {
-      MgArgs<float> A{c->A.view<float>(), n, c->e1p64.p, c->e2p64.p, c->nx, c->ny, G, c->sas32.p, dt, refl,
+      MgArgs<float> A{c->A.view<float>(), n, c->e1p32.p, c->e2p32.p, c->nx, c->ny, G, c->sas32.p, dt, refl,
                       max_events, c->scal.p + 2, key, c->step, c->next, c->batch, c->dep_raw.p, c->path_raw.p, nb,
                       c->tally_copies - 1, c->stats, Q};
       if (c->geometry == IMC_GEOM_XY)
@@ -2068,7 +2088,7 @@ int mgimc_sphere_transport(mgimc_ctx* c, double dt, const int32_t* reflect, doub
                        c->tally_copies - 1, c->stats, pool_order_queue(n)};
       k_m1_transport<double><<<grid, block, 0, s>>>(A);
     } else {
-      MgArgs<float> A{c->A.view<float>(), n, c->e1p64.p, c->e2p64.p, c->nx, 1, G, c->sas32.p, dt, refl, 0,
+      MgArgs<float> A{c->A.view<float>(), n, c->e1p32.p, c->e2p32.p, c->nx, 1, G, c->sas32.p, dt, refl, 0,
                       c->scal.p + 2, key, c->step, c->next, c->batch, c->dep_raw.p, c->path_raw.p, nb,
                       c->tally_copies - 1, c->stats, pool_order_queue(n)};
       k_m1_transport<float><<<grid, block, 0, s>>>(A);
