@@ -1002,11 +1002,12 @@ __device__ __forceinline__ int block_sum_256i(int v, int* wsum) {
 
 template <typename R>
 __global__ __launch_bounds__(256) void k_comb_copies(PoolView<R> P, int64_t n, int nx, int ny,
-                                                     const double* __restrict__ ew, int passthrough, Key key,
-                                                     uint32_t step, int32_t* __restrict__ copies,
+                                                     const double* __restrict__ ew, const double* __restrict__ Etot,
+                                                     Key key, uint32_t step, int32_t* __restrict__ copies,
                                                      int64_t* __restrict__ bsum) {
 #pragma clang fp contract(off)
   __shared__ int wsum[4];
+  const bool passthrough = !(*Etot > 0.0);  // empty census: every live particle kept as it is
   int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int32_t m = 0;
   if (k < n && comb_alive(P, k, nx, ny)) {
@@ -1026,18 +1027,22 @@ __global__ __launch_bounds__(256) void k_comb_copies(PoolView<R> P, int64_t n, i
 }
 
 // Comb output: workgroup b expands particles [256 b, 256 b + 256).  Their
-// copy offsets are staged in LDS and the workgroup's threads stride over its
-// contiguous output range, each finding its parent by binary search in LDS:
-// coalesced writes, and a parent with many copies is shared by the whole
-// workgroup instead of one lane.
+// copy offsets are staged in LDS.  When no parent of the block has more than
+// kCombDirectMax copies each thread writes its own parent's copies; otherwise
+// the workgroup's threads stride over its contiguous output range, each
+// finding its parent by binary search in LDS, so that a parent with many
+// copies is shared by the whole workgroup instead of one lane.  Both write
+// the same outputs.
+constexpr int kCombDirectMax = 4;
+
 template <typename R>
 __global__ __launch_bounds__(256) void k_comb_write(PoolView<R> S, PoolView<R> D, int64_t n, int nx, int ny,
                                                     const int32_t* __restrict__ copies,
                                                     const int64_t* __restrict__ boff, const double* __restrict__ ew,
-                                                    int passthrough, int copy_mask,
-                                                    unsigned int* __restrict__ count_part) {
+                                                    const double* __restrict__ Etot, int64_t cap, int* fault,
+                                                    int copy_mask, unsigned int* __restrict__ count_part) {
   __shared__ int64_t so[257];
-  __shared__ int wsum[4];
+  __shared__ int wsum[4], wmax[4];
   const int t = threadIdx.x;
   const int64_t k0 = (int64_t)blockIdx.x * 256;
   const int64_t k = k0 + t;
@@ -1045,13 +1050,15 @@ __global__ __launch_bounds__(256) void k_comb_write(PoolView<R> S, PoolView<R> D
   {
     // this block's copy offsets: block offset + exclusive scan of the counts
     const int lane = t & 63, wid = t >> 6;
-    int inc = m;
+    int inc = m, mx = m;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
       int y = __shfl_up(inc, o, 64);
       if (lane >= o) inc += y;
+      mx = max(mx, __shfl_xor(mx, o, 64));
     }
     if (lane == 63) wsum[wid] = inc;
+    if (lane == 0) wmax[wid] = mx;
     __syncthreads();
     int wofs = 0;
 #pragma unroll
@@ -1070,6 +1077,34 @@ __global__ __launch_bounds__(256) void k_comb_write(PoolView<R> S, PoolView<R> D
   const int cnt = (n - k0 < 256) ? (int)(n - k0) : 256;
   __syncthreads();  // so[cnt] is the block's end: threads past n carry zero copies
   const int64_t lo = so[0], hi = so[cnt];
+  if (hi > cap) {  // beyond the output capacity the host sized from its bound
+    if (t == 0) atomicOr(fault, 1);
+    return;
+  }
+  const bool passthrough = !(*Etot > 0.0);
+  if (max(max(wmax[0], wmax[1]), max(wmax[2], wmax[3])) <= kCombDirectMax) {
+    // few copies per parent (the usual case: census weights near the cell's
+    // comb weight): each thread writes its own parent's copies, reading the
+    // parent once; consecutive parents fill consecutive outputs
+    if (m > 0) {
+      const int32_t ci = S.ci[k], cj = S.cj[k];
+      const double w = passthrough ? S.w[k] : ew[(int64_t)ci * ny + cj];
+      const R d1 = S.d1[k], d2 = S.d2[k], p1 = S.p1[k], p2 = S.p2[k];
+      const int64_t o0 = so[t];
+      for (int q = 0; q < m; ++q) {
+        const int64_t o = o0 + q;
+        D.w[o] = w;
+        D.d1[o] = d1;
+        D.d2[o] = d2;
+        D.p1[o] = p1;
+        D.p2[o] = p2;
+        D.t[o] = (R)0;  // times reset after the comb (IMC/IMC2D.py:1831)
+        D.ci[o] = ci;
+        D.cj[o] = cj;
+      }
+    }
+    return;
+  }
   for (int64_t o = lo + t; o < hi; o += 256) {
     int a = 0, b = cnt - 1;  // last parent p with so[p] <= o
     while (a < b) {
@@ -1093,10 +1128,11 @@ __global__ __launch_bounds__(256) void k_comb_write(PoolView<R> S, PoolView<R> D
 }
 
 __global__ void k_comb_disc(int64_t ncell, const double* __restrict__ ecen, const double* __restrict__ ew,
-                            const unsigned long long* __restrict__ cellcount, int passthrough,
+                            const unsigned long long* __restrict__ cellcount, const double* __restrict__ Etot,
                             double* __restrict__ disc) {
   int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncell) return;
+  const bool passthrough = !(*Etot > 0.0);
   disc[c] = passthrough ? 0.0 : ecen[c] - ew[c] * (double)cellcount[c];
 }
 
@@ -2123,12 +2159,13 @@ int imc2d_transport(imc2d_ctx* c, double dt, const int32_t* reflect, int64_t max
   IMC_HIP(hipSetDevice(c->device));
   hipStream_t s = c->stream;
   int64_t n = c->n;
-  IMC_HIP(hipMemsetAsync(c->dep_raw.p, 0, sizeof(double) * c->ncell * c->tally_copies, s));
-  IMC_HIP(hipMemsetAsync(c->si_raw.p, 0, sizeof(double) * c->ncell * c->tally_copies, s));
-  IMC_HIP(hipMemsetAsync(c->stats, 0, sizeof(unsigned long long) * 8, s));
-  IMC_HIP(hipMemsetAsync(c->scal + 3, 0, sizeof(double), s));
-  IMC_HIP(hipMemsetAsync(c->next, 0, sizeof(unsigned long long), s));
   int rc;
+  {
+    const int64_t tb = (int64_t)sizeof(double) * c->ncell * c->tally_copies;
+    ZeroJob z[5] = {{c->dep_raw.p, tb}, {c->si_raw.p, tb}, {c->stats, (int64_t)sizeof(unsigned long long) * 8},
+                    {c->scal + 3, 8}, {c->next, 8}};
+    if ((rc = zero_jobs(z, 5, s))) return rc;
+  }
   if (weight_floor < 0.0) {
     if ((rc = tree_sum(c->A.w, n, c->scal + 1, c->tree_scr.p, s))) return rc;
     k_weight_floor<<<1, 1, 0, s>>>(c->scal + 1, n, c->scal + 2);
@@ -2331,12 +2368,13 @@ int imc2d_transport_cf(imc2d_ctx* c, double dt, const int32_t* reflect, int64_t 
   IMC_HIP(hipSetDevice(c->device));
   hipStream_t s = c->stream;
   int64_t n = c->n;
-  IMC_HIP(hipMemsetAsync(c->dep_raw.p, 0, sizeof(double) * c->ncell * c->tally_copies, s));
-  IMC_HIP(hipMemsetAsync(c->si_raw.p, 0, sizeof(double) * c->ncell * c->tally_copies, s));
-  IMC_HIP(hipMemsetAsync(c->stats, 0, sizeof(unsigned long long) * 8, s));
-  IMC_HIP(hipMemsetAsync(c->scal + 3, 0, sizeof(double), s));
-  IMC_HIP(hipMemsetAsync(c->next, 0, sizeof(unsigned long long), s));
   int rc;
+  {
+    const int64_t tb = (int64_t)sizeof(double) * c->ncell * c->tally_copies;
+    ZeroJob z[5] = {{c->dep_raw.p, tb}, {c->si_raw.p, tb}, {c->stats, (int64_t)sizeof(unsigned long long) * 8},
+                    {c->scal + 3, 8}, {c->next, 8}};
+    if ((rc = zero_jobs(z, 5, s))) return rc;
+  }
   if (weight_floor < 0.0) {
     if ((rc = tree_sum(c->A.w, n, c->scal + 1, c->tree_scr.p, s))) return rc;
     k_weight_floor<<<1, 1, 0, s>>>(c->scal + 1, n, c->scal + 2);
@@ -2433,81 +2471,86 @@ int imc2d_comb(imc2d_ctx* c, int64_t Nmax, double* ecen_out, double* disc_out, d
   int rc;
   if ((rc = census_histogram(c))) return rc;
   if ((rc = canonical_sum(c->ecen.p, nc, c->scal + 4, c->sum_scr.p, s))) return rc;
-  double* hs = stage_scalars(c);
-  double* hsd = stage_scalars_dev(c);
-  {
-    CopyJob out{c->scal + 4, hsd, 8};
-    if ((rc = copy_jobs(&out, 1, s))) return rc;
-  }
-  IMC_HIP(hipStreamSynchronize(s));
-  double E = hs[0];
-  int passthrough = !(E > 0.0);
   k_comb_ew<<<launch_blocks(nc, 256), 256, 0, s>>>(nc, c->ecen.p, c->scal + 4, Nmax, c->ew.p);
   IMC_HIP(hipGetLastError());
+  // Output capacity without reading the count back first: a particle's copy
+  // count is floor(w / ew + u) <= w / ew + 1, so the comb keeps at most
+  // sum_c max(1, rint(Nmax e_c / E)) + n <= Nmax + 2 ncell + n particles
+  // (a kernel-side check guards the bound all the same).  The bound also
+  // covers the next step's pool of about the same size, so its sources need
+  // no reallocation.  A buffer short of the bound grows to 1.5 times it,
+  // which absorbs the pool's growth over the first steps, so each of the two
+  // buffers is reallocated about once.  (Growing to the other buffer's
+  // capacity instead lets the two outgrow each other step after step once
+  // their capacities differ.)
+  const int64_t bound = Nmax + 2 * nc + n + 256;
+  if (c->B.cap < bound && (rc = c->B.grow(bound + bound / 2, 0, s))) return rc;
   const int64_t nblk = (n + 255) / 256;
-  if ((rc = c->copies.reserve(n + 1)) || (rc = c->counts.reserve(2 * nblk + 4)) ||
+  if ((rc = c->copies.reserve(n + 1)) || (rc = c->counts.reserve(2 * nblk + 6)) ||
       (rc = c->scan_scr.reserve(scan_scratch(nblk) + 8)))
     return rc;
   int64_t* bsum = c->counts.p;             // block copy totals
   int64_t* boff = c->counts.p + nblk + 1;  // their exclusive scan, then the total
+  int64_t* total_dev = boff + nblk;
+  int* fault = (int*)(boff + nblk + 1);
+  {
+    ZeroJob z[1] = {{fault, 8}};
+    if ((rc = zero_jobs(z, 1, s))) return rc;
+  }
+  const bool want_counts = disc_out != nullptr;
+  if (want_counts) IMC_HIP(hipMemsetAsync(c->count_part.p, 0, sizeof(unsigned int) * nc * c->tally_copies, s));
+  unsigned int* cp = want_counts ? c->count_part.p : nullptr;
   Key key = make_key(c->seed);
   if (n > 0) {
     if (c->precision == IMC_FP64)
       k_comb_copies<double><<<(unsigned)nblk, 256, 0, s>>>(c->A.view<double>(), n, c->nx, c->ny, c->ew.p,
-                                                           passthrough, key, c->step, c->copies.p, bsum);
+                                                           c->scal + 4, key, c->step, c->copies.p, bsum);
     else
-      k_comb_copies<float><<<(unsigned)nblk, 256, 0, s>>>(c->A.view<float>(), n, c->nx, c->ny, c->ew.p, passthrough,
+      k_comb_copies<float><<<(unsigned)nblk, 256, 0, s>>>(c->A.view<float>(), n, c->nx, c->ny, c->ew.p, c->scal + 4,
                                                           key, c->step, c->copies.p, bsum);
     IMC_HIP(hipGetLastError());
   }
   if ((rc = exclusive_scan_i64(bsum, nblk, boff, c->scan_scr.p, s))) return rc;
-  {
-    CopyJob out{boff + nblk, hsd + 1, 8};
-    if ((rc = copy_jobs(&out, 1, s))) return rc;
-  }
-  IMC_HIP(hipStreamSynchronize(s));
-  int64_t total = *(int64_t*)(hs + 1);
-  // the comb output becomes the next pool: give it the capacity the current
-  // pool reached, so the next step's sources do not reallocate
-  if ((rc = c->B.grow(std::max<int64_t>(std::max<int64_t>(total, 1), c->A.cap), 0, s))) return rc;
-  bool want_counts = disc_out != nullptr && !passthrough;
-  if (want_counts) IMC_HIP(hipMemsetAsync(c->count_part.p, 0, sizeof(unsigned int) * nc * c->tally_copies, s));
-  unsigned int* cp = want_counts ? c->count_part.p : nullptr;
   if (n > 0) {
     if (c->precision == IMC_FP64)
       k_comb_write<double><<<(unsigned)nblk, 256, 0, s>>>(c->A.view<double>(), c->B.view<double>(), n, c->nx, c->ny,
-                                                          c->copies.p, boff, c->ew.p, passthrough, c->tally_copies - 1,
-                                                          cp);
+                                                          c->copies.p, boff, c->ew.p, c->scal + 4, c->B.cap, fault,
+                                                          c->tally_copies - 1, cp);
     else
       k_comb_write<float><<<(unsigned)nblk, 256, 0, s>>>(c->A.view<float>(), c->B.view<float>(), n, c->nx, c->ny,
-                                                         c->copies.p, boff, c->ew.p, passthrough, c->tally_copies - 1,
-                                                         cp);
+                                                         c->copies.p, boff, c->ew.p, c->scal + 4, c->B.cap, fault,
+                                                         c->tally_copies - 1, cp);
     IMC_HIP(hipGetLastError());
   }
   size_t bytes = sizeof(double) * nc;
   if (disc_out) {
-    if (want_counts) {
-      k_reduce_copies<unsigned int, unsigned long long><<<launch_blocks(nc, 256), 256, 0, s>>>(
-          nc, c->tally_copies, c->count_part.p, c->cellcount.p);
-      IMC_HIP(hipGetLastError());
-    }
-    k_comb_disc<<<launch_blocks(nc, 256), 256, 0, s>>>(nc, c->ecen.p, c->ew.p, c->cellcount.p, passthrough, c->disc.p);
+    k_reduce_copies<unsigned int, unsigned long long><<<launch_blocks(nc, 256), 256, 0, s>>>(
+        nc, c->tally_copies, c->count_part.p, c->cellcount.p);
+    IMC_HIP(hipGetLastError());
+    k_comb_disc<<<launch_blocks(nc, 256), 256, 0, s>>>(nc, c->ecen.p, c->ew.p, c->cellcount.p, c->scal + 4,
+                                                       c->disc.p);
     IMC_HIP(hipGetLastError());
   }
   std::swap(c->A, c->B);
-  c->n = total;
-  if ((rc = tree_sum(c->A.w, total, c->scal + 5, c->tree_scr.p, s))) return rc;
+  if ((rc = tree_sum_dev(c->A.w, total_dev, c->scal + 5, c->tree_scr.p, s))) return rc;
+  double* hs = stage_scalars(c);
+  double* hsd = stage_scalars_dev(c);
   {
-    CopyJob out[3] = {disc_out ? stage_out(c, 4, c->disc.p) : CopyJob{}, ecen_out ? stage_out(c, 3, c->ecen.p) : CopyJob{},
-                      CopyJob{c->scal + 5, hsd + 2, 8}};
-    if ((rc = copy_jobs(out, 3, s))) return rc;
+    CopyJob out[4] = {disc_out ? stage_out(c, 4, c->disc.p) : CopyJob{}, ecen_out ? stage_out(c, 3, c->ecen.p) : CopyJob{},
+                      CopyJob{c->scal + 4, hsd, 16}, CopyJob{total_dev, hsd + 2, 16}};
+    static_assert(sizeof(int64_t) == sizeof(double), "scalar staging slots are 8 bytes");
+    if ((rc = copy_jobs(out, 4, s))) return rc;
   }
   IMC_HIP(hipStreamSynchronize(s));
+  const int64_t total = *(const int64_t*)(hs + 2);
+  const int fault_bits = *(const int*)(hs + 3);
+  IMC_REQUIRE(fault_bits == 0, kErrState,
+              "imc2d_comb: comb output exceeded its capacity bound or a tile look-back did not complete");
+  c->n = total;
   if (disc_out) std::memcpy(disc_out, c->stage.slot(nc, 4), bytes);
   if (ecen_out) std::memcpy(ecen_out, c->stage.slot(nc, 3), bytes);
-  double tr = hs[2];
-  if (total_rad_out) *total_rad_out = tr;
-  if (ecen_total_out) *ecen_total_out = E;
+  if (total_rad_out) *total_rad_out = hs[1];
+  if (ecen_total_out) *ecen_total_out = hs[0];
   return kOk;
 }
 

@@ -118,6 +118,39 @@ int copy_jobs(const CopyJob* jobs, int njobs, hipStream_t s) {
   return kOk;
 }
 
+struct ZeroJobs8 {
+  ZeroJob j[8];
+  int n;
+};
+
+__global__ void k_zero_jobs(ZeroJobs8 J) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (int q = 0; q < J.n; ++q) {
+    unsigned long long* p = (unsigned long long*)J.j[q].p;
+    const int64_t words = J.j[q].bytes >> 3;
+    for (int64_t i = i0; i < words; i += stride) p[i] = 0ull;
+  }
+}
+
+int zero_jobs(const ZeroJob* jobs, int njobs, hipStream_t s) {
+  IMC_REQUIRE(njobs <= 8, kErrArg, "zero_jobs: more than eight ranges");
+  ZeroJobs8 J{};
+  int64_t most = 0;
+  for (int q = 0; q < njobs; ++q) {
+    if (!jobs[q].p || jobs[q].bytes <= 0) continue;
+    IMC_REQUIRE((jobs[q].bytes & 7) == 0 && ((uintptr_t)jobs[q].p & 7) == 0, kErrArg,
+                "zero_jobs: range not 8-byte aligned");
+    J.j[J.n++] = jobs[q];
+    most = std::max<int64_t>(most, jobs[q].bytes >> 3);
+  }
+  if (J.n == 0) return kOk;
+  int blocks = (int)std::min<int64_t>((most + 255) / 256, 2048);
+  k_zero_jobs<<<blocks, 256, 0, s>>>(J);
+  IMC_HIP(hipGetLastError());
+  return kOk;
+}
+
 // ---------------------------------------------------------------------------
 // canonical sum
 // ---------------------------------------------------------------------------
@@ -185,10 +218,21 @@ __device__ __forceinline__ double block_sum_256(double v) {
 }
 
 __global__ __launch_bounds__(256) void k_tree_partial(const double* __restrict__ x, int64_t n,
+                                                      const int64_t* __restrict__ n_dev,
                                                       double* __restrict__ part) {
+  if (n_dev) n = *n_dev;
   double v = 0.0;
   const int64_t stride = (int64_t)kTreeBlocks * 256;
-  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) v += x[i];
+  int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  // four loads in flight, added in the same order as the one-at-a-time loop
+  for (; i + 3 * stride < n; i += 4 * stride) {
+    const double a = x[i], b = x[i + stride], c = x[i + 2 * stride], d = x[i + 3 * stride];
+    v += a;
+    v += b;
+    v += c;
+    v += d;
+  }
+  for (; i < n; i += stride) v += x[i];
   double r = block_sum_256(v);
   if (threadIdx.x == 0) part[blockIdx.x] = r;
 }
@@ -201,7 +245,15 @@ __global__ __launch_bounds__(256) void k_tree_final(const double* __restrict__ p
 }
 
 int tree_sum(const double* x, int64_t n, double* out, double* scratch, hipStream_t s) {
-  k_tree_partial<<<kTreeBlocks, 256, 0, s>>>(x, n, scratch);
+  k_tree_partial<<<kTreeBlocks, 256, 0, s>>>(x, n, nullptr, scratch);
+  IMC_HIP(hipGetLastError());
+  k_tree_final<<<1, 256, 0, s>>>(scratch, out);
+  IMC_HIP(hipGetLastError());
+  return kOk;
+}
+
+int tree_sum_dev(const double* x, const int64_t* n_dev, double* out, double* scratch, hipStream_t s) {
+  k_tree_partial<<<kTreeBlocks, 256, 0, s>>>(x, 0, n_dev, scratch);
   IMC_HIP(hipGetLastError());
   k_tree_final<<<1, 256, 0, s>>>(scratch, out);
   IMC_HIP(hipGetLastError());

@@ -117,6 +117,15 @@ struct CopyJob {
 };
 int copy_jobs(const CopyJob* jobs, int njobs, hipStream_t s);
 
+// Up to eight device ranges cleared in one kernel launch (8-byte
+// granularity).  A kernel rather than hipMemsetAsync: one launch for all of a
+// step's tally buffers, and safe under graph capture.
+struct ZeroJob {
+  void* p;
+  int64_t bytes;
+};
+int zero_jobs(const ZeroJob* jobs, int njobs, hipStream_t s);
+
 // Deterministic "canonical" sum: 1024-element chunks summed left to right,
 // repeated on the chunk sums until one value is left.  The result lands in
 // *out (device).  `scratch` must hold ceil(n/1024) + ceil(n/2^20) + 2 doubles.
@@ -129,6 +138,8 @@ int64_t canonical_sum_scratch(int64_t n);
 // the oracle does not need the canonical order (weight floor, census totals).
 // `scratch` must hold 1024 doubles.
 int tree_sum(const double* x, int64_t n, double* out, double* scratch, hipStream_t s);
+// The same with the length read from device memory when the kernels run.
+int tree_sum_dev(const double* x, const int64_t* n_dev, double* out, double* scratch, hipStream_t s);
 
 // Exclusive scan of int64 counts; writes n+1 offsets (last = total).
 // `scratch` must hold scan_scratch(n) int64.
