@@ -446,6 +446,11 @@ struct TransportArgs {
 // stores left out).  Each of the four waves of a workgroup owns kStageSlots
 // entries; the wave flushes at the top of its loop once kStageFlushAt wait,
 // so at most kStageFlushAt - 1 + 64 are ever held.
+// LDS tally cache slots of the gray tracker.  With the census staging below
+// they set the occupancy; on the 200x200 heterogeneous Marshak FP32 step:
+// 512 slots 7 waves per SIMD 2.40 ms, 1024 slots 6 waves 2.30 ms, 2048 slots
+// 4 waves 2.36 ms (72 staging slots flushed at 9, 7 waves: 2.53 ms).
+constexpr int kTrackSlots = 1024;
 constexpr int kStageSlots = 96;
 constexpr int kStageFlushAt = 32;
 static_assert(kStageFlushAt - 1 + 64 <= kStageSlots, "a full wave of finishes must fit after a flush check");
@@ -484,11 +489,11 @@ __global__ __launch_bounds__(256) void k_transport(TransportArgs<R> A) {
   const int64_t copy_off = (int64_t)(blockIdx.x & A.copy_mask) * A.ncell;
   double* const dep_t = A.dep_raw + copy_off;
   double* const si_t = A.si_raw + copy_off;
-  __shared__ int lkey[kLdsSlots];
-  __shared__ double lval[kLdsSlots];
+  __shared__ int lkey[kTrackSlots];
+  __shared__ double lval[kTrackSlots];
   __shared__ CensusStage<R, Off> stg;
   const int wv = threadIdx.x >> 6;
-  lds_tally_init(lkey, lval);
+  lds_tally_init<kTrackSlots>(lkey, lval);
   if (__lane_id() == 0) stg.n[wv] = 0;
   __syncthreads();
   const int ny = A.ny, nx = A.nx;
@@ -567,7 +572,7 @@ __global__ __launch_bounds__(256) void k_transport(TransportArgs<R> A) {
       st_cen += o.evt == kEvtCensus;
       if (c != acc_cell) {
         if (acc_cell >= 0) {
-          if (!lds_tally_add(lkey, lval, acc_cell, acc_dep)) atomic_add_f64(&elem(dep_t, (uint32_t)acc_cell), acc_dep);
+          if (!lds_tally_add<kTrackSlots>(lkey, lval, acc_cell, acc_dep)) atomic_add_f64(&elem(dep_t, (uint32_t)acc_cell), acc_dep);
           if (acc_si != 0.0) atomic_add_f64(&elem(si_t, (uint32_t)acc_cell), acc_si);
         }
         acc_cell = c;
@@ -632,11 +637,11 @@ __global__ __launch_bounds__(256) void k_transport(TransportArgs<R> A) {
   }
   stage_flush(stg, wv, stg.n[wv], A.P);
   if (acc_cell >= 0) {
-    if (!lds_tally_add(lkey, lval, acc_cell, acc_dep)) atomic_add_f64(&elem(dep_t, (uint32_t)acc_cell), acc_dep);
+    if (!lds_tally_add<kTrackSlots>(lkey, lval, acc_cell, acc_dep)) atomic_add_f64(&elem(dep_t, (uint32_t)acc_cell), acc_dep);
     if (acc_si != 0.0) atomic_add_f64(&elem(si_t, (uint32_t)acc_cell), acc_si);
   }
   __syncthreads();
-  lds_tally_flush(lkey, lval, dep_t);
+  lds_tally_flush<kTrackSlots>(lkey, lval, dep_t);
   // wave reductions of counters
   unsigned long long s0 = wave_sum<unsigned long long>(st_ev);
   unsigned long long s1 = wave_sum<unsigned long long>(st_bnd);
