@@ -15,7 +15,12 @@ event count within 5%.
 2. The checks of the reference's 2-D Marshak-wave driver
 (IMC/MarshakWave2D.py): x- vs y-driven waves on a square (FC and CF), an rz
 shell far from the axis against the 1-D slab, and the self-similar Marshak
-solution."""
+solution.
+
+3. The multigroup power-law Marshak wave against the reference's published
+10-group run (mgimc_10.out): per-step radiation energy, the 0.05 ns
+summaries (radiation, internal, outgoing and face energies) and the census
+size over the first ten steps."""
 import json
 import os
 
@@ -242,3 +247,92 @@ def test_crooked_pipe_fc_matches_published_quick_run():
         tmax.append(float(np.max(st.temperature)))
     print(f"crooked pipe FC: max T {np.mean(tmax):.6f} (published {published:.6f})", flush=True)
     assert np.mean(tmax) == pytest.approx(published, rel=0.035)
+
+
+# ---------------------------------------------------------------------------
+# Multigroup power-law Marshak wave against the reference's published
+# 10-group run (MG_IMC/problems/test_marshak_wave_multigroup_powerlaw.py
+# --mode publication; results/marshak_wave_powerlaw/imc_10g_publication/
+# mgimc_10.out, its per-step lines and 0.05 ns summaries extracted by
+# tests/golden/extract_mgimc_10.py into tests/golden/mgimc_10_powerlaw.json).
+# ---------------------------------------------------------------------------
+@pytest.fixture(scope="module")
+def published_mg():
+    with open(os.path.join(HERE, "golden", "mgimc_10_powerlaw.json")) as fh:
+        return json.load(fh)
+
+
+def _run_mg_powerlaw(prob, seed, nsteps):
+    from radtranbook_amd import MG_IMC2D
+
+    MG_IMC2D.VERBOSE = False
+    T0 = np.full(prob.shape, prob.T0)
+    st = MG_IMC2D.init_simulation(prob.Ntarget, T0, T0, prob.edges1, prob.edges2, prob.energy_edges, prob.eos,
+                                  prob.inv_eos, Ntarget_ic=prob.Ntarget, geometry="xy", precision="fp64", seed=seed)
+    rows = []
+    nmax = prob.Nmax
+    zeros = np.zeros(prob.shape)
+    for _ in range(nsteps):
+        st, info = MG_IMC2D.step(st, prob.Ntarget, prob.Nboundary, 0, nmax, prob.T_boundary, prob.dt, prob.edges1,
+                                 prob.edges2, prob.energy_edges, prob.sigma_a_funcs, prob.inv_eos, prob.cv_func, zeros,
+                                 reflect=prob.reflect, theta=1.0, use_scalar_intensity_Tr=True,
+                                 conserve_comb_energy=False, geometry="xy", max_events_per_particle=prob.max_events,
+                                 T_emit_floor=prob.T_emit_floor, Ntotal=0, Ntotal_T_floor=0.02,
+                                 Nmax_growth=prob.Nmax_growth, Nmax_final=prob.Nmax_final)
+        assert abs(info["balance_residual"]) <= 1e-12 * info["total_energy"]
+        nmax = info["Nmax_next"]
+        rows.append((info["N_particles"], info["total_internal_energy"], info["total_radiation_energy"],
+                     info["boundary_emission"], info["boundary_outgoing"]))
+    st._engine.close()
+    return np.array(rows, dtype=np.float64)
+
+
+def test_mg_powerlaw_marshak_matches_published_run(published_mg):
+    """The first ten steps (0.1 ns) at the published particle counts, eight
+    seeds.  Each published figure is one run, so a difference is judged as
+    the t statistic (ours - published) / (sd sqrt(1 + 1/8)) with 7 degrees of
+    freedom (sd the spread of one run, from the seeds; the rounding of the
+    printed digits added in quadrature), converted to the Gaussian z of the
+    same tail probability; the per-step face energy is deterministic and the
+    census size follows from the comb target."""
+    from radtranbook_amd import _lib
+    from radtranbook_amd.problems import mg_powerlaw_marshak
+
+    _lib.load()
+    _lib.require_gpu()
+    prob = mg_powerlaw_marshak()
+    from scipy import stats
+
+    nsteps, seeds = 10, tuple(range(1, 9))
+    print(f"test_mg_powerlaw_marshak_matches_published_run: 140 cells x 10 groups, {nsteps} steps, "
+          f"Ntarget = Nboundary = 5e5, Nmax 2e6 + 1e5/step, {len(seeds)} seeds", flush=True)
+    runs = np.stack([_run_mg_powerlaw(prob, s, nsteps) for s in seeds])  # (seeds, steps, 5)
+    mean, sd = runs.mean(0), runs.std(0, ddof=1)
+    f1 = np.sqrt(1.0 + 1.0 / len(seeds))
+    dof = len(seeds) - 1
+
+    def zstat(diff, sdev, rounding):
+        t = np.abs(diff) / np.sqrt((sdev * f1) ** 2 + rounding ** 2)
+        return stats.norm.isf(np.clip(stats.t.sf(t, dof), 1e-300, 0.5))
+
+    pub = np.array(published_mg["steps"][:nsteps], dtype=np.float64)  # step, t, N, E_mat, E_rad
+    summ = {round(r[0], 6): r for r in published_mg["summary"]}
+    # census size after the comb: the target plus the bins that keep one particle
+    np.testing.assert_allclose(mean[:, 0], pub[:, 2], rtol=2e-3)
+    # radiation energy per step (5 printed digits)
+    half_ulp = 0.5 * 10.0 ** (np.floor(np.log10(pub[:, 4])) - 4)
+    z = zstat(mean[:, 2] - pub[:, 4], sd[:, 2], half_ulp)
+    print("  E_rad per step, ours vs published:", " ".join(f"{a:.4e}/{b:.4e}" for a, b in zip(mean[:, 2], pub[:, 4])),
+          flush=True)
+    print(f"  E_rad |z| max {z.max():.2f}", flush=True)
+    assert z.max() < 4.0
+    for k, t in ((4, 0.05), (9, 0.1)):
+        s = summ[t]  # t, N, total, internal, radiation, boundary_emission, boundary_outgoing, source, residual
+        assert mean[k, 3] == pytest.approx(s[5], rel=1e-6)  # face energy a c T_b^4 / 4 x area x dt
+        for col, name, digits in ((2, "radiation", 7), (4, "outgoing", 7), (1, "internal", 6)):
+            ref = s[{2: 4, 4: 6, 1: 3}[col]]
+            hu = 0.5 * 10.0 ** (np.floor(np.log10(abs(ref))) - (digits - 1))
+            zz = float(zstat(mean[k, col] - ref, sd[k, col], hu))
+            print(f"  t = {t} ns {name}: ours {mean[k, col]:.7e} +- {sd[k, col]:.1e} (one run), published {ref:.7e},"
+                  f" |z| {zz:.2f}", flush=True)
+            assert zz < 4.0, (t, name)
