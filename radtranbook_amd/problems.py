@@ -367,3 +367,113 @@ class MGPowerLawMarshak:
 
 def mg_powerlaw_marshak(**kw):
     return MGPowerLawMarshak(**kw)
+
+
+@dataclass
+class CrookedPipeMGRZ:
+    """The reference's multigroup crooked-pipe driver
+    (MG_IMC/problems/crooked_pipe_multigroup_imc.py, default arguments,
+    ``--mode quick``): rz, r in [0, 2] cm (40 cells), z in [0, 7] cm (140
+    cells); thin where r < 0.5 and (z < 3 or z > 4), r < 1.5 and 2.5 < z < 3,
+    r < 1.5 and 4 < z < 4.5, or 1 <= r < 1.5 and 2.5 < z < 4.5 (density 0.1),
+    thick elsewhere (density 2; is_optically_thick and material_density,
+    172-218); sigma_a(T, E) = min(10 rho max(T, 1e-2)^-1/2 E^-3, 1e14) with
+    each group the geometric mean of its edge values (237-263), 10 groups
+    with edges logspace(-2, 1, 11) keV (809); c_v = 0.05 rho (72); T0 = 0.05
+    keV (69); a 0.5 keV surface source on z = 0 for r < 0.5, given by
+    boundary_source_func (919-931), every other face open except the axis
+    (reflect = (True, False, False, False), 975); dt from 1e-4 ns growing by
+    1.1 per step to 0.01 ns; T_emit_floor 0.075 keV; 2e4 emission and 2e4
+    face particles per step, census combed to 1e5 growing by 1e4 per step to
+    5e5 (759-780)."""
+
+    nr: int = 40
+    nz: int = 140
+    n_groups: int = 10
+    T0: float = 0.05
+    T_bc: float = 0.5
+    dt_initial: float = 1e-4
+    dt_growth: float = 1.1
+    dt_max: float = 0.01
+    T_emit_floor: float = 0.075
+    Ntarget: int = 20_000
+    Nboundary: int = 20_000
+    Nmax: int = 100_000
+    Nmax_growth: int = 10_000
+    Nmax_final: int = 500_000
+    rho_thick: float = 2.0
+    rho_thin: float = 0.1
+    cv_mass: float = 0.05
+
+    @property
+    def edges1(self):
+        return np.linspace(0.0, 2.0, self.nr + 1)
+
+    @property
+    def edges2(self):
+        return np.linspace(0.0, 7.0, self.nz + 1)
+
+    @property
+    def energy_edges(self):
+        return np.logspace(-2.0, 1.0, self.n_groups + 1)
+
+    @property
+    def shape(self):
+        return (self.nr, self.nz)
+
+    @property
+    def thick(self):
+        r = 0.5 * (self.edges1[:-1] + self.edges1[1:])[:, None]
+        z = 0.5 * (self.edges2[:-1] + self.edges2[1:])[None, :]
+        thin = ((r < 0.5) & ((z < 3.0) | (z > 4.0))) | ((r < 1.5) & (z > 2.5) & (z < 3.0)) | \
+               ((r < 1.5) & (z > 4.0) & (z < 4.5)) | ((r >= 1.0) & (r < 1.5) & (z > 2.5) & (z < 4.5))
+        return ~thin
+
+    @property
+    def rho(self):
+        return np.where(self.thick, self.rho_thick, self.rho_thin)
+
+    def sigma_at(self, T, E):
+        return np.minimum(10.0 * self.rho * np.maximum(T, 1e-2) ** -0.5 * E ** -3.0, 1e14)
+
+    @property
+    def sigma_a_funcs(self):
+        e = self.energy_edges
+
+        def make(lo, hi):
+            return lambda T: np.sqrt(self.sigma_at(T, lo) * self.sigma_at(T, hi))
+        return [make(e[g], e[g + 1]) for g in range(self.n_groups)]
+
+    @property
+    def cv(self):
+        return self.cv_mass * self.rho
+
+    def eos(self, T):
+        return self.cv * T
+
+    def inv_eos(self, u):
+        return u / self.cv
+
+    def cv_func(self, T):
+        return self.cv + 0.0 * T
+
+    def boundary_source_func(self, r, z, side):
+        return self.T_bc if (side == "zmin" and r < 0.5) else 0.0
+
+    T_boundary = (0.0, 0.0, 0.0, 0.0)
+    reflect = (True, False, False, False)
+
+    def dt_schedule(self, t_final):
+        """The driver's steps: dt_initial growing by dt_growth to dt_max, the
+        last step cut to land on t_final."""
+        t, dt, out = 0.0, self.dt_initial, []
+        while t < t_final - 1e-15:
+            d = min(dt, t_final - t)
+            out.append(d)
+            t += d
+            dt = min(dt * self.dt_growth, self.dt_max)
+        return out
+
+
+def crooked_pipe_mg_rz(**kw):
+    return CrookedPipeMGRZ(**kw)
