@@ -49,6 +49,16 @@ const char* imc_last_error(void);
 /* Number of visible GPUs (0 if none). */
 int imc_device_count(void);
 
+/* Page-locked host memory for per-step outputs.  An output array that lies
+ * in a block from imc_host_alloc receives its values from a kernel directly
+ * (no staging slot, no host copy after the stream synchronises); any other
+ * host pointer still works through the context's staging.  No counterpart in
+ * the reference, whose outputs are NumPy arrays in pageable memory: it is
+ * the host side of this library's output path.  imc_host_free must not be
+ * called while a call that writes into the block is in flight. */
+int imc_host_alloc(uint64_t bytes, void** out);
+int imc_host_free(void* p);
+
 /* ---- gray 2-D (xy / rz) ------------------------------------------------ */
 
 /* Context for one mesh.  Replaces the module-level state set up by

@@ -39,6 +39,8 @@ _vp = C.c_void_p
 SIGNATURES = {
     "imc_last_error": (C.c_char_p, []),
     "imc_device_count": (C.c_int, []),
+    "imc_host_alloc": (C.c_int, [C.c_uint64, C.POINTER(_vp)]),
+    "imc_host_free": (C.c_int, [_vp]),
     "imc2d_create": (C.c_int, [C.POINTER(_vp), C.c_int, C.c_int, C.c_int, _dp, _dp, C.c_int, C.c_uint64, C.c_int]),
     "imc2d_destroy": (None, [_vp]),
     "imc2d_set_step": (C.c_int, [_vp, C.c_uint32]),
@@ -183,39 +185,72 @@ def _pf_worker():
 
 
 class HostArrays:
-    """Fresh float64 arrays of one shape whose pages are already resident.
+    """Fresh float64 arrays of one shape for per-step outputs.
 
     Every step returns new per-cell arrays (temperatures, tallies, emission
-    energies), which a caller may keep.  The first write to each 4 KiB page
-    of a new allocation faults (about 1 us a page, ~0.3 ms per step for the
-    six 200x200 arrays of a gray 2-D step), on the calling thread, inside the
-    device-to-host copy.  One helper thread (shared by all engines) allocates
-    and touches arrays ahead of use; the touch is a libc memset through
-    ctypes, which releases the GIL, so it runs while the calling thread waits
-    on the device.  ``take`` never blocks: with the reserve empty it returns a
-    plain ``np.empty``."""
+    energies), which a caller may keep.  These arrays live in page-locked
+    host blocks from ``imc_host_alloc``, into which the library's kernels
+    write their results directly (no staging copy, no host memcpy after the
+    stream synchronises; a fresh pageable array would also pay ~1 us per
+    4 KiB page of first-touch faults).  One helper thread (shared by all
+    engines) allocates blocks ahead of use -- the ctypes call releases the
+    GIL, so this runs while the calling thread waits on the device -- and a
+    block comes back to its pool when the last array that uses it is
+    garbage-collected.  ``take`` never blocks: with nothing in reserve it
+    returns a plain ``np.empty``, which the library handles through its
+    staging.  Without a GPU (no page-locked memory) the reserve holds
+    pageable arrays whose pages the helper thread has already touched."""
 
     def __init__(self, shape, depth=12, max_bytes=64 << 20, min_bytes=64 << 10):
         self.shape = tuple(int(x) for x in shape)
-        nbytes = int(np.prod(self.shape)) * 8
-        self.depth = 0 if nbytes < min_bytes else max(0, min(int(depth), max_bytes // max(nbytes, 1)))
-        self._q = collections.deque()
+        self.count = int(np.prod(self.shape))
+        self.nbytes = self.count * 8
+        self.depth = 0 if self.nbytes < min_bytes else max(0, min(int(depth), max_bytes // max(self.nbytes, 1)))
+        self._q = collections.deque()     # arrays ready to hand out
+        self._free = collections.deque()  # addresses of blocks no array uses any more
         self._queued = False
         self._closed = False
+        self._pinned = True  # until an allocation fails (no GPU): then pre-faulted pageable arrays
         self._ref = weakref.ref(self)
+
+    # -- blocks ------------------------------------------------------------------
+    def _alloc(self):
+        p = C.c_void_p()
+        try:
+            rc = load().imc_host_alloc(C.c_uint64(self.nbytes), C.byref(p))
+        except ImcError:
+            return None
+        return p.value if rc == 0 and p.value else None
+
+    def _wrap(self, addr):
+        buf = (C.c_double * self.count).from_address(addr)
+        fin = weakref.finalize(buf, _host_block_released, self._ref, addr)
+        fin.atexit = False  # the process's memory goes with it
+        return np.frombuffer(buf, dtype=np.float64).reshape(self.shape)
 
     def _refill(self):
         self._queued = False
         while not self._closed and len(self._q) < self.depth:
-            a = np.empty(self.shape)
-            C.memset(a.ctypes.data, 0, a.nbytes)
-            self._q.append(a)
+            if self._free:
+                self._q.append(self._wrap(self._free.popleft()))
+                continue
+            addr = self._alloc() if self._pinned else None
+            if addr is None:
+                self._pinned = False
+                a = np.empty(self.shape)
+                C.memset(a.ctypes.data, 0, a.nbytes)  # fault the pages in here, not in the caller's copy
+                self._q.append(a)
+            else:
+                self._q.append(self._wrap(addr))
 
     def take(self):
         try:
             a = self._q.popleft()
         except IndexError:
-            a = np.empty(self.shape)
+            try:
+                a = self._wrap(self._free.popleft())
+            except IndexError:
+                a = np.empty(self.shape)
         if self.depth and not self._queued and not self._closed:
             self._queued = True
             global _PF_THREAD
@@ -228,7 +263,17 @@ class HostArrays:
 
     def close(self):
         self._closed = True
-        self._q.clear()
+        self._q.clear()  # their blocks come back through _host_block_released
+        while self._free:
+            load().imc_host_free(C.c_void_p(self._free.popleft()))
+
+
+def _host_block_released(pool_ref, addr):
+    pool = pool_ref()
+    if pool is not None and not pool._closed:
+        pool._free.append(addr)
+    else:
+        load().imc_host_free(C.c_void_p(addr))
 
 
 class Engine2D:

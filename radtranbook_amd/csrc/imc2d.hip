@@ -1472,6 +1472,26 @@ static CopyJob stage_out(Imc2dCtx* c, int slot, const double* src) {
   return CopyJob{src, c->stage.dslot(c->ncell, slot), (int64_t)(sizeof(double) * c->ncell)};
 }
 
+// Device per-cell array -> the caller's host array.  Straight into it when it
+// lies in a block of imc_host_alloc (page-locked, kernel-visible), else
+// through staging slot `slot`, copied out by finish_out once the stream has
+// synchronised.  A null host pointer gives an empty job.
+struct HostOut {
+  double* host = nullptr;
+  const double* staged = nullptr;
+};
+static CopyJob out_to_host(Imc2dCtx* c, int slot, const double* src, double* host, HostOut* h) {
+  *h = HostOut{};
+  if (!host) return CopyJob{};
+  const int64_t bytes = (int64_t)(sizeof(double) * c->ncell);
+  if (void* d = host_device_alias(host, (size_t)bytes)) return CopyJob{src, d, bytes};
+  *h = HostOut{host, c->stage.slot(c->ncell, slot)};
+  return CopyJob{src, c->stage.dslot(c->ncell, slot), bytes};
+}
+static void finish_out(const Imc2dCtx* c, const HostOut& h) {
+  if (h.host) std::memcpy(h.host, h.staged, sizeof(double) * c->ncell);
+}
+
 static double* stage_scalars(Imc2dCtx* c) { return c->stage.slot(c->ncell, 6); }
 static double* stage_scalars_dev(Imc2dCtx* c) { return c->stage.dslot(c->ncell, 6); }
 
@@ -1869,7 +1889,6 @@ int imc2d_set_material(imc2d_ctx* c, const double* T, const double* sigma_true, 
                        double dt, int clip, double* sigma_a_out, double* sigma_s_out) {
   IMC_REQUIRE(c && T && sigma_true && cv, kErrArg, "imc2d_set_material: null argument");
   IMC_HIP(hipSetDevice(c->device));
-  size_t bytes = sizeof(double) * c->ncell;
   int rc;
   {
     CopyJob in[3] = {stage_in(c, 0, c->T.p, T), stage_in(c, 1, c->sig.p, sigma_true), stage_in(c, 2, c->cv.p, cv)};
@@ -1883,14 +1902,15 @@ int imc2d_set_material(imc2d_ctx* c, const double* T, const double* sigma_true, 
                                                                c->bad);
   IMC_HIP(hipGetLastError());
   int* hbad = (int*)stage_scalars(c);
+  HostOut ho[2];
   {
-    CopyJob out[3] = {CopyJob{c->bad, stage_scalars_dev(c), 4}, sigma_a_out ? stage_out(c, 3, c->sa.p) : CopyJob{},
-                      sigma_s_out ? stage_out(c, 4, c->ss.p) : CopyJob{}};
+    CopyJob out[3] = {CopyJob{c->bad, stage_scalars_dev(c), 4}, out_to_host(c, 3, c->sa.p, sigma_a_out, &ho[0]),
+                      out_to_host(c, 4, c->ss.p, sigma_s_out, &ho[1])};
     if ((rc = copy_jobs(out, 3, c->stream))) return rc;
   }
   IMC_HIP(hipStreamSynchronize(c->stream));
-  if (sigma_a_out) std::memcpy(sigma_a_out, c->stage.slot(c->ncell, 3), bytes);
-  if (sigma_s_out) std::memcpy(sigma_s_out, c->stage.slot(c->ncell, 4), bytes);
+  finish_out(c, ho[0]);
+  finish_out(c, ho[1]);
   IMC_REQUIRE(!*hbad, kErrArg, "Fleck factor out of bounds");
   c->emit_lo = c->emit_hi = 0;
   return kOk;
@@ -1970,16 +1990,14 @@ int imc2d_update_material(imc2d_ctx* c, double* T_out, double* Tr_out, double* u
   // (a left-to-right sum over 1024-cell chunks left it near 4e-14)
   if ((rc = tree_sum(c->uV.p, nc, c->scal + 6, c->tree_scr.p, s))) return rc;
   double* hs = stage_scalars(c);
+  HostOut ho[3];
   {
-    CopyJob out[4] = {T_out ? stage_out(c, 3, c->T.p) : CopyJob{}, Tr_out ? stage_out(c, 4, c->Tr.p) : CopyJob{},
-                      u_out ? stage_out(c, 5, c->u.p) : CopyJob{}, CopyJob{c->scal + 6, stage_scalars_dev(c) + 3, 8}};
+    CopyJob out[4] = {out_to_host(c, 3, c->T.p, T_out, &ho[0]), out_to_host(c, 4, c->Tr.p, Tr_out, &ho[1]),
+                      out_to_host(c, 5, c->u.p, u_out, &ho[2]), CopyJob{c->scal + 6, stage_scalars_dev(c) + 3, 8}};
     if ((rc = copy_jobs(out, 4, s))) return rc;
   }
   IMC_HIP(hipStreamSynchronize(s));
-  const size_t bytes = sizeof(double) * nc;
-  if (T_out) std::memcpy(T_out, c->stage.slot(nc, 3), bytes);
-  if (Tr_out) std::memcpy(Tr_out, c->stage.slot(nc, 4), bytes);
-  if (u_out) std::memcpy(u_out, c->stage.slot(nc, 5), bytes);
+  for (const HostOut& h : ho) finish_out(c, h);
   if (internal_total_out) *internal_total_out = hs[3];
   return kOk;
 }
@@ -2106,14 +2124,15 @@ int imc2d_sample_volume(imc2d_ctx* c, int kind, int64_t N, const double* QV_host
     IMC_HIP(hipMemcpyAsync(c->fcounts.p, c->counts.p, sizeof(int64_t) * c->ncell, hipMemcpyDeviceToDevice,
                            c->stream));  // the counts used, for imc2d_emission_counts
   double* hs = stage_scalars(c);
+  HostOut ho_E;
   {
-    CopyJob out[2] = {CopyJob{c->scal + 0, stage_scalars_dev(c), 8}, E_out ? stage_out(c, 3, QV) : CopyJob{}};
+    CopyJob out[2] = {CopyJob{c->scal + 0, stage_scalars_dev(c), 8}, out_to_host(c, 3, QV, E_out, &ho_E)};
     int rc1 = copy_jobs(out, 2, c->stream);
     if (rc1) return rc1;
   }
   IMC_HIP(hipStreamSynchronize(c->stream));
   double Etot = hs[0];
-  if (E_out) std::memcpy(E_out, c->stage.slot(c->ncell, 3), sizeof(double) * c->ncell);
+  finish_out(c, ho_E);
   if (total_out) *total_out = Etot;
   if (count_out) *count_out = (Etot > 0.0 && N > 0) ? total : 0;
   if (!(Etot > 0.0) || N <= 0 || total <= 0) return kOk;
@@ -2225,22 +2244,20 @@ int imc2d_transport(imc2d_ctx* c, double dt, const int32_t* reflect, int64_t max
                                                                   c->si_raw.p, c->sa.p, f32 ? c->saf.p : nullptr,
                                                                   c->vol, dt, c->dep.p, c->si.p);
   IMC_HIP(hipGetLastError());
-  size_t bytes = sizeof(double) * c->ncell;
-  double* h_dep = c->stage.slot(c->ncell, 3);
-  double* h_si = c->stage.slot(c->ncell, 4);
   double* hs = stage_scalars(c);
   unsigned long long* hst = (unsigned long long*)(hs + 1);
+  HostOut ho[2];
   {
     double* hsd = stage_scalars_dev(c);
-    CopyJob out[4] = {dep_out ? stage_out(c, 3, c->dep.p) : CopyJob{}, si_out ? stage_out(c, 4, c->si.p) : CopyJob{},
+    CopyJob out[4] = {out_to_host(c, 3, c->dep.p, dep_out, &ho[0]), out_to_host(c, 4, c->si.p, si_out, &ho[1]),
                       CopyJob{c->scal + 3, hsd, 8}, CopyJob{c->stats, hsd + 1, 64}};
     int rc1 = copy_jobs(out, 4, s);
     if (rc1) return rc1;
   }
   IMC_HIP(hipStreamSynchronize(s));
   IMC_HIP(hipEventElapsedTime(&c->last_transport_ms, c->ev0, c->ev1));
-  if (dep_out) std::memcpy(dep_out, h_dep, bytes);
-  if (si_out) std::memcpy(si_out, h_si, bytes);
+  finish_out(c, ho[0]);
+  finish_out(c, ho[1]);
   if (bl_out) *bl_out = hs[0];
   if (stats_out)
     for (int a = 0; a < 8; ++a) stats_out[a] = (int64_t)hst[a];
@@ -2423,18 +2440,18 @@ int imc2d_transport_cf(imc2d_ctx* c, double dt, const int32_t* reflect, int64_t 
   IMC_HIP(hipGetLastError());
   double* hs = stage_scalars(c);
   unsigned long long* hst = (unsigned long long*)(hs + 1);
+  HostOut ho[2];
   {
     double* hsd = stage_scalars_dev(c);
-    CopyJob out[4] = {dep_out ? stage_out(c, 3, c->dep.p) : CopyJob{}, si_out ? stage_out(c, 4, c->si.p) : CopyJob{},
+    CopyJob out[4] = {out_to_host(c, 3, c->dep.p, dep_out, &ho[0]), out_to_host(c, 4, c->si.p, si_out, &ho[1]),
                       CopyJob{c->scal + 3, hsd, 8}, CopyJob{c->stats, hsd + 1, 64}};
     int rc1 = copy_jobs(out, 4, s);
     if (rc1) return rc1;
   }
   IMC_HIP(hipStreamSynchronize(s));
   IMC_HIP(hipEventElapsedTime(&c->last_transport_ms, c->ev0, c->ev1));
-  size_t bytes = sizeof(double) * c->ncell;
-  if (dep_out) std::memcpy(dep_out, c->stage.slot(c->ncell, 3), bytes);
-  if (si_out) std::memcpy(si_out, c->stage.slot(c->ncell, 4), bytes);
+  finish_out(c, ho[0]);
+  finish_out(c, ho[1]);
   if (bl_out) *bl_out = hs[0];
   if (stats_out)
     for (int a = 0; a < 8; ++a) stats_out[a] = (int64_t)hst[a];
@@ -2527,7 +2544,6 @@ int imc2d_comb(imc2d_ctx* c, int64_t Nmax, double* ecen_out, double* disc_out, d
                                                          c->tally_copies - 1, cp);
     IMC_HIP(hipGetLastError());
   }
-  size_t bytes = sizeof(double) * nc;
   if (disc_out) {
     k_reduce_copies<unsigned int, unsigned long long><<<launch_blocks(nc, 256), 256, 0, s>>>(
         nc, c->tally_copies, c->count_part.p, c->cellcount.p);
@@ -2540,8 +2556,9 @@ int imc2d_comb(imc2d_ctx* c, int64_t Nmax, double* ecen_out, double* disc_out, d
   if ((rc = tree_sum_dev(c->A.w, total_dev, c->scal + 5, c->tree_scr.p, s))) return rc;
   double* hs = stage_scalars(c);
   double* hsd = stage_scalars_dev(c);
+  HostOut ho[2];
   {
-    CopyJob out[4] = {disc_out ? stage_out(c, 4, c->disc.p) : CopyJob{}, ecen_out ? stage_out(c, 3, c->ecen.p) : CopyJob{},
+    CopyJob out[4] = {out_to_host(c, 4, c->disc.p, disc_out, &ho[0]), out_to_host(c, 3, c->ecen.p, ecen_out, &ho[1]),
                       CopyJob{c->scal + 4, hsd, 16}, CopyJob{total_dev, hsd + 2, 16}};
     static_assert(sizeof(int64_t) == sizeof(double), "scalar staging slots are 8 bytes");
     if ((rc = copy_jobs(out, 4, s))) return rc;
@@ -2552,8 +2569,8 @@ int imc2d_comb(imc2d_ctx* c, int64_t Nmax, double* ecen_out, double* disc_out, d
   IMC_REQUIRE(fault_bits == 0, kErrState,
               "imc2d_comb: comb output exceeded its capacity bound or a tile look-back did not complete");
   c->n = total;
-  if (disc_out) std::memcpy(disc_out, c->stage.slot(nc, 4), bytes);
-  if (ecen_out) std::memcpy(ecen_out, c->stage.slot(nc, 3), bytes);
+  finish_out(c, ho[0]);
+  finish_out(c, ho[1]);
   if (total_rad_out) *total_rad_out = hs[1];
   if (ecen_total_out) *ecen_total_out = hs[0];
   return kOk;

@@ -3,7 +3,10 @@
 #include <cstdio>
 
 #include <algorithm>
+#include <map>
+#include <mutex>
 
+#include "../../include/imc_hip.h"
 #include "imc_util.hpp"
 
 namespace imc {
@@ -18,6 +21,21 @@ void set_error(const char* fmt, ...) {
 }
 
 const char* get_error() { return g_err; }
+
+// Blocks of imc_host_alloc: host base -> (bytes, kernel-visible base).
+static std::mutex g_host_mu;
+static std::map<uintptr_t, std::pair<size_t, char*>> g_host_blocks;
+
+void* host_device_alias(const void* p, size_t bytes) {
+  if (!p) return nullptr;
+  const uintptr_t a = (uintptr_t)p;
+  std::lock_guard<std::mutex> lock(g_host_mu);
+  auto it = g_host_blocks.upper_bound(a);
+  if (it == g_host_blocks.begin()) return nullptr;
+  --it;
+  if (a + bytes > it->first + it->second.first) return nullptr;
+  return it->second.second + (a - it->first);
+}
 
 int chunk_grid(int64_t nchunk) {
   return (int)std::max<int64_t>(1, std::min<int64_t>(kChunkGrid, (nchunk + kChunkBlock - 1) / kChunkBlock));
@@ -361,3 +379,40 @@ int exclusive_scan_i32_to_i64(const int32_t* in, int64_t n, int64_t* out, int64_
 }
 
 }  // namespace imc
+
+extern "C" {
+
+int imc_host_alloc(uint64_t bytes, void** out) {
+  using namespace imc;
+  IMC_REQUIRE(out && bytes > 0, kErrArg, "imc_host_alloc: bad arguments");
+  *out = nullptr;
+  void* p = nullptr;
+  void* d = nullptr;
+  IMC_HIP(hipHostMalloc(&p, (size_t)bytes, hipHostMallocMapped | hipHostMallocPortable));
+  if (hipHostGetDevicePointer(&d, p, 0) != hipSuccess || !d) {
+    (void)hipHostFree(p);
+    set_error("imc_host_alloc: no device address for the block");
+    return kErrHip;
+  }
+  {
+    std::lock_guard<std::mutex> lock(g_host_mu);
+    g_host_blocks[(uintptr_t)p] = std::make_pair((size_t)bytes, (char*)d);
+  }
+  *out = p;
+  return kOk;
+}
+
+int imc_host_free(void* p) {
+  using namespace imc;
+  IMC_REQUIRE(p, kErrArg, "imc_host_free: null pointer");
+  {
+    std::lock_guard<std::mutex> lock(g_host_mu);
+    auto it = g_host_blocks.find((uintptr_t)p);
+    IMC_REQUIRE(it != g_host_blocks.end(), kErrArg, "imc_host_free: not a block of imc_host_alloc");
+    g_host_blocks.erase(it);
+  }
+  IMC_HIP(hipHostFree(p));
+  return kOk;
+}
+
+}  // extern "C"

@@ -117,6 +117,10 @@ struct CopyJob {
 };
 int copy_jobs(const CopyJob* jobs, int njobs, hipStream_t s);
 
+// Kernel-visible address of [p, p + bytes) when it lies inside one block of
+// imc_host_alloc, else null.
+void* host_device_alias(const void* p, size_t bytes);
+
 // Up to eight device ranges cleared in one kernel launch (8-byte
 // granularity).  A kernel rather than hipMemsetAsync: one launch for all of a
 // step's tally buffers, and safe under graph capture.
