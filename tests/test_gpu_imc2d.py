@@ -328,3 +328,59 @@ def test_fp32_vs_fp64_ensemble(L, oracle):
     assert z.max() < 5.5
     if det.any():
         np.testing.assert_allclose(mg[det], mc[det], rtol=1e-5)
+
+
+# Drop-in step options beyond the defaults, each against the oracle's step
+# on the same seed (FP64): a fixed volume source (xy; rz with the linear
+# tilt and uniform), the census-bincount T_r (use_scalar_intensity_Tr=False)
+# and the comb discrepancy put back into the material (conserve_comb_energy).
+@pytest.mark.parametrize("case", ["fixed_xy", "fixed_rz_linear", "fixed_rz_uniform", "census_Tr", "conserve_comb"])
+def test_step_options_match_oracle(L, oracle, case):
+    from radtranbook_amd import IMC2D
+
+    _announce(f"test_step_options_match_oracle {case}")
+    IMC2D.VERBOSE = False
+    geometry = "rz" if "rz" in case else "xy"
+    n, m = 16, 16
+    e1, e2 = _mesh(n, m)
+    if geometry == "rz":
+        e1 = np.linspace(0.0, 1.0, n + 1)
+    g = np.random.default_rng(1).standard_normal((n, m))
+    sig0 = np.exp(2.3 + 1.5 * (g - g.mean()) / g.std())
+    cvv = 0.3
+    sf = lambda T: sig0 * np.maximum(T, 1e-3) ** -3  # noqa: E731
+    eos = lambda T: cvv * T  # noqa: E731
+    inv = lambda u: u / cvv  # noqa: E731
+    cvf = lambda T: cvv + 0 * T  # noqa: E731
+    T0 = np.full((n, m), 1e-3)
+    src = np.zeros((n, m))
+    Ns = 0
+    if case.startswith("fixed"):
+        src[4:10, 3:12] = 50.0
+        Ns = 8000
+    kw = dict(use_scalar_intensity_Tr=case != "census_Tr", conserve_comb_energy=case == "conserve_comb",
+              rz_linear_source=case != "fixed_rz_uniform")
+    Tb = (0.5, 0, 0, 0) if geometry == "xy" else (0, 0, 0.5, 0)
+    refl = (False, False, False, False) if geometry == "xy" else (True, False, False, False)
+    N, seed = 16000, 21
+    st = IMC2D.init_simulation(N // 2, T0, T0, e1, e2, eos, inv, geometry=geometry, precision="fp64", seed=seed)
+    orc = oracle.Oracle2D(e1, e2, geometry, seed=seed)
+    so = orc.init_simulation(N // 2, T0, T0, eos, inv)
+    for k in range(3):
+        st, a = IMC2D.step(st, N // 2, N // 2, Ns, N, Tb, 0.005, e1, e2, sf, inv, cvf, src, reflect=refl,
+                           geometry=geometry, **kw)
+        so, b = orc.step(so, N // 2, N // 2, Ns, N, Tb, 0.005, sf, inv, cvf, src, reflect=refl, **kw)
+        assert abs(a["energy_residual"]) <= 1e-12 * a["total_energy"]
+        if k == 0:  # identical streams end to end on the first step
+            assert a["source_emission"] == pytest.approx(b["source_emission"], rel=1e-13, abs=0)
+            if Ns:
+                assert a["source_emission"] > 0
+            np.testing.assert_allclose(a["deposited"], b["dep"], rtol=1e-6, atol=1e-9 * np.abs(b["dep"]).max())
+            close = np.isclose(a["radiation_temperature"], b["radiation_temperature"], rtol=1e-6, atol=1e-12)
+            assert close.mean() > 0.99
+            np.testing.assert_allclose(a["temperature"], b["temperature"], rtol=1e-6)
+            assert a["comb_discrepancy"] == pytest.approx(b["comb_discrepancy"], rel=1e-6, abs=1e-12 * b["total_energy"])
+        assert a["total_energy"] == pytest.approx(b["total_energy"], rel=1e-4)
+        assert a["total_internal_energy"] == pytest.approx(b["total_internal_energy"], rel=1e-4)
+    if case == "conserve_comb":
+        assert a["comb_discrepancy"] == 0.0
