@@ -185,6 +185,35 @@ int imc2d_transport_cf(imc2d_ctx* ctx, double dt, const int32_t* reflect, int64_
                        double fastpath_threshold, double no_progress_rel_tol, int64_t no_progress_max_streak,
                        double* dep_out, double* si_out, double* bl_out, int64_t* stats_out);
 
+/* One whole step of IMC/IMC2D.py:1591-1897 (step() with
+ * use_scalar_intensity_Tr=True, conserve_comb_energy=False, no fixed source,
+ * uniform face temperatures) on the device-held power-law material of
+ * imc2d_set_material_powerlaw: the Fleck block, face sources
+ * (_sample_boundary_*, Nboundary particles on each side with face_E[side] >
+ * 0, face_E = a c T_b^4 / 4 x area x dt), emission (_sample_emission_*,
+ * Ntarget particles), transport (_transport_particles_*), the material and
+ * T_r update of IMC/IMC2D.py:1800-1810 and the comb to about Nmax particles
+ * (_comb).  The same kernels and random streams as the staged entry points
+ * (imc2d_fleck_powerlaw, imc2d_sample_boundary, imc2d_sample_volume,
+ * imc2d_transport, imc2d_update_material, imc2d_comb) in that order, enqueued
+ * as one device-resident sequence: the pool sizes the random numbers decide
+ * live in device memory, so no host round trip splits the step.  use_graph
+ * != 0 captures the sequence into a HIP graph on first use and replays it
+ * while the parameters, the buffers and a pool bound (1/8 above the step
+ * that captured it) hold; one graph is kept per pool-buffer parity.
+ * reflect[4] as imc2d_transport; rz_linear as imc2d_sample_volume.
+ * Outputs (host, nullable; per cell): T, T_r, u after the update, dep
+ * (deposited energy density), si (scalar intensity), E (emission energies).
+ * scalars_out[6]: emission energy, boundary loss, sum(u V), pre-comb census
+ * energy, post-comb census energy, transport kernel ms (device wall clock).
+ * counts_out[11]: particles transported, census after the comb, emission
+ * particles, then imc2d_transport's stats[8].  Synchronises the stream.
+ * Refused while the adjoint replay or pinned emission counts are on. */
+int imc2d_step(imc2d_ctx* ctx, double theta, double dt, int64_t Ntarget, int64_t Nboundary, const double* face_E,
+               int64_t Nmax, const int32_t* reflect, int64_t max_events, int rz_linear, int use_graph, double* T_out,
+               double* Tr_out, double* u_out, double* dep_out, double* si_out, double* E_out, double* scalars_out,
+               int64_t* counts_out);
+
 /* Per-cell stochastic comb to about Nmax particles: _comb (IMC/IMC2D.py:1448-1511).
  * Optional outputs: ecen_out, pre-comb census energy per cell; disc_out,
  * ecen - ecen_after per cell (the comb discrepancy of IMC/IMC2D.py:1511);

@@ -96,6 +96,19 @@ def _face_profile(geometry, side, e1, e2, dt, bfunc):
     return np.cumsum(em)
 
 
+def _device_material(eng, sigma_a_func, cv, inv_eos):
+    """device_material() of the step's callables, checked once per set of
+    callable objects (the check scans every cell of sigma0 and c_v)."""
+    key = (sigma_a_func, getattr(cv, "__self__", cv), getattr(cv, "__func__", None),
+           getattr(inv_eos, "__self__", inv_eos), getattr(inv_eos, "__func__", None))
+    hit = getattr(eng, "_dm_check", None)
+    if hit is not None and all(a is b for a, b in zip(hit[0], key)):
+        return hit[1]
+    dm = device_material(sigma_a_func, cv, inv_eos, eng.shape)
+    eng._dm_check = (key, dm)
+    return dm
+
+
 def init_simulation(Ntarget, Tinit, Tr_init, edges1, edges2, eos, inv_eos, Ntarget_ic=None, geometry="xy",
                     precision="fp64", seed=1, device=0, engine=None):
     """Initialise material state and the equilibrium census (IMC/IMC2D.py:1514-1588).
@@ -151,13 +164,18 @@ def step(state, Ntarget, Nboundary, Nsource, Nmax, T_boundary, dt, edges1, edges
     eng.set_step(state.count)
 
     # a power-law opacity with a linear EOS runs on the device-held T and u
-    dm = device_material(sigma_a_func, cv, inv_eos, eng.shape) if (use_scalar_intensity_Tr and
-                                                                   not conserve_comb_energy) else None
+    dm = _device_material(eng, sigma_a_func, cv, inv_eos) if (use_scalar_intensity_Tr and
+                                                             not conserve_comb_energy) else None
     if dm is not None:
         refs = getattr(eng, "_device_material", None)
         cur = (sigma_a_func, cv.__self__, state.temperature, state.internal_energy)
         if refs is None or any(a is not b for a, b in zip(refs, cur)):
             eng.set_material_powerlaw(dm[0], dm[1], dm[2], dm[3], state.temperature, state.internal_energy)
+        src = np.asarray(source, dtype=np.float64)
+        if boundary_source_func is None and not (Nsource > 0 and np.max(src) > 0.0):
+            # the whole step as one device-resident sequence (a replayed graph)
+            return _step_device(state, eng, Ntarget, Nboundary, Nmax, T_boundary, dt, sigma_a_func, cv, reflect,
+                                theta, geometry, rz_linear_source, max_events_per_particle, t_step_start)
         eng.fleck_powerlaw(theta, dt, clip=True)
     else:
         eng._device_material = None
@@ -261,6 +279,88 @@ def step(state, Ntarget, Nboundary, Nsource, Nmax, T_boundary, dt, edges1, edges
                 "total": t_end - t_step_start,
             },
             "transport_kernel_ms": eng.last_transport_ms(),
+            "transport_events": {
+                "total": events_total,
+                "boundary_crossings": int(stats[1]),
+                "absorption_continue_events": int(stats[2]),
+                "census_events": int(stats[3]),
+                "absorption_capture_events": int(stats[4]),
+                "weight_floor_kills": int(stats[5]),
+                "reflections": int(stats[6]),
+                "event_cap_hits": int(stats[7]),
+                "avg_events_per_particle": events_total / n_tr,
+                "n_particles_transported": int(n_tr),
+            },
+        },
+    }
+    return state, info
+
+
+def _step_device(state, eng, Ntarget, Nboundary, Nmax, T_boundary, dt, sigma_a_func, cv, reflect, theta, geometry,
+                 rz_linear_source, max_events_per_particle, t_step_start):
+    """step() for the device-held power-law material with uniform face
+    temperatures and no fixed source: ``imc2d_step`` runs the Fleck block,
+    sourcing, transport, material update and comb on the device in one call
+    (the same kernels and random streams as the staged path)."""
+    e1, e2 = eng.e1, eng.e2
+    face_E = np.zeros(4)
+    if Nboundary > 0:
+        for side in range(4):
+            Tb = T_boundary[side]
+            Tb = Tb(state.time) if callable(Tb) else Tb
+            if Tb > 0.0:
+                face_E[side] = _face_energy(geometry, side, float(Tb), dt, e1, e2)
+    boundary_emission = float(np.sum(face_E))
+    t_transport_start = _time.perf_counter()
+    r = eng.step_device(theta, dt, Ntarget, Nboundary, face_E, Nmax, reflect, max_events_per_particle,
+                        rz_linear=rz_linear_source)
+    t_post_start = _time.perf_counter()
+    temperature, radiation_temperature, internal_energy = r["T"], r["Tr"], r["u"]
+    eng._device_material = (sigma_a_func, cv.__self__, temperature, internal_energy)
+    total_internal = r["internal_total"]
+    total_rad = r["rad_total"]
+    boundary_loss = r["boundary_loss"]
+    total_energy = total_internal + total_rad
+    energy_loss = total_energy - state.previous_total_energy - boundary_emission + boundary_loss
+    disc_total = r["ecen_total"] - total_rad
+
+    state.internal_energy = internal_energy
+    state.temperature = temperature
+    state.radiation_temperature = radiation_temperature
+    state.time += dt
+    state.previous_total_energy = total_energy
+    state.count += 1
+    state._invalidate()
+
+    t_end = _time.perf_counter()
+    stats = r["stats"]
+    events_total = int(stats[0])
+    n_tr = max(int(r["n_transported"]), 1)
+    info = {
+        "time": state.time,
+        "temperature": temperature,
+        "radiation_temperature": radiation_temperature,
+        "N_particles": r["n"],
+        "total_energy": total_energy,
+        "total_internal_energy": total_internal,
+        "total_radiation_energy": total_rad,
+        "boundary_emission": boundary_emission,
+        "boundary_loss": boundary_loss,
+        "source_emission": 0.0,
+        "energy_loss": energy_loss,
+        "comb_discrepancy": disc_total,
+        "energy_residual": energy_loss + disc_total,
+        "deposited": r["dep"],
+        "scalar_intensity": r["si"],
+        "emitted_energies": r["E_emit"],
+        "profiling": {
+            "phase_times_s": {
+                "sampling": t_transport_start - t_step_start,
+                "transport": t_post_start - t_transport_start,
+                "postprocess": t_end - t_post_start,
+                "total": t_end - t_step_start,
+            },
+            "transport_kernel_ms": r["transport_ms"],
             "transport_events": {
                 "total": events_total,
                 "boundary_crossings": int(stats[1]),

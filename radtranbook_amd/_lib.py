@@ -65,6 +65,8 @@ SIGNATURES = {
                                      C.c_int64, _dp, _dp, _dp, _i64p]),
     "imc2d_last_transport_ms": (C.c_float, [_vp]),
     "imc2d_comb": (C.c_int, [_vp, C.c_int64, _dp, _dp, _dp, _dp]),
+    "imc2d_step": (C.c_int, [_vp, C.c_double, C.c_double, C.c_int64, C.c_int64, _dp, C.c_int64, _i32p, C.c_int64,
+                             C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _i64p]),
     "imc2d_enable_adjoint": (C.c_int, [_vp, C.c_int]),
     "imc2d_adjoint": (C.c_int, [_vp, C.c_double, _i32p, C.c_int64, _dp, _dp, _dp]),
     "imc2d_emission_counts": (C.c_int, [_vp, _i64p, _i64p]),
@@ -297,6 +299,8 @@ class Engine2D:
         self._h = h
         self._L = L
         self._host = HostArrays(self.shape)
+        # whole steps of the device-held material replay a captured HIP graph
+        self.use_graph = True
 
     def close(self):
         if getattr(self, "_host", None) is not None:
@@ -444,6 +448,26 @@ class Engine2D:
         tr = C.c_double()
         check(self._L.imc2d_comb(self._h, int(Nmax), _p(ecen), _p(disc), C.byref(et), C.byref(tr)))
         return ecen, disc, et.value, tr.value
+
+    def step_device(self, theta, dt, Ntarget, Nboundary, face_E, Nmax, reflect, max_events, rz_linear=True,
+                    graph=None):
+        """``imc2d_step``: one whole step on the device-held power-law
+        material.  Returns a dict of the per-cell outputs (T, Tr, u, dep, si,
+        E_emit), the scalars and the counts."""
+        refl = np.array([1 if r else 0 for r in reflect], dtype=np.int32)
+        fe = np.ascontiguousarray(face_E, dtype=np.float64)
+        out = {k: self._host.take() for k in ("T", "Tr", "u", "dep", "si", "E_emit")}
+        sc = np.zeros(6)
+        cn = np.zeros(11, dtype=np.int64)
+        use_graph = self.use_graph if graph is None else bool(graph)
+        check(self._L.imc2d_step(self._h, float(theta), float(dt), int(Ntarget), int(Nboundary), _p(fe), int(Nmax),
+                                 _p(refl, _i32p), int(max_events), 1 if rz_linear else 0, 1 if use_graph else 0,
+                                 _p(out["T"]), _p(out["Tr"]), _p(out["u"]), _p(out["dep"]), _p(out["si"]),
+                                 _p(out["E_emit"]), _p(sc), _p(cn, _i64p)))
+        out.update(e_emit=sc[0], boundary_loss=sc[1], internal_total=sc[2], ecen_total=sc[3], rad_total=sc[4],
+                   transport_ms=sc[5], n_transported=int(cn[0]), n=int(cn[1]), emit_count=int(cn[2]),
+                   stats=cn[3:].copy())
+        return out
 
     # -- differentiable step ----------------------------------------------------
     def enable_adjoint(self, on=True):

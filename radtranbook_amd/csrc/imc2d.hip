@@ -79,17 +79,29 @@ struct PoolBuf {
   }
 };
 
+// Sizes and per-step values of the device-resident step (imc2d_step) in
+// device memory: its kernels read them when they run, so one enqueued
+// sequence, or one graph, serves every step whatever the pool size.
+struct StepDev {
+  int64_t n0;        // census size at the start of the step
+  int64_t n_src;     // first emission slot: n0 + the face particles
+  int64_t n;         // pool size after the sources (set on the device)
+  int64_t n_out;     // census size after the comb (set on the device)
+  uint32_t step;     // Philox step index
+  int fault;         // bit 1: the emission sample would pass the pool bound
+  double face_E[4];  // face energies of the step, per side
+};
+
 // ---------------------------------------------------------------------------
 // per-cell kernels (no FMA contraction: counts and weights must match the
 // oracle bit for bit)
 // ---------------------------------------------------------------------------
-__global__ void k_fleck(int64_t n, const double* __restrict__ T, const double* __restrict__ sig,
-                        const double* __restrict__ cv, double theta, double dt, int clip, double* __restrict__ sa,
-                        double* __restrict__ ss, float* __restrict__ saf, float* __restrict__ ssf,
-                        double2* __restrict__ sas64, float2* __restrict__ sas32, int* __restrict__ bad) {
+__device__ __forceinline__ void fleck_cell(int64_t c, const double* __restrict__ T, const double* __restrict__ sig,
+                                           const double* __restrict__ cv, double theta, double dt, int clip,
+                                           double* __restrict__ sa, double* __restrict__ ss, float* __restrict__ saf,
+                                           float* __restrict__ ssf, double2* __restrict__ sas64,
+                                           float2* __restrict__ sas32, int* __restrict__ bad) {
 #pragma clang fp contract(off)
-  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= n) return;
   double t = T[c];
   double beta = 4.0 * kA * (t * t * t) / cv[c];
   double f = 1.0 / (1.0 + theta * beta * sig[c] * kC * dt);
@@ -111,27 +123,44 @@ __global__ void k_fleck(int64_t n, const double* __restrict__ T, const double* _
   }
 }
 
-__global__ void k_emission_energy(int64_t n, const double* __restrict__ T, const double* __restrict__ sa,
-                                  const double* __restrict__ vol, double dt, double* __restrict__ E) {
-#pragma clang fp contract(off)
+__global__ void k_fleck(int64_t n, const double* __restrict__ T, const double* __restrict__ sig,
+                        const double* __restrict__ cv, double theta, double dt, int clip, double* __restrict__ sa,
+                        double* __restrict__ ss, float* __restrict__ saf, float* __restrict__ ssf,
+                        double2* __restrict__ sas64, float2* __restrict__ sas32, int* __restrict__ bad) {
   int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
-  double t = T[c] > 0.0 ? T[c] : 0.0;
+  fleck_cell(c, T, sig, cv, theta, dt, clip, sa, ss, saf, ssf, sas64, sas32, bad);
+}
+
+__device__ __forceinline__ double emission_energy_cell(double T, double sa, double vol, double dt) {
+#pragma clang fp contract(off)
+  double t = T > 0.0 ? T : 0.0;
   double t2 = t * t;
   double t4 = t2 * t2;
-  E[c] = kA * kC * t4 * sa[c] * dt * vol[c];
+  return kA * kC * t4 * sa * dt * vol;
+}
+
+__global__ void k_emission_energy(int64_t n, const double* __restrict__ T, const double* __restrict__ sa,
+                                  const double* __restrict__ vol, double dt, double* __restrict__ E) {
+  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  E[c] = emission_energy_cell(T[c], sa[c], vol[c], dt);
 }
 
 // Power-law opacity on the device: sigma = sigma0 * max(T, T_floor)^-power,
 // with power 3 evaluated as sigma0 / (m m m), the form of the host
 // PowerLawOpacity (radtranbook_amd/materials.py), so both give the same bits.
+__device__ __forceinline__ double powerlaw_sigma_cell(double T, double sigma0, double power, double T_floor) {
+#pragma clang fp contract(off)
+  const double m = T > T_floor ? T : T_floor;
+  return power == 3.0 ? sigma0 / (m * m * m) : sigma0 * pow(m, -power);
+}
+
 __global__ void k_powerlaw_sigma(int64_t n, const double* __restrict__ T, const double* __restrict__ sigma0,
                                  double power, double T_floor, double* __restrict__ sig) {
-#pragma clang fp contract(off)
   int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
-  const double m = T[c] > T_floor ? T[c] : T_floor;
-  sig[c] = power == 3.0 ? sigma0[c] / (m * m * m) : sigma0[c] * pow(m, -power);
+  sig[c] = powerlaw_sigma_cell(T[c], sigma0[c], power, T_floor);
 }
 
 // Material update of IMC/IMC2D.py:1800-1810 for u = c_v T: u += dep -
@@ -282,15 +311,56 @@ __global__ void k_sample_volume(PoolView<R> P, int64_t base, int64_t total, cons
                                 const double* __restrict__ QV, int kind, int geometry, const double* __restrict__ Ar,
                                 const double* __restrict__ Mr, const double* __restrict__ Pn, double dt, Key key,
                                 uint32_t step, uint32_t tag, const double* __restrict__ cf_sa,
-                                const double* __restrict__ cf_beta) {
+                                const double* __restrict__ cf_beta, const StepDev* __restrict__ sd) {
 #pragma clang fp contract(off)
-  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (sd) {  // the device-resident step: emission slots [n_src, n)
+    base = sd->n_src;
+    total = sd->n - sd->n_src;
+    step = sd->step;
+  }
+  // one thread per particle (a grid-stride loop doubled the registers and
+  // spilled)
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= total) return;
   int64_t c = find_bin(offs, ncell, k);
   int64_t n = offs[c + 1] - offs[c];
   int i = (int)(c / ny), j = (int)(c % ny);
   double x0 = e1[i], x1 = e1[i + 1], y0 = e2[j], y1 = e2[j + 1];
   Ctr ctr = make_ctr(tag, (uint64_t)k, step);
+  if constexpr (kF32<R>) {
+    if (geometry != 2 && kind != 2) {
+      // FP32 pools: the particle's FP32 geometry from 24-bit uniforms and
+      // the hardware square root and sin/cos (two Philox blocks instead of
+      // three, no FP64 transcendentals); same distributions as below
+      const QuadF a = block4_f(ctr, key, 0), b = block4_f(ctr, key, 1);
+      const float w1 = (float)(x1 - x0), w2 = (float)(y1 - y0);
+      float q1, q2, f1, f2;
+      if (geometry == 0) {
+        q1 = (float)x0 + w1 * a.u1;
+        q2 = (float)y0 + w2 * a.u2;
+        const float uz = 2.0f * a.u3 - 1.0f;
+        const float rxy = __builtin_amdgcn_sqrtf(fmaxf(0.0f, 1.0f - uz * uz));
+        f1 = rxy * __builtin_amdgcn_cosf(b.u0);
+        f2 = rxy * __builtin_amdgcn_sinf(b.u0);
+      } else {
+        const float a0 = (float)x0, a1 = (float)x1;
+        q1 = __builtin_amdgcn_sqrtf(a0 * a0 + (a1 * a1 - a0 * a0) * a.u1);
+        q2 = (float)y0 + w2 * a.u2;
+        f2 = 2.0f * a.u3 - 1.0f;
+        f1 = __builtin_amdgcn_cosf(b.u0);
+      }
+      const int64_t o = base + k;
+      P.w[o] = QV[c] / (double)n;
+      P.t[o] = kind == 1 ? 0.0f : (float)dt * a.u0;
+      P.p1[o] = q1;
+      P.p2[o] = q2;
+      P.d1[o] = f1;
+      P.d2[o] = f2;
+      P.ci[o] = i;
+      P.cj[o] = j;
+      return;
+    }
+  }
   Pair b0 = block_d(ctr, key, 0), b1 = block_d(ctr, key, 1), b2 = block_d(ctr, key, 2);
   // kind 0 uniform time, 1 time 0 (census), 2 Carter-Forest emission time
   // from the same uniform (IMC/IMC2D_CarterForest.py:900-903)
@@ -345,14 +415,66 @@ __device__ __forceinline__ int locate_edge(const double* __restrict__ e, int n, 
 template <typename R>
 __global__ void k_sample_boundary(PoolView<R> P, int64_t base, int64_t n, int geometry, int side, double E_total,
                                   double dt, const double* __restrict__ e1, int nx, const double* __restrict__ e2,
-                                  int ny, const double* __restrict__ prof, Key key, uint32_t step) {
+                                  int ny, const double* __restrict__ prof, Key key, uint32_t step,
+                                  const StepDev* __restrict__ sd) {
 #pragma clang fp contract(off)
   int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
+  if (sd) {  // the device-resident step: `base` is the offset past the census
+    base += sd->n0;
+    E_total = sd->face_E[side];
+    step = sd->step;
+  }
   double x0 = e1[0], x1 = e1[nx], y0 = e2[0], y1 = e2[ny];
   int nface = side < 2 ? ny : nx;
   const double* fe = side < 2 ? e2 : e1;
   Ctr ctr = make_ctr(kTagBoundary0 + side, (uint64_t)k, step);
+  if constexpr (kF32<R>) {
+    if (!prof) {
+      // FP32 pools: 24-bit uniforms and the hardware square root and cos
+      // (one Philox block, two for an rz end face); same distributions
+      const QuadF u = block4_f(ctr, key, 0);
+      const float a = (float)(side < 2 ? y0 : x0), b = (float)(side < 2 ? y1 : x1);
+      float q1, q2, f1, f2;
+      if (geometry == 0) {
+        const float mun = __builtin_amdgcn_sqrtf(u.u1);
+        const float ct = __builtin_amdgcn_sqrtf(1.0f - u.u1) * __builtin_amdgcn_cosf(u.u2);
+        const float along = a + (b - a) * u.u3;
+        if (side == 0) {
+          f1 = mun; f2 = ct; q1 = (float)(x0 + 1e-12); q2 = along;
+        } else if (side == 1) {
+          f1 = -mun; f2 = ct; q1 = (float)(x1 - 1e-12); q2 = along;
+        } else if (side == 2) {
+          f2 = mun; f1 = ct; q2 = (float)(y0 + 1e-12); q1 = along;
+        } else {
+          f2 = -mun; f1 = ct; q2 = (float)(y1 - 1e-12); q1 = along;
+        }
+      } else if (side < 2) {
+        f2 = 2.0f * u.u1 - 1.0f;
+        const float muabs = __builtin_amdgcn_sqrtf(u.u2);
+        f1 = side == 0 ? muabs : -muabs;
+        q1 = (float)(side == 0 ? x0 + 1e-12 : x1 - 1e-12);
+        q2 = a + (b - a) * u.u3;
+      } else {
+        const QuadF v = block4_f(ctr, key, 1);
+        const float e = __builtin_amdgcn_sqrtf(u.u2);
+        f2 = side == 2 ? e : -e;
+        f1 = __builtin_amdgcn_cosf(u.u3);
+        q2 = (float)(side == 2 ? y0 + 1e-12 : y1 - 1e-12);
+        q1 = __builtin_amdgcn_sqrtf(a * a + (b * b - a * a) * v.u0);
+      }
+      const int64_t o = base + k;
+      P.w[o] = E_total / (double)n;
+      P.t[o] = (float)dt * u.u0;
+      P.p1[o] = q1;
+      P.p2[o] = q2;
+      P.d1[o] = f1;
+      P.d2[o] = f2;
+      P.ci[o] = locate_edge(e1, nx, (double)q1);
+      P.cj[o] = locate_edge(e2, ny, (double)q2);
+      return;
+    }
+  }
   Pair b0 = block_d(ctr, key, 0), b1 = block_d(ctr, key, 1), b2 = block_d(ctr, key, 2);
   double a = side < 2 ? y0 : x0, b = side < 2 ? y1 : x1;
   if (prof) {
@@ -436,6 +558,7 @@ struct TransportArgs {
   double* bl;
   unsigned long long* stats;  // [8]
   QueueMap Q;
+  const StepDev* sd;  // non-null: pool size and step index read from the device
 };
 
 // Finished histories of a wave are staged in LDS and written back to their
@@ -485,6 +608,14 @@ template <typename R, int GEOM, typename Off>
 __global__ __launch_bounds__(256) void k_transport(TransportArgs<R> A) {
   using R2 = typename Vec2<R>::type;
   const double wfloor = *A.floor_ptr;
+  int64_t n = A.n;
+  uint32_t step = A.step;
+  QueueMap Q = A.Q;
+  if (A.sd) {
+    n = A.sd->n;
+    step = A.sd->step;
+    Q.len = Q.order ? ((n + (1ll << Q.shift) - 1) >> Q.shift) << Q.shift : n;
+  }
   // per-workgroup tally copy: spreads same-cell atomics over copies and XCDs
   const int64_t copy_off = (int64_t)(blockIdx.x & A.copy_mask) * A.ncell;
   double* const dep_t = A.dep_raw + copy_off;
@@ -524,7 +655,7 @@ __global__ __launch_bounds__(256) void k_transport(TransportArgs<R> A) {
     }
     uint64_t need = __ballot(!active);
     if (need) {
-      const int64_t kk = wave_take(wq, !active, need, A.next, A.batch, A.Q, A.n);
+      const int64_t kk = wave_take(wq, !active, need, A.next, A.batch, Q, n);
       if (kk >= 0) {
         k = kk;
         const Off ko = (Off)k;
@@ -564,7 +695,7 @@ __global__ __launch_bounds__(256) void k_transport(TransportArgs<R> A) {
       const R2 ex = elem(A.e1p, (uint32_t)i);
       const R2 ey = elem(A.e2p, (uint32_t)j);
       MoveOut<R> o;
-      Ctr ctr = make_ctr(kTagTransport, (uint64_t)k, A.step);
+      Ctr ctr = make_ctr(kTagTransport, (uint64_t)k, step);
       segment_move<GEOM>(ctr, A.key, ev - 1u, w, d1, d2, p1, p2, fresh, ex, ey, mat, dcen, o);
       dcen = o.dcen;
       st_bnd += o.evt == kEvtBoundary;
@@ -686,8 +817,10 @@ __global__ __launch_bounds__(kChunkBlock) void k_chunk_key(PoolView<R> P, int64_
                                                            const double* __restrict__ sa,
                                                            const double* __restrict__ ss,
                                                            const double* __restrict__ beta, double dt,
-                                                           uint8_t* __restrict__ key, int32_t* __restrict__ hist) {
+                                                           uint8_t* __restrict__ key, int32_t* __restrict__ hist,
+                                                           const int64_t* __restrict__ n_dev = nullptr) {
   __shared__ int h[kChunkKeys];
+  if (n_dev) n = *n_dev;
   chunk_hist_zero(h);
   const int64_t nchunk = (n + (1ll << shift) - 1) >> shift;
   for (int64_t c = blockIdx.x * (int64_t)kChunkBlock + threadIdx.x; c < nchunk;
@@ -719,8 +852,10 @@ __global__ __launch_bounds__(kChunkBlock) void k_chunk_key(PoolView<R> P, int64_
   chunk_hist_publish(h, hist);
 }
 
-__global__ void k_weight_floor(const double* __restrict__ sumw, int64_t n, double* __restrict__ out) {
+__global__ void k_weight_floor(const double* __restrict__ sumw, int64_t n, double* __restrict__ out,
+                               const int64_t* __restrict__ n_dev = nullptr) {
 #pragma clang fp contract(off)
+  if (n_dev) n = *n_dev;
   *out = 1e-10 * (*sumw) / (double)(n > 1 ? n : 1);
 }
 
@@ -953,7 +1088,9 @@ __device__ __forceinline__ bool comb_alive(const PoolView<R>& P, int64_t k, int 
 // workgroup's private copy of the histogram at the end.
 template <typename R>
 __global__ __launch_bounds__(256) void k_comb_ecen(PoolView<R> P, int64_t n, int nx, int ny, int copy_mask,
-                                                   double* __restrict__ ecen_part) {
+                                                   double* __restrict__ ecen_part,
+                                                   const int64_t* __restrict__ n_dev = nullptr) {
+  if (n_dev) n = *n_dev;
   __shared__ int lkey[kLdsSlots];
   __shared__ double lval[kLdsSlots];
   lds_tally_init(lkey, lval);
@@ -995,8 +1132,8 @@ __global__ void k_comb_ew(int64_t ncell, const double* __restrict__ ecen, const 
   }
 }
 
-// Comb decisions, one thread per particle, and each 256-particle block's
-// copy total (the block sums are scanned instead of every particle's count).
+// Comb decisions, one thread per particle, and each 256-particle tile's copy
+// total (the tile sums are scanned instead of every particle's count).
 __device__ __forceinline__ int block_sum_256i(int v, int* wsum) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -1006,14 +1143,24 @@ __device__ __forceinline__ int block_sum_256i(int v, int* wsum) {
 }
 
 template <typename R>
-__global__ __launch_bounds__(256) void k_comb_copies(PoolView<R> P, int64_t n, int nx, int ny,
+__global__ __launch_bounds__(256) void k_comb_copies(PoolView<R> P, int64_t n, int64_t ntile, int nx, int ny,
                                                      const double* __restrict__ ew, const double* __restrict__ Etot,
                                                      Key key, uint32_t step, int32_t* __restrict__ copies,
-                                                     int64_t* __restrict__ bsum) {
+                                                     int64_t* __restrict__ bsum, const StepDev* __restrict__ sd) {
 #pragma clang fp contract(off)
   __shared__ int wsum[4];
+  // workgroup b takes tile b; the device-resident step launches one
+  // workgroup per tile of its pool bound and reads the pool size and step
+  // index from the device, so tiles past the pool get a zero total and one
+  // scan over all ntile totals serves any pool of up to 256 ntile
+  const int64_t b = blockIdx.x;
+  if (b >= ntile) return;
+  if (sd) {
+    n = sd->n;
+    step = sd->step;
+  }
   const bool passthrough = !(*Etot > 0.0);  // empty census: every live particle kept as it is
-  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t k = b * 256 + threadIdx.x;
   int32_t m = 0;
   if (k < n && comb_alive(P, k, nx, ny)) {
     if (passthrough) {
@@ -1028,32 +1175,38 @@ __global__ __launch_bounds__(256) void k_comb_copies(PoolView<R> P, int64_t n, i
   }
   if (k < n) copies[k] = m;
   int tot = block_sum_256i(m, wsum);
-  if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+  if (threadIdx.x == 0) bsum[b] = tot;
 }
 
-// Comb output: workgroup b expands particles [256 b, 256 b + 256).  Their
-// copy offsets are staged in LDS.  When no parent of the block has more than
-// kCombDirectMax copies each thread writes its own parent's copies; otherwise
-// the workgroup's threads stride over its contiguous output range, each
-// finding its parent by binary search in LDS, so that a parent with many
-// copies is shared by the whole workgroup instead of one lane.  Both write
-// the same outputs.
+// Comb output: workgroup b expands the particles of tile b, [256 b,
+// 256 b + 256).  Their copy offsets are staged in LDS.  When no parent of the
+// tile has more than kCombDirectMax copies each thread writes its own
+// parent's copies; otherwise the workgroup's threads stride over the tile's
+// contiguous output range, each finding its parent by binary search in LDS,
+// so that a parent with many copies is shared by the whole workgroup instead
+// of one lane.  Both write the same outputs.
 constexpr int kCombDirectMax = 4;
 
 template <typename R>
-__global__ __launch_bounds__(256) void k_comb_write(PoolView<R> S, PoolView<R> D, int64_t n, int nx, int ny,
-                                                    const int32_t* __restrict__ copies,
+__global__ __launch_bounds__(256) void k_comb_write(PoolView<R> S, PoolView<R> D, int64_t n, int64_t ntile, int nx,
+                                                    int ny, const int32_t* __restrict__ copies,
                                                     const int64_t* __restrict__ boff, const double* __restrict__ ew,
                                                     const double* __restrict__ Etot, int64_t cap, int* fault,
-                                                    int copy_mask, unsigned int* __restrict__ count_part) {
+                                                    int copy_mask, unsigned int* __restrict__ count_part,
+                                                    const int64_t* __restrict__ n_dev) {
   __shared__ int64_t so[257];
   __shared__ int wsum[4], wmax[4];
+  // workgroup blk takes tile blk; with n_dev the pool size is read from the
+  // device and tiles past it exit at once
+  const int64_t blk = blockIdx.x;
+  if (n_dev) n = *n_dev;
+  if (blk >= ntile || blk * 256 >= n) return;
   const int t = threadIdx.x;
-  const int64_t k0 = (int64_t)blockIdx.x * 256;
+  const int64_t k0 = blk * 256;
   const int64_t k = k0 + t;
   int32_t m = k < n ? copies[k] : 0;
   {
-    // this block's copy offsets: block offset + exclusive scan of the counts
+    // this tile's copy offsets: tile offset + exclusive scan of the counts
     const int lane = t & 63, wid = t >> 6;
     int inc = m, mx = m;
 #pragma unroll
@@ -1068,7 +1221,7 @@ __global__ __launch_bounds__(256) void k_comb_write(PoolView<R> S, PoolView<R> D
     int wofs = 0;
 #pragma unroll
     for (int q = 0; q < 4; ++q) wofs += q < wid ? wsum[q] : 0;
-    const int64_t b0 = boff[blockIdx.x];
+    const int64_t b0 = boff[blk];
     so[t] = b0 + wofs + inc - m;
     if (t == 255) so[256] = b0 + wofs + inc;
   }
@@ -1080,7 +1233,7 @@ __global__ __launch_bounds__(256) void k_comb_write(PoolView<R> S, PoolView<R> D
       atomicAdd(count_part + (int64_t)(blockIdx.x & copy_mask) * ((int64_t)nx * ny) + key, cnt);
   }
   const int cnt = (n - k0 < 256) ? (int)(n - k0) : 256;
-  __syncthreads();  // so[cnt] is the block's end: threads past n carry zero copies
+  __syncthreads();  // so[cnt] is the tile's end: threads past n carry zero copies
   const int64_t lo = so[0], hi = so[cnt];
   if (hi > cap) {  // beyond the output capacity the host sized from its bound
     if (t == 0) atomicOr(fault, 1);
@@ -1458,6 +1611,20 @@ struct Imc2dCtx {
   // page-locked staging: slots of ncell doubles (0 T, 1 sigma, 2 cv, 3 and 4
   // per-cell outputs, 5 source energies), then 16 scalars
   PinnedBuf stage;
+  // device-resident step (imc2d_step): its device block, the page-locked
+  // input/output block, and one captured graph per pool-buffer parity with
+  // the launch signature it was captured for
+  StepDev* sd = nullptr;
+  PinnedBuf step_io;
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
+  std::vector<uint64_t> graph_sig[2];
+  int64_t graph_lim[2] = {0, 0};  // pool size each capture covers
+  int graph_next = 0;              // slot the next capture replaces
+  int wall_khz = 0;                // device wall-clock rate
+  // a second stream for the step's host outputs, which overlap the comb
+  hipStream_t side = nullptr;
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
 };
 
 // Host array -> staging slot (host memcpy); the returned job copies it on to
@@ -1617,6 +1784,131 @@ __global__ void k_noalloc_outputs(const double* __restrict__ bl_in, const unsign
   if (a < 8 && st_out) st_out[a] = (int64_t)st_in[a];
 }
 
+// ---------------------------------------------------------------------------
+// device-resident step (imc2d_step)
+// ---------------------------------------------------------------------------
+// Page-locked blocks: the host fills StepHostIn before each step (census
+// size, step index, face energies, kernel-visible output destinations); the
+// step's last kernels fill StepHostOut for the host.
+struct StepHostIn {
+  int64_t n0, n_src;
+  uint32_t step, pad;
+  double face_E[4];
+  double* out[6];  // T, T_r, u, dep, si, emission energies (null: not wanted)
+};
+struct StepHostOut {
+  double e_emit, bl, internal_total, ecen_total, rad_total;
+  int64_t n, n_out, emit_count;
+  unsigned long long stats[8];
+  int64_t t0, t1;  // device wall clock before and after the transport launch
+  int fault;
+};
+constexpr size_t kStepInBytes = 256;
+static_assert(sizeof(StepHostIn) <= kStepInBytes, "step input block");
+
+// The device copy of the step's inputs, with the output destinations.
+struct StepOutDst {
+  double* out[6];
+};
+
+// First kernel of the step: the step inputs into the device block
+// (workgroup 0), then per cell the power-law sigma, the Fleck factor and
+// effective opacities (IMC/IMC2D.py:1646-1651, clipped) and the emission
+// energies a c T^4 sigma_a dt V: k_powerlaw_sigma, k_fleck and
+// k_emission_energy in one launch, the same bits.
+__global__ void k_step_prologue(const StepHostIn* __restrict__ in, StepDev* __restrict__ sd,
+                                StepOutDst* __restrict__ dst, int64_t n, const double* __restrict__ T,
+                                const double* __restrict__ sigma0, double power, double T_floor,
+                                double* __restrict__ sig, const double* __restrict__ cv, double theta, double dt,
+                                double* __restrict__ sa, double* __restrict__ ss, float* __restrict__ saf,
+                                float* __restrict__ ssf, double2* __restrict__ sas64, float2* __restrict__ sas32,
+                                const double* __restrict__ vol, double* __restrict__ E) {
+  if (blockIdx.x == 0) {
+    const int a = threadIdx.x;
+    if (a == 0) {
+      sd->n0 = in->n0;
+      sd->n_src = in->n_src;
+      sd->n = in->n_src;
+      sd->n_out = 0;
+      sd->step = in->step;
+      sd->fault = 0;
+    }
+    if (a < 4) sd->face_E[a] = in->face_E[a];
+    if (a < 6) dst->out[a] = in->out[a];
+  }
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  sig[c] = powerlaw_sigma_cell(T[c], sigma0[c], power, T_floor);
+  fleck_cell(c, T, sig, cv, theta, dt, 1, sa, ss, saf, ssf, sas64, sas32, nullptr);
+  E[c] = emission_energy_cell(T[c], sa[c], vol[c], dt);
+}
+
+// Pool size after the emission sample: offs[ncell] particles from n_src on,
+// none when the emission energy is zero or no particles were asked for (the
+// rule of imc2d_sample_volume).  A count past the pool bound `lim` (which
+// the ceil rule cannot give for finite energies) samples nothing and is
+// reported.
+__global__ void k_step_pool(StepDev* __restrict__ sd, const int64_t* __restrict__ total,
+                            const double* __restrict__ Etot, int64_t N, int64_t lim) {
+  if (threadIdx.x == 0) {
+    const int64_t m = (*Etot > 0.0 && N > 0) ? *total : 0;
+    if (m < 0 || m > lim - sd->n_src) {
+      sd->fault |= 2;
+      sd->n = sd->n_src;
+    } else {
+      sd->n = sd->n_src + m;
+    }
+  }
+}
+
+// Census size after the comb, clamped to the output capacity (an overflow
+// is reported through the comb's fault word), for the sums that follow.
+__global__ void k_comb_total(StepDev* __restrict__ sd, const int64_t* __restrict__ total, int64_t cap,
+                             int* __restrict__ fault) {
+  if (threadIdx.x == 0) {
+    const int64_t t = *total;
+    if (t > cap) atomicOr(fault, 1);
+    sd->n_out = t > cap ? cap : t;
+  }
+}
+
+__global__ void k_stamp(int64_t* __restrict__ t) {
+  if (threadIdx.x == 0) *t = (int64_t)wall_clock64();
+}
+
+__global__ void k_step_end(const StepDev* __restrict__ sd, const int* __restrict__ fault,
+                           const double* __restrict__ scal, const unsigned long long* __restrict__ stats,
+                           StepHostOut* __restrict__ out) {
+  const int a = threadIdx.x;
+  if (a < 8) out->stats[a] = stats[a];
+  if (a == 0) {
+    const int64_t nout = sd->n_out;
+    out->e_emit = scal[0];
+    out->bl = scal[3];
+    out->internal_total = scal[6];
+    out->ecen_total = scal[4];
+    out->rad_total = scal[5];
+    out->n = sd->n;
+    out->n_out = nout;
+    out->emit_count = sd->n - sd->n_src;
+    out->fault = *fault | sd->fault;
+  }
+}
+
+// Per-cell outputs of the step to their host destinations (blockIdx.y picks
+// the array; a null destination is skipped).
+__global__ void k_step_outputs(const StepOutDst* __restrict__ dst, const double* __restrict__ T,
+                               const double* __restrict__ Tr, const double* __restrict__ u,
+                               const double* __restrict__ dep, const double* __restrict__ si,
+                               const double* __restrict__ E, int64_t n) {
+  const int a = blockIdx.y;
+  double* d = dst->out[a];
+  if (!d) return;
+  const double* src = a == 0 ? T : a == 1 ? Tr : a == 2 ? u : a == 3 ? dep : a == 4 ? si : E;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += stride) d[c] = src[c];
+}
+
 static int zero_words(void* p, size_t bytes, hipStream_t s) {
   int64_t n = (int64_t)(bytes / 8);
   if (n <= 0) return kOk;
@@ -1637,9 +1929,234 @@ static void launch_transport(int geometry, int grid, int block, hipStream_t s, c
     k_transport<R, 1, Off><<<grid, block, 0, s>>>(A);
 }
 
+struct imc2d_ctx : public Imc2dCtx {};
+
+// Parameters and launch shape of one device-resident step.
+struct StepLaunch {
+  double theta, dt;
+  int64_t Ntarget, Nb, Nmax, max_events;
+  int face_mask, refl, rz_linear;
+  int64_t lim;  // largest pool the launch covers: tile count and buffer sizes
+  int grid_tr;  // tracker workgroups
+  bool narrow;  // 32-bit byte offsets in the tracker
+};
+
+// The step's kernels in order, on the context's stream; every size that
+// depends on the step's random numbers is read from the device block.
+template <typename R>
+static int step_enqueue_t(imc2d_ctx* c, const StepLaunch& L) {
+  hipStream_t s = c->stream;
+  const int64_t nc = c->ncell;
+  const int gc = launch_blocks(nc, 256);
+  StepDev* sd = c->sd;
+  StepOutDst* dst = (StepOutDst*)((char*)c->sd + 256);
+  const StepHostIn* hin = (const StepHostIn*)c->step_io.dev;
+  StepHostOut* hout = (StepHostOut*)(c->step_io.dev + kStepInBytes);
+  const Key key = make_key(c->seed);
+  const bool f32 = c->precision == IMC_FP32_TRACK;
+  int rc;
+  // step inputs, Fleck factor on the device-held temperature
+  // (IMC/IMC2D.py:1646-1651) and emission energies
+  k_step_prologue<<<gc, 256, 0, s>>>(hin, sd, dst, nc, c->T.p, c->sigma0.p, c->pl_power, c->pl_floor, c->sig.p,
+                                     c->cv.p, L.theta, L.dt, c->sa.p, c->ss.p, f32 ? c->saf.p : nullptr,
+                                     f32 ? c->ssf.p : nullptr, c->sas64.p, c->sas32.p, c->vol, c->E.p);
+  IMC_HIP(hipGetLastError());
+  // face sources, appended after the census side by side (IMC/IMC2D.py:1653-1700)
+  int64_t off = 0;
+  for (int side = 0; side < 4; ++side) {
+    if (!((L.face_mask >> side) & 1)) continue;
+    k_sample_boundary<R><<<launch_blocks(L.Nb, 256), 256, 0, s>>>(c->A.view<R>(), off, L.Nb, c->geometry, side, 0.0,
+                                                                  L.dt, c->e1, c->nx, c->e2, c->ny, nullptr, key, 0u,
+                                                                  sd);
+    IMC_HIP(hipGetLastError());
+    off += L.Nb;
+  }
+  // emission (IMC/IMC2D.py:1191-1445): ceil counts, offsets, sample
+  if ((rc = canonical_sum(c->E.p, nc, c->scal + 0, c->sum_scr.p, s))) return rc;
+  k_counts<<<gc, 256, 0, s>>>(nc, c->E.p, L.Ntarget, 0, c->scal + 0, c->counts.p);
+  IMC_HIP(hipGetLastError());
+  if ((rc = exclusive_scan_i64(c->counts.p, nc, c->offs.p, c->scan_scr.p, s))) return rc;
+  k_step_pool<<<1, 64, 0, s>>>(sd, c->offs.p + nc, c->scal + 0, L.Ntarget, L.lim);
+  IMC_HIP(hipGetLastError());
+  const int geom = c->geometry == IMC_GEOM_XY ? 0 : (L.rz_linear ? 2 : 1);
+  if (geom == 2) {
+    k_rz_params<<<gc, 256, 0, s>>>(c->nx, c->ny, c->T.p, c->e1, c->e2, c->Ar.p, c->Mr.p, c->Pn.p);
+    IMC_HIP(hipGetLastError());
+  }
+  // one thread per emission particle of the bound
+  const int gsrc = launch_blocks(L.Ntarget + nc + 16, 256);
+  k_sample_volume<R><<<gsrc, 256, 0, s>>>(c->A.view<R>(), 0, 0, c->offs.p, nc, c->ny, c->e1, c->e2, c->E.p, 0, geom,
+                                          c->Ar.p, c->Mr.p, c->Pn.p, L.dt, key, 0u, kTagEmission, c->sa.p, c->beta.p,
+                                          sd);
+  IMC_HIP(hipGetLastError());
+  // transport (IMC/IMC2D.py:938-1107) with the weight floor 1e-10 sum(w) / N
+  {
+    const int64_t tb = (int64_t)sizeof(double) * nc * c->tally_copies;
+    ZeroJob z[5] = {{c->dep_raw.p, tb}, {c->si_raw.p, tb}, {c->stats, (int64_t)sizeof(unsigned long long) * 8},
+                    {c->scal + 3, 8}, {c->next, 8}};
+    if ((rc = zero_jobs(z, 5, s))) return rc;
+  }
+  if ((rc = tree_sum_dev(c->A.w, &sd->n, c->scal + 1, c->tree_scr.p, s))) return rc;
+  k_weight_floor<<<1, 1, 0, s>>>(c->scal + 1, 0, c->scal + 2, &sd->n);
+  IMC_HIP(hipGetLastError());
+  QueueMap Q = pool_order_queue(0);
+  if ((c->batch & (c->batch - 1)) == 0) {
+    int shift = 0;
+    while ((1 << shift) < c->batch) ++shift;
+    k_chunk_key<R><<<kChunkGrid, kChunkBlock, 0, s>>>(c->A.view<R>(), 0, shift, c->nx, c->ny, c->e1, c->e2, c->sa.p,
+                                                      c->ss.p, nullptr, L.dt, c->chunk_key.p, c->chunk_hist.p,
+                                                      &sd->n);
+    IMC_HIP(hipGetLastError());
+    if ((rc = chunk_order_dev(c->chunk_key.p, &sd->n, shift, kChunkGrid, c->chunk_hist.p, c->chunk_ord.p, s)))
+      return rc;
+    Q = QueueMap{c->chunk_ord.p, 0, shift};
+  }
+  k_stamp<<<1, 64, 0, s>>>(&hout->t0);
+  IMC_HIP(hipGetLastError());
+  {
+    using R2 = typename Vec2<R>::type;
+    const R2* e1p = f32 ? (const R2*)c->e1p32.p : (const R2*)c->e1p64.p;
+    const R2* e2p = f32 ? (const R2*)c->e2p32.p : (const R2*)c->e2p64.p;
+    const R2* sas = f32 ? (const R2*)c->sas32.p : (const R2*)c->sas64.p;
+    TransportArgs<R> A{c->A.view<R>(), 0, e1p, e2p, c->nx, c->ny, sas, L.dt, L.refl, L.max_events, c->scal + 2,
+                       key, 0u, c->next, c->batch, c->dep_raw.p, c->si_raw.p, nc, c->tally_copies - 1, c->scal + 3,
+                       c->stats, Q, sd};
+    if (L.narrow)
+      launch_transport<R, uint32_t>(c->geometry, L.grid_tr, 256, s, A);
+    else
+      launch_transport<R, uint64_t>(c->geometry, L.grid_tr, 256, s, A);
+    IMC_HIP(hipGetLastError());
+  }
+  k_stamp<<<1, 64, 0, s>>>(&hout->t1);
+  IMC_HIP(hipGetLastError());
+  k_finalize_tallies<<<gc, 256, 0, s>>>(nc, c->tally_copies, c->dep_raw.p, c->si_raw.p, c->sa.p,
+                                        f32 ? c->saf.p : nullptr, c->vol, L.dt, c->dep.p, c->si.p);
+  IMC_HIP(hipGetLastError());
+  // material update (IMC/IMC2D.py:1800-1810)
+  k_update_material<<<gc, 256, 0, s>>>(nc, c->u.p, c->T.p, c->Tr.p, c->dep.p, c->E.p, c->vol, c->cv.p, c->si.p,
+                                       c->uV.p);
+  IMC_HIP(hipGetLastError());
+  if ((rc = tree_sum(c->uV.p, nc, c->scal + 6, c->tree_scr.p, s))) return rc;
+  // the per-cell outputs are final: copy them to the host on the side
+  // stream while the comb runs
+  IMC_HIP(hipEventRecord(c->fork_ev, s));
+  IMC_HIP(hipStreamWaitEvent(c->side, c->fork_ev, 0));
+  k_step_outputs<<<dim3((unsigned)std::min(gc, 64), 6), 256, 0, c->side>>>(dst, c->T.p, c->Tr.p, c->u.p, c->dep.p,
+                                                                           c->si.p, c->E.p, nc);
+  IMC_HIP(hipGetLastError());
+  IMC_HIP(hipEventRecord(c->join_ev, c->side));
+  // comb (IMC/IMC2D.py:1448-1511) over the tiles of the largest pool the
+  // launch covers; tiles past the pool count zero copies
+  const int64_t ntile = (L.lim + 255) / 256;
+  int64_t* bsum = c->counts.p;
+  int64_t* boff = c->counts.p + ntile + 1;
+  int64_t* total_dev = boff + ntile;
+  int* fault = (int*)(boff + ntile + 1);
+  {
+    ZeroJob z[2] = {{c->ecen_part.p, (int64_t)sizeof(double) * nc * c->tally_copies}, {fault, 8}};
+    if ((rc = zero_jobs(z, 2, s))) return rc;
+  }
+  k_comb_ecen<R><<<c->n_cu * 4, 256, 0, s>>>(c->A.view<R>(), 0, c->nx, c->ny, c->tally_copies - 1, c->ecen_part.p,
+                                             &sd->n);
+  IMC_HIP(hipGetLastError());
+  k_reduce_copies<double, double><<<gc, 256, 0, s>>>(nc, c->tally_copies, c->ecen_part.p, c->ecen.p);
+  IMC_HIP(hipGetLastError());
+  if ((rc = canonical_sum(c->ecen.p, nc, c->scal + 4, c->sum_scr.p, s))) return rc;
+  k_comb_ew<<<gc, 256, 0, s>>>(nc, c->ecen.p, c->scal + 4, L.Nmax, c->ew.p);
+  IMC_HIP(hipGetLastError());
+  // one workgroup per tile; tiles past the pool exit at once
+  const int gcomb = (int)ntile;
+  k_comb_copies<R><<<gcomb, 256, 0, s>>>(c->A.view<R>(), 0, ntile, c->nx, c->ny, c->ew.p, c->scal + 4, key, 0u,
+                                         c->copies.p, bsum, sd);
+  IMC_HIP(hipGetLastError());
+  if ((rc = exclusive_scan_i64(bsum, ntile, boff, c->scan_scr.p, s))) return rc;
+  k_comb_write<R><<<gcomb, 256, 0, s>>>(c->A.view<R>(), c->B.view<R>(), 0, ntile, c->nx, c->ny, c->copies.p, boff,
+                                        c->ew.p, c->scal + 4, c->B.cap, fault, c->tally_copies - 1, nullptr, &sd->n);
+  IMC_HIP(hipGetLastError());
+  k_comb_total<<<1, 64, 0, s>>>(sd, total_dev, c->B.cap, fault);
+  IMC_HIP(hipGetLastError());
+  if ((rc = tree_sum_dev(c->B.w, &sd->n_out, c->scal + 5, c->tree_scr.p, s))) return rc;
+  k_step_end<<<1, 64, 0, s>>>(sd, fault, c->scal, c->stats, hout);
+  IMC_HIP(hipGetLastError());
+  IMC_HIP(hipStreamWaitEvent(s, c->join_ev, 0));
+  return kOk;
+}
+
+static int step_enqueue(imc2d_ctx* c, const StepLaunch& L) {
+  return c->precision == IMC_FP64 ? step_enqueue_t<double>(c, L) : step_enqueue_t<float>(c, L);
+}
+
+// Buffers of a step whose pool stays within `lim` particles.
+static int step_reserve(imc2d_ctx* c, const StepLaunch& L) {
+  const int64_t nc = c->ncell, lim = L.lim;
+  const int64_t ntile = (lim + 255) / 256;
+  int rc;
+  if ((rc = c->A.grow(lim, c->n, c->stream))) return rc;
+  const int64_t comb_bound = L.Nmax + 2 * nc + lim + 256;
+  if (c->B.cap < comb_bound && (rc = c->B.grow(comb_bound + comb_bound / 2, 0, c->stream))) return rc;
+  const int64_t nchunk = (lim + c->batch - 1) / c->batch + 1;
+  if ((rc = c->copies.reserve(lim + 1)) || (rc = c->counts.reserve(std::max<int64_t>(nc + 1, 2 * ntile + 6))) ||
+      (rc = c->scan_scr.reserve(std::max(scan_scratch(nc), scan_scratch(ntile)) + 8)) ||
+      (rc = c->chunk_key.reserve(nchunk)) || (rc = c->chunk_ord.reserve(nchunk)) ||
+      (rc = c->chunk_hist.reserve((int64_t)kChunkGrid * kChunkKeys)))
+    return rc;
+  return kOk;
+}
+
+// What a captured step depends on: its parameters and every buffer address.
+static std::vector<uint64_t> step_signature(const imc2d_ctx* c, const StepLaunch& L) {
+  auto bits = [](double x) {
+    uint64_t u;
+    std::memcpy(&u, &x, 8);
+    return u;
+  };
+  auto P = [](const void* p) { return (uint64_t)(uintptr_t)p; };
+  return std::vector<uint64_t>{
+      bits(L.theta), bits(L.dt), (uint64_t)L.Ntarget, (uint64_t)L.Nb, (uint64_t)L.Nmax, (uint64_t)L.max_events,
+      (uint64_t)L.face_mask, (uint64_t)L.refl, (uint64_t)L.rz_linear, (uint64_t)L.grid_tr, (uint64_t)L.narrow,
+      c->seed, (uint64_t)c->batch, (uint64_t)c->tally_copies, bits(c->pl_power), bits(c->pl_floor),
+      P(c->A.w), P(c->A.d1), P(c->A.d2), P(c->A.t), P(c->A.p1), P(c->A.p2), P(c->A.ci), P(c->A.cj),
+      (uint64_t)c->A.cap, P(c->B.w), P(c->B.d1), P(c->B.d2), P(c->B.t), P(c->B.p1), P(c->B.p2), P(c->B.ci),
+      P(c->B.cj), (uint64_t)c->B.cap, P(c->copies.p), P(c->counts.p), P(c->scan_scr.p), P(c->chunk_key.p),
+      P(c->chunk_ord.p), P(c->chunk_hist.p)};
+}
+
+static void step_drop_graph(imc2d_ctx* c, int slot) {
+  if (c->graph_exec[slot]) (void)hipGraphExecDestroy(c->graph_exec[slot]);
+  if (c->graph[slot]) (void)hipGraphDestroy(c->graph[slot]);
+  c->graph_exec[slot] = nullptr;
+  c->graph[slot] = nullptr;
+  c->graph_sig[slot].clear();
+  c->graph_lim[slot] = 0;
+}
+
+// Capture the step into graph slot `slot` (the sequence is recorded, not run).
+static int step_capture(imc2d_ctx* c, const StepLaunch& L, int slot) {
+  step_drop_graph(c, slot);
+  IMC_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  const int rc = step_enqueue(c, L);
+  hipGraph_t g = nullptr;
+  const hipError_t ec = hipStreamEndCapture(c->stream, &g);
+  if (rc || ec != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    if (rc) return rc;
+    IMC_HIP(ec);
+  }
+  hipGraphExec_t x = nullptr;
+  const hipError_t ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+  if (ei != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    IMC_HIP(ei);
+  }
+  c->graph[slot] = g;
+  c->graph_exec[slot] = x;
+  c->graph_sig[slot] = step_signature(c, L);
+  c->graph_lim[slot] = L.lim;
+  return kOk;
+}
+
 extern "C" {
 
-struct imc2d_ctx : public Imc2dCtx {};
 
 const char* imc_last_error(void) { return imc::get_error(); }
 
@@ -1790,7 +2307,16 @@ void imc2d_destroy(imc2d_ctx* c) {
   c->fcounts.release();
   c->g_raw.release();
   c->g.release();
-  void* ptrs[] = {c->e1, c->e2, c->e1f, c->e2f, c->vol, c->scal, c->stats, c->next, c->bad};
+  step_drop_graph(c, 0);
+  step_drop_graph(c, 1);
+  c->step_io.release();
+  if (c->side) {
+    (void)hipStreamSynchronize(c->side);
+    (void)hipStreamDestroy(c->side);
+  }
+  if (c->fork_ev) (void)hipEventDestroy(c->fork_ev);
+  if (c->join_ev) (void)hipEventDestroy(c->join_ev);
+  void* ptrs[] = {c->e1, c->e2, c->e1f, c->e2f, c->vol, c->scal, c->stats, c->next, c->bad, c->sd};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   (void)hipEventDestroy(c->ev0);
@@ -2010,11 +2536,11 @@ static int launch_volume(imc2d_ctx* c, int64_t base, int64_t total, const double
   if (c->precision == IMC_FP64)
     k_sample_volume<double><<<g, 256, 0, c->stream>>>(c->A.view<double>(), base, total, c->offs.p, c->ncell, c->ny,
                                                       c->e1, c->e2, QV, kind, geom, c->Ar.p, c->Mr.p, c->Pn.p, dt,
-                                                      key, step, tag, c->sa.p, c->beta.p);
+                                                      key, step, tag, c->sa.p, c->beta.p, nullptr);
   else
     k_sample_volume<float><<<g, 256, 0, c->stream>>>(c->A.view<float>(), base, total, c->offs.p, c->ncell, c->ny,
                                                      c->e1, c->e2, QV, kind, geom, c->Ar.p, c->Mr.p, c->Pn.p, dt,
-                                                     key, step, tag, c->sa.p, c->beta.p);
+                                                     key, step, tag, c->sa.p, c->beta.p, nullptr);
   IMC_HIP(hipGetLastError());
   return kOk;
 }
@@ -2080,10 +2606,10 @@ int imc2d_sample_boundary(imc2d_ctx* c, int side, int64_t n, double E_total, dou
   Key key = make_key(c->seed);
   if (c->precision == IMC_FP64)
     k_sample_boundary<double><<<g, 256, 0, c->stream>>>(c->A.view<double>(), c->n, n, c->geometry, side, E_total, dt,
-                                                        c->e1, c->nx, c->e2, c->ny, prof, key, c->step);
+                                                        c->e1, c->nx, c->e2, c->ny, prof, key, c->step, nullptr);
   else
     k_sample_boundary<float><<<g, 256, 0, c->stream>>>(c->A.view<float>(), c->n, n, c->geometry, side, E_total, dt,
-                                                       c->e1, c->nx, c->e2, c->ny, prof, key, c->step);
+                                                       c->e1, c->nx, c->e2, c->ny, prof, key, c->step, nullptr);
   IMC_HIP(hipGetLastError());
   if (prof) IMC_HIP(hipStreamSynchronize(c->stream));
   c->n += n;
@@ -2525,23 +3051,23 @@ int imc2d_comb(imc2d_ctx* c, int64_t Nmax, double* ecen_out, double* disc_out, d
   Key key = make_key(c->seed);
   if (n > 0) {
     if (c->precision == IMC_FP64)
-      k_comb_copies<double><<<(unsigned)nblk, 256, 0, s>>>(c->A.view<double>(), n, c->nx, c->ny, c->ew.p,
-                                                           c->scal + 4, key, c->step, c->copies.p, bsum);
+      k_comb_copies<double><<<(unsigned)nblk, 256, 0, s>>>(c->A.view<double>(), n, nblk, c->nx, c->ny, c->ew.p,
+                                                           c->scal + 4, key, c->step, c->copies.p, bsum, nullptr);
     else
-      k_comb_copies<float><<<(unsigned)nblk, 256, 0, s>>>(c->A.view<float>(), n, c->nx, c->ny, c->ew.p, c->scal + 4,
-                                                          key, c->step, c->copies.p, bsum);
+      k_comb_copies<float><<<(unsigned)nblk, 256, 0, s>>>(c->A.view<float>(), n, nblk, c->nx, c->ny, c->ew.p,
+                                                          c->scal + 4, key, c->step, c->copies.p, bsum, nullptr);
     IMC_HIP(hipGetLastError());
   }
   if ((rc = exclusive_scan_i64(bsum, nblk, boff, c->scan_scr.p, s))) return rc;
   if (n > 0) {
     if (c->precision == IMC_FP64)
-      k_comb_write<double><<<(unsigned)nblk, 256, 0, s>>>(c->A.view<double>(), c->B.view<double>(), n, c->nx, c->ny,
-                                                          c->copies.p, boff, c->ew.p, c->scal + 4, c->B.cap, fault,
-                                                          c->tally_copies - 1, cp);
+      k_comb_write<double><<<(unsigned)nblk, 256, 0, s>>>(c->A.view<double>(), c->B.view<double>(), n, nblk, c->nx,
+                                                          c->ny, c->copies.p, boff, c->ew.p, c->scal + 4, c->B.cap,
+                                                          fault, c->tally_copies - 1, cp, nullptr);
     else
-      k_comb_write<float><<<(unsigned)nblk, 256, 0, s>>>(c->A.view<float>(), c->B.view<float>(), n, c->nx, c->ny,
-                                                         c->copies.p, boff, c->ew.p, c->scal + 4, c->B.cap, fault,
-                                                         c->tally_copies - 1, cp);
+      k_comb_write<float><<<(unsigned)nblk, 256, 0, s>>>(c->A.view<float>(), c->B.view<float>(), n, nblk, c->nx,
+                                                         c->ny, c->copies.p, boff, c->ew.p, c->scal + 4, c->B.cap,
+                                                         fault, c->tally_copies - 1, cp, nullptr);
     IMC_HIP(hipGetLastError());
   }
   if (disc_out) {
@@ -2573,6 +3099,137 @@ int imc2d_comb(imc2d_ctx* c, int64_t Nmax, double* ecen_out, double* disc_out, d
   finish_out(c, ho[1]);
   if (total_rad_out) *total_rad_out = hs[1];
   if (ecen_total_out) *ecen_total_out = hs[0];
+  return kOk;
+}
+
+// One whole step of the device-held power-law material as one
+// device-resident sequence (IMC/IMC2D.py:1591-1897 with
+// use_scalar_intensity_Tr and no comb-energy conservation); see imc_hip.h.
+int imc2d_step(imc2d_ctx* c, double theta, double dt, int64_t Ntarget, int64_t Nboundary, const double* face_E,
+               int64_t Nmax, const int32_t* reflect, int64_t max_events, int rz_linear, int use_graph, double* T_out,
+               double* Tr_out, double* u_out, double* dep_out, double* si_out, double* E_out, double* scalars_out,
+               int64_t* counts_out) {
+  IMC_REQUIRE(c && face_E && reflect && Ntarget >= 0 && Nboundary >= 0 && Nmax >= 0 && max_events >= 0 && dt > 0.0,
+              kErrArg, "imc2d_step: bad arguments");
+  IMC_REQUIRE(c->pl_ready, kErrState, "imc2d_step: call imc2d_set_material_powerlaw first");
+  IMC_REQUIRE(!c->adjoint && !c->fixed_counts, kErrState,
+              "imc2d_step: the adjoint replay and pinned emission counts need the staged entry points");
+  IMC_HIP(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  if (!c->sd) {
+    // the device block, the output destinations at +256
+    IMC_HIP(hipMalloc(&c->sd, 512));
+    IMC_HIP(hipMemset(c->sd, 0, 512));
+    int rc0 = c->step_io.reserve(kStepInBytes + sizeof(StepHostOut));
+    if (rc0) return rc0;
+    IMC_HIP(hipDeviceGetAttribute(&c->wall_khz, hipDeviceAttributeWallClockRate, c->device));
+    IMC_HIP(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+    IMC_HIP(hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming));
+    IMC_HIP(hipEventCreateWithFlags(&c->join_ev, hipEventDisableTiming));
+  }
+  StepLaunch L{};
+  L.theta = theta;
+  L.dt = dt;
+  L.Ntarget = Ntarget;
+  L.Nb = Nboundary;
+  L.Nmax = Nmax;
+  L.max_events = max_events;
+  L.rz_linear = (c->geometry == IMC_GEOM_RZ && rz_linear) ? 1 : 0;
+  int nface = 0;
+  for (int a = 0; a < 4; ++a) {
+    if (face_E[a] > 0.0 && Nboundary > 0) {
+      L.face_mask |= 1 << a;
+      ++nface;
+    }
+    L.refl |= (reflect[a] ? 1 : 0) << a;
+  }
+  const int64_t n0 = c->n;
+  const int64_t n_src = n0 + nface * Nboundary;
+  // emission counts are ceil(N E_c / E) <= N E_c / E + 1 per cell
+  const int64_t need = n_src + Ntarget + c->ncell + 16;
+  {
+    int grid;
+    int rc0 = transport_grid(c, INT64_MAX / 2, 256, &grid);
+    if (rc0) return rc0;
+    L.grid_tr = grid;
+  }
+  int slot = -1;
+  if (use_graph) {
+    for (int k = 0; k < 2; ++k) {
+      if (!c->graph_exec[k] || c->graph_lim[k] < need) continue;
+      StepLaunch Lk = L;
+      Lk.lim = c->graph_lim[k];
+      Lk.narrow = c->A.cap <= kNarrowMax;
+      if (step_signature(c, Lk) == c->graph_sig[k]) {
+        slot = k;
+        L = Lk;
+      }
+    }
+  }
+  int rc;
+  if (slot < 0) {
+    // a capture covers 1/8 more than this step needs, so that steady steps
+    // replay it; a direct launch covers this step
+    L.lim = use_graph ? need + need / 8 : need;
+    if ((rc = step_reserve(c, L))) return rc;
+    L.narrow = c->A.cap <= kNarrowMax;
+  }
+  // inputs of the step
+  StepHostIn* hin = (StepHostIn*)c->step_io.p;
+  StepHostOut* hout = (StepHostOut*)(c->step_io.p + kStepInBytes);
+  hin->n0 = n0;
+  hin->n_src = n_src;
+  hin->step = c->step;
+  for (int a = 0; a < 4; ++a) hin->face_E[a] = ((L.face_mask >> a) & 1) ? face_E[a] : 0.0;
+  double* outs[6] = {T_out, Tr_out, u_out, dep_out, si_out, E_out};
+  HostOut ho[6];
+  const int64_t bytes = (int64_t)sizeof(double) * c->ncell;
+  for (int a = 0; a < 6; ++a) {
+    ho[a] = HostOut{};
+    hin->out[a] = nullptr;
+    if (!outs[a]) continue;
+    if (void* d = host_device_alias(outs[a], (size_t)bytes)) {
+      hin->out[a] = (double*)d;
+    } else {
+      ho[a] = HostOut{outs[a], c->stage.slot(c->ncell, a)};
+      hin->out[a] = c->stage.dslot(c->ncell, a);
+    }
+  }
+  std::memset(hout, 0, sizeof(StepHostOut));
+  if (use_graph) {
+    if (slot < 0) {
+      slot = c->graph_next;
+      c->graph_next ^= 1;
+      if ((rc = step_capture(c, L, slot))) return rc;
+    }
+    IMC_HIP(hipGraphLaunch(c->graph_exec[slot], s));
+  } else {
+    if ((rc = step_enqueue(c, L))) return rc;
+  }
+  IMC_HIP(hipStreamSynchronize(s));
+  IMC_REQUIRE(!(hout->fault & 2), kErrState, "imc2d_step: emission counts passed the pool bound (non-finite energies?)");
+  IMC_REQUIRE(hout->fault == 0, kErrState, "imc2d_step: comb output exceeded its capacity bound");
+  IMC_REQUIRE(hout->n <= L.lim, kErrState, "imc2d_step: pool of %lld particles exceeded its bound %lld",
+              (long long)hout->n, (long long)L.lim);
+  std::swap(c->A, c->B);
+  c->n = hout->n_out;
+  c->emit_lo = c->emit_hi = 0;
+  c->last_transport_ms = (float)((double)(hout->t1 - hout->t0) / (double)(c->wall_khz > 0 ? c->wall_khz : 1));
+  for (const HostOut& h : ho) finish_out(c, h);
+  if (scalars_out) {
+    scalars_out[0] = hout->e_emit;
+    scalars_out[1] = hout->bl;
+    scalars_out[2] = hout->internal_total;
+    scalars_out[3] = hout->ecen_total;
+    scalars_out[4] = hout->rad_total;
+    scalars_out[5] = c->last_transport_ms;
+  }
+  if (counts_out) {
+    counts_out[0] = hout->n;
+    counts_out[1] = hout->n_out;
+    counts_out[2] = hout->emit_count;
+    for (int a = 0; a < 8; ++a) counts_out[3 + a] = (int64_t)hout->stats[a];
+  }
   return kOk;
 }
 

@@ -73,8 +73,10 @@ __global__ __launch_bounds__(1024) void k_chunk_scan(int32_t* __restrict__ hist,
 
 __global__ __launch_bounds__(kChunkBlock) void k_chunk_scatter(const uint8_t* __restrict__ key, int64_t nchunk,
                                                                const int32_t* __restrict__ off,
-                                                               int32_t* __restrict__ order) {
+                                                               int32_t* __restrict__ order,
+                                                               const int64_t* __restrict__ n_dev, int shift) {
   __shared__ int cur[kChunkKeys];
+  if (n_dev) nchunk = (*n_dev + (1ll << shift) - 1) >> shift;
   if (threadIdx.x < kChunkKeys) cur[threadIdx.x] = off[blockIdx.x * kChunkKeys + threadIdx.x];
   __syncthreads();
   for (int64_t c = blockIdx.x * (int64_t)kChunkBlock + threadIdx.x; c < nchunk;
@@ -88,7 +90,17 @@ int chunk_order(const uint8_t* key, int64_t nchunk, int grid, int32_t* hist, int
   if (nchunk == 0) return kOk;
   k_chunk_scan<<<1, 1024, 0, s>>>(hist, grid);
   IMC_HIP(hipGetLastError());
-  k_chunk_scatter<<<grid, kChunkBlock, 0, s>>>(key, nchunk, hist, order);
+  k_chunk_scatter<<<grid, kChunkBlock, 0, s>>>(key, nchunk, hist, order, nullptr, 0);
+  IMC_HIP(hipGetLastError());
+  return kOk;
+}
+
+int chunk_order_dev(const uint8_t* key, const int64_t* n_dev, int shift, int grid, int32_t* hist, int32_t* order,
+                    hipStream_t s) {
+  IMC_REQUIRE(grid >= 1 && grid <= kChunkGrid, kErrArg, "chunk_order_dev: grid %d", grid);
+  k_chunk_scan<<<1, 1024, 0, s>>>(hist, grid);
+  IMC_HIP(hipGetLastError());
+  k_chunk_scatter<<<grid, kChunkBlock, 0, s>>>(key, 0, hist, order, n_dev, shift);
   IMC_HIP(hipGetLastError());
   return kOk;
 }

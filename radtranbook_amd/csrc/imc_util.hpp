@@ -164,6 +164,10 @@ constexpr int kChunkGrid = 128;
 constexpr int kChunkBlock = 256;
 int chunk_grid(int64_t nchunk);
 int chunk_order(const uint8_t* key, int64_t nchunk, int grid, int32_t* hist, int32_t* order, hipStream_t s);
+// The same for ceil(n / 2^shift) chunks with n read from device memory when
+// the kernels run (the key kernel ran with that n as well).
+int chunk_order_dev(const uint8_t* key, const int64_t* n_dev, int shift, int grid, int32_t* hist, int32_t* order,
+                    hipStream_t s);
 
 // Block-local key histogram for the key kernels: zero it, count, publish.
 __device__ __forceinline__ void chunk_hist_zero(int* h) {
