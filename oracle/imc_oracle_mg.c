@@ -11,7 +11,10 @@
  * Differences from the reference, all deliberate:
  *  - Philox streams (philox_ref.h) replace np.random; every multinomial
  *    np.random.choice becomes an i.i.d. inverse-CDF draw per particle
- *    (the same distribution).
+ *    (the same distribution).  The material emission's cells come from the
+ *    order statistics of its N uniforms (exponential spacings), so they are
+ *    drawn in non-decreasing cell order; the multiset of draws has the same
+ *    distribution.
  *  - B_g is the Planck group integral computed here from its series; the
  *    reference calls the external planck_integrals module, absent here
  *    (its gray fallback would make b*_g = 1/G).
@@ -218,6 +221,45 @@ void om_prefix(const double* x, int64_t n, double* cdf) {
   }
 }
 
+/* Running sum in three levels, each left to right: 1024-element chunks,
+ * chunk offsets within groups of 1024 chunks, group offsets;
+ * S[i] = (group offset + chunk offset) + local prefix (the device's
+ * prefix_three_level). */
+void om_prefix3(const double* x, int64_t n, double* S) {
+  const int64_t nch = (n + 1023) / 1024, ngroup = (nch + 1023) / 1024;
+  double* tot = (double*)calloc((size_t)(nch > 0 ? nch : 1), sizeof(double));
+  double* choff = (double*)calloc((size_t)(nch > 0 ? nch : 1), sizeof(double));
+  double* gtot = (double*)calloc((size_t)(ngroup > 0 ? ngroup : 1), sizeof(double));
+  if (!tot || !choff || !gtot) abort();
+  for (int64_t b = 0; b < nch; ++b) {
+    const int64_t c0 = b * 1024, c1 = c0 + 1024 < n ? c0 + 1024 : n;
+    double s = 0.0;
+    for (int64_t i = c0; i < c1; ++i) {
+      s += x[i];
+      S[i] = s;
+    }
+    tot[b] = s;
+  }
+  for (int64_t q = 0; q < ngroup; ++q) {
+    const int64_t b0 = q * 1024, b1 = b0 + 1024 < nch ? b0 + 1024 : nch;
+    double o = 0.0;
+    for (int64_t b = b0; b < b1; ++b) {
+      choff[b] = o;
+      o += tot[b];
+    }
+    gtot[q] = o;
+  }
+  for (int64_t i = 0; i < n; ++i) {
+    const int64_t b = i >> 10, q = b >> 10;
+    double go = 0.0;
+    for (int64_t r = 0; r < q; ++r) go += gtot[r];
+    S[i] = (go + choff[b]) + S[i];
+  }
+  free(tot);
+  free(choff);
+  free(gtot);
+}
+
 /* first index with cdf[index] > u (np.random.choice), clamped to n-1 */
 static int64_t om_pick(const double* cdf, int64_t n, double u) {
   int64_t lo = 0, hi = n - 1;
@@ -287,15 +329,58 @@ static void om_pos(int geometry, const double* e1, const double* e2, int i, int 
 /* cell from the cell-total CDF, group from the cell's group energies.        */
 /* Slots: 0 cell, 1 group, 2 pos1, 3 pos2, 4/5 direction, 6 time.             */
 /* ------------------------------------------------------------------------- */
+/* ln x for x in (0, 1] from frexp and the atanh series to s^23, in plain
+ * FP64 arithmetic (built without FMA contraction): the device draws its
+ * exponential spacings with the same operations, so both get the same bits. */
+static double om_ln01(double x) {
+  int e;
+  double m = frexp(x, &e);
+  if (m < 0.70710678118654752440) {
+    m = m * 2.0;
+    e -= 1;
+  }
+  const double s = (m - 1.0) / (m + 1.0);
+  const double s2 = s * s;
+  double p = 1.0 / 23.0;
+  p = 1.0 / 21.0 + s2 * p;
+  p = 1.0 / 19.0 + s2 * p;
+  p = 1.0 / 17.0 + s2 * p;
+  p = 1.0 / 15.0 + s2 * p;
+  p = 1.0 / 13.0 + s2 * p;
+  p = 1.0 / 11.0 + s2 * p;
+  p = 1.0 / 9.0 + s2 * p;
+  p = 1.0 / 7.0 + s2 * p;
+  p = 1.0 / 5.0 + s2 * p;
+  p = 1.0 / 3.0 + s2 * p;
+  p = 1.0 + s2 * p;
+  return (double)e * 0.69314718055994530942 + 2.0 * s * p;
+}
+
+/* Material emission: N particles whose cells are the order statistics of N
+ * uniforms, u_k = S_k / S_N with S the running sum (om_prefix3 order) of the
+ * N + 1 spacings -ln(1 - u0_k), u0_k slot 0 of stream k; the group, position,
+ * direction and time from slots 1..6 of stream k. */
 void om_sample_emission(int geometry, int nx, int ny, int G, const double* e1, const double* e2, const double* E,
                         const double* cdf, int64_t N, double dt, uint64_t seed, uint32_t step, double* w, double* d1,
                         double* d2, double* t, double* p1, double* p2, int32_t* ci, int32_t* cj, int32_t* grp) {
   int64_t ncell = (int64_t)nx * ny;
   double Etot = cdf[ncell - 1];
+  if (N <= 0) return;
+  double* x = (double*)calloc((size_t)(N + 1), sizeof(double));
+  double* S = (double*)calloc((size_t)(N + 1), sizeof(double));
+  if (!x || !S) abort();
+  for (int64_t k = 0; k <= N; ++k) {
+    orc_stream s;
+    orc_stream_init(&s, seed, ORC_TAG_EMISSION, (uint64_t)k, step);
+    x[k] = -om_ln01(1.0 - orc_u01(&s, 0));
+  }
+  om_prefix3(x, N + 1, S);
   for (int64_t k = 0; k < N; ++k) {
     orc_stream s;
     orc_stream_init(&s, seed, ORC_TAG_EMISSION, (uint64_t)k, step);
-    int64_t c = om_pick(cdf, ncell, orc_u01(&s, 0) * Etot);
+    double u = S[k] / S[N];
+    if (!(u < 1.0)) u = nextafter(1.0, 0.0);
+    int64_t c = om_pick(cdf, ncell, u * Etot);
     int i = (int)(c / ny), j = (int)(c % ny);
     double tot = 0.0;
     for (int g = 0; g < G; ++g) tot += E[(int64_t)g * ncell + c];
@@ -318,6 +403,8 @@ void om_sample_emission(int geometry, int nx, int ny, int G, const double* e1, c
     cj[k] = j;
     grp[k] = gs;
   }
+  free(x);
+  free(S);
 }
 
 /* Equilibrium census (MG_IMC2D.py:992-1095): energy a Tr^4 V per cell (cdf),

@@ -171,6 +171,8 @@ def group_cdf(tot):
 
 
 def sample_emission(geometry, e1, e2, E, cdf, N, dt, seed, step):
+    """N emission particles (MG_IMC2D.py:1896-1943) in non-decreasing cell
+    order: their cells are the order statistics of N uniforms (om_sample_emission)."""
     G, n = E.shape; nx, ny = len(e1) - 1, len(e2) - 1
     p = PoolMG.empty(int(N))
     _lib().om_sample_emission(0 if geometry == "xy" else 1, nx, ny, G, _p(_f64(e1)), _p(_f64(e2)), _p(_f64(E)),

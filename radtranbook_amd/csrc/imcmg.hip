@@ -163,6 +163,38 @@ __global__ void k_prefix_finish(const double* __restrict__ local, const double* 
   if (i < n) cdf[i] = off[i >> 10] + local[i];
 }
 
+// Running sum for long arrays in three levels, each left to right: the
+// 1024-element chunks of k_prefix_chunks, chunk offsets within each group of
+// 1024 chunks (one lane per group), then the group offsets (one lane);
+// S[i] = (group offset + chunk offset) + local prefix.  The oracle's
+// om_prefix3 restates the same order.
+__global__ void k_prefix_groups(const double* __restrict__ chunk_tot, int64_t nch, double* __restrict__ choff,
+                                double* __restrict__ group_tot) {
+#pragma clang fp contract(off)
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t b0 = q * 1024;
+  if (b0 >= nch) return;
+  const int64_t b1 = b0 + 1024 < nch ? b0 + 1024 : nch;
+  double o = 0.0;
+  for (int64_t b = b0; b < b1; ++b) {
+    choff[b] = o;
+    o += chunk_tot[b];
+  }
+  group_tot[q] = o;
+}
+
+__global__ void k_prefix_finish3(const double* __restrict__ local, const double* __restrict__ choff,
+                                 const double* __restrict__ group_tot, int64_t ngroup, int64_t n,
+                                 double* __restrict__ S) {
+#pragma clang fp contract(off)
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t b = i >> 10, q = b >> 10;
+  double go = 0.0;  // the group offset, summed left to right (few groups)
+  for (int64_t r = 0; r < q; ++r) go += group_tot[r];
+  S[i] = (go + choff[b]) + local[i];
+}
+
 __device__ __forceinline__ int64_t mg_pick(const double* __restrict__ cdf, int64_t n, double u) {
   int64_t lo = 0, hi = n - 1;
   while (lo < hi) {
@@ -229,35 +261,111 @@ __device__ __forceinline__ void mg_store(MgView<R> P, int64_t o, double w, doubl
   P.g[o] = g;
 }
 
-// Material emission (MG_IMC2D.py:1896-1943).  Slots: 0 cell, 1 group, 2 pos1,
-// 3 pos2, 4/5 direction, 6 time.
-template <typename R>
-__global__ void k_mg_sample_emission(MgView<R> P, int64_t base, int64_t N, int geom, int nx, int ny, int G,
-                                     const double* __restrict__ e1, const double* __restrict__ e2,
-                                     const double* __restrict__ E, const double* __restrict__ cdf,
-                                     const int32_t* __restrict__ guide, double dt, Key key, uint32_t step) {
+// Group energies cell-major, Ecg[c G + g] = E[g ncell + c]: the group draw
+// of an emission particle reads its cell's G values from one or two cache
+// lines instead of G lines ncell apart.
+__global__ void k_mg_cell_major(const double* __restrict__ E, int64_t ncell, int G, double* __restrict__ Ecg) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= ncell * G) return;
+  const int64_t c = q / G;
+  const int g = (int)(q - c * G);
+  Ecg[q] = E[(int64_t)g * ncell + c];
+}
+
+// Cell and group of emission particle k (slot 0: the cell by the guided
+// inverse CDF, then the group within the cell by its group energies, Ecg
+// cell-major).
+__device__ __forceinline__ void mg_emission_bin(Pair b0, int64_t ncell, int G, const double* __restrict__ Ecg,
+                                                const double* __restrict__ cdf, const int32_t* __restrict__ guide,
+                                                int64_t* cell, int* grp) {
 #pragma clang fp contract(off)
-  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= N) return;
-  const int64_t ncell = (int64_t)nx * ny;
   const double Etot = cdf[ncell - 1];
-  Ctr ctr = make_ctr(kTagEmission, (uint64_t)k, step);
-  Pair b0 = block_d(ctr, key, 0), b1 = block_d(ctr, key, 1), b2 = block_d(ctr, key, 2), b3 = block_d(ctr, key, 3);
   int64_t c = mg_pick_guided(cdf, guide, b0.u0, b0.u0 * Etot);
-  int i = (int)(c / ny), j = (int)(c % ny);
+  const double* e = Ecg + c * G;
   double tot = 0.0;
-  for (int g = 0; g < G; ++g) tot += E[(int64_t)g * ncell + c];
+  for (int g = 0; g < G; ++g) tot += e[g];
   int gs = 0;
   if (tot > 0.0) {
     double xi = b0.u1 * tot, cum = 0.0;
     for (int g = 0; g < G; ++g) {
-      cum += E[(int64_t)g * ncell + c];
+      cum += e[g];
       if (xi <= cum) {
         gs = g;
         break;
       }
     }
   }
+  *cell = c;
+  *grp = gs;
+}
+
+// ln x for x in (0, 1] in plain FP64 arithmetic (no library call, no FMA
+// contraction), so that the oracle's identical restatement gives the same
+// bits: x = m 2^e with m in [1/sqrt 2, sqrt 2), ln m = 2 atanh(s) with s =
+// (m - 1) / (m + 1), |s| < 0.172, summed to s^23 (truncation below 1e-19).
+__device__ __forceinline__ double mg_ln01(double x) {
+#pragma clang fp contract(off)
+  int e;
+  double m = frexp(x, &e);
+  if (m < 0.70710678118654752440) {
+    m = m * 2.0;
+    e -= 1;
+  }
+  const double s = (m - 1.0) / (m + 1.0);
+  const double s2 = s * s;
+  double p = 1.0 / 23.0;
+  p = 1.0 / 21.0 + s2 * p;
+  p = 1.0 / 19.0 + s2 * p;
+  p = 1.0 / 17.0 + s2 * p;
+  p = 1.0 / 15.0 + s2 * p;
+  p = 1.0 / 13.0 + s2 * p;
+  p = 1.0 / 11.0 + s2 * p;
+  p = 1.0 / 9.0 + s2 * p;
+  p = 1.0 / 7.0 + s2 * p;
+  p = 1.0 / 5.0 + s2 * p;
+  p = 1.0 / 3.0 + s2 * p;
+  p = 1.0 + s2 * p;
+  return (double)e * 0.69314718055994530942 + 2.0 * s * p;
+}
+
+// The N emission cells are the order statistics of N i.i.d. uniforms, taken
+// as u_k = S_k / S_N from the running sums S of N + 1 exponential spacings
+// x_k = -ln(1 - u0_k) (slot 0 of particle k's stream; k = N is the closing
+// spacing): cells then come out in non-decreasing order, which keeps a
+// wave's histories, and so its tally adds, in few cells, and the multiset of
+// (cell, group) draws has the reference's distribution (N i.i.d. draws,
+// MG_IMC2D.py:1896-1943).
+__global__ void k_mg_emission_spacings(int64_t n1, Key key, uint32_t step, double* __restrict__ x) {
+#pragma clang fp contract(off)
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n1) return;
+  const Pair b0 = block_d(make_ctr(kTagEmission, (uint64_t)k, step), key, 0);
+  x[k] = -mg_ln01(1.0 - b0.u0);
+}
+
+// Material emission (MG_IMC2D.py:1896-1943).  Slots: 0 cell (the sorted
+// uniform u_k = S_k / S_N of k_mg_emission_spacings) and group, 1 position,
+// 2 direction, 3 time.
+template <typename R>
+__global__ void k_mg_sample_emission(MgView<R> P, int64_t base, int64_t N, int geom, int nx, int ny, int G,
+                                     const double* __restrict__ e1, const double* __restrict__ e2,
+                                     const double* __restrict__ Ecg, const double* __restrict__ cdf,
+                                     const int32_t* __restrict__ guide, double dt, Key key, uint32_t step,
+                                     const double* __restrict__ S) {
+#pragma clang fp contract(off)
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= N) return;
+  const int64_t ncell = (int64_t)nx * ny;
+  Ctr ctr = make_ctr(kTagEmission, (uint64_t)k, step);
+  Pair b0 = block_d(ctr, key, 0), b1 = block_d(ctr, key, 1), b2 = block_d(ctr, key, 2), b3 = block_d(ctr, key, 3);
+  double u = S[k] / S[N];
+  if (!(u < 1.0)) u = 0x1.fffffffffffffp-1;  // the largest double below 1
+  b0.u0 = u;
+  int64_t c;
+  int gs;
+  mg_emission_bin(b0, ncell, G, Ecg, cdf, guide, &c, &gs);
+  const double Etot = cdf[ncell - 1];
+  int i = (int)(c / ny), j = (int)(c % ny);
   double p1, p2, d1, d2;
   mg_pos(geom, e1, e2, i, j, b1.u0, b1.u1, p1, p2);
   mg_isotropic(geom, b2, d1, d2);
@@ -1193,6 +1301,8 @@ struct MgCtx {
   DevBuf<int64_t> counts, scan_scr;
   DevBuf<int32_t> copies, guide;
   DevBuf<unsigned long long> gcount;
+  DevBuf<double> Ecg;                       // group energies cell-major
+  DevBuf<double> spacing, spacing_sum;      // emission cells: exponential spacings, running sum
   unsigned long long* stats = nullptr;
   unsigned long long* next = nullptr;
   MgPool A, B;
@@ -1214,6 +1324,24 @@ static int prefix_canonical(MgCtx* c, const double* x, int64_t n, double* out) {
   k_prefix_offsets<<<1, 1, 0, s>>>(c->chunk.p, nch, c->choff.p);
   IMC_HIP(hipGetLastError());
   k_prefix_finish<<<launch_blocks(n, 256), 256, 0, s>>>(c->local.p, c->choff.p, n, out);
+  IMC_HIP(hipGetLastError());
+  return kOk;
+}
+
+// Three-level running sum (k_prefix_groups); scratch: c->local (n), c->chunk
+// and c->choff (chunks), c->tmp (groups).
+static int prefix_three_level(MgCtx* c, const double* x, int64_t n, double* out) {
+  const int64_t nch = (n + 1023) / 1024, ngroup = (nch + 1023) / 1024;
+  int rc;
+  if ((rc = c->local.reserve(n)) || (rc = c->chunk.reserve(nch)) || (rc = c->choff.reserve(nch)) ||
+      (rc = c->tmp.reserve(ngroup)))
+    return rc;
+  hipStream_t s = c->stream;
+  k_prefix_chunks<<<(unsigned)nch, 256, 0, s>>>(x, n, c->local.p, c->chunk.p);
+  IMC_HIP(hipGetLastError());
+  k_prefix_groups<<<launch_blocks(ngroup, 64), 64, 0, s>>>(c->chunk.p, nch, c->choff.p, c->tmp.p);
+  IMC_HIP(hipGetLastError());
+  k_prefix_finish3<<<launch_blocks(n, 256), 256, 0, s>>>(c->local.p, c->choff.p, c->tmp.p, ngroup, n, out);
   IMC_HIP(hipGetLastError());
   return kOk;
 }
@@ -1388,6 +1516,9 @@ void mgimc_destroy(mgimc_ctx* c) {
   c->copies.release();
   c->guide.release();
   c->gcount.release();
+  c->Ecg.release();
+  c->spacing.release();
+  c->spacing_sum.release();
   c->stage.release();
   c->A.release();
   c->B.release();
@@ -1695,12 +1826,24 @@ int mgimc_sample_emission(mgimc_ctx* c, int64_t N, double dt, double* Etot_out) 
   IMC_HIP(hipGetLastError());
   int g = launch_blocks(N, 256);
   Key key = make_key(c->seed);
+  // sorted uniforms for the cells: N + 1 exponential spacings and their
+  // running sum in three levels (the oracle's om_prefix3 order)
+  const int64_t n1 = N + 1;
+  if ((rc = c->Ecg.reserve(c->nb)) || (rc = c->spacing.reserve(n1)) || (rc = c->spacing_sum.reserve(n1)))
+    return rc;
+  k_mg_cell_major<<<launch_blocks(c->nb, 256), 256, 0, s>>>(c->E.p, c->ncell, c->G, c->Ecg.p);
+  IMC_HIP(hipGetLastError());
+  k_mg_emission_spacings<<<launch_blocks(n1, 256), 256, 0, s>>>(n1, key, c->step, c->spacing.p);
+  IMC_HIP(hipGetLastError());
+  if ((rc = prefix_three_level(c, c->spacing.p, n1, c->spacing_sum.p))) return rc;
   if (c->precision == IMC_FP64)
     k_mg_sample_emission<double><<<g, 256, 0, s>>>(c->A.view<double>(), c->n, N, c->geometry, c->nx, c->ny, c->G,
-                                                   c->e1.p, c->e2.p, c->E.p, c->cdf.p, c->guide.p, dt, key, c->step);
+                                                   c->e1.p, c->e2.p, c->Ecg.p, c->cdf.p, c->guide.p, dt, key, c->step,
+                                                   c->spacing_sum.p);
   else
     k_mg_sample_emission<float><<<g, 256, 0, s>>>(c->A.view<float>(), c->n, N, c->geometry, c->nx, c->ny, c->G,
-                                                  c->e1.p, c->e2.p, c->E.p, c->cdf.p, c->guide.p, dt, key, c->step);
+                                                  c->e1.p, c->e2.p, c->Ecg.p, c->cdf.p, c->guide.p, dt, key, c->step,
+                                                  c->spacing_sum.p);
   IMC_HIP(hipGetLastError());
   c->n += N;
   return kOk;

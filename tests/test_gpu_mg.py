@@ -128,6 +128,9 @@ def test_mg_material_and_sourcing(L, M, geometry):
     for part, nm, wr, mis in ((ref, "ic", 0.0, 0), (b1, "left", 0.0, 0), (b2, "bottom", 0.0, 0), (sp, "src", 0.0, 0),
                               (em, "emission", 1e-13, 2)):
         sl = [a[k:k + len(part)] for a in got]
+        if nm == "emission":
+            # emission cells are drawn in non-decreasing order (sorted uniforms)
+            assert np.all(np.diff(sl[6].astype(np.int64) * m + sl[7]) >= 0), "emission cells not in order"
         _cmp_pool(sl, part, nm, w_rtol=wr, allow_mismatch=mis)
         k += len(part)
     assert k == len(got[0])

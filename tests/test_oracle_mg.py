@@ -85,6 +85,8 @@ def test_emission_sampler_distribution(M):
     p = M.sample_emission("xy", e1, e2, E, cdf, N, 0.02, 7, 3)
     assert np.allclose(p.w, cdf[-1] / N, rtol=0)
     cell = p.ci * 3 + p.cj
+    # cells from the order statistics of the uniforms: drawn in cell order
+    assert np.all(np.diff(cell) >= 0)
     cnt = np.bincount(cell, minlength=12)
     exp = N * tot / tot.sum()
     assert stats.chi2.sf(np.sum((cnt - exp) ** 2 / exp), 11) > 1e-3
@@ -98,6 +100,18 @@ def test_emission_sampler_distribution(M):
     assert np.all((p.p1 >= e1[p.ci]) & (p.p1 < e1[p.ci + 1]) & (p.p2 >= e2[p.cj]) & (p.p2 < e2[p.cj + 1]))
     assert np.mean(p.d1 ** 2 + p.d2 ** 2) == pytest.approx(2.0 / 3.0, abs=0.01)
     assert np.all((p.t >= 0) & (p.t < 0.02))
+    # the counts are multinomial (i.i.d. draws, not stratified): over seeds,
+    # a cell's count has variance N p (1 - p)
+    n_small, nseed = 3000, 120
+    pc = tot[c0] / tot.sum()
+    k = []
+    for seed in range(100, 100 + nseed):
+        q = M.sample_emission("xy", e1, e2, E, cdf, n_small, 0.02, seed, 3)
+        k.append(np.sum(q.ci * 3 + q.cj == c0))
+    k = np.array(k)
+    stat = (nseed - 1) * k.var(ddof=1) / (n_small * pc * (1 - pc))
+    assert 1e-3 < stats.chi2.cdf(stat, nseed - 1) < 1 - 1e-3
+    assert k.mean() == pytest.approx(n_small * pc, abs=5 * np.sqrt(n_small * pc * (1 - pc) / nseed))
 
 
 def test_source_sampler(M):
