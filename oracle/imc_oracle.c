@@ -765,6 +765,114 @@ void orc_transport_2d_adjoint(int64_t n, const double* w, const double* d1, cons
 }
 
 /* ------------------------------------------------------------------------- */
+/* Forward-mode likelihood-ratio gradient of the gray 2-D transport tallies   */
+/* (reference for the replay adjoint).  For L = sum_c lam_c E_abs,c and a     */
+/* direction (vA, vS) of per-cell perturbations of (sigma_a, sigma_s), one    */
+/* forward pass per history accumulates                                       */
+/*   X     = sum_j lam_{c_j} dep_j            (the history's contribution)    */
+/*   dX/de = sum_j lam_{c_j} d dep_j / de     (paths held fixed: dep_j =      */
+/*           w_{j-1} - w_j, w_j = w_{j-1} e^{-sigma_a s_j}, so d w_j / de =   */
+/*           e^{-sigma_a s_j} d w_{j-1} / de - w_j s_j vA_{c_j})              */
+/*   S     = sum_j vS_{c_j} ([scatter_j] / sigma_s - s_j), the derivative of  */
+/*           the log density of the sampled path (segments whose scatter      */
+/*           rate is not capped and sigma_s > 1e-12)                          */
+/* with an emission-born history (k in [emit_lo, emit_hi)) starting from      */
+/* d w_0 / de = w_0 vA_b / sigma_a,b (its weight is proportional to the birth */
+/* cell's sigma_a).  E[sum_h dX/de + X S] is the directional derivative; the  */
+/* two sums are returned separately (out[0] pathwise, out[1] score, out[2]    */
+/* sum X).  The same histories and draw slots as orc_transport_2d.            */
+/* ------------------------------------------------------------------------- */
+static void orc_lr_history(int64_t k, double w0, double u1, double u2, const double* times, double x, double y,
+                           int i, int j, const double* e1, const double* e2, int nx, int ny, const double* sigma_a,
+                           const double* sigma_s, double dt, const int32_t* reflect, int64_t max_events,
+                           int32_t geometry, double weight_floor, uint64_t seed, uint32_t step, const double* lam,
+                           const double* vA, const double* vS, double dw0, double* X_out, double* dX_out,
+                           double* S_out) {
+  orc_stream st;
+  orc_stream_init(&st, seed, ORC_TAG_TRANSPORT, (uint64_t)k, step);
+  double dcen = (dt - times[k]) * ORC_C;
+  double w = w0, dw = dw0, X = 0.0, dX = 0.0, S = 0.0;
+  int64_t events = 0;
+  while (dcen > 0.0) {
+    events += 1;
+    if (events > max_events) break;
+    if (i < 0 || i >= nx || j < 0 || j >= ny) break;
+    int64_t c = (int64_t)i * ny + j;
+    double sa = sigma_a[c], ss = sigma_s[c];
+    orc_move_out o;
+    uint32_t ev = (uint32_t)(events - 1);
+    if (geometry == 0)
+      orc_move_xy(&st, ev, w, u1, u2, x, y, i, j, e1, e2, sa, ss, dcen, &o);
+    else
+      orc_move_rz(&st, ev, w, u1, u2, x, y, i, j, e1, e2, sa, ss, dcen, &o);
+    dcen = o.dcen;
+    /* weight derivative across the segment and the deposit's derivative */
+    double dw_next = dw * exp(-sa * o.s) - o.w * o.s * vA[c];
+    X += lam[c] * o.dep_w;
+    dX += lam[c] * (dw - dw_next);
+    double xl = e1[i], xr = e1[i + 1], yl = e2[j], yr = e2[j + 1];
+    double cl = (xr - xl) < (yr - yl) ? (xr - xl) : (yr - yl);
+    if (ss > 1e-12 && !(ss * cl > 10000.0)) S += vS[c] * ((o.evt == E_SCATTER ? 1.0 / ss : 0.0) - o.s);
+    w = o.w;
+    dw = dw_next;
+    if (o.w <= weight_floor) break;
+    u1 = o.d1; u2 = o.d2; x = o.p1; y = o.p2;
+    if (o.crossed != X_NONE) {
+      if (o.crossed == X_IP) i += 1;
+      else if (o.crossed == X_IM) i -= 1;
+      else if (o.crossed == X_JP) j += 1;
+      else j -= 1;
+      if (i < 0) {
+        if (reflect[0]) { u1 = -u1; i = 0; } else { w = 0.0; dcen = 0.0; }
+      } else if (i >= nx) {
+        if (reflect[1]) { u1 = -u1; i = nx - 1; } else { w = 0.0; dcen = 0.0; }
+      } else if (j < 0) {
+        if (reflect[2]) { u2 = -u2; j = 0; } else { w = 0.0; dcen = 0.0; }
+      } else if (j >= ny) {
+        if (reflect[3]) { u2 = -u2; j = ny - 1; } else { w = 0.0; dcen = 0.0; }
+      }
+    }
+    if (w <= 1e-300) {
+      w = 0.0;
+      dcen = 0.0;
+    }
+  }
+  *X_out = X;
+  *dX_out = dX;
+  *S_out = S;
+}
+
+void orc_transport_2d_lr(int64_t n, const double* w, const double* d1, const double* d2, const double* times,
+                         const double* p1, const double* p2, const int32_t* ci, const int32_t* cj, const double* e1,
+                         const double* e2, int nx, int ny, const double* sigma_a, const double* sigma_s, double dt,
+                         const int32_t* reflect, int64_t max_events, int32_t geometry, double weight_floor,
+                         uint64_t seed, uint32_t step, const double* lam, const double* vA, const double* vS,
+                         int64_t emit_lo, int64_t emit_hi, double* out, int32_t nthreads) {
+  int nt = 1;
+#ifdef _OPENMP
+  nt = nthreads > 0 ? nthreads : omp_get_max_threads();
+#endif
+  double path = 0.0, score = 0.0, xsum = 0.0;
+#pragma omp parallel for schedule(dynamic, 256) num_threads(nt) reduction(+ : path, score, xsum)
+  for (int64_t k = 0; k < n; ++k) {
+    double dw0 = 0.0;
+    if (k >= emit_lo && k < emit_hi && ci[k] >= 0 && ci[k] < nx && cj[k] >= 0 && cj[k] < ny) {
+      int64_t b = (int64_t)ci[k] * ny + cj[k];
+      if (sigma_a[b] > 0.0) dw0 = w[k] * vA[b] / sigma_a[b];
+    }
+    double X, dX, S;
+    orc_lr_history(k, w[k], d1[k], d2[k], times, p1[k], p2[k], ci[k], cj[k], e1, e2, nx, ny, sigma_a, sigma_s, dt,
+                   reflect, max_events, geometry, weight_floor, seed, step, lam, vA, vS, dw0, &X, &dX, &S);
+    path += dX;
+    score += X * S;
+    xsum += X;
+  }
+  out[0] = path;
+  out[1] = score;
+  out[2] = xsum;
+}
+
+/* ------------------------------------------------------------------------- */
 /* a18: per-cell stochastic comb, IMC/IMC2D.py:1448-1511, IMC/IMC1D.py:770-828 */
 /* cell[k] is the flat cell id (any bijection; dead particles have w <= 0).   */
 /* stream_base offsets the Philox index (problem << 32 in batched runs).      */

@@ -424,6 +424,33 @@ def transport_2d_adjoint(pool, e1, e2, geometry, sigma_a, sigma_s, dt, reflect, 
     return gA.reshape(nx, ny), gS.reshape(nx, ny), Lam
 
 
+def transport_2d_lr(pool, e1, e2, geometry, sigma_a, sigma_s, dt, reflect, max_events, weight_floor, seed, step,
+                    lam, vA, vS, emit_lo=0, emit_hi=0, nthreads=0):
+    """Forward-mode likelihood-ratio estimate of the directional derivative of
+    L = sum_c lam_c E_abs,c along (vA, vS), perturbations of (sigma_a,
+    sigma_s) per cell, over one transport of ``pool`` (unchanged): returns
+    (pathwise part, score-function part, sum of the histories' L)."""
+    L = lib()
+    if not hasattr(L, "_lr_ready"):
+        L.orc_transport_2d_lr.argtypes = [C.c_int64, _dp, _dp, _dp, _dp, _dp, _dp, _i32p, _i32p, _dp, _dp, C.c_int,
+                                          C.c_int, _dp, _dp, C.c_double, _i32p, C.c_int64, C.c_int32, C.c_double,
+                                          C.c_uint64, C.c_uint32, _dp, _dp, _dp, C.c_int64, C.c_int64, _dp,
+                                          C.c_int32]
+        L._lr_ready = True
+    e1 = _f64(e1); e2 = _f64(e2); nx, ny = e1.size - 1, e2.size - 1
+    gcode = 0 if geometry in ("xy", 0) else 1
+    ci = np.ascontiguousarray(pool.ci, dtype=np.int32); cj = np.ascontiguousarray(pool.cj, dtype=np.int32)
+    refl = np.array([1 if r else 0 for r in reflect], dtype=np.int32)
+    out = np.zeros(3)
+    L.orc_transport_2d_lr(len(pool), _p(_f64(pool.w)), _p(_f64(pool.d1)), _p(_f64(pool.d2)), _p(_f64(pool.t)),
+                          _p(_f64(pool.p1)), _p(_f64(pool.p2)), _p(ci, _i32p), _p(cj, _i32p), _p(e1), _p(e2), nx, ny,
+                          _p(_f64(sigma_a).ravel()), _p(_f64(sigma_s).ravel()), float(dt), _p(refl, _i32p),
+                          int(max_events), gcode, float(weight_floor), int(seed), int(step), _p(_f64(lam).ravel()),
+                          _p(_f64(vA).ravel()), _p(_f64(vS).ravel()), int(emit_lo), int(emit_hi), _p(out),
+                          int(nthreads))
+    return float(out[0]), float(out[1]), float(out[2])
+
+
 # ----------------------------------------------------------------------------
 # gray 1-D (slab / sphere), IMC/IMC1D.py:859-1124
 # ----------------------------------------------------------------------------

@@ -127,3 +127,73 @@ def test_adjoint_scatter_score_matches_fd_in_mean(oracle):
     assert z < 4.0, (z, lr.mean(), fd.mean())
     # more scattering keeps photons longer in the absorber: clearly positive
     assert lr.mean() > 3 * lr.std(ddof=1) / np.sqrt(len(lr))
+
+
+# ---------------------------------------------------------------------------
+# forward-mode likelihood-ratio reference (oracle.transport_2d_lr)
+# ---------------------------------------------------------------------------
+def test_lr_zero_d_closed_form(oracle):
+    """One reflecting cell: E_abs = sum w (1 - e^{-sigma_a L}), L = c (dt - t),
+    so the pathwise sigma_a derivative is sum w L e^{-sigma_a L} exactly; the
+    tally does not depend on the path, so the score part has zero mean."""
+    e1 = np.array([0.0, 0.4]); e2 = np.array([0.0, 0.3])
+    p = _pool(oracle, 2000, e1, e2, np.random.default_rng(1))
+    sa = np.array([[7.0]]); ss = np.array([[25.0]]); dt = 0.01
+    one = np.ones((1, 1))
+    path, score, X = oracle.transport_2d_lr(p, e1, e2, "xy", sa, ss, dt, (True,) * 4, 10**7, 0.0, 3, 1, one, one,
+                                            0 * one)
+    L = C * (dt - p.t)
+    assert path == pytest.approx(np.sum(p.w * L * np.exp(-sa[0, 0] * L)), rel=1e-10)
+    assert X == pytest.approx(np.sum(p.w * (1 - np.exp(-sa[0, 0] * L))), rel=1e-12)
+    assert score == 0.0
+    vals = [oracle.transport_2d_lr(p, e1, e2, "xy", sa, ss, dt, (True,) * 4, 10**7, 0.0, 100 + s, 1, one, 0 * one,
+                                   one)[1] for s in range(30)]
+    vals = np.array(vals)
+    assert abs(vals.mean()) < 4 * vals.std(ddof=1) / np.sqrt(len(vals))
+
+
+def test_lr_pathwise_equals_replay_adjoint(oracle):
+    """The pathwise (sigma_a) and emission-weight parts of the forward-mode
+    estimate and of the replay adjoint are the same estimator accumulated in
+    opposite directions: on the same histories they agree to rounding for
+    any direction vA."""
+    e1, e2 = _mesh()
+    rng = np.random.default_rng(6)
+    p = _pool(oracle, 3000, e1, e2, rng)
+    sa = rng.uniform(2, 20, (4, 3)); ss = rng.uniform(1, 40, (4, 3))
+    lam = rng.normal(size=(4, 3))
+    dt, refl, lo, hi = 0.01, (False, True, False, True), 700, 2400
+    gA, gS, _ = oracle.transport_2d_adjoint(p, e1, e2, "xy", sa, ss, dt, refl, 10**7, 0.0, 5, 1, lam, lo, hi)
+    for trial in range(3):
+        vA = rng.normal(size=(4, 3))
+        path, _, _ = oracle.transport_2d_lr(p, e1, e2, "xy", sa, ss, dt, refl, 10**7, 0.0, 5, 1, lam, vA,
+                                            np.zeros((4, 3)), lo, hi)
+        assert path == pytest.approx(np.sum(gA * vA), rel=1e-10, abs=1e-12 * np.sum(np.abs(gA * vA)))
+
+
+def test_lr_score_matches_fd_and_adjoint_in_mean(oracle):
+    """The score-function (sigma_s) part: its mean over seeds agrees with
+    central differences of a global sigma_s scale (common random numbers,
+    3%) and with the replay adjoint's causal score estimate, a different
+    estimator of the same derivative."""
+    e1, e2 = _mesh()
+    rng = np.random.default_rng(4)
+    sa = rng.uniform(1, 5, (4, 3)); ss = rng.uniform(5, 30, (4, 3))
+    lam = np.ones((4, 3))
+    dt, refl, h = 0.01, (False,) * 4, 0.03
+    lr, adj, fd = [], [], []
+    for seed in range(60):
+        p = _pool(oracle, 800, e1, e2, np.random.default_rng(100 + seed))
+        _, score, _ = oracle.transport_2d_lr(p, e1, e2, "xy", sa, ss, dt, refl, 10**7, 0.0, seed, 1, lam,
+                                             np.zeros((4, 3)), ss)
+        _, gS, _ = oracle.transport_2d_adjoint(p, e1, e2, "xy", sa, ss, dt, refl, 10**7, 0.0, seed, 1, lam)
+        lr.append(score)
+        adj.append(np.sum(gS * ss))
+        fd.append((_lam_tally(oracle, p, e1, e2, sa, ss * (1 + h), dt, refl, lam, seed)
+                   - _lam_tally(oracle, p, e1, e2, sa, ss * (1 - h), dt, refl, lam, seed)) / (2 * h))
+    lr, adj, fd = np.array(lr), np.array(adj), np.array(fd)
+    for other, name in ((fd, "fd"), (adj, "adjoint")):
+        d = lr - other
+        z = abs(d.mean()) / (d.std(ddof=1) / np.sqrt(len(d)))
+        assert z < 4.0, (name, z, lr.mean(), other.mean())
+    assert lr.mean() > 3 * lr.std(ddof=1) / np.sqrt(len(lr))
