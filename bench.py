@@ -103,8 +103,50 @@ def roofline(kernel, precision, seg_per_launch, hist_per_launch, avg_launch_s, m
     }
 
 
-def run_gpu(prob, precision, steps, warmup, fom_seeds):
+def _sample_steps(steps):
+    return sorted({max(steps // 4, 0), steps // 2, (3 * steps) // 4, steps - 1})
+
+
+def tune_2d(prob, precision, steps):
+    """Tuning pass: the workload's own run (timed seed) with the launch-shape
+    candidates timed on the staged pool of four of its steps."""
     from radtranbook_amd import IMC2D
+    from radtranbook_amd import autotune as AT
+
+    IMC2D.VERBOSE = False
+    t = AT.gray2d_tuner(prob.dt, prob.reflect, prob.max_events)
+    e1, e2 = prob.edges1, prob.edges2
+    T0 = np.full(prob.shape, prob.T0)
+    src = np.zeros(prob.shape)
+    samples = _sample_steps(steps)
+    cur = [0]
+
+    def hook(eng):
+        if cur[0] in samples:
+            t.sample(eng)
+
+    st = IMC2D.init_simulation(prob.Ntarget, T0, T0, e1, e2, prob.eos.eos, prob.eos.inv_eos, geometry=prob.geometry,
+                               precision=precision, seed=1)
+    IMC2D._TRANSPORT_HOOK = hook
+    try:
+        for s in range(steps):
+            cur[0] = s
+            st, _ = IMC2D.step(st, prob.Ntarget, prob.Nboundary, 0, prob.Nmax, prob.T_boundary, prob.dt, e1, e2,
+                               prob.opacity, prob.eos.inv_eos, prob.eos.cv_func, src, reflect=prob.reflect,
+                               geometry=prob.geometry, max_events_per_particle=prob.max_events)
+    finally:
+        IMC2D._TRANSPORT_HOOK = None
+        st._engine.close()
+    rep = t.report()
+    AT.store_shape("gray2d", prob.name, precision, rep)
+    log(f"[{precision}] autotune: {rep['shape']} {rep['tuned_ms']:.2f} ms over {rep['sample_steps']} sample steps "
+        f"(default {rep['default_ms']:.2f} ms, {rep['candidates']} candidates x {rep['repeats']})")
+    return t.best(), rep
+
+
+def run_gpu(prob, precision, steps, warmup, fom_seeds, launch=None):
+    from radtranbook_amd import IMC2D
+    from radtranbook_amd import autotune as AT
 
     IMC2D.VERBOSE = False
     e1, e2 = prob.edges1, prob.edges2
@@ -120,6 +162,8 @@ def run_gpu(prob, precision, steps, warmup, fom_seeds):
         # (same buffers, new Philox seed), so no allocation is timed
         st = IMC2D.init_simulation(prob.Ntarget, T0, T0, e1, e2, eos.eos, eos.inv_eos, geometry=prob.geometry,
                                    precision=precision, seed=seed, engine=engine[0])
+        if engine[0] is None and launch is not None:
+            AT.apply(st._engine, launch)
         engine[0] = st._engine
         infos = []
         t0 = time.perf_counter()
@@ -237,13 +281,26 @@ def run_gpu_batch1d(probs, precision, steps, warmup, tune):
     meshes, T0, eos, inv, cvf, sf, N, Tb = _batch_inputs(probs)
     Nt, Nb = N // 2, N - N // 2
     dt = probs[0].dt
-    b = BatchIMC1D(meshes, T0, T0, eos, inv, N, precision=precision, seed=1000)
+    from radtranbook_amd import autotune as AT
+
+    b = BatchIMC1D(meshes, T0, T0, eos, inv, N, precision=precision, seed=1)
     tuned = None
+    if tune:
+        # tuning pass: the timed run itself (seed 1), candidates timed on the
+        # staged step at four of its steps
+        tuner = AT.gray1d_tuner((False, False))
+        samples = _sample_steps(steps)
+        for s in range(steps):
+            b.step(Nt, Nb, N, Tb, dt, sf, cvf, tune=(tuner.sample if s in samples else False))
+        tuned = tuner.report()
+        AT.apply(b.eng, tuner.best())
+        AT.store_shape("gray1d", f"batch_slabs_{len(probs)}", precision, tuned)
+        log(f"[{precision}] autotune: {tuned['shape']} {tuned['tuned_ms']:.2f} ms over {tuned['sample_steps']} "
+            f"sample steps (default {tuned['default_ms']:.2f} ms, {tuned['candidates']} candidates x "
+            f"{tuned['repeats']})")
+    b.reset(T0, T0, eos, inv, N, seed=1000)
     for s in range(warmup):
-        out = b.step(Nt, Nb, N, Tb, dt, sf, cvf, tune=(tune and s == warmup - 1))
-        if out["tuned"] is not None:
-            tuned = out["tuned"][0]
-            log(f"[{precision}] autotuned transport launch (grid, batch, copies) = {tuned}")
+        b.step(Nt, Nb, N, Tb, dt, sf, cvf)
     b.reset(T0, T0, eos, inv, N, seed=1)
     infos = []
     t0 = time.perf_counter()
@@ -301,7 +358,7 @@ def main_1d(args, workload):
         desc = f"gray IMC 1D slab: 100 cells on [0,0.2] cm, 1e5 particles/step, dt=0.01 ns x {steps} steps"
         metric = "IMC segments/s, 1D slab 100 cells 1e5 particles/step"
     precs = ["fp32", "fp64"] if args.precision == "both" else [args.precision]
-    res = {p: run_gpu_batch1d(probs, p, steps, warmup, args.tune and warmup > 0) for p in precs}
+    res = {p: run_gpu_batch1d(probs, p, steps, warmup, args.tune and workload == "batch1d") for p in precs}
     head = res[precs[0]]
     value = head["segments"] / head["wall"]
     avg_launch_s = head["kernel_ms"].mean() / 1e3
@@ -326,7 +383,7 @@ def main_1d(args, workload):
                              float(head["seg_per_launch"].mean()), head["histories"] / steps, avg_launch_s,
                              BYTES_PER_SEGMENT_1D[precs[0]], "k1_transport_batch1d_256slabs.md",
                              workload == "batch1d" and args.problems == 256 and not args.tune),
-        "autotuned_launch": head["tuned"],
+        "autotune": {p: res[p]["tuned"] for p in precs} if args.tune and workload == "batch1d" else None,
     }
     for p in precs:
         r = res[p]
@@ -357,24 +414,36 @@ def main_1d(args, workload):
 BYTES_PER_SEGMENT_MG = {"fp32": 2 * 44 + 8 + 16, "fp64": 2 * 64 + 16 + 16}
 
 
-def run_gpu_mg(prob, precision, steps, warmup, n_particles):
+def run_gpu_mg(prob, precision, steps, warmup, n_particles, tune=False):
     from radtranbook_amd import MG_IMC2D
+    from radtranbook_amd import autotune as AT
 
     MG_IMC2D.VERBOSE = False
     T0 = np.full(prob.shape, prob.T0)
     zeros = np.zeros(prob.shape)
     sf = prob.sigma_funcs()
     engine = [None]
+    shape = [None]
+    tuner = AT.mg2d_tuner(prob.dt, (False,) * 4, 10**6) if tune else None
+    samples = _sample_steps(steps)
+    cur = [-1]
 
-    def one_run(seed, nsteps, timed):
+    def hook(eng):
+        if cur[0] in samples:
+            tuner.sample(eng)
+
+    def one_run(seed, nsteps, timed, tuning=False):
         st = MG_IMC2D.init_simulation(n_particles // 2, T0, T0, prob.edges1, prob.edges2, prob.energy_edges, prob.eos,
                                       prob.inv_eos, geometry=prob.geometry, precision=precision, seed=seed,
                                       engine=engine[0])
+        if shape[0] is not None:
+            AT.apply(st._engine, shape[0])
         engine[0] = st._engine
         infos = []
         t0 = time.perf_counter()
         for s in range(nsteps):
             ts = time.perf_counter()
+            cur[0] = s if tuning else -1
             st, info = MG_IMC2D.step(st, 0, 0, 0, n_particles, (0.0, 0.0, 0.0, 0.0), prob.dt, prob.edges1, prob.edges2,
                                      prob.energy_edges, sf, prob.inv_eos, prob.cv_func, zeros,
                                      boundary_source_func=prob.boundary_source_func, Ntotal=n_particles,
@@ -387,6 +456,19 @@ def run_gpu_mg(prob, precision, steps, warmup, n_particles):
                     f"histories {te['n_particles_transported']:.3e}, split {info['N_boundary']}/{info['N_target']}")
         return infos, time.perf_counter() - t0
 
+    rep = None
+    if tune:
+        MG_IMC2D._TRANSPORT_HOOK = hook
+        try:
+            one_run(1, steps, False, tuning=True)
+        finally:
+            MG_IMC2D._TRANSPORT_HOOK = None
+        rep = tuner.report()
+        shape[0] = tuner.best()
+        AT.store_shape("mg2d", prob.name, precision, rep)
+        log(f"[{precision}] autotune: {rep['shape']} {rep['tuned_ms']:.2f} ms over {rep['sample_steps']} sample "
+            f"steps (default {rep['default_ms']:.2f} ms, {rep['candidates']} candidates x {rep['repeats']})")
+        AT.apply(engine[0], shape[0])
     if warmup > 0:
         one_run(1000, warmup, False)
     infos, wall = one_run(1, steps, True)
@@ -397,7 +479,7 @@ def run_gpu_mg(prob, precision, steps, warmup, n_particles):
     seg_launch = np.array([i["profiling"]["transport_events"]["total"] for i in infos], dtype=np.float64)
     resid = max(abs(i["balance_residual"]) / i["total_energy"] for i in infos)
     return dict(wall=wall, segments=segs, histories=hist, kernel_ms=kern, seg_per_launch=seg_launch, residual=resid,
-                final_T=infos[-1]["temperature"])
+                final_T=infos[-1]["temperature"], tuned=rep)
 
 
 def run_cpu_mg(prob, n_particles, nsteps):
@@ -431,7 +513,7 @@ def main_mg(args):
     npart = int(args.particles) if args.particles is not None else 20_000_000
     prob = crooked_pipe_mg(n_particles=npart, n_steps=steps)
     precs = ["fp32", "fp64"] if args.precision == "both" else [args.precision]
-    res = {p: run_gpu_mg(prob, p, steps, warmup, npart) for p in precs}
+    res = {p: run_gpu_mg(prob, p, steps, warmup, npart, args.tune) for p in precs}
     head = res[precs[0]]
     value = head["segments"] / head["wall"]
     out = {
@@ -451,6 +533,8 @@ def main_mg(args):
                              head["kernel_ms"].mean() / 1e3, BYTES_PER_SEGMENT_MG[precs[0]],
                              "k_mg_transport_crooked_pipe_240x240x10.md", npart == 20_000_000),
     }
+    if args.tune:
+        out["autotune"] = {p: res[p]["tuned"] for p in precs}
     for p in precs:
         r = res[p]
         out[p] = {"segments_per_s": r["segments"] / r["wall"], "ms_per_step": r["wall"] / steps * 1e3,
@@ -753,10 +837,10 @@ def main():
                          "differentiable rz step; cf2d: Carter-Forest variant of marshak2d; cf1d: the reference's "
                          "FC vs Carter-Forest 1D Marshak-wave comparison case")
     ap.add_argument("--problems", type=int, default=256)
-    ap.add_argument("--tune", action=argparse.BooleanOptionalAction, default=False,
-                    help="autotune the 1D transport launch shape on the last warm-up step (off by default: on "
-                         "batch1d the shape picked there ran 66-67 ms/step of FP32 transport against 48.5 ms "
-                         "for the default shape, grid auto, batch 256, 8 tally copies)")
+    ap.add_argument("--tune", action=argparse.BooleanOptionalAction, default=True,
+                    help="before the warm-up, run the workload once (timed seed) and time the transport "
+                         "launch-shape candidates on four of its staged steps; the timed runs use the fastest "
+                         "(marshak2d, mg2d, batch1d; --no-tune keeps the default shape)")
     ap.add_argument("--cpu-problems", type=int, default=32)
     ap.add_argument("--cpu-scale", type=float, default=0.1)
     ap.add_argument("--steps", type=int, default=None)
@@ -809,8 +893,12 @@ def main():
     prob = marshak_2d_heterogeneous(n=args.cells, n_particles=int(args.particles), n_steps=args.steps)
     precs = ["fp32", "fp64"] if args.precision == "both" else [args.precision]
     res = {}
+    tuned = {}
     for p in precs:
-        res[p] = run_gpu(prob, p, args.steps, args.warmup, args.fom_seeds)
+        launch = None
+        if args.tune:
+            launch, tuned[p] = tune_2d(prob, p, args.steps)
+        res[p] = run_gpu(prob, p, args.steps, args.warmup, args.fom_seeds, launch)
     head = res[precs[0]]
     value = head["segments"] / head["wall"]
     kern_s = head["kernel_ms"].sum() / 1e3
@@ -844,6 +932,8 @@ def main():
                              BYTES_PER_SEGMENT[precs[0]], "k_transport_marshak2d_200x200_1e7.md",
                              args.cells == 200 and int(args.particles) == 10_000_000),
     }
+    if tuned:
+        out["autotune"] = tuned
     mask = fom_cells(res, prob.T0)
     out["fom_cells"] = int(mask.sum())
     # the FOM-cell set of the FOM definition itself: every cell with E_abs

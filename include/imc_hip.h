@@ -78,6 +78,13 @@ int imc2d_set_seed(imc2d_ctx* ctx, uint64_t seed);
  * batch = particles a wave claims per queue visit; tally_copies = private
  * copies of the per-cell tallies (power of two, 1..64). */
 int imc2d_set_launch(imc2d_ctx* ctx, int grid, int batch, int tally_copies);
+/* Tracker shape (no reference counterpart: a launch parameter of this
+ * library, picked by radtranbook_amd.autotune): block = 256 or 512 threads
+ * per workgroup of the gray tracker (its waves share the LDS tally cache);
+ * refill_min = idle lanes (1..64) a wave with active histories waits for
+ * before it takes new particles from the queue (a wave with none always
+ * refills).  Results do not depend on either. */
+int imc2d_set_tracker(imc2d_ctx* ctx, int block, int refill_min);
 
 /* Census access: the particle arrays of SimulationState2D
  * (weights, dir1, dir2, times, pos1, pos2, cell_i, cell_j; IMC/IMC2D.py:68-75). */
@@ -257,6 +264,9 @@ int imc1d_set_step(imc1d_ctx* ctx, uint32_t step);
 int imc1d_set_seed(imc1d_ctx* ctx, uint64_t seed);
 /* Transport launch shape, as imc2d_set_launch. */
 int imc1d_set_launch(imc1d_ctx* ctx, int grid, int batch, int tally_copies);
+/* Tracker shape as imc2d_set_tracker (no reference counterpart): block must
+ * be 256; refill_min = idle lanes (1..64) before a wave refills. */
+int imc1d_set_tracker(imc1d_ctx* ctx, int block, int refill_min);
 
 /* Census particles per problem (nullable output) and in total. */
 int64_t imc1d_census_counts(imc1d_ctx* ctx, int64_t* per_problem);
@@ -358,6 +368,13 @@ void mgimc_destroy(mgimc_ctx* ctx);
 int mgimc_set_step(mgimc_ctx* ctx, uint32_t step);
 int mgimc_set_seed(mgimc_ctx* ctx, uint64_t seed);
 int mgimc_set_launch(mgimc_ctx* ctx, int grid, int batch, int tally_copies);
+/* Tracker shape of the 2-D multigroup tracker as imc2d_set_tracker (no
+ * reference counterpart): block must be 256; refill_min 1..64. */
+int mgimc_set_tracker(mgimc_ctx* ctx, int block, int refill_min);
+/* Copy of the particle pool on the device (no reference counterpart; lets
+ * a staged step be transported several times, as the autotuner does):
+ * save = 1 keeps the current pool, save = 0 puts the kept one back. */
+int mgimc_snapshot(mgimc_ctx* ctx, int save);
 
 /* Census: weights, dir1, dir2, times, pos1, pos2, cell_i, cell_j, groups
  * (MG_IMC2D.py:119-127). */

@@ -322,8 +322,9 @@ class BatchIMC1D:
     def step(self, Ntarget, Nboundary, NMax, T_boundary, dt, sigma_a_func, cv, reflect=(False, False), theta=1.0,
              T_emit_floor=0.0, tune=False):
         """One step of every problem.  T_boundary: (P, 2) array or pair.
-        ``tune``: time the transport launch shapes on this step first and keep
-        the fastest (results do not depend on the shape).
+        ``tune``: True times the transport launch shapes on this step first
+        and keeps the fastest (results do not depend on the shape); a
+        callable is called with the engine before the transport instead.
         Returns a dict of per-problem energy diagnostics and device timings."""
         eng = self.eng
         P = self.P
@@ -348,7 +349,9 @@ class BatchIMC1D:
         Ec, _ = eng.sample_volume(SRC_EMISSION, self._per(Ntarget, np.int64), None, self._per(T_emit_floor),
                                   want_energy=True)
         tuned = None
-        if tune:
+        if callable(tune):
+            tune(eng)
+        elif tune:
             from .autotune import tune_engine1d
 
             tuned = tune_engine1d(eng, reflect)

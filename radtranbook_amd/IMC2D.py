@@ -27,6 +27,11 @@ __a = A_RAD
 
 VERBOSE = True
 
+# Called with the engine right before each staged transport (the launch-shape
+# autotuner times its candidates there); while set, steps take the staged
+# path.
+_TRANSPORT_HOOK = None
+
 _SIDES = {"xy": ("left", "right", "bottom", "top"), "rz": ("rmin", "rmax", "zmin", "zmax")}
 
 
@@ -172,7 +177,7 @@ def step(state, Ntarget, Nboundary, Nsource, Nmax, T_boundary, dt, edges1, edges
         if refs is None or any(a is not b for a, b in zip(refs, cur)):
             eng.set_material_powerlaw(dm[0], dm[1], dm[2], dm[3], state.temperature, state.internal_energy)
         src = np.asarray(source, dtype=np.float64)
-        if boundary_source_func is None and not (Nsource > 0 and np.max(src) > 0.0):
+        if boundary_source_func is None and not (Nsource > 0 and np.max(src) > 0.0) and _TRANSPORT_HOOK is None:
             # the whole step as one device-resident sequence (a replayed graph)
             return _step_device(state, eng, Ntarget, Nboundary, Nmax, T_boundary, dt, sigma_a_func, cv, reflect,
                                 theta, geometry, rz_linear_source, max_events_per_particle, t_step_start)
@@ -213,6 +218,8 @@ def step(state, Ntarget, Nboundary, Nsource, Nmax, T_boundary, dt, edges1, edges
     E_emit_cells, E_emit, _ = eng.sample_volume(SRC_EMISSION, Ntarget, None, dt, rz_linear_source,
                                                 want_energy=True)
 
+    if _TRANSPORT_HOOK is not None:
+        _TRANSPORT_HOOK(eng)
     t_transport_start = _time.perf_counter()
     n_transported = eng.n
     dep_cell, si_cell, boundary_loss, stats = eng.transport(dt, reflect, max_events_per_particle, -1.0)

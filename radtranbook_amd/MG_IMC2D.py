@@ -36,6 +36,10 @@ __a = A_RAD
 
 VERBOSE = True
 
+# Called with the engine right before each transport (the launch-shape
+# autotuner times its candidates there).
+_TRANSPORT_HOOK = None
+
 _SIDES = {"xy": ("left", "right", "bottom", "top"), "rz": ("rmin", "rmax", "zmin", "zmax")}
 
 
@@ -249,6 +253,8 @@ def step(state, Ntarget, Nboundary, Nsource, Nmax, T_boundary, dt, edges1, edges
 
     t_transport_start = _time.perf_counter()
     n_transported = eng.n
+    if _TRANSPORT_HOOK is not None:
+        _TRANSPORT_HOOK(eng)
     dep, si, bl, stats = eng.transport(dt, reflect, max_events_per_particle, -1.0, want_dep=diagnostics)
     t_post_start = _time.perf_counter()
     dep_cell = eng.group_sum("dep")

@@ -46,6 +46,7 @@ SIGNATURES = {
     "imc2d_set_step": (C.c_int, [_vp, C.c_uint32]),
     "imc2d_set_seed": (C.c_int, [_vp, C.c_uint64]),
     "imc2d_set_launch": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
+    "imc2d_set_tracker": (C.c_int, [_vp, C.c_int, C.c_int]),
     "imc2d_census_size": (C.c_int64, [_vp]),
     "imc2d_put_census": (C.c_int, [_vp, C.c_int64, _dp, _dp, _dp, _dp, _dp, _dp, _i64p, _i64p]),
     "imc2d_get_census": (C.c_int, [_vp, C.c_int64, _dp, _dp, _dp, _dp, _dp, _dp, _i64p, _i64p]),
@@ -76,6 +77,7 @@ SIGNATURES = {
     "imc1d_set_step": (C.c_int, [_vp, C.c_uint32]),
     "imc1d_set_seed": (C.c_int, [_vp, C.c_uint64]),
     "imc1d_set_launch": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
+    "imc1d_set_tracker": (C.c_int, [_vp, C.c_int, C.c_int]),
     "imc1d_census_counts": (C.c_int64, [_vp, _i64p]),
     "imc1d_census_energy": (C.c_int, [_vp, _dp, _dp]),
     "imc1d_put_census": (C.c_int, [_vp, _i64p, _dp, _dp, _dp, _dp, _i64p]),
@@ -97,6 +99,8 @@ SIGNATURES = {
     "mgimc_set_step": (C.c_int, [_vp, C.c_uint32]),
     "mgimc_set_seed": (C.c_int, [_vp, C.c_uint64]),
     "mgimc_set_launch": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
+    "mgimc_set_tracker": (C.c_int, [_vp, C.c_int, C.c_int]),
+    "mgimc_snapshot": (C.c_int, [_vp, C.c_int]),
     "mgimc_census_size": (C.c_int64, [_vp]),
     "mgimc_put_census": (C.c_int, [_vp, C.c_int64, _dp, _dp, _dp, _dp, _dp, _dp, _i64p, _i64p, _i32p]),
     "mgimc_get_census": (C.c_int, [_vp, C.c_int64, _dp, _dp, _dp, _dp, _dp, _dp, _i64p, _i64p, _i32p]),
@@ -325,6 +329,9 @@ class Engine2D:
 
     def set_launch(self, grid=0, batch=256, tally_copies=8):
         check(self._L.imc2d_set_launch(self._h, int(grid), int(batch), int(tally_copies)))
+
+    def set_tracker(self, block=256, refill_min=1):
+        check(self._L.imc2d_set_tracker(self._h, int(block), int(refill_min)))
 
     # -- census ---------------------------------------------------------------
     @property
@@ -558,6 +565,9 @@ class Engine1D:
     def set_launch(self, grid=0, batch=256, tally_copies=8):
         check(self._L.imc1d_set_launch(self._h, int(grid), int(batch), int(tally_copies)))
 
+    def set_tracker(self, block=256, refill_min=1):
+        check(self._L.imc1d_set_tracker(self._h, int(block), int(refill_min)))
+
     def census_counts(self):
         per = np.zeros(self.P, dtype=np.int64)
         tot = self._L.imc1d_census_counts(self._h, _p(per, _i64p))
@@ -724,6 +734,12 @@ class EngineMG:
 
     def set_launch(self, grid=0, batch=256, tally_copies=8):
         check(self._L.mgimc_set_launch(self._h, int(grid), int(batch), int(tally_copies)))
+
+    def set_tracker(self, block=256, refill_min=1):
+        check(self._L.mgimc_set_tracker(self._h, int(block), int(refill_min)))
+
+    def snapshot(self, save=True):
+        check(self._L.mgimc_snapshot(self._h, 1 if save else 0))
 
     @property
     def n(self):

@@ -428,6 +428,7 @@ struct Transport1Args {
   int copy_mask;
   unsigned long long* stats;  // [8]
   QueueMap Q;
+  int refill_min;  // a wave with active lanes refills once this many lanes are idle (0 or 1: any)
 };
 
 template <typename R>
@@ -465,7 +466,7 @@ __global__ __launch_bounds__(256) void k1_transport(Transport1Args<R> A) {
   uint32_t st_ev = 0, st_bnd = 0, st_sc = 0, st_cen = 0, st_floor = 0, st_refl = 0, st_bad = 0;
   while (true) {
     uint64_t need = __ballot(!active);
-    if (need) {
+    if (need && (__popcll(need) >= A.refill_min || need == ~0ull)) {
       const int64_t kk = wave_take(wq, !active, need, A.next, A.batch, A.Q, A.n);
       if (kk >= 0) {
         k = kk;
@@ -1067,6 +1068,7 @@ struct Imc1dCtx {
   int64_t nA = 0, nN = 0, nB = 0;
   bool staged = false;  // B holds the grouped pool of the current census + blocks
   int grid_override = 0, batch = 256, tally_copies = kTallyCopies;
+  int refill_min = 1;  // idle lanes before a wave of the tracker refills
 };
 
 }  // namespace imc
@@ -1211,6 +1213,14 @@ int imc1d_set_launch(imc1d_ctx* c, int grid, int batch, int tally_copies) {
   c->grid_override = grid;
   c->batch = batch;
   c->tally_copies = tally_copies;
+  return kOk;
+}
+
+int imc1d_set_tracker(imc1d_ctx* c, int block, int refill_min) {
+  IMC_REQUIRE(c, kErrArg, "null ctx");
+  IMC_REQUIRE(block == 256, kErrArg, "imc1d_set_tracker: block must be 256");
+  IMC_REQUIRE(refill_min >= 1 && refill_min <= 64, kErrArg, "imc1d_set_tracker: refill_min must be in [1, 64]");
+  c->refill_min = refill_min;
   return kOk;
 }
 
@@ -1728,7 +1738,7 @@ int imc1d_transport(imc1d_ctx* c, const int32_t* reflect, const double* weight_f
     if (c->precision == IMC_FP64) {
       Transport1Args<double> A{c->B.view<double>(), n, c->poff.p, c->coff.p, c->ep64.p, c->sas64.p, c->dt.p,
                                c->floor_v.p, c->refl.p, key, c->step, c->next, c->batch, c->dep_raw.p,
-                               c->path_raw.p, c->ncell, P, c->tally_copies - 1, c->stats, Q};
+                               c->path_raw.p, c->ncell, P, c->tally_copies - 1, c->stats, Q, c->refill_min};
       if (c->sph)
         k1_transport<double, 1><<<grid, block, 0, s>>>(A);
       else
@@ -1736,7 +1746,7 @@ int imc1d_transport(imc1d_ctx* c, const int32_t* reflect, const double* weight_f
     } else {
       Transport1Args<float> A{c->B.view<float>(), n, c->poff.p, c->coff.p, c->ep32.p, c->sas32.p, c->dt.p,
                               c->floor_v.p, c->refl.p, key, c->step, c->next, c->batch, c->dep_raw.p,
-                              c->path_raw.p, c->ncell, P, c->tally_copies - 1, c->stats, Q};
+                              c->path_raw.p, c->ncell, P, c->tally_copies - 1, c->stats, Q, c->refill_min};
       if (c->sph)
         k1_transport<float, 1><<<grid, block, 0, s>>>(A);
       else

@@ -90,6 +90,7 @@ struct StepDev {
   uint32_t step;     // Philox step index
   int fault;         // bit 1: the emission sample would pass the pool bound
   double face_E[4];  // face energies of the step, per side
+  Key key;           // Philox key (the context's seed)
 };
 
 // ---------------------------------------------------------------------------
@@ -317,6 +318,7 @@ __global__ void k_sample_volume(PoolView<R> P, int64_t base, int64_t total, cons
     base = sd->n_src;
     total = sd->n - sd->n_src;
     step = sd->step;
+    key = sd->key;
   }
   // one thread per particle (a grid-stride loop doubled the registers and
   // spilled)
@@ -424,6 +426,7 @@ __global__ void k_sample_boundary(PoolView<R> P, int64_t base, int64_t n, int ge
     base += sd->n0;
     E_total = sd->face_E[side];
     step = sd->step;
+    key = sd->key;
   }
   double x0 = e1[0], x1 = e1[nx], y0 = e2[0], y1 = e2[ny];
   int nface = side < 2 ? ny : nx;
@@ -559,6 +562,7 @@ struct TransportArgs {
   unsigned long long* stats;  // [8]
   QueueMap Q;
   const StepDev* sd;  // non-null: pool size and step index read from the device
+  int refill_min;     // a wave with active lanes refills once this many lanes are idle
 };
 
 // Finished histories of a wave are staged in LDS and written back to their
@@ -578,18 +582,18 @@ constexpr int kStageSlots = 96;
 constexpr int kStageFlushAt = 32;
 static_assert(kStageFlushAt - 1 + 64 <= kStageSlots, "a full wave of finishes must fit after a flush check");
 
-template <typename R, typename Off>
+template <typename R, typename Off, int WAVES>
 struct CensusStage {
-  double w[4][kStageSlots];
-  R d1[4][kStageSlots], d2[4][kStageSlots], p1[4][kStageSlots], p2[4][kStageSlots];
-  int32_t ci[4][kStageSlots], cj[4][kStageSlots];
-  Off k[4][kStageSlots];
-  int n[4];
+  double w[WAVES][kStageSlots];
+  R d1[WAVES][kStageSlots], d2[WAVES][kStageSlots], p1[WAVES][kStageSlots], p2[WAVES][kStageSlots];
+  int32_t ci[WAVES][kStageSlots], cj[WAVES][kStageSlots];
+  Off k[WAVES][kStageSlots];
+  int n[WAVES];
 };
 
 // Every lane of wave `wv` calls it with the same nb.
-template <typename R, typename Off>
-__device__ __forceinline__ void stage_flush(CensusStage<R, Off>& S, int wv, int nb, const PoolView<R>& P) {
+template <typename R, typename Off, int WAVES>
+__device__ __forceinline__ void stage_flush(CensusStage<R, Off, WAVES>& S, int wv, int nb, const PoolView<R>& P) {
   for (int e = __lane_id(); e < nb; e += 64) {
     const Off ko = S.k[wv][e];
     elem(P.w, ko) = S.w[wv][e];
@@ -603,17 +607,20 @@ __device__ __forceinline__ void stage_flush(CensusStage<R, Off>& S, int wv, int 
 }
 
 // Off: offset type of the particle-array accesses (uint32_t while every
-// byte offset fits 32 bits, else uint64_t).
-template <typename R, int GEOM, typename Off>
-__global__ __launch_bounds__(256) void k_transport(TransportArgs<R> A) {
+// byte offset fits 32 bits, else uint64_t).  WAVES: waves per workgroup (4
+// or 8), which share the LDS tally cache.
+template <typename R, int GEOM, typename Off, int WAVES = 4>
+__global__ __launch_bounds__(64 * WAVES) void k_transport(TransportArgs<R> A) {
   using R2 = typename Vec2<R>::type;
   const double wfloor = *A.floor_ptr;
   int64_t n = A.n;
   uint32_t step = A.step;
   QueueMap Q = A.Q;
+  Key key = A.key;
   if (A.sd) {
     n = A.sd->n;
     step = A.sd->step;
+    key = A.sd->key;
     Q.len = Q.order ? ((n + (1ll << Q.shift) - 1) >> Q.shift) << Q.shift : n;
   }
   // per-workgroup tally copy: spreads same-cell atomics over copies and XCDs
@@ -622,7 +629,7 @@ __global__ __launch_bounds__(256) void k_transport(TransportArgs<R> A) {
   double* const si_t = A.si_raw + copy_off;
   __shared__ int lkey[kTrackSlots];
   __shared__ double lval[kTrackSlots];
-  __shared__ CensusStage<R, Off> stg;
+  __shared__ CensusStage<R, Off, WAVES> stg;
   const int wv = threadIdx.x >> 6;
   lds_tally_init<kTrackSlots>(lkey, lval);
   if (__lane_id() == 0) stg.n[wv] = 0;
@@ -654,7 +661,8 @@ __global__ __launch_bounds__(256) void k_transport(TransportArgs<R> A) {
       }
     }
     uint64_t need = __ballot(!active);
-    if (need) {
+    // refill once refill_min lanes are idle (always when none is active)
+    if (need && (__popcll(need) >= A.refill_min || need == ~0ull)) {
       const int64_t kk = wave_take(wq, !active, need, A.next, A.batch, Q, n);
       if (kk >= 0) {
         k = kk;
@@ -696,7 +704,7 @@ __global__ __launch_bounds__(256) void k_transport(TransportArgs<R> A) {
       const R2 ey = elem(A.e2p, (uint32_t)j);
       MoveOut<R> o;
       Ctr ctr = make_ctr(kTagTransport, (uint64_t)k, step);
-      segment_move<GEOM>(ctr, A.key, ev - 1u, w, d1, d2, p1, p2, fresh, ex, ey, mat, dcen, o);
+      segment_move<GEOM>(ctr, key, ev - 1u, w, d1, d2, p1, p2, fresh, ex, ey, mat, dcen, o);
       dcen = o.dcen;
       st_bnd += o.evt == kEvtBoundary;
       st_sc += o.evt == kEvtScatter;
@@ -1158,6 +1166,7 @@ __global__ __launch_bounds__(256) void k_comb_copies(PoolView<R> P, int64_t n, i
   if (sd) {
     n = sd->n;
     step = sd->step;
+    key = sd->key;
   }
   const bool passthrough = !(*Etot > 0.0);  // empty census: every live particle kept as it is
   const int64_t k = b * 256 + threadIdx.x;
@@ -1603,6 +1612,8 @@ struct Imc2dCtx {
   DevBuf<double> g_raw, g;
   int grid_override = 0;
   int batch = 256;
+  int block = 256;      // gray tracker workgroup: 256 or 512 threads
+  int refill_min = 1;   // idle lanes before a wave of the gray tracker refills
   DevBuf<uint8_t> chunk_key;
   DevBuf<int32_t> chunk_ord, chunk_hist;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -1795,6 +1806,7 @@ struct StepHostIn {
   uint32_t step, pad;
   double face_E[4];
   double* out[6];  // T, T_r, u, dep, si, emission energies (null: not wanted)
+  Key key;
 };
 struct StepHostOut {
   double e_emit, bl, internal_total, ecen_total, rad_total;
@@ -1832,6 +1844,7 @@ __global__ void k_step_prologue(const StepHostIn* __restrict__ in, StepDev* __re
       sd->n_out = 0;
       sd->step = in->step;
       sd->fault = 0;
+      sd->key = in->key;
     }
     if (a < 4) sd->face_E[a] = in->face_E[a];
     if (a < 6) dst->out[a] = in->out[a];
@@ -1921,12 +1934,20 @@ static int zero_words(void* p, size_t bytes, hipStream_t s) {
 
 using namespace imc;
 
+// block: 256 (four waves per workgroup) or 512 (eight)
 template <typename R, typename Off>
 static void launch_transport(int geometry, int grid, int block, hipStream_t s, const TransportArgs<R>& A) {
-  if (geometry == IMC_GEOM_XY)
-    k_transport<R, 0, Off><<<grid, block, 0, s>>>(A);
-  else
-    k_transport<R, 1, Off><<<grid, block, 0, s>>>(A);
+  if (block == 512) {
+    if (geometry == IMC_GEOM_XY)
+      k_transport<R, 0, Off, 8><<<grid, block, 0, s>>>(A);
+    else
+      k_transport<R, 1, Off, 8><<<grid, block, 0, s>>>(A);
+  } else {
+    if (geometry == IMC_GEOM_XY)
+      k_transport<R, 0, Off, 4><<<grid, block, 0, s>>>(A);
+    else
+      k_transport<R, 1, Off, 4><<<grid, block, 0, s>>>(A);
+  }
 }
 
 struct imc2d_ctx : public Imc2dCtx {};
@@ -2020,11 +2041,11 @@ static int step_enqueue_t(imc2d_ctx* c, const StepLaunch& L) {
     const R2* sas = f32 ? (const R2*)c->sas32.p : (const R2*)c->sas64.p;
     TransportArgs<R> A{c->A.view<R>(), 0, e1p, e2p, c->nx, c->ny, sas, L.dt, L.refl, L.max_events, c->scal + 2,
                        key, 0u, c->next, c->batch, c->dep_raw.p, c->si_raw.p, nc, c->tally_copies - 1, c->scal + 3,
-                       c->stats, Q, sd};
+                       c->stats, Q, sd, c->refill_min};
     if (L.narrow)
-      launch_transport<R, uint32_t>(c->geometry, L.grid_tr, 256, s, A);
+      launch_transport<R, uint32_t>(c->geometry, L.grid_tr, c->block, s, A);
     else
-      launch_transport<R, uint64_t>(c->geometry, L.grid_tr, 256, s, A);
+      launch_transport<R, uint64_t>(c->geometry, L.grid_tr, c->block, s, A);
     IMC_HIP(hipGetLastError());
   }
   k_stamp<<<1, 64, 0, s>>>(&hout->t1);
@@ -2091,9 +2112,12 @@ static int step_reserve(imc2d_ctx* c, const StepLaunch& L) {
   const int64_t nc = c->ncell, lim = L.lim;
   const int64_t ntile = (lim + 255) / 256;
   int rc;
-  if ((rc = c->A.grow(lim, c->n, c->stream))) return rc;
+  // Both pool buffers hold the comb's output bound (which is above `lim`),
+  // so that either serves as the step's pool or as the comb's output after
+  // the swap and the captures of the two buffer parities stay valid.
   const int64_t comb_bound = L.Nmax + 2 * nc + lim + 256;
-  if (c->B.cap < comb_bound && (rc = c->B.grow(comb_bound + comb_bound / 2, 0, c->stream))) return rc;
+  if (c->A.cap < comb_bound && (rc = c->A.grow(comb_bound + comb_bound / 8, c->n, c->stream))) return rc;
+  if (c->B.cap < comb_bound && (rc = c->B.grow(comb_bound + comb_bound / 8, 0, c->stream))) return rc;
   const int64_t nchunk = (lim + c->batch - 1) / c->batch + 1;
   if ((rc = c->copies.reserve(lim + 1)) || (rc = c->counts.reserve(std::max<int64_t>(nc + 1, 2 * ntile + 6))) ||
       (rc = c->scan_scr.reserve(std::max(scan_scratch(nc), scan_scratch(ntile)) + 8)) ||
@@ -2114,7 +2138,8 @@ static std::vector<uint64_t> step_signature(const imc2d_ctx* c, const StepLaunch
   return std::vector<uint64_t>{
       bits(L.theta), bits(L.dt), (uint64_t)L.Ntarget, (uint64_t)L.Nb, (uint64_t)L.Nmax, (uint64_t)L.max_events,
       (uint64_t)L.face_mask, (uint64_t)L.refl, (uint64_t)L.rz_linear, (uint64_t)L.grid_tr, (uint64_t)L.narrow,
-      c->seed, (uint64_t)c->batch, (uint64_t)c->tally_copies, bits(c->pl_power), bits(c->pl_floor),
+      (uint64_t)c->block, (uint64_t)c->refill_min,
+      (uint64_t)c->batch, (uint64_t)c->tally_copies, bits(c->pl_power), bits(c->pl_floor),
       P(c->A.w), P(c->A.d1), P(c->A.d2), P(c->A.t), P(c->A.p1), P(c->A.p2), P(c->A.ci), P(c->A.cj),
       (uint64_t)c->A.cap, P(c->B.w), P(c->B.d1), P(c->B.d2), P(c->B.t), P(c->B.p1), P(c->B.p2), P(c->B.ci),
       P(c->B.cj), (uint64_t)c->B.cap, P(c->copies.p), P(c->counts.p), P(c->scan_scr.p), P(c->chunk_key.p),
@@ -2345,6 +2370,15 @@ int imc2d_set_launch(imc2d_ctx* c, int grid, int batch, int tally_copies) {
   c->grid_override = grid;
   c->batch = batch;
   c->tally_copies = tally_copies;
+  return kOk;
+}
+
+int imc2d_set_tracker(imc2d_ctx* c, int block, int refill_min) {
+  IMC_REQUIRE(c, kErrArg, "null ctx");
+  IMC_REQUIRE(block == 256 || block == 512, kErrArg, "imc2d_set_tracker: block must be 256 or 512");
+  IMC_REQUIRE(refill_min >= 1 && refill_min <= 64, kErrArg, "imc2d_set_tracker: refill_min must be in [1, 64]");
+  c->block = block;
+  c->refill_min = refill_min;
   return kOk;
 }
 
@@ -2686,11 +2720,15 @@ static int transport_grid(imc2d_ctx* c, int64_t n, int block, int* grid_out) {
     grid = c->grid_override;
   } else {
     int per_cu = 0;
-    const void* fn = c->precision == IMC_FP64
-                         ? (c->geometry == IMC_GEOM_XY ? (const void*)k_transport<double, 0, uint32_t>
-                                                       : (const void*)k_transport<double, 1, uint32_t>)
-                         : (c->geometry == IMC_GEOM_XY ? (const void*)k_transport<float, 0, uint32_t>
-                                                       : (const void*)k_transport<float, 1, uint32_t>);
+    const bool w8 = block == 512;
+    const void* fn =
+        c->precision == IMC_FP64
+            ? (c->geometry == IMC_GEOM_XY
+                   ? (w8 ? (const void*)k_transport<double, 0, uint32_t, 8> : (const void*)k_transport<double, 0, uint32_t, 4>)
+                   : (w8 ? (const void*)k_transport<double, 1, uint32_t, 8> : (const void*)k_transport<double, 1, uint32_t, 4>))
+            : (c->geometry == IMC_GEOM_XY
+                   ? (w8 ? (const void*)k_transport<float, 0, uint32_t, 8> : (const void*)k_transport<float, 0, uint32_t, 4>)
+                   : (w8 ? (const void*)k_transport<float, 1, uint32_t, 8> : (const void*)k_transport<float, 1, uint32_t, 4>));
     IMC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, block, 0));
     if (per_cu < 1) per_cu = 1;
     grid = per_cu * c->n_cu;
@@ -2725,7 +2763,7 @@ int imc2d_transport(imc2d_ctx* c, double dt, const int32_t* reflect, int64_t max
   }
   int refl = 0;
   for (int a = 0; a < 4; ++a) refl |= (reflect[a] ? 1 : 0) << a;
-  const int block = 256;
+  const int block = c->block;
   int grid;
   if ((rc = transport_grid(c, n, block, &grid))) return rc;
   Key key = make_key(c->seed);
@@ -2748,7 +2786,7 @@ int imc2d_transport(imc2d_ctx* c, double dt, const int32_t* reflect, int64_t max
     if (c->precision == IMC_FP64) {
       TransportArgs<double> A{c->A.view<double>(), n, c->e1p64.p, c->e2p64.p, c->nx, c->ny, c->sas64.p, dt, refl,
                               max_events, c->scal + 2, key, c->step, c->next, c->batch, c->dep_raw.p, c->si_raw.p,
-                              c->ncell, c->tally_copies - 1, c->scal + 3, c->stats, Q};
+                              c->ncell, c->tally_copies - 1, c->scal + 3, c->stats, Q, nullptr, c->refill_min};
       if (narrow)
         launch_transport<double, uint32_t>(c->geometry, grid, block, s, A);
       else
@@ -2756,7 +2794,7 @@ int imc2d_transport(imc2d_ctx* c, double dt, const int32_t* reflect, int64_t max
     } else {
       TransportArgs<float> A{c->A.view<float>(), n, c->e1p32.p, c->e2p32.p, c->nx, c->ny, c->sas32.p, dt, refl,
                              max_events, c->scal + 2, key, c->step, c->next, c->batch, c->dep_raw.p, c->si_raw.p,
-                             c->ncell, c->tally_copies - 1, c->scal + 3, c->stats, Q};
+                             c->ncell, c->tally_copies - 1, c->scal + 3, c->stats, Q, nullptr, c->refill_min};
       if (narrow)
         launch_transport<float, uint32_t>(c->geometry, grid, block, s, A);
       else
@@ -2845,7 +2883,7 @@ int imc2d_transport_noalloc(imc2d_ctx* c, double dt, const int32_t* reflect, int
   IMC_HIP(hipGetLastError());
   int refl = 0;
   for (int a = 0; a < 4; ++a) refl |= (reflect[a] ? 1 : 0) << a;
-  const int block = 256;
+  const int block = c->block;
   int grid;
   if ((rc = transport_grid(c, n, block, &grid))) return rc;
   QueueMap Q;
@@ -2857,7 +2895,7 @@ int imc2d_transport_noalloc(imc2d_ctx* c, double dt, const int32_t* reflect, int
     if (c->precision == IMC_FP64) {
       TransportArgs<double> A{c->A.view<double>(), n, c->e1p64.p, c->e2p64.p, c->nx, c->ny, c->sas64.p, dt, refl,
                               max_events, scal + 2, key, c->step, next, c->batch, dep_raw, si_raw,
-                              c->ncell, c->tally_copies - 1, scal + 3, stats, Q};
+                              c->ncell, c->tally_copies - 1, scal + 3, stats, Q, nullptr, c->refill_min};
       if (narrow)
         launch_transport<double, uint32_t>(c->geometry, grid, block, s, A);
       else
@@ -2865,7 +2903,7 @@ int imc2d_transport_noalloc(imc2d_ctx* c, double dt, const int32_t* reflect, int
     } else {
       TransportArgs<float> A{c->A.view<float>(), n, c->e1p32.p, c->e2p32.p, c->nx, c->ny, c->sas32.p, dt, refl,
                              max_events, scal + 2, key, c->step, next, c->batch, dep_raw, si_raw,
-                             c->ncell, c->tally_copies - 1, scal + 3, stats, Q};
+                             c->ncell, c->tally_copies - 1, scal + 3, stats, Q, nullptr, c->refill_min};
       if (narrow)
         launch_transport<float, uint32_t>(c->geometry, grid, block, s, A);
       else
@@ -3149,7 +3187,7 @@ int imc2d_step(imc2d_ctx* c, double theta, double dt, int64_t Ntarget, int64_t N
   const int64_t need = n_src + Ntarget + c->ncell + 16;
   {
     int grid;
-    int rc0 = transport_grid(c, INT64_MAX / 2, 256, &grid);
+    int rc0 = transport_grid(c, INT64_MAX / 2, c->block, &grid);
     if (rc0) return rc0;
     L.grid_tr = grid;
   }
@@ -3168,9 +3206,14 @@ int imc2d_step(imc2d_ctx* c, double theta, double dt, int64_t Ntarget, int64_t N
   }
   int rc;
   if (slot < 0) {
-    // a capture covers 1/8 more than this step needs, so that steady steps
-    // replay it; a direct launch covers this step
-    L.lim = use_graph ? need + need / 8 : need;
+    // A capture covers 1/8 more than the larger of this step's pool and the
+    // pool of a step after a comb to Nmax particles (at most Nmax + 2 ncell
+    // in census, but for the comb's rounding), so that the steps of a run
+    // replay the two captures (one per pool-buffer parity); a direct launch
+    // covers this step.
+    const int64_t steady = L.Nmax + 2 * c->ncell + (n_src - n0) + Ntarget + c->ncell + 16;
+    const int64_t cover = std::max(need, steady);
+    L.lim = use_graph ? cover + cover / 8 : need;
     if ((rc = step_reserve(c, L))) return rc;
     L.narrow = c->A.cap <= kNarrowMax;
   }
@@ -3180,6 +3223,7 @@ int imc2d_step(imc2d_ctx* c, double theta, double dt, int64_t Ntarget, int64_t N
   hin->n0 = n0;
   hin->n_src = n_src;
   hin->step = c->step;
+  hin->key = make_key(c->seed);
   for (int a = 0; a < 4; ++a) hin->face_E[a] = ((L.face_mask >> a) & 1) ? face_E[a] : 0.0;
   double* outs[6] = {T_out, Tr_out, u_out, dep_out, si_out, E_out};
   HostOut ho[6];

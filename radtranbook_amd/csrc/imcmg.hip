@@ -544,6 +544,7 @@ struct MgArgs {
   int copy_mask;
   unsigned long long* stats;
   QueueMap Q;
+  int refill_min;  // a wave with active lanes refills once this many lanes are idle (0 or 1: any)
 };
 
 constexpr int kMgWaveSlots = 512;  // tally cache slots per wave of the 2-D tracker
@@ -584,7 +585,7 @@ __global__ __launch_bounds__(256) void k_mg_transport(MgArgs<R> A) {
   uint32_t st_ev = 0, st_bnd = 0, st_sc = 0, st_cen = 0, st_floor = 0, st_refl = 0, st_cap = 0;
   while (true) {
     uint64_t need = __ballot(!active);
-    if (need) {
+    if (need && (__popcll(need) >= A.refill_min || need == ~0ull)) {
       const int64_t kk = wave_take(wq, !active, need, A.next, A.batch, A.Q, A.n);
       if (kk >= 0) {
         k = kk;
@@ -1308,6 +1309,9 @@ struct MgCtx {
   MgPool A, B;
   int64_t n = 0;
   int grid_override = 0, batch = 256, tally_copies = kMgTallyCopies;
+  int refill_min = 1;  // idle lanes before a wave of the 2-D tracker refills
+  MgPool snap;         // mgimc_snapshot
+  int64_t n_snap = 0;
   bool have_material = false;
   // page-locked staging of the (G, ncell) transfers: slot 0 opacities in,
   // slot 1 scalar intensity out, slot 2 post-comb census energy out
@@ -1476,7 +1480,7 @@ static int mg_create(mgimc_ctx** out, int geometry, int nx, int ny, const double
   }
   IMC_HIP(hipMalloc(&c->stats, sizeof(unsigned long long) * 8));
   IMC_HIP(hipMalloc(&c->next, sizeof(unsigned long long)));
-  c->A.rsz = c->B.rsz = precision == IMC_FP64 ? 8 : 4;
+  c->A.rsz = c->B.rsz = c->snap.rsz = precision == IMC_FP64 ? 8 : 4;
   *out = c;
   return kOk;
 }
@@ -1522,6 +1526,7 @@ void mgimc_destroy(mgimc_ctx* c) {
   c->stage.release();
   c->A.release();
   c->B.release();
+  c->snap.release();
   if (c->stats) (void)hipFree(c->stats);
   if (c->next) (void)hipFree(c->next);
   (void)hipEventDestroy(c->ev0);
@@ -1549,6 +1554,35 @@ int mgimc_set_launch(mgimc_ctx* c, int grid, int batch, int tally_copies) {
   c->grid_override = grid;
   c->batch = batch;
   c->tally_copies = tally_copies;
+  return kOk;
+}
+
+int mgimc_set_tracker(mgimc_ctx* c, int block, int refill_min) {
+  IMC_REQUIRE(c, kErrArg, "null ctx");
+  IMC_REQUIRE(block == 256, kErrArg, "mgimc_set_tracker: block must be 256");
+  IMC_REQUIRE(refill_min >= 1 && refill_min <= 64, kErrArg, "mgimc_set_tracker: refill_min must be in [1, 64]");
+  c->refill_min = refill_min;
+  return kOk;
+}
+
+// Copy of the pool: save = 1 keeps the current pool, save = 0 puts the kept
+// one back (it stays kept).
+int mgimc_snapshot(mgimc_ctx* c, int save) {
+  IMC_REQUIRE(c, kErrArg, "null ctx");
+  IMC_HIP(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  MgPool& from = save ? c->A : c->snap;
+  MgPool& to = save ? c->snap : c->A;
+  const int64_t n = save ? c->n : c->n_snap;
+  int rc = to.grow(std::max<int64_t>(n, 1), 0, s);
+  if (rc) return rc;
+  for (int k = 0; k < 9; ++k)
+    if (n > 0) IMC_HIP(hipMemcpyAsync(to.a[k], from.a[k], from.esz(k) * (size_t)n, hipMemcpyDeviceToDevice, s));
+  IMC_HIP(hipStreamSynchronize(s));
+  if (save)
+    c->n_snap = n;
+  else
+    c->n = n;
   return kOk;
 }
 
@@ -1900,7 +1934,7 @@ int mgimc_transport(mgimc_ctx* c, double dt, const int32_t* reflect, int64_t max
     if (c->precision == IMC_FP64) {
       MgArgs<double> A{c->A.view<double>(), n, c->e1p64.p, c->e2p64.p, c->nx, c->ny, G, c->sas64.p, dt, refl,
                        max_events, c->scal.p + 2, key, c->step, c->next, c->batch, c->dep_raw.p, c->path_raw.p, nb,
-                       c->tally_copies - 1, c->stats, Q};
+                       c->tally_copies - 1, c->stats, Q, c->refill_min};
       if (c->geometry == IMC_GEOM_XY)
         k_mg_transport<double, 0><<<grid, block, 0, s>>>(A);
       else
@@ -1908,7 +1942,7 @@ int mgimc_transport(mgimc_ctx* c, double dt, const int32_t* reflect, int64_t max
     } else {
       MgArgs<float> A{c->A.view<float>(), n, c->e1p32.p, c->e2p32.p, c->nx, c->ny, G, c->sas32.p, dt, refl,
                       max_events, c->scal.p + 2, key, c->step, c->next, c->batch, c->dep_raw.p, c->path_raw.p, nb,
-                      c->tally_copies - 1, c->stats, Q};
+                      c->tally_copies - 1, c->stats, Q, c->refill_min};
       if (c->geometry == IMC_GEOM_XY)
         k_mg_transport<float, 0><<<grid, block, 0, s>>>(A);
       else

@@ -1,0 +1,155 @@
+"""The launch-shape autotuner on the device: a run whose staged steps are
+transported again and again under candidate shapes (the pool put back from
+the engine's snapshot before each trial) must step exactly as the same run
+untuned.  Every history draws from its own Philox stream, so the two runs
+differ only by the summation order of the FP64 tally atomics (1e-10
+relative); census sizes and event counts are equal.  Covers the gray 2-D,
+multigroup 2-D and batched 1-D engines, with block-512 trackers and refill
+thresholds among the candidates."""
+import sys
+
+import numpy as np
+import pytest
+
+from radtranbook_amd import autotune as AT
+
+pytestmark = pytest.mark.gpu
+
+
+def _announce(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _close(a, b, rtol=1e-10):
+    a, b = np.asarray(a, dtype=np.float64), np.asarray(b, dtype=np.float64)
+    np.testing.assert_allclose(a, b, rtol=rtol, atol=rtol * max(np.abs(b).max(), 1e-300))
+
+
+def _gray2d_run(precision, tuner):
+    from radtranbook_amd import IMC2D
+    from radtranbook_amd.materials import LinearEOS
+
+    IMC2D.VERBOSE = False
+    nx, ny = 20, 16
+    e1, e2 = np.linspace(0.0, 0.5, nx + 1), np.linspace(0.0, 0.4, ny + 1)
+    sigma0 = np.exp(2.0 + np.random.default_rng(3).standard_normal((nx, ny)))
+    eos = LinearEOS(np.full((nx, ny), 0.3))
+    T0 = np.full((nx, ny), 1e-3)
+    sf = lambda T: sigma0 / np.maximum(T, 1e-3) ** 3  # noqa: E731
+    st = IMC2D.init_simulation(8000, T0, T0, e1, e2, eos.eos, eos.inv_eos, precision=precision, seed=5)
+    IMC2D._TRANSPORT_HOOK = (lambda eng: tuner.sample(eng)) if tuner else (lambda eng: None)
+    out = []
+    try:
+        for _ in range(3):
+            st, info = IMC2D.step(st, 6000, 4000, 0, 12000, (1.0, 0, 0, 0), 0.005, e1, e2, sf, eos.inv_eos,
+                                  eos.cv_func, np.zeros((nx, ny)))
+            out.append(info)
+    finally:
+        IMC2D._TRANSPORT_HOOK = None
+        st._engine.close()
+    return out
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_gray2d_tuned_run_steps_as_untuned(precision):
+    _announce(f"test_gray2d_tuned_run_steps_as_untuned {precision}")
+    shapes = [AT.default_shape("gray2d"), AT.Shape(0, 128, 8, 512, 8), AT.Shape(0, 512, 4, 256, 32),
+              AT.Shape(64, 64, 8, 512, 1)]
+    tuner = AT.gray2d_tuner(0.005, (False,) * 4, 10**6, repeats=2)
+    tuner.shapes = shapes
+    ref = _gray2d_run(precision, None)
+    got = _gray2d_run(precision, tuner)
+    assert len(tuner.samples) == 3 and all(len(s) == len(shapes) for s in tuner.samples)
+    assert all(v > 0 for smp in tuner.samples for v in smp.values())
+    assert tuner.best() in shapes
+    for a, b in zip(got, ref):
+        _close(a["temperature"], b["temperature"])
+        _close(a["radiation_temperature"], b["radiation_temperature"])
+        _close(a["emitted_energies"], b["emitted_energies"])
+        assert a["N_particles"] == b["N_particles"]
+        ta, tb = a["profiling"]["transport_events"], b["profiling"]["transport_events"]
+        assert ta["n_particles_transported"] == tb["n_particles_transported"]
+        assert ta["total"] == tb["total"]
+
+
+def _mg_run(precision, tuner):
+    from radtranbook_amd import MG_IMC2D
+
+    MG_IMC2D.VERBOSE = False
+    edges = np.logspace(-4, 1, 7)
+    nu = np.sqrt(edges[:-1] * edges[1:])
+    n, m = 10, 8
+    e1, e2 = np.linspace(0.0, 1.0, n + 1), np.linspace(0.0, 0.8, m + 1)
+    sig0 = np.exp(np.random.default_rng(1).uniform(np.log(0.5), np.log(20.0), (n, m)))
+    sf = [(lambda g: (lambda T: np.minimum(sig0 * (nu[g] / np.maximum(T, 1e-3)) ** -3, 2e3)))(g) for g in range(6)]
+    inv = lambda u: u / 0.3  # noqa: E731
+    cvf = lambda T: 0.3 + 0 * T  # noqa: E731
+    T0 = np.full((n, m), 0.05)
+    st = MG_IMC2D.init_simulation(3000, T0, T0, e1, e2, edges, lambda T: 0.3 * T, inv, precision=precision, seed=7)
+    MG_IMC2D._TRANSPORT_HOOK = (lambda eng: tuner.sample(eng)) if tuner else None
+    out = []
+    try:
+        for _ in range(3):
+            st, info = MG_IMC2D.step(st, 3000, 3000, 0, 6000, (0.5, 0, 0, 0), 0.004, e1, e2, edges, sf, inv, cvf,
+                                     np.zeros((n, m)))
+            out.append(info)
+    finally:
+        MG_IMC2D._TRANSPORT_HOOK = None
+        st._engine.close()
+    return out
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_mg2d_tuned_run_steps_as_untuned(precision):
+    _announce(f"test_mg2d_tuned_run_steps_as_untuned {precision}")
+    shapes = [AT.default_shape("mg2d"), AT.Shape(0, 128, 16, 256, 8), AT.Shape(0, 512, 32, 256, 32)]
+    tuner = AT.mg2d_tuner(0.004, (False,) * 4, 10**6, repeats=2)
+    tuner.shapes = shapes
+    ref = _mg_run(precision, None)
+    got = _mg_run(precision, tuner)
+    assert len(tuner.samples) == 3
+    for a, b in zip(got, ref):
+        _close(a["temperature"], b["temperature"])
+        _close(a["radiation_temperature"], b["radiation_temperature"])
+        assert a["N_particles"] == b["N_particles"]
+        ta, tb = a["profiling"]["transport_events"], b["profiling"]["transport_events"]
+        assert ta["total"] == tb["total"]
+
+
+def _batch1d_run(tune):
+    from radtranbook_amd.IMC1D import BatchIMC1D
+
+    sizes = (12, 30, 7)
+    meshes = [np.column_stack([np.linspace(0, 0.1 * (k + 1), I + 1)[:-1], np.linspace(0, 0.1 * (k + 1), I + 1)[1:]])
+              for k, I in enumerate(sizes)]
+    T0 = [np.full(I, 1e-3) for I in sizes]
+    eos = lambda T: 0.3 * T  # noqa: E731
+    inv = lambda u: u / 0.3  # noqa: E731
+    cvf = lambda T: 0.3 + 0 * T  # noqa: E731
+    sf = lambda T: 200.0 * np.maximum(T, 1e-3) ** -3  # noqa: E731
+    N = np.array([4000, 9000, 3000], dtype=np.int64)
+    b = BatchIMC1D(meshes, T0, T0, eos, inv, N, precision="fp64", seed=2)
+    out = []
+    try:
+        for s in range(3):
+            o = b.step(N // 2, N - N // 2, N, np.array([[1.0, 0.0]] * 3), 0.01, sf, cvf,
+                       tune=tune if s > 0 else False)
+            out.append((o, [t.copy() for t in b.T]))
+    finally:
+        b.eng.close()
+    return out
+
+
+def test_batch1d_tuned_run_steps_as_untuned():
+    _announce("test_batch1d_tuned_run_steps_as_untuned")
+    ref = _batch1d_run(False)
+    got = _batch1d_run(True)
+    for (a, Ta), (b, Tb) in zip(got, ref):
+        for x, y in zip(Ta, Tb):
+            _close(x, y)
+        assert np.array_equal(a["N_particles"], b["N_particles"])
+        assert np.array_equal(np.asarray(a["stats"]), np.asarray(b["stats"]))
+    shape, rep = got[1][0]["tuned"]
+    assert shape in AT.candidates("gray1d")
+    assert rep["tuned_ms"] <= rep["default_ms"]
