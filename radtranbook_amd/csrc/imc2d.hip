@@ -1090,10 +1090,14 @@ __device__ __forceinline__ bool comb_alive(const PoolView<R>& P, int64_t k, int 
   return P.w[k] > 0.0 && ci >= 0 && ci < nx && cj >= 0 && cj < ny;
 }
 
-// Census energy per cell: a grid-stride pass over the pool; runs of equal
-// cells among consecutive lanes are summed in registers, the run totals go to
-// the workgroup's LDS tally cache, and the cache is flushed to this
-// workgroup's private copy of the histogram at the end.
+// Census energy per cell.  Each workgroup sums one contiguous range of the
+// pool, four tiles of 256 particles per trip (loads of the four in flight
+// together); runs of equal cells among consecutive lanes are summed in
+// registers and the run totals go to the workgroup's LDS tally cache, flushed
+// to this workgroup's private copy of the histogram at the end.  The pool is
+// laid out cell by cell, so a range covers few cells and nearly every add
+// stays in LDS (a grid-stride walk spread every workgroup over the whole mesh
+// and overflowed the cache into global atomics).
 template <typename R>
 __global__ __launch_bounds__(256) void k_comb_ecen(PoolView<R> P, int64_t n, int nx, int ny, int copy_mask,
                                                    double* __restrict__ ecen_part,
@@ -1105,20 +1109,30 @@ __global__ __launch_bounds__(256) void k_comb_ecen(PoolView<R> P, int64_t n, int
   __syncthreads();
   const int64_t ncell = (int64_t)nx * ny;
   double* const dst = ecen_part + (int64_t)(blockIdx.x & copy_mask) * ncell;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  // every lane runs the same number of trips so the wave-wide run sum sees
-  // all 64 lanes
-  const int64_t trips = (n + stride - 1) / stride;
-  for (int64_t t = 0; t < trips; ++t) {
-    int64_t k = t * stride + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int key = -1;
-    double v = 0.0;
-    if (k < n && comb_alive(P, k, nx, ny)) {
-      key = P.ci[k] * ny + P.cj[k];
-      v = P.w[k];
+  constexpr int kU = 4;
+  const int64_t span = (int64_t)kU * blockDim.x;
+  const int64_t per = ((n + gridDim.x - 1) / gridDim.x + span - 1) / span * span;
+  const int64_t lo = (int64_t)blockIdx.x * per;
+  const int64_t hi = n < lo + per ? n : lo + per;
+  // uniform trip count over the workgroup: the run sum needs all 64 lanes
+  for (int64_t k0 = lo; k0 < hi; k0 += span) {
+    int key[kU];
+    double v[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int64_t k = k0 + u * (int64_t)blockDim.x + threadIdx.x;
+      key[u] = -1;
+      v[u] = 0.0;
+      if (k < hi && comb_alive(P, k, nx, ny)) {
+        key[u] = P.ci[k] * ny + P.cj[k];
+        v[u] = P.w[k];
+      }
     }
-    bool head = wave_run_sum(key, v);
-    if (head && key >= 0 && !lds_tally_add(lkey, lval, key, v)) atomic_add_f64(dst + key, v);
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      bool head = wave_run_sum(key[u], v[u]);
+      if (head && key[u] >= 0 && !lds_tally_add(lkey, lval, key[u], v[u])) atomic_add_f64(dst + key[u], v[u]);
+    }
   }
   __syncthreads();
   lds_tally_flush(lkey, lval, dst);

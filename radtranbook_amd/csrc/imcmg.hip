@@ -798,18 +798,33 @@ __global__ __launch_bounds__(256) void k_mg_ecen(MgView<R> P, int64_t n, int nx,
   __shared__ __align__(8) char tally_lds[4 * Tally::kBytes];
   Tally tally(tally_lds + (threadIdx.x >> 6) * Tally::kBytes);
   double* const dst = part + (int64_t)(blockIdx.x & copy_mask) * nb;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  const int64_t trips = (n + stride - 1) / stride;
-  for (int64_t tr = 0; tr < trips; ++tr) {
-    int64_t k = tr * stride + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int key = -1;
-    double v = 0.0;
-    if (k < n) {
-      key = mg_bin(P, k, nx, ny);
-      if (key >= 0) v = P.w[k];
+  // one contiguous range of the pool per workgroup, four tiles per trip (the
+  // pool comes cell by cell, so a range covers few bins and the wave caches
+  // keep them; a grid-stride walk spread every wave over the whole mesh)
+  constexpr int kU = 4;
+  const int64_t span = (int64_t)kU * blockDim.x;
+  const int64_t per = ((n + gridDim.x - 1) / gridDim.x + span - 1) / span * span;
+  const int64_t lo = (int64_t)blockIdx.x * per;
+  const int64_t hi = n < lo + per ? n : lo + per;
+  // uniform trip count over the workgroup: the run sum needs all 64 lanes
+  for (int64_t k0 = lo; k0 < hi; k0 += span) {
+    int key[kU];
+    double v[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int64_t k = k0 + u * (int64_t)blockDim.x + threadIdx.x;
+      key[u] = -1;
+      v[u] = 0.0;
+      if (k < hi) {
+        key[u] = mg_bin(P, k, nx, ny);
+        if (key[u] >= 0) v[u] = P.w[k];
+      }
     }
-    bool head = wave_run_sum(key, v);
-    if (head && key >= 0) tally.add(key, v, dst);
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      bool head = wave_run_sum(key[u], v[u]);
+      if (head && key[u] >= 0) tally.add(key[u], v[u], dst);
+    }
   }
   tally.flush(dst);
 }
