@@ -107,6 +107,27 @@ def _sample_steps(steps):
     return sorted({max(steps // 4, 0), steps // 2, (3 * steps) // 4, steps - 1})
 
 
+RETUNE = False  # --retune: tune even where a pick is cached
+
+
+def cached_pick(kind, workload, precision):
+    """(shape, report) kept by an earlier tuning run of this workload and
+    precision, or None (none kept, or --retune)."""
+    from radtranbook_amd import autotune as AT
+
+    if RETUNE:
+        return None
+    rep = AT.load_report(kind, workload, precision)
+    if rep is None:
+        return None
+    log(f"[{precision}] autotune: cached pick {rep['shape']} ({AT.cache_path()})")
+    return AT.Shape(**rep["shape"]), dict(rep, cached=True)
+
+
+def tune_key_2d(prob):
+    return f"{prob.name}_N{prob.Ntarget}_Nb{prob.Nboundary}_Nmax{prob.Nmax}"
+
+
 def tune_2d(prob, precision, steps):
     """Tuning pass: the workload's own run (timed seed) with the launch-shape
     candidates timed on the staged pool of four of its steps."""
@@ -138,7 +159,7 @@ def tune_2d(prob, precision, steps):
         IMC2D._TRANSPORT_HOOK = None
         st._engine.close()
     rep = t.report()
-    AT.store_shape("gray2d", prob.name, precision, rep)
+    AT.store_shape("gray2d", tune_key_2d(prob), precision, rep)
     log(f"[{precision}] autotune: {rep['shape']} {rep['tuned_ms']:.2f} ms over {rep['sample_steps']} sample steps "
         f"(default {rep['default_ms']:.2f} ms, {rep['candidates']} candidates x {rep['repeats']})")
     return t.best(), rep
@@ -285,7 +306,12 @@ def run_gpu_batch1d(probs, precision, steps, warmup, tune):
 
     b = BatchIMC1D(meshes, T0, T0, eos, inv, N, precision=precision, seed=1)
     tuned = None
-    if tune:
+    key = f"batch_slabs_{len(probs)}_N{int(N.sum())}"
+    cached = cached_pick("gray1d", key, precision) if tune else None
+    if cached:
+        AT.apply(b.eng, cached[0])
+        tuned = cached[1]
+    elif tune:
         # tuning pass: the timed run itself (seed 1), candidates timed on the
         # staged step at four of its steps
         tuner = AT.gray1d_tuner((False, False))
@@ -294,7 +320,7 @@ def run_gpu_batch1d(probs, precision, steps, warmup, tune):
             b.step(Nt, Nb, N, Tb, dt, sf, cvf, tune=(tuner.sample if s in samples else False))
         tuned = tuner.report()
         AT.apply(b.eng, tuner.best())
-        AT.store_shape("gray1d", f"batch_slabs_{len(probs)}", precision, tuned)
+        AT.store_shape("gray1d", key, precision, tuned)
         log(f"[{precision}] autotune: {tuned['shape']} {tuned['tuned_ms']:.2f} ms over {tuned['sample_steps']} "
             f"sample steps (default {tuned['default_ms']:.2f} ms, {tuned['candidates']} candidates x "
             f"{tuned['repeats']})")
@@ -457,7 +483,11 @@ def run_gpu_mg(prob, precision, steps, warmup, n_particles, tune=False):
         return infos, time.perf_counter() - t0
 
     rep = None
-    if tune:
+    key = f"{prob.name}_N{n_particles}"
+    cached = cached_pick("mg2d", key, precision) if tune else None
+    if cached:
+        shape[0], rep = cached
+    elif tune:
         MG_IMC2D._TRANSPORT_HOOK = hook
         try:
             one_run(1, steps, False, tuning=True)
@@ -465,7 +495,7 @@ def run_gpu_mg(prob, precision, steps, warmup, n_particles, tune=False):
             MG_IMC2D._TRANSPORT_HOOK = None
         rep = tuner.report()
         shape[0] = tuner.best()
-        AT.store_shape("mg2d", prob.name, precision, rep)
+        AT.store_shape("mg2d", key, precision, rep)
         log(f"[{precision}] autotune: {rep['shape']} {rep['tuned_ms']:.2f} ms over {rep['sample_steps']} sample "
             f"steps (default {rep['default_ms']:.2f} ms, {rep['candidates']} candidates x {rep['repeats']})")
         AT.apply(engine[0], shape[0])
@@ -839,8 +869,10 @@ def main():
     ap.add_argument("--problems", type=int, default=256)
     ap.add_argument("--tune", action=argparse.BooleanOptionalAction, default=True,
                     help="before the warm-up, run the workload once (timed seed) and time the transport "
-                         "launch-shape candidates on four of its staged steps; the timed runs use the fastest "
-                         "(marshak2d, mg2d, batch1d; --no-tune keeps the default shape)")
+                         "launch-shape candidates on four of its staged steps; the timed runs use the fastest, "
+                         "which is kept in the launch-shape cache and reused by later runs of the same workload "
+                         "and precision (marshak2d, mg2d, batch1d; --no-tune keeps the default shape)")
+    ap.add_argument("--retune", action="store_true", help="tune even where the cache holds a pick")
     ap.add_argument("--cpu-problems", type=int, default=32)
     ap.add_argument("--cpu-scale", type=float, default=0.1)
     ap.add_argument("--steps", type=int, default=None)
@@ -853,6 +885,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
+    global RETUNE
+    RETUNE = args.retune
 
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU; --gpus {args.gpus} is not supported "
@@ -897,7 +931,7 @@ def main():
     for p in precs:
         launch = None
         if args.tune:
-            launch, tuned[p] = tune_2d(prob, p, args.steps)
+            launch, tuned[p] = cached_pick("gray2d", tune_key_2d(prob), p) or tune_2d(prob, p, args.steps)
         res[p] = run_gpu(prob, p, args.steps, args.warmup, args.fom_seeds, launch)
     head = res[precs[0]]
     value = head["segments"] / head["wall"]

@@ -182,8 +182,11 @@ def step(state, Ntarget, Nboundary, Nsource, Nmax, T_boundary, dt, edges1, edges
     eng.set_step(state.count)
 
     temperature = state.temperature
-    sigma = np.stack([np.asarray(sigma_a_funcs[g](temperature), dtype=np.float64) * np.ones((nx, ny))
-                      for g in range(G)])
+    # the opacities go straight into a page-locked block (uploaded by one DMA
+    # copy); a callable may return a scalar or an (nx, ny) array
+    sigma = eng.group_block()
+    for g in range(G):
+        sigma[g] = sigma_a_funcs[g](temperature)
     cvv = np.asarray(cv(temperature), dtype=np.float64) * np.ones((nx, ny))
     want = ("Eplanck",) if Ntotal > 0 else ()
     if diagnostics:
@@ -263,14 +266,16 @@ def step(state, Ntarget, Nboundary, Nsource, Nmax, T_boundary, dt, edges1, edges
 
     ecen, _, post, ecen_total, total_rad = eng.comb(int(Nmax), want_ecen=not use_scalar_intensity_Tr,
                                                     want_post=True)
+    # per-group arrays of the step written into page-locked blocks (no fresh
+    # 8-byte x G x ncell allocations); post is not returned, so in place
     if use_scalar_intensity_Tr:
-        erg_pre = si / __c
+        erg_pre = np.divide(si, __c, out=eng.group_block())
     else:
-        erg_pre = ecen / volumes[None, :, :]
+        erg_pre = np.divide(ecen, volumes[None, :, :], out=ecen)
     if conserve_comb_energy:
         internal_energy = internal_energy + eng.group_sum("disc") / volumes
         temperature = inv_eos(internal_energy)
-    erg_post = post / volumes[None, :, :]
+    erg_post = np.divide(post, volumes[None, :, :], out=post)
     radiation_temperature = quarter_root(np.sum(erg_pre, axis=0) / __a)
 
     total_internal = float(np.sum(internal_energy * volumes))

@@ -829,6 +829,11 @@ class EngineMG:
         check(self._L.mgimc_comb(self._h, int(Nmax), _p(ecen), _p(disc), _p(post), C.byref(et), C.byref(tr)))
         return ecen, disc, post, et.value, tr.value
 
+    def group_block(self):
+        """A fresh (G, nx, ny) float64 array in a page-locked block (the
+        library copies such arrays by DMA, with no staging copy)."""
+        return self._ghost.take()
+
     _GROUP_SUM = {"emission": 0, "dep": 1, "disc": 2}
 
     def group_sum(self, what):

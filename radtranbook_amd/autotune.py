@@ -169,14 +169,22 @@ def cache_key(kind, workload, precision):
     return f"{kind}|{workload}|{precision}"
 
 
-def load_shape(kind, workload, precision):
-    """The cached pick, or None."""
+def load_report(kind, workload, precision):
+    """The cached tuner report (its "shape" checked), or None."""
     try:
         with open(cache_path()) as f:
             d = json.load(f)
-        return Shape(**d[cache_key(kind, workload, precision)]["shape"])
+        rep = d[cache_key(kind, workload, precision)]
+        Shape(**rep["shape"])
+        return rep
     except (OSError, KeyError, TypeError, ValueError):
         return None
+
+
+def load_shape(kind, workload, precision):
+    """The cached pick, or None."""
+    rep = load_report(kind, workload, precision)
+    return None if rep is None else Shape(**rep["shape"])
 
 
 def store_shape(kind, workload, precision, report):

@@ -1691,7 +1691,10 @@ int mgimc_set_material(mgimc_ctx* c, const double* T, const double* sigma, const
   hipStream_t s = c->stream;
   int64_t nc = c->ncell, nb = c->nb;
   IMC_HIP(hipMemcpyAsync(c->T.p, T, sizeof(double) * nc, hipMemcpyHostToDevice, s));
-  {
+  if (host_device_alias(sigma, sizeof(double) * nb)) {
+    // page-locked (imc_host_alloc): one DMA copy, no host memcpy
+    IMC_HIP(hipMemcpyAsync(c->sig.p, sigma, sizeof(double) * nb, hipMemcpyHostToDevice, s));
+  } else {
     double* h = c->stage.slot(nb, 0);
     std::memcpy(h, sigma, sizeof(double) * nb);
     IMC_HIP(hipMemcpyAsync(c->sig.p, h, sizeof(double) * nb, hipMemcpyHostToDevice, s));
@@ -1975,14 +1978,19 @@ int mgimc_transport(mgimc_ctx* c, double dt, const int32_t* reflect, int64_t max
   IMC_HIP(hipGetLastError());
   c->dep_valid = true;
   if (dep_out) IMC_HIP(hipMemcpyAsync(dep_out, c->dep.p, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
-  double* h_si = c->stage.slot(nb, 1);
-  if (si_out) IMC_HIP(hipMemcpyAsync(h_si, c->si.p, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
+  // page-locked outputs (imc_host_alloc) take a direct DMA copy, others go
+  // through the staging slot and a host copy after the synchronisation
+  double* h_si = nullptr;
+  if (si_out) {
+    h_si = host_device_alias(si_out, sizeof(double) * nb) ? si_out : c->stage.slot(nb, 1);
+    IMC_HIP(hipMemcpyAsync(h_si, c->si.p, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
+  }
   if (bl_out) IMC_HIP(hipMemcpyAsync(bl_out, c->bl.p, sizeof(double) * (1 + G + 4), hipMemcpyDeviceToHost, s));
   unsigned long long st[8];
   IMC_HIP(hipMemcpyAsync(st, c->stats, sizeof(st), hipMemcpyDeviceToHost, s));
   IMC_HIP(hipStreamSynchronize(s));
   IMC_HIP(hipEventElapsedTime(&c->last_ms, c->ev0, c->ev1));
-  if (si_out) std::memcpy(si_out, h_si, sizeof(double) * nb);
+  if (si_out && h_si != si_out) std::memcpy(si_out, h_si, sizeof(double) * nb);
   if (stats_out)
     for (int a = 0; a < 8; ++a) stats_out[a] = (int64_t)st[a];
   return kOk;
@@ -2053,6 +2061,7 @@ int mgimc_comb(mgimc_ctx* c, int64_t Nmax, double* ecen_out, double* disc_out, d
   std::swap(c->A, c->B);
   c->n = total;
   c->post_valid = disc_out || post_out;
+  double* h_post = nullptr;
   if (disc_out || post_out) {
     if ((rc = mg_histogram(c, c->A, total, c->post.p))) return rc;
     if (disc_out) {
@@ -2060,14 +2069,17 @@ int mgimc_comb(mgimc_ctx* c, int64_t Nmax, double* ecen_out, double* disc_out, d
       IMC_HIP(hipGetLastError());
       IMC_HIP(hipMemcpyAsync(disc_out, c->tmp.p, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
     }
-    if (post_out) IMC_HIP(hipMemcpyAsync(c->stage.slot(nb, 2), c->post.p, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
+    if (post_out) {
+      h_post = host_device_alias(post_out, sizeof(double) * nb) ? post_out : c->stage.slot(nb, 2);
+      IMC_HIP(hipMemcpyAsync(h_post, c->post.p, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
+    }
   }
   if (ecen_out) IMC_HIP(hipMemcpyAsync(ecen_out, c->ecen.p, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
   if ((rc = tree_sum((const double*)c->A.a[0], total, c->scal.p + 5, c->tree_scr.p, s))) return rc;
   double tr = 0.0;
   IMC_HIP(hipMemcpyAsync(&tr, c->scal.p + 5, sizeof(double), hipMemcpyDeviceToHost, s));
   IMC_HIP(hipStreamSynchronize(s));
-  if (post_out) std::memcpy(post_out, c->stage.slot(nb, 2), sizeof(double) * nb);
+  if (post_out && h_post != post_out) std::memcpy(post_out, h_post, sizeof(double) * nb);
   if (total_rad_out) *total_rad_out = tr;
   if (ecen_total_out) *ecen_total_out = E;
   return kOk;
