@@ -206,8 +206,10 @@ int imc2d_transport_cf(imc2d_ctx* ctx, double dt, const int32_t* reflect, int64_
  * as one device-resident sequence: the pool sizes the random numbers decide
  * live in device memory, so no host round trip splits the step.  use_graph
  * != 0 captures the sequence into a HIP graph on first use and replays it
- * while the parameters, the buffers and a pool bound (1/8 above the step
- * that captured it) hold; one graph is kept per pool-buffer parity.
+ * while the parameters, the buffers and a pool bound hold (1/8 above the
+ * larger of the capturing step's pool and Nmax + 3 ncell + the step's
+ * sources, so steady steps replay); one graph is kept per pool-buffer
+ * parity.
  * reflect[4] as imc2d_transport; rz_linear as imc2d_sample_volume.
  * Outputs (host, nullable; per cell): T, T_r, u after the update, dep
  * (deposited energy density), si (scalar intensity), E (emission energies).
