@@ -176,13 +176,16 @@ def test_mg_transport_fp64_history_parity(L, M, geometry):
     # that must stay rare, and the totals must not notice it
     assert same.mean() > 0.999
     vol = r["vol"][None, :]
-    assert np.sum(dep_d.reshape(10, -1) * vol) == pytest.approx(np.sum(dep * vol), rel=1e-5)
-    assert bl_d[0] == pytest.approx(bl[0], rel=1e-5, abs=1e-30)
-    assert np.sum(got[0]) == pytest.approx(np.sum(ref.w), rel=1e-5)
+    # totals differ by at most what the divergent histories carry: each one's
+    # census weight, deposit and leak lie between 0 and its starting weight
+    w0 = np.sum(pool.w)
+    slack = float(np.sum(pool.w[~same])) + 1e-12 * w0
+    assert abs(np.sum(dep_d.reshape(10, -1) * vol) - np.sum(dep * vol)) <= slack
+    assert abs(bl_d[0] - bl[0]) <= slack
+    assert abs(np.sum(got[0]) - np.sum(ref.w)) <= slack
     close = np.isclose(dep_d.reshape(10, -1), dep, rtol=1e-6, atol=1e-9 * np.abs(dep).max())
     assert close.mean() > 0.98
     np.testing.assert_allclose(st_d, st, rtol=2e-3, atol=2)
-    w0 = np.sum(pool.w)
     assert np.sum(got[0]) + np.sum(dep_d.reshape(10, -1) * vol) + bl_d[0] == pytest.approx(w0, rel=1e-12)
 
 
@@ -307,7 +310,9 @@ def test_mg_full_step_fp64_matches_oracle(L, M, geometry):
     np.testing.assert_allclose(a["boundary_emission_by_group"], b["boundary_emission_by_group"], rtol=1e-12)
     np.testing.assert_allclose(a["emission_by_group"], b["emission_by_group"], rtol=1e-12, atol=1e-300)
     vol = M.volumes(*_mesh(10, 8), geometry).reshape(1, 10, 8)
-    assert np.sum(a["deposited_by_group"] * vol) == pytest.approx(np.sum(b["dep"] * vol), rel=1e-5)
+    # a history that diverges on a device-libm ulp (about 2 in 10^4) moves the
+    # total by up to its weight, ~1/16000 of the step's energy: allow five
+    assert np.sum(a["deposited_by_group"] * vol) == pytest.approx(np.sum(b["dep"] * vol), rel=3e-4)
     close = np.isclose(a["deposited_by_group"], b["dep"], rtol=1e-6, atol=1e-9 * np.abs(b["dep"]).max())
     assert close.mean() > 0.98
     for x in a_all:
