@@ -248,7 +248,8 @@ int imc2d_adjoint(imc2d_ctx* ctx, double dt, const int32_t* reflect, int64_t max
  * emission sample; fix (nullable) pins them for later samples (common random
  * numbers across perturbed runs); both null returns to the ceil rule. */
 int imc2d_emission_counts(imc2d_ctx* ctx, int64_t* out, const int64_t* fix);
-/* Copy the census into a device snapshot (save != 0) or back from it. */
+/* Copy the census into a device snapshot (save != 0) or back from it (an
+ * error before the first save). */
 int imc2d_snapshot(imc2d_ctx* ctx, int save);
 
 /* ---- gray 1-D (slab / sphere), batched ------------------------------------ */
@@ -375,7 +376,8 @@ int mgimc_set_launch(mgimc_ctx* ctx, int grid, int batch, int tally_copies);
 int mgimc_set_tracker(mgimc_ctx* ctx, int block, int refill_min);
 /* Copy of the particle pool on the device (no reference counterpart; lets
  * a staged step be transported several times, as the autotuner does):
- * save = 1 keeps the current pool, save = 0 puts the kept one back. */
+ * save = 1 keeps the current pool, save = 0 puts the kept one back (an
+ * error before the first save). */
 int mgimc_snapshot(mgimc_ctx* ctx, int save);
 
 /* Census: weights, dir1, dir2, times, pos1, pos2, cell_i, cell_j, groups

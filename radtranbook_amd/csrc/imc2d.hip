@@ -1620,7 +1620,7 @@ struct Imc2dCtx {
   // range of the step's pool, optional fixed emission counts, a snapshot
   PoolBuf S, snap;
   bool adjoint = false;
-  int64_t n_saved = 0, n_snap = 0, emit_lo = 0, emit_hi = 0;
+  int64_t n_saved = 0, n_snap = -1, emit_lo = 0, emit_hi = 0;  // n_snap < 0: no snapshot saved
   bool fixed_counts = false;
   DevBuf<int64_t> fcounts;
   DevBuf<double> g_raw, g;
@@ -3346,12 +3346,13 @@ int imc2d_emission_counts(imc2d_ctx* c, int64_t* out, const int64_t* fix) {
 // Copy the census to a snapshot (save != 0) or restore it from the snapshot.
 int imc2d_snapshot(imc2d_ctx* c, int save) {
   IMC_REQUIRE(c, kErrArg, "null ctx");
+  IMC_REQUIRE(save || c->n_snap >= 0, kErrState, "imc2d_snapshot: no snapshot saved to restore");
   IMC_HIP(hipSetDevice(c->device));
   hipStream_t s = c->stream;
   PoolBuf& from = save ? c->A : c->snap;
   PoolBuf& to = save ? c->snap : c->A;
   int64_t n = save ? c->n : c->n_snap;
-  int rc = to.grow(std::max<int64_t>(n, 1), save ? 0 : 0, s);
+  int rc = to.grow(std::max<int64_t>(n, 1), 0, s);
   if (rc) return rc;
   int re = c->precision == IMC_FP64 ? 8 : 4;
   void* src[8] = {from.w, from.d1, from.d2, from.t, from.p1, from.p2, from.ci, from.cj};

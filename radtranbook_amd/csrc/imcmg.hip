@@ -1326,7 +1326,7 @@ struct MgCtx {
   int grid_override = 0, batch = 256, tally_copies = kMgTallyCopies;
   int refill_min = 1;  // idle lanes before a wave of the 2-D tracker refills
   MgPool snap;         // mgimc_snapshot
-  int64_t n_snap = 0;
+  int64_t n_snap = -1;  // < 0: no snapshot saved
   bool have_material = false;
   // page-locked staging of the (G, ncell) transfers: slot 0 opacities in,
   // slot 1 scalar intensity out, slot 2 post-comb census energy out
@@ -1584,6 +1584,7 @@ int mgimc_set_tracker(mgimc_ctx* c, int block, int refill_min) {
 // one back (it stays kept).
 int mgimc_snapshot(mgimc_ctx* c, int save) {
   IMC_REQUIRE(c, kErrArg, "null ctx");
+  IMC_REQUIRE(save || c->n_snap >= 0, kErrState, "mgimc_snapshot: no snapshot saved to restore");
   IMC_HIP(hipSetDevice(c->device));
   hipStream_t s = c->stream;
   MgPool& from = save ? c->A : c->snap;

@@ -153,3 +153,33 @@ def test_batch1d_tuned_run_steps_as_untuned():
     shape, rep = got[1][0]["tuned"]
     assert shape in AT.candidates("gray1d")
     assert rep["tuned_ms"] <= rep["default_ms"]
+
+
+def test_snapshot_restore_needs_a_saved_snapshot():
+    """Putting back a pool that was never saved is an error, not an empty
+    pool; after a save the restore brings back the saved census."""
+    _announce("test_snapshot_restore_needs_a_saved_snapshot")
+    from radtranbook_amd import _lib as L
+
+    e1, e2 = np.linspace(0.0, 1.0, 5), np.linspace(0.0, 1.0, 4)
+    eng = L.Engine2D(e1, e2, "xy", "fp64", seed=3)
+    mg = L.EngineMG(e1, e2, np.logspace(-4, 1, 4), "xy", "fp64", seed=3)
+    try:
+        with pytest.raises(L.ImcError):
+            eng.snapshot(False)
+        with pytest.raises(L.ImcError):
+            mg.snapshot(False)
+        n = 37
+        rng = np.random.default_rng(1)
+        ci = rng.integers(0, 4, n).astype(np.int64)
+        cj = rng.integers(0, 3, n).astype(np.int64)
+        arrs = [rng.random(n) for _ in range(6)]
+        eng.put_census(*arrs, ci, cj)
+        eng.snapshot(True)
+        eng.put_census(*[a[:5] for a in arrs], ci[:5], cj[:5])
+        eng.snapshot(False)
+        got = eng.get_census()
+        assert len(got[0]) == n and np.array_equal(got[0], arrs[0]) and np.array_equal(got[6], ci)
+    finally:
+        eng.close()
+        mg.close()
