@@ -294,6 +294,7 @@ class BatchIMC1D:
         self.eng = Engine1D([_edges(m) for m in self.meshes], geometry=geometry, precision=precision, seed=seed,
                             device=device)
         self.volumes = [_cell_volumes(m, geometry) for m in self.meshes]
+        self._vcat = np.concatenate(self.volumes)
         self.reset(Tinit, Tr_init, eos, inv_eos, Ntarget_ic, T_emit_floor, seed)
 
     def reset(self, Tinit, Tr_init, eos, inv_eos, Ntarget_ic, T_emit_floor=0.0, seed=None):
@@ -330,10 +331,14 @@ class BatchIMC1D:
         P = self.P
         eng.set_step(self.count)
         dtv = self._per(dt)
-        sig = np.concatenate([np.asarray(self._fn(sigma_a_func, p)(self.T[p]), dtype=np.float64)
-                              * np.ones(self.T[p].size) for p in range(P)])
-        cvv = np.concatenate([np.asarray(self._fn(cv, p)(self.T[p]), dtype=np.float64) * np.ones(self.T[p].size)
-                              for p in range(P)])
+        # the callables are per problem; their values go straight into the
+        # concatenated per-cell arrays (a scalar fills its problem's cells)
+        coff = eng.coff
+        sig = np.empty(int(coff[-1]))
+        cvv = np.empty(int(coff[-1]))
+        for p in range(P):
+            sig[coff[p]:coff[p + 1]] = self._fn(sigma_a_func, p)(self.T[p])
+            cvv[coff[p]:coff[p + 1]] = self._fn(cv, p)(self.T[p])
         _set_material(eng, np.concatenate(self.T), sig, cvv, self._per(theta), dtv)
         Tb = np.broadcast_to(np.asarray(T_boundary, dtype=np.float64).reshape(-1, 2), (P, 2))
         nb = self._per(Nboundary, np.int64)
@@ -360,13 +365,16 @@ class BatchIMC1D:
         if stats[6] > 0:
             raise AssertionError("non-positive distance in move_particle")
         _, _, ecen_total, total_rad = eng.comb(self._per(NMax, np.int64))
-        dep_p, si_p, ec_p = eng.split(dep), eng.split(si), eng.split(Ec)
-        total = np.zeros(P)
-        for p in range(P):
-            self.u[p] = self.u[p] + dep_p[p] - ec_p[p] / self.volumes[p]
-            self.T[p] = self._fn(self._inv, p)(self.u[p])
-            self.Tr[p] = quarter_root(si_p[p] / A_RAD / C_LIGHT)
-            total[p] = float(np.sum(self.u[p] * self.volumes[p])) + total_rad[p]
+        # material update over all cells at once (IMC/IMC1D.py:1021-1035); only
+        # inv_eos is called per problem.  Fresh arrays every step: the lists
+        # handed out earlier keep their values.
+        u = np.concatenate(self.u)
+        u += dep
+        u -= Ec / self._vcat
+        self.u = eng.split(u)
+        self.T = [np.asarray(self._fn(self._inv, p)(self.u[p]), dtype=np.float64) for p in range(P)]
+        self.Tr = eng.split(quarter_root(si / A_RAD / C_LIGHT))
+        total = np.add.reduceat(u * self._vcat, coff[:-1]) + total_rad
         loss = total - self.previous_total - be + leak
         disc = ecen_total - total_rad
         self.previous_total = total
