@@ -371,19 +371,37 @@ __global__ void k1_gather(Pool1View<R> A, Pool1View<R> N, Pool1View<R> Dst, int 
   Dst.prob[k] = S.prob[i];
 }
 
-// Deterministic per-problem sums over grouped pool segments (one workgroup
-// per problem, fixed tree).
-__global__ __launch_bounds__(256) void k1_segment_sums(const double* __restrict__ w, const int64_t* __restrict__ poff,
-                                                       double* __restrict__ out) {
-  __shared__ double ws[4];
-  int p = blockIdx.x;
-  double v = 0.0;
-  for (int64_t i = poff[p] + threadIdx.x; i < poff[p + 1]; i += 256) v += w[i];
+// Deterministic per-problem sums over grouped pool segments: one workgroup
+// of kSegThreads per problem, four independent accumulators per thread (a
+// single dependent add chain per thread left the largest segments, ~2e6
+// particles, waiting on one load at a time), combined in a fixed order.
+constexpr int kSegThreads = 1024;
+__global__ __launch_bounds__(kSegThreads) void k1_segment_sums(const double* __restrict__ w,
+                                                               const int64_t* __restrict__ poff,
+                                                               double* __restrict__ out) {
+  __shared__ double ws[kSegThreads / 64];
+  const int p = blockIdx.x;
+  const int64_t lo = poff[p], hi = poff[p + 1];
+  const int64_t bd = kSegThreads;
+  double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
+  int64_t i = lo + threadIdx.x;
+  for (; i + 3 * bd < hi; i += 4 * bd) {
+    v0 += w[i];
+    v1 += w[i + bd];
+    v2 += w[i + 2 * bd];
+    v3 += w[i + 3 * bd];
+  }
+  for (; i < hi; i += bd) v0 += w[i];
+  double v = (v0 + v1) + (v2 + v3);
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = v;
   __syncthreads();
-  if (threadIdx.x == 0) out[p] = (ws[0] + ws[1]) + (ws[2] + ws[3]);
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int q = 0; q < kSegThreads / 64; ++q) t += ws[q];
+    out[p] = t;
+  }
 }
 
 __global__ void k1_floor(int P, const double* __restrict__ sums, const int64_t* __restrict__ poff,
@@ -1337,7 +1355,7 @@ int imc1d_census_energy(imc1d_ctx* c, double* per_cell, double* per_problem) {
     for (int p = 0; p < P; ++p) off[p + 1] = off[p] + c->h_ncen[p];
     IMC_HIP(hipMemcpyAsync(c->taken.p, off.data(), sizeof(int64_t) * (P + 1), hipMemcpyHostToDevice, s));
     if (n > 0) {
-      k1_segment_sums<<<P, 256, 0, s>>>(c->A.w, c->taken.p, c->psum.p);
+      k1_segment_sums<<<P, kSegThreads, 0, s>>>(c->A.w, c->taken.p, c->psum.p);
       IMC_HIP(hipGetLastError());
     } else {
       IMC_HIP(hipMemsetAsync(c->psum.p, 0, sizeof(double) * P, s));
@@ -1683,7 +1701,7 @@ int imc1d_transport(imc1d_ctx* c, const int32_t* reflect, const double* weight_f
   for (int p = 0; p < P; ++p) any_auto |= fl[p] < 0.0;
   if (any_auto) {
     if (n > 0) {
-      k1_segment_sums<<<P, 256, 0, s>>>(c->B.w, c->poff.p, c->psum.p);
+      k1_segment_sums<<<P, kSegThreads, 0, s>>>(c->B.w, c->poff.p, c->psum.p);
       IMC_HIP(hipGetLastError());
     } else {
       IMC_HIP(hipMemsetAsync(c->psum.p, 0, sizeof(double) * P, s));
@@ -1831,7 +1849,7 @@ int imc1d_transport_cf(imc1d_ctx* c, const int32_t* reflect, const double* weigh
   for (int p = 0; p < P; ++p) any_auto |= fl[p] < 0.0;
   if (any_auto) {
     if (n > 0) {
-      k1_segment_sums<<<P, 256, 0, s>>>(c->B.w, c->poff.p, c->psum.p);
+      k1_segment_sums<<<P, kSegThreads, 0, s>>>(c->B.w, c->poff.p, c->psum.p);
       IMC_HIP(hipGetLastError());
     } else {
       IMC_HIP(hipMemsetAsync(c->psum.p, 0, sizeof(double) * P, s));
@@ -1972,7 +1990,7 @@ int imc1d_comb(imc1d_ctx* c, const int64_t* Nmax, double* ecen_out, double* disc
   if (total_rad_out) {
     IMC_HIP(hipMemcpyAsync(c->poff.p, newoff.data(), sizeof(int64_t) * (P + 1), hipMemcpyHostToDevice, s));
     if (total > 0) {
-      k1_segment_sums<<<P, 256, 0, s>>>(c->A.w, c->poff.p, c->psum.p);
+      k1_segment_sums<<<P, kSegThreads, 0, s>>>(c->A.w, c->poff.p, c->psum.p);
       IMC_HIP(hipGetLastError());
     } else {
       IMC_HIP(hipMemsetAsync(c->psum.p, 0, sizeof(double) * P, s));
