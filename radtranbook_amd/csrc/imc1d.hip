@@ -354,13 +354,25 @@ __global__ void k1_sample_boundary(Pool1View<R> D, int64_t base, int64_t total, 
 }
 
 // Gather segments into the grouped transport pool: destination range
-// [dst[s], dst[s+1]) comes from src_buf[s] starting at src[s].
+// [dst[s], dst[s+1]) comes from src_buf[s] starting at src[s].  One thread
+// per destination; two threads of the workgroup find the segments of its
+// first and last destinations by binary search, and every thread walks
+// forward from the first (a workgroup spans few segments), instead of each
+// running the whole search through global memory.
 template <typename R>
-__global__ void k1_gather(Pool1View<R> A, Pool1View<R> N, Pool1View<R> Dst, int nseg, const int64_t* __restrict__ dst,
-                          const int64_t* __restrict__ src, const int32_t* __restrict__ from_new, int64_t total) {
-  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k1_gather(Pool1View<R> A, Pool1View<R> N, Pool1View<R> Dst, int nseg,
+                                                 const int64_t* __restrict__ dst, const int64_t* __restrict__ src,
+                                                 const int32_t* __restrict__ from_new, int64_t total) {
+  __shared__ int64_t sb;
+  const int64_t k0 = (int64_t)blockIdx.x * blockDim.x;
+  const int64_t k = k0 + threadIdx.x;
+  if (threadIdx.x == 0) sb = bin_of(dst, nseg, k0);
+  __syncthreads();
   if (k >= total) return;
-  int s = (int)bin_of(dst, nseg, k);
+  // the first segment whose end passes k (bin_of's answer): the walk ends by
+  // the last destination's segment, since k < total = dst[nseg]
+  int64_t s = sb;
+  while (dst[s + 1] <= k) ++s;
   int64_t i = src[s] + (k - dst[s]);
   const Pool1View<R>& S = from_new[s] ? N : A;
   Dst.w[k] = S.w[i];
