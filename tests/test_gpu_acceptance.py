@@ -318,7 +318,7 @@ def _run_mg_powerlaw(prob, seed, nsteps):
 
 def test_mg_powerlaw_marshak_matches_published_run(published_mg):
     """The first ten steps (0.1 ns) at the published particle counts, eight
-    seeds.  Each published figure is one run, so a difference is judged as
+    seeds.  Each published figure is one run, so a difference is read as
     the t statistic (ours - published) / (sd sqrt(1 + 1/8)) with 7 degrees of
     freedom (sd the spread of one run, from the seeds; the rounding of the
     printed digits added in quadrature), converted to the Gaussian z of the
